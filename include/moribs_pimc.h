@@ -1,0 +1,125 @@
+/*
+ * moribs_pimc.h -- C ABI of the MI355X (gfx950) path-integral Monte Carlo sampler.
+ *
+ * The library replaces MoRiBS-PIMC's sampling hot path: the translational centroid and
+ * multilevel-bisection moves, the rotor moves on tabulated free-rotor density matrices,
+ * the pair / rotor-particle potential-energy differences they need, and the energy
+ * estimators evaluated on the sampled paths.  Plain C types only; every buffer passed in
+ * is caller-owned host memory and is copied.  All device work of one context is ordered
+ * on one HIP stream.  Errors are reported by a negative return value and a message from
+ * pimc_last_error(); nothing calls exit() (the reference exits from nrerror(),
+ * mc_utils.cc:99-108, and _io_error(), mc_input.h:35-42).
+ *
+ * Reference interfaces replaced (file:line in upstream MoRiBS-PIMC):
+ *   moves        MCMolecularMove mc_piqmc.cc:54, MCBisectionMove :194, MCRotationsMove :421,
+ *                MCRotations3D :692 (declared mc_piqmc.h:4-23), scheduled by PIMCPass
+ *                mc_main.cc:510-522 inside the pass loop mc_main.cc:348-381
+ *   energies     PotEnergy(int,double**,int) mc_piqmc.cc:1796, PotEnergy(int,double**) :1201,
+ *                PotRotEnergy :1967, PotRotE3D :2046
+ *   tables       InitPotentials/InitRotDensity mc_poten.cc:85-167 reading init_pot1D :379,
+ *                init_pot2D :305, init_pot3D :254, init_rotdens :508, init_rot3D :440
+ *   state        MCCoords/MCAngles/MCCosine mc_setup.h:176-190, laid out [3][NumbAtoms*NumbTimes]
+ *   counters     MCTotal/MCAccep[type][MCMOLEC|MCMULTI|MCROTAT] mc_piqmc.h:25-26, mc_setup.h:161-164
+ *   estimators   MCGetAverage mc_main.cc:551-616 (GetKinEnergy mc_estim.cc:876,
+ *                GetPotEnergy_Densities :500, GetRotEnergy :939, GetRotE3D :989)
+ */
+#ifndef MORIBS_PIMC_H
+#define MORIBS_PIMC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pimc_ctx pimc_ctx;
+
+#define PIMC_MAX_TYPES 4
+
+/* context flags */
+#define PIMC_FLAG_FP64        0x1u  /* pair terms in FP64 instead of FP32 (parity runs) */
+#define PIMC_FLAG_REF_ORDER   0x2u  /* move order of mc_main.cc:348-381 (one segment at a time) */
+#define PIMC_FLAG_PROFILE     0x4u  /* time every kernel launch with HIP events */
+
+/* move kinds, the second index of MCTotal/MCAccep (mc_setup.h:161-164) */
+#define PIMC_MCMOLEC 0
+#define PIMC_MCMULTI 1
+#define PIMC_MCROTAT 2
+
+/* One replica (an independent Markov chain): the system of qmc.input.
+   Types follow the reference: atoms first, then at most one molecule type
+   (mc_input.cc:194-224, 386-387). */
+typedef struct {
+    int    ntypes;                   /* NumbTypes */
+    int    numb[PIMC_MAX_TYPES];     /* MCAtom[t].numb */
+    int    molecule[PIMC_MAX_TYPES]; /* 0 atom, 1 linear rotor, 2 non-linear rotor (MCAtom[t].molecule) */
+    int    levels[PIMC_MAX_TYPES];   /* bisection levels (MCAtom[t].levels) */
+    double mcstep[PIMC_MAX_TYPES];   /* centroid step, Angstrom (MCAtom[t].mcstep) */
+    double mass[PIMC_MAX_TYPES];     /* amu (MCInitParams mc_setup.cc:244-319) */
+    int    pot1d[PIMC_MAX_TYPES];    /* 1D table used when the partner is of this type, or -1 */
+    int    pot2d[PIMC_MAX_TYPES];    /* 2D table of the linear-rotor type, or -1 */
+    int    rotdens;                  /* .rot table of the linear rotor, or -1 */
+    int    P;                        /* NUMBEROFSLICES */
+    int    Prot;                     /* rotational slices (ROTATION line), 0 = no rotation */
+    double T;                        /* TEMPERATURE, Kelvin */
+    double rtstep;                   /* rotational step (ROTATION line) */
+} pimc_replica_desc;
+
+/* ---- lifetime ---- */
+/* seed and replica index form the key of the counter-based generator (Philox4x32-10)
+   that replaces SPRNG streams (mc_randg.cc) and RngStream (omprng.cc). */
+int  pimc_create(int n_replicas, unsigned flags, unsigned seed, pimc_ctx** out);
+void pimc_destroy(pimc_ctx* ctx);
+const char* pimc_last_error(const pimc_ctx* ctx);
+int  pimc_device_name(char* buf, int len);
+
+/* ---- tables (host arrays in Kelvin / Angstrom / reference file order) ---- */
+/* init_pot1D mc_poten.cc:379-438: clamped spline + U0 exp(-alpha r) and -C6/r^6 tails */
+int  pimc_add_pot1d(pimc_ctx* ctx, int n, const double* r, const double* v);
+/* init_pot2D mc_poten.cc:305-377: header sizes/steps, grids, V[r][cos] */
+int  pimc_add_pot2d(pimc_ctx* ctx, int nr, int nc, double dr, double dc, const double* rgrid,
+                    const double* cgrid, const double* v);
+/* init_rotdens mc_poten.cc:508-546: the four .rot columns */
+int  pimc_add_rotdens(pimc_ctx* ctx, int n, const double* cosg, const double* rho,
+                      const double* erot, const double* esq);
+
+/* ---- replicas and state ---- */
+int  pimc_set_replica(pimc_ctx* ctx, int replica, const pimc_replica_desc* desc);
+int  pimc_commit(pimc_ctx* ctx);   /* allocates device state after all replicas are set */
+/* coords/angles [3][natoms*P] like MCCoords/MCAngles; cosines follow from angles
+   (mc_main.cc:186-197).  angles may be NULL (phi=0, cos(theta)=1, chi=0, mc_setup.cc:476-489). */
+int  pimc_set_state(pimc_ctx* ctx, int replica, const double* coords, const double* angles);
+int  pimc_get_state(pimc_ctx* ctx, int replica, double* coords, double* angles, double* cosines);
+
+/* ---- sampling ---- */
+/* n_passes passes of PIMCPass over all replicas; one MCGetAverage sample per replica
+   every sample_every passes (0: none).  Pass numbers continue across calls. */
+int  pimc_run(pimc_ctx* ctx, int n_passes, int sample_every);
+/* samples of one replica, 8 doubles each: K, V, E_rot, E_rot^2 term (ErotSQ), Cv, Cv_trans,
+   Cv_rot, Erot_termSQ (mc_main.cc:551-616).  Returns the number copied. */
+int  pimc_get_samples(pimc_ctx* ctx, int replica, double* out, int max_samples);
+int  pimc_clear_samples(pimc_ctx* ctx);
+/* MCTotal/MCAccep of one replica, [ntypes][3] */
+int  pimc_get_counters(pimc_ctx* ctx, int replica, double* total, double* accep);
+/* accumulated g(r) (300 bins) and g(r,theta) (300x50) counts of one replica
+   (bin_1Ddensity/bin_2Ddensity mc_estim.cc:1233-1286) */
+int  pimc_get_histograms(pimc_ctx* ctx, int replica, double* gr1d, double* gr2d);
+/* pair evaluations performed and bead moves proposed since creation (all replicas) */
+int  pimc_get_work(pimc_ctx* ctx, double* pair_evals, double* bead_moves);
+
+/* ---- energies on the current state (batched) ---- */
+/* PotEnergy(atom, pos, it) mc_piqmc.cc:1796: out[q] for queries (replica, atom, it) with
+   the atom's bead at slice it replaced by pos[3q..3q+2] */
+int  pimc_pot_energy_slice(pimc_ctx* ctx, int nq, const int* replica, const int* atom,
+                           const int* it, const double* pos, double* out);
+/* one MCGetAverage sample of every replica on the current state, out[8*replica] */
+int  pimc_estimate(pimc_ctx* ctx, double* out);
+
+/* ---- measurement ---- */
+/* accumulated device time (ms) and launch count of a kernel family when PIMC_FLAG_PROFILE is
+   set: "bisect", "centroid", "rotor", "estimate" */
+int  pimc_kernel_time(pimc_ctx* ctx, const char* family, double* ms, long* launches);
+int  pimc_synchronize(pimc_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

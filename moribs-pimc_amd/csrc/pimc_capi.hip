@@ -1,0 +1,750 @@
+// pimc_capi.hip -- host side of the C ABI declared in include/moribs_pimc.h.
+//
+// A context holds any number of independent replicas (Markov chains).  pimc_run() issues,
+// per pass, the move kernels in the order of PIMCPass (mc_main.cc:510-522) grouped by
+// segment offset: for u in 0..S-1 { [u==0: centroid(type), bisection(type) per type],
+// bisection(all types), linear-rotor sweeps u, u+S, ... } -- every atom gets one bisection
+// proposal per segment start and the rotor P sweeps per pass, as in mc_main.cc:348-381.
+// One pass is captured once as a HIP graph and replayed.
+#include "pimc_kernels.hpp"
+#include "../../include/moribs_pimc.h"
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <map>
+#include <algorithm>
+
+using namespace pimc;
+
+namespace {
+
+// physical constants as in mc_const.h:12-14 (lambda = hbar^2/2m in Angstrom^2 K)
+constexpr double kHBAR = 1.05457266, kAMU = 1.6605402, kKB = 1.380658;
+
+// Numerical-Recipes clamped cubic spline (second derivatives), end slopes given.
+void spline_y2(const std::vector<double>& x, const std::vector<double>& y, double yp1, double ypn,
+               std::vector<double>& y2) {
+    const int n = (int)x.size();
+    std::vector<double> u(n);
+    y2.assign(n, 0.0);
+    y2[0] = -0.5;
+    u[0] = (3.0 / (x[1] - x[0])) * ((y[1] - y[0]) / (x[1] - x[0]) - yp1);
+    for (int i = 1; i < n - 1; ++i) {
+        double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+        double p = sig * y2[i - 1] + 2.0;
+        y2[i] = (sig - 1.0) / p;
+        double d = (y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1]);
+        u[i] = (6.0 * d / (x[i + 1] - x[i - 1]) - sig * u[i - 1]) / p;
+    }
+    double qn = 0.5;
+    double un = (3.0 / (x[n - 1] - x[n - 2])) * (ypn - (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]));
+    y2[n - 1] = (un - qn * u[n - 2]) / (qn * y2[n - 2] + 1.0);
+    for (int k = n - 2; k >= 0; --k) y2[k] = y2[k] * y2[k + 1] + u[k];
+}
+
+// end slopes from first differences at both ends (init_spline, mc_utils.cc:188-201)
+void spline_fd(const std::vector<double>& x, const std::vector<double>& y, std::vector<double>& y2) {
+    const int n = (int)x.size();
+    double dpl = (y[1] - y[0]) / (x[1] - x[0]);
+    double dpr = (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]);
+    spline_y2(x, y, dpl, dpr, y2);
+}
+
+struct Family {
+    double ms = 0;
+    long n = 0;
+};
+
+template <typename T>
+T* dalloc(size_t n) {
+    T* p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+    return p;
+}
+
+}  // namespace
+
+struct pimc_ctx {
+    int nrep = 0;
+    unsigned flags = 0, seed = 0;
+    bool fp64 = false, ref_order = false, profile = false, committed = false;
+    std::string err;
+    std::vector<pimc_replica_desc> desc;
+    std::vector<char> desc_set;
+    std::vector<Rep> rep;
+    // host tables
+    std::vector<Tab1D> t1;
+    std::vector<double4> k1;
+    std::vector<Tab2D> t2;
+    std::vector<double> v2, g2;
+    std::vector<TabRot> tr;
+    std::vector<double4> rk, re;
+    // device
+    hipStream_t stream = nullptr;
+    Rep* d_rep = nullptr;
+    Mut* d_mut = nullptr;
+    double4* d_beads = nullptr;
+    RotBead* d_rotb = nullptr;
+    unsigned* d_hist = nullptr;
+    double* d_samples = nullptr;
+    int sample_cap = 0;
+    float4* d_k1f = nullptr;
+    double4* d_k1d = nullptr;
+    Tab1D* d_t1 = nullptr;
+    float* d_v2f = nullptr;
+    double* d_v2d = nullptr;
+    double* d_g2d = nullptr;
+    float* d_g2f = nullptr;
+    Tab2D* d_t2 = nullptr;
+    double4* d_rk = nullptr;
+    double4* d_re = nullptr;
+    TabRot* d_tr = nullptr;
+    int2* d_work = nullptr;
+    int nwork = 0;
+    int* d_all = nullptr;
+    int* d_rotreps = nullptr;
+    int nrotreps = 0;
+    double* d_est = nullptr;
+    size_t lds_bisect = 0, lds_rotor = 0;
+    int Smax = 1, Pmax = 1, max_types = 1;
+    int rotor_block = 256;
+    long long nbeads = 0, nrotb = 0, nhist = 0;
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_ok = false;
+    std::map<std::string, Family> fam;
+    double bead_moves_per_pass = 0;
+    double bead_moves = 0;
+    Tables tables() const {
+        Tables T;
+        T.k1f = d_k1f;
+        T.k1d = d_k1d;
+        T.t1 = d_t1;
+        T.v2f = d_v2f;
+        T.v2d = d_v2d;
+        T.g2d = d_g2d;
+        T.g2f = d_g2f;
+        T.t2 = d_t2;
+        T.rk = d_rk;
+        T.re = d_re;
+        T.tr = d_tr;
+        return T;
+    }
+};
+
+static int fail(pimc_ctx* c, const std::string& m) {
+    if (c) c->err = m;
+    return -1;
+}
+
+#define HIPCHECK(ctx, x)                                                                   \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return fail(ctx, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" {
+
+int pimc_device_name(char* buf, int len) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, 0) != hipSuccess) return -1;
+    std::snprintf(buf, (size_t)len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return 0;
+}
+
+int pimc_create(int n_replicas, unsigned flags, unsigned seed, pimc_ctx** out) {
+    if (!out || n_replicas < 1) return -1;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return -2;  // no HIP device: no fallback
+    pimc_ctx* c = new pimc_ctx();
+    c->nrep = n_replicas;
+    c->flags = flags;
+    c->seed = seed;
+    c->fp64 = flags & PIMC_FLAG_FP64;
+    c->ref_order = flags & PIMC_FLAG_REF_ORDER;
+    c->profile = flags & PIMC_FLAG_PROFILE;
+    c->desc.resize(n_replicas);
+    c->desc_set.assign(n_replicas, 0);
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return -3;
+    }
+    *out = c;
+    return 0;
+}
+
+void pimc_destroy(pimc_ctx* c) {
+    if (!c) return;
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    void* ptrs[] = {c->d_rep, c->d_mut, c->d_beads, c->d_rotb, c->d_hist, c->d_samples, c->d_k1f, c->d_k1d,
+                    c->d_t1, c->d_v2f, c->d_v2d, c->d_g2d, c->d_g2f, c->d_t2, c->d_rk, c->d_re, c->d_tr,
+                    c->d_work, c->d_all, c->d_rotreps, c->d_est};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* pimc_last_error(const pimc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int pimc_add_pot1d(pimc_ctx* c, int n, const double* r, const double* v) {
+    if (c->committed) return fail(c, "tables must be added before pimc_commit");
+    if (n < 2) return fail(c, "init_pot1D: Not enough data points for extrapolation");
+    std::vector<double> x(r, r + n), y(v, v + n), y2;
+    for (int i = 1; i < n; ++i)
+        if (!(x[i] > x[i - 1])) return fail(c, "init_pot1D: grid must increase");
+    spline_fd(x, y, y2);
+    Tab1D t;
+    t.n = n;
+    t.off = (int)c->k1.size();
+    t.x0 = x[0];
+    t.xmax = x[n - 1];
+    t.inv_h = (double)(n - 1) / (x[n - 1] - x[0]);
+    // U0 exp(-alpha r) below the grid, -C6/r^6 above (mc_poten.cc:420-437)
+    t.alpha = std::log(y[0] / y[1]) / (x[1] - x[0]);
+    t.unode = y[0] * std::exp(t.alpha * x[0]);
+    double r0 = std::pow(x[n - 2], 6.0), r1 = std::pow(x[n - 1], 6.0);
+    t.c6 = (y[n - 1] - y[n - 2]) / (1.0 / r0 - 1.0 / r1);
+    for (int i = 0; i < n; ++i) c->k1.push_back(make_double4(x[i], y[i], y2[i], 0.0));
+    c->t1.push_back(t);
+    return (int)c->t1.size() - 1;
+}
+
+int pimc_add_pot2d(pimc_ctx* c, int nr, int nc, double dr, double dc, const double* rg, const double* cg,
+                   const double* v) {
+    if (c->committed) return fail(c, "tables must be added before pimc_commit");
+    if (nr < 2 || nc < 2) return fail(c, "init_pot2D: grid needs two points per axis");
+    Tab2D t;
+    t.nr = nr;
+    t.nc = nc;
+    t.off = (long long)c->v2.size();
+    t.goff = (int)c->g2.size();
+    t.pad = 0;
+    t.dr = dr;
+    t.dc = dc;
+    c->g2.insert(c->g2.end(), rg, rg + nr);
+    c->g2.insert(c->g2.end(), cg, cg + nc);
+    c->v2.insert(c->v2.end(), v, v + (size_t)nr * nc);
+    c->t2.push_back(t);
+    return (int)c->t2.size() - 1;
+}
+
+int pimc_add_rotdens(pimc_ctx* c, int n, const double* x, const double* rho, const double* erot, const double* esq) {
+    if (c->committed) return fail(c, "tables must be added before pimc_commit");
+    if (n < 2) return fail(c, "init_rotdens: too few points");
+    std::vector<double> gx(x, x + n), yr(rho, rho + n), ye(erot, erot + n), ys(esq, esq + n), s0, s1, s2;
+    spline_fd(gx, yr, s0);
+    spline_fd(gx, ye, s1);
+    spline_fd(gx, ys, s2);
+    TabRot t;
+    t.n = n;
+    t.off = (int)c->rk.size();
+    t.x0 = gx[0];
+    t.xmax = gx[n - 1];
+    t.inv_h = (double)(n - 1) / (gx[n - 1] - gx[0]);
+    for (int i = 0; i < n; ++i) {
+        c->rk.push_back(make_double4(gx[i], yr[i], s0[i], 0.0));
+        c->re.push_back(make_double4(ye[i], s1[i], ys[i], s2[i]));
+    }
+    c->tr.push_back(t);
+    return (int)c->tr.size() - 1;
+}
+
+int pimc_set_replica(pimc_ctx* c, int r, const pimc_replica_desc* d) {
+    if (c->committed) return fail(c, "replicas must be set before pimc_commit");
+    if (r < 0 || r >= c->nrep) return fail(c, "replica index out of range");
+    if (d->ntypes < 1 || d->ntypes > PIMC_MAX_TYPES) return fail(c, "ntypes out of range");
+    if (d->P < 2 || d->P > 65535) return fail(c, "P out of range");
+    int natoms = 0, nmolt = 0;
+    for (int t = 0; t < d->ntypes; ++t) {
+        if (d->numb[t] < 1) return fail(c, "every type needs at least one atom");
+        if (d->molecule[t] != 0 && t != d->ntypes - 1) return fail(c, "Molecules should follow atoms in input file");
+        if (d->molecule[t] == 2) return fail(c, "non-linear rotors are not supported by this build");
+        if (d->molecule[t] == 1) {
+            ++nmolt;
+            if (d->numb[t] != 1) return fail(c, "No more than one linear dopant molecule so far");
+        }
+        if (d->levels[t] < 1 || (1 << d->levels[t]) >= d->P)
+            return fail(c, "Segment size is larger then a number of time slices");
+        natoms += d->numb[t];
+    }
+    if (natoms > 4096) return fail(c, "too many atoms");
+    if (d->Prot > 0 && d->P % d->Prot) return fail(c, "NumbTimes is not proportional to NumbRotTimes");
+    c->desc[r] = *d;
+    c->desc_set[r] = 1;
+    return 0;
+}
+
+int pimc_commit(pimc_ctx* c) {
+    if (c->committed) return fail(c, "already committed");
+    for (int r = 0; r < c->nrep; ++r)
+        if (!c->desc_set[r]) return fail(c, "replica " + std::to_string(r) + " not set");
+    const double lam = 100.0 * (kHBAR * kHBAR) / (kAMU * kKB);
+    c->rep.resize(c->nrep);
+    long long boff = 0, roff = 0, hoff = 0;
+    std::vector<int2> work;
+    std::vector<int> rotreps;
+    size_t lds_b = 0, lds_r = 0;
+    c->bead_moves_per_pass = 0;
+    for (int r = 0; r < c->nrep; ++r) {
+        const pimc_replica_desc& d = c->desc[r];
+        Rep& p = c->rep[r];
+        std::memset(&p, 0, sizeof p);
+        p.ntypes = d.ntypes;
+        p.P = d.P;
+        p.Prot = d.Prot > 0 ? d.Prot : d.P;
+        p.RotRatio = p.P / p.Prot;
+        p.T = d.T;
+        p.beta = 1.0 / d.T;
+        p.tau = p.beta / (double)d.P;
+        p.rtstep = d.rtstep;
+        p.imtype = -1;
+        p.rotkind = 0;
+        p.rot_atom = -1;
+        p.key = r;
+        p.rotdens = d.rotdens;
+        int g = 0, S = 1;
+        for (int t = 0; t < d.ntypes; ++t) {
+            p.tnumb[t] = d.numb[t];
+            p.tmol[t] = d.molecule[t];
+            p.tlev[t] = d.levels[t];
+            p.tbegin[t] = g;
+            p.lambda[t] = 0.5 * lam / d.mass[t];
+            p.mcstep[t] = d.mcstep[t];
+            p.pot1d[t] = d.pot1d[t];
+            p.pot2d[t] = d.pot2d[t];
+            S = std::max(S, 1 << d.levels[t]);
+            if (d.molecule[t] == 1 && d.Prot > 0) {
+                p.imtype = t;
+                p.rotkind = 1;
+                p.rot_atom = g;
+            }
+            g += d.numb[t];
+            c->bead_moves_per_pass += (double)d.numb[t] * d.P * ((1 << d.levels[t]) - 1) + (double)d.numb[t] * d.P;
+        }
+        p.natoms = g;
+        p.S = S;
+        if (!c->ref_order && d.P % S) return fail(c, "segment order needs P divisible by the segment size");
+        // table checks
+        for (int t = 0; t < d.ntypes; ++t) {
+            if (d.molecule[t] == 0 && (d.pot1d[t] < 0 || d.pot1d[t] >= (int)c->t1.size()))
+                return fail(c, "atom type without a 1D potential table");
+            if (d.molecule[t] == 1 && (d.pot2d[t] < 0 || d.pot2d[t] >= (int)c->t2.size()))
+                return fail(c, "linear rotor without a 2D potential table");
+        }
+        if (p.rotkind == 1) {
+            if (d.rotdens < 0 || d.rotdens >= (int)c->tr.size()) return fail(c, "linear rotor without a .rot table");
+            c->bead_moves_per_pass += (double)p.P * p.Prot;
+            rotreps.push_back(r);
+            lds_r = std::max(lds_r, (size_t)5 * p.Prot * sizeof(double));
+        }
+        p.bead_off = boff;
+        boff += (long long)p.natoms * p.P;
+        p.rot_off = roff;
+        roff += (p.rotkind ? (long long)p.tnumb[p.imtype] * p.Prot : 0);
+        p.hist_off = hoff;
+        hoff += BINSR + BINSR * BINST;
+        int W = S + 1;
+        lds_b = std::max(lds_b, (size_t)(3 * p.natoms * W + 6 * W) * sizeof(double));
+        int nseg = c->ref_order ? 1 : p.P / S;
+        for (int k = 0; k < nseg; ++k) work.push_back(make_int2(r, k));
+        c->Smax = std::max(c->Smax, S);
+        c->Pmax = std::max(c->Pmax, p.P);
+        c->max_types = std::max(c->max_types, p.ntypes);
+    }
+    if (lds_b > 160 * 1024) return fail(c, "segment window exceeds LDS (too many atoms for this segment size)");
+    if (lds_r > 150 * 1024) return fail(c, "rotor chain exceeds LDS");
+    c->lds_bisect = lds_b;
+    c->lds_rotor = lds_r;
+    c->nbeads = boff;
+    c->nrotb = roff;
+    c->nhist = hoff;
+    c->nwork = (int)work.size();
+    c->nrotreps = (int)rotreps.size();
+    // device allocations
+    c->d_rep = dalloc<Rep>(c->nrep);
+    c->d_mut = dalloc<Mut>(c->nrep);
+    c->d_beads = dalloc<double4>(boff);
+    c->d_rotb = dalloc<RotBead>(roff);
+    c->d_hist = dalloc<unsigned>(hoff);
+    c->d_work = dalloc<int2>(work.size());
+    c->d_all = dalloc<int>(c->nrep);
+    c->d_rotreps = dalloc<int>(rotreps.size());
+    c->d_est = dalloc<double>(8 * (size_t)c->nrep);
+    c->sample_cap = 4096;
+    c->d_samples = dalloc<double>((size_t)8 * c->sample_cap * c->nrep);
+    for (int r = 0; r < c->nrep; ++r) c->rep[r].sample_off = (long long)8 * c->sample_cap * r;
+    if (!c->d_rep || !c->d_mut || !c->d_beads || !c->d_rotb || !c->d_hist || !c->d_work || !c->d_all ||
+        !c->d_rotreps || !c->d_est || !c->d_samples)
+        return fail(c, "hipMalloc failed");
+    HIPCHECK(c, hipMemcpy(c->d_rep, c->rep.data(), sizeof(Rep) * c->nrep, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemset(c->d_mut, 0, sizeof(Mut) * c->nrep));
+    HIPCHECK(c, hipMemset(c->d_hist, 0, sizeof(unsigned) * std::max<long long>(hoff, 1)));
+    HIPCHECK(c, hipMemcpy(c->d_work, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice));
+    std::vector<int> all(c->nrep);
+    for (int r = 0; r < c->nrep; ++r) all[r] = r;
+    HIPCHECK(c, hipMemcpy(c->d_all, all.data(), sizeof(int) * c->nrep, hipMemcpyHostToDevice));
+    if (!rotreps.empty())
+        HIPCHECK(c, hipMemcpy(c->d_rotreps, rotreps.data(), sizeof(int) * rotreps.size(), hipMemcpyHostToDevice));
+    // default state: every bead at the origin of its own lattice site is the caller's job;
+    // rotor axes start along z (mc_setup.cc:476-489)
+    std::vector<RotBead> rb(std::max<long long>(roff, 1));
+    for (auto& b : rb) {
+        b.phi = 0; b.cth = 1; b.chi = 0; b.pad0 = 0;
+        b.nx = 0; b.ny = 0; b.nz = 1; b.pad1 = 0;
+    }
+    HIPCHECK(c, hipMemcpy(c->d_rotb, rb.data(), sizeof(RotBead) * rb.size(), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemset(c->d_beads, 0, sizeof(double4) * std::max<long long>(boff, 1)));
+    // tables
+    std::vector<float4> k1f(c->k1.size());
+    for (size_t i = 0; i < c->k1.size(); ++i)
+        k1f[i] = make_float4((float)c->k1[i].x, (float)c->k1[i].y, (float)c->k1[i].z, 0.f);
+    std::vector<float> v2f(c->v2.begin(), c->v2.end()), g2f(c->g2.begin(), c->g2.end());
+    c->d_k1f = dalloc<float4>(k1f.size());
+    c->d_k1d = dalloc<double4>(c->k1.size());
+    c->d_t1 = dalloc<Tab1D>(c->t1.size());
+    c->d_v2f = dalloc<float>(v2f.size());
+    c->d_v2d = dalloc<double>(c->v2.size());
+    c->d_g2d = dalloc<double>(c->g2.size());
+    c->d_g2f = dalloc<float>(g2f.size());
+    c->d_t2 = dalloc<Tab2D>(c->t2.size());
+    c->d_rk = dalloc<double4>(c->rk.size());
+    c->d_re = dalloc<double4>(c->re.size());
+    c->d_tr = dalloc<TabRot>(c->tr.size());
+    auto up = [&](void* d, const void* h, size_t bytes) -> int {
+        if (bytes == 0) return 0;
+        return hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+    };
+    if (up(c->d_k1f, k1f.data(), k1f.size() * sizeof(float4)) || up(c->d_k1d, c->k1.data(), c->k1.size() * sizeof(double4)) ||
+        up(c->d_t1, c->t1.data(), c->t1.size() * sizeof(Tab1D)) || up(c->d_v2f, v2f.data(), v2f.size() * sizeof(float)) ||
+        up(c->d_v2d, c->v2.data(), c->v2.size() * sizeof(double)) ||
+        up(c->d_g2d, c->g2.data(), c->g2.size() * sizeof(double)) ||
+        up(c->d_g2f, g2f.data(), g2f.size() * sizeof(float)) || up(c->d_t2, c->t2.data(), c->t2.size() * sizeof(Tab2D)) ||
+        up(c->d_rk, c->rk.data(), c->rk.size() * sizeof(double4)) ||
+        up(c->d_re, c->re.data(), c->re.size() * sizeof(double4)) || up(c->d_tr, c->tr.data(), c->tr.size() * sizeof(TabRot)))
+        return fail(c, "table upload failed");
+    // rotor block: enough lanes for one group per slice move of the largest chain
+    int pr = 1;
+    for (auto& p : c->rep)
+        if (p.rotkind) pr = std::max(pr, p.Prot);
+    c->rotor_block = std::min(1024, std::max(64, ((pr / 2 + 63) / 64) * 64));
+    c->committed = true;
+    return 0;
+}
+
+int pimc_set_state(pimc_ctx* c, int r, const double* coords, const double* angles) {
+    if (!c->committed) return fail(c, "pimc_commit first");
+    if (r < 0 || r >= c->nrep) return fail(c, "replica index out of range");
+    const Rep& p = c->rep[r];
+    const int n = p.natoms * p.P;
+    if (coords) {
+        std::vector<double4> b(n);
+        for (int i = 0; i < n; ++i) b[i] = make_double4(coords[i], coords[n + i], coords[2 * n + i], 0.0);
+        HIPCHECK(c, hipMemcpy(c->d_beads + p.bead_off, b.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    }
+    if (p.rotkind && angles) {
+        const int nr = p.tnumb[p.imtype];
+        std::vector<RotBead> rb((size_t)nr * p.Prot);
+        for (int a = 0; a < nr; ++a)
+            for (int k = 0; k < p.Prot; ++k) {
+                int i = (p.tbegin[p.imtype] + a) * p.P + k;  // rotor slice k lives at offset + k
+                RotBead& b = rb[(size_t)a * p.Prot + k];
+                b.phi = angles[i];
+                b.cth = angles[n + i];
+                b.chi = angles[2 * n + i];
+                double sint = std::sqrt(1.0 - b.cth * b.cth);
+                b.nx = sint * std::cos(b.phi);
+                b.ny = sint * std::sin(b.phi);
+                b.nz = b.cth;
+                b.pad0 = b.pad1 = 0;
+            }
+        HIPCHECK(c, hipMemcpy(c->d_rotb + p.rot_off, rb.data(), sizeof(RotBead) * rb.size(), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int pimc_get_state(pimc_ctx* c, int r, double* coords, double* angles, double* cosines) {
+    if (!c->committed) return fail(c, "pimc_commit first");
+    if (r < 0 || r >= c->nrep) return fail(c, "replica index out of range");
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    const Rep& p = c->rep[r];
+    const int n = p.natoms * p.P;
+    if (coords) {
+        std::vector<double4> b(n);
+        HIPCHECK(c, hipMemcpy(b.data(), c->d_beads + p.bead_off, sizeof(double4) * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) {
+            coords[i] = b[i].x;
+            coords[n + i] = b[i].y;
+            coords[2 * n + i] = b[i].z;
+        }
+    }
+    if (angles || cosines) {
+        for (int i = 0; i < n; ++i) {  // non-rotor entries keep the initial values
+            if (angles) { angles[i] = 0; angles[n + i] = 1; angles[2 * n + i] = 0; }
+            if (cosines) { cosines[i] = 0; cosines[n + i] = 0; cosines[2 * n + i] = 1; }
+        }
+        if (p.rotkind) {
+            const int nr = p.tnumb[p.imtype];
+            std::vector<RotBead> rb((size_t)nr * p.Prot);
+            HIPCHECK(c, hipMemcpy(rb.data(), c->d_rotb + p.rot_off, sizeof(RotBead) * rb.size(), hipMemcpyDeviceToHost));
+            for (int a = 0; a < nr; ++a)
+                for (int k = 0; k < p.Prot; ++k) {
+                    int i = (p.tbegin[p.imtype] + a) * p.P + k;
+                    const RotBead& b = rb[(size_t)a * p.Prot + k];
+                    if (angles) { angles[i] = b.phi; angles[n + i] = b.cth; angles[2 * n + i] = b.chi; }
+                    if (cosines) { cosines[i] = b.nx; cosines[n + i] = b.ny; cosines[2 * n + i] = b.nz; }
+                }
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ launches
+namespace {
+
+struct Timed {
+    pimc_ctx* c;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(pimc_ctx* c_, const char* n) : c(c_), name(n) {
+        if (c->profile) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~Timed() {
+        if (c->profile) {
+            (void)hipEventRecord(b, c->stream);
+            (void)hipEventSynchronize(b);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, a, b);
+            Family& f = c->fam[name];
+            f.ms += ms;
+            f.n += 1;
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    }
+};
+
+template <typename R>
+void launch_bisect(pimc_ctx* c, int u, int tb, int te) {
+    Run run{c->seed, u, c->ref_order ? 1 : 0, tb, te};
+    Timed t(c, "bisect");
+    hipLaunchKernelGGL(k_bisect<R>, dim3(c->nwork), dim3(64), c->lds_bisect, c->stream, c->d_rep, c->d_mut,
+                       c->d_beads, c->d_rotb, c->tables(), c->d_work, run);
+}
+
+template <typename R>
+void launch_centroid(pimc_ctx* c, int type) {
+    Run run{c->seed, 0, c->ref_order ? 1 : 0, type, type + 1};
+    Timed t(c, "centroid");
+    hipLaunchKernelGGL(k_centroid<R>, dim3(c->nrep), dim3(256), 0, c->stream, c->d_rep, c->d_mut, c->d_beads,
+                       c->d_rotb, c->tables(), c->d_all, run);
+}
+
+template <typename R>
+void launch_rotor(pimc_ctx* c, int u) {
+    if (c->nrotreps == 0) return;
+    Run run{c->seed, u, c->ref_order ? 1 : 0, 0, 0};
+    Timed t(c, "rotor");
+    hipLaunchKernelGGL(k_rotor<R>, dim3(c->nrotreps), dim3(c->rotor_block), c->lds_rotor, c->stream, c->d_rep,
+                       c->d_mut, c->d_beads, c->d_rotb, c->tables(), c->d_rotreps, run);
+}
+
+template <typename R>
+void launch_estimate(pimc_ctx* c, double* out) {
+    Timed t(c, "estimate");
+    hipLaunchKernelGGL(k_estimate<R>, dim3(c->nrep), dim3(256), 0, c->stream, c->d_rep, c->d_mut, c->d_beads,
+                       c->d_rotb, c->tables(), c->d_all, out ? nullptr : c->d_hist, c->d_samples, c->sample_cap, out);
+}
+
+// one pass in the order described at the top of this file
+template <typename R>
+void issue_pass(pimc_ctx* c) {
+    if (c->ref_order) {
+        for (int t = 0; t < c->Pmax; ++t) {
+            for (int ty = 0; ty < c->max_types; ++ty) {
+                if (t == 0) launch_centroid<R>(c, ty);
+                launch_bisect<R>(c, t, ty, ty + 1);
+            }
+            launch_rotor<R>(c, t);
+        }
+    } else {
+        for (int u = 0; u < c->Smax; ++u) {
+            if (u == 0) {
+                for (int ty = 0; ty < c->max_types; ++ty) {
+                    launch_centroid<R>(c, ty);
+                    launch_bisect<R>(c, 0, ty, ty + 1);
+                }
+            } else {
+                launch_bisect<R>(c, u, 0, PIMC_MAX_TYPES);
+            }
+            launch_rotor<R>(c, u);
+        }
+    }
+    hipLaunchKernelGGL(k_pass_end, dim3((c->nrep + 255) / 256), dim3(256), 0, c->stream, c->d_mut, c->nrep);
+}
+
+int check_errors(pimc_ctx* c) {
+    std::vector<Mut> m(c->nrep);
+    HIPCHECK(c, hipMemcpyAsync(m.data(), c->d_mut, sizeof(Mut) * c->nrep, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    for (int r = 0; r < c->nrep; ++r)
+        if (m[r].err) return fail(c, "Rotational Moves: Negative rot density (replica " + std::to_string(r) + ")");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pimc_run(pimc_ctx* c, int n_passes, int sample_every) {
+    if (!c->committed) return fail(c, "pimc_commit first");
+    for (int p = 0; p < n_passes; ++p) {
+        bool use_graph = !c->profile;
+        if (use_graph && !c->graph_exec) {
+            hipGraph_t g;
+            HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            if (c->fp64) issue_pass<double>(c);
+            else issue_pass<float>(c);
+            HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
+            HIPCHECK(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(g);
+        }
+        if (use_graph) HIPCHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
+        else if (c->fp64) issue_pass<double>(c);
+        else issue_pass<float>(c);
+        HIPCHECK(c, hipGetLastError());
+        if (sample_every > 0 && (p + 1) % sample_every == 0) {
+            if (c->fp64) launch_estimate<double>(c, nullptr);
+            else launch_estimate<float>(c, nullptr);
+        }
+        c->bead_moves += c->bead_moves_per_pass;
+    }
+    HIPCHECK(c, hipGetLastError());
+    return check_errors(c);
+}
+
+int pimc_synchronize(pimc_ctx* c) {
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pimc_get_samples(pimc_ctx* c, int r, double* out, int max_samples) {
+    if (!c->committed || r < 0 || r >= c->nrep) return fail(c, "bad replica");
+    Mut m;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(&m, c->d_mut + r, sizeof(Mut), hipMemcpyDeviceToHost));
+    int n = std::min<int>(std::min<int>((int)m.nsamples, c->sample_cap), max_samples);
+    if (n > 0)
+        HIPCHECK(c, hipMemcpy(out, c->d_samples + c->rep[r].sample_off, sizeof(double) * 8 * n, hipMemcpyDeviceToHost));
+    return n;
+}
+
+int pimc_clear_samples(pimc_ctx* c) {
+    std::vector<Mut> m(c->nrep);
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(m.data(), c->d_mut, sizeof(Mut) * c->nrep, hipMemcpyDeviceToHost));
+    for (auto& x : m) x.nsamples = 0;
+    HIPCHECK(c, hipMemcpy(c->d_mut, m.data(), sizeof(Mut) * c->nrep, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemset(c->d_hist, 0, sizeof(unsigned) * std::max<long long>(c->nhist, 1)));
+    return 0;
+}
+
+int pimc_get_counters(pimc_ctx* c, int r, double* total, double* accep) {
+    if (!c->committed || r < 0 || r >= c->nrep) return fail(c, "bad replica");
+    Mut m;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(&m, c->d_mut + r, sizeof(Mut), hipMemcpyDeviceToHost));
+    for (int t = 0; t < c->rep[r].ntypes; ++t)
+        for (int k = 0; k < 3; ++k) {
+            total[3 * t + k] = (double)m.total[t][k];
+            accep[3 * t + k] = (double)m.accep[t][k];
+        }
+    return 0;
+}
+
+int pimc_get_histograms(pimc_ctx* c, int r, double* gr1d, double* gr2d) {
+    if (!c->committed || r < 0 || r >= c->nrep) return fail(c, "bad replica");
+    std::vector<unsigned> h(BINSR + BINSR * BINST);
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(h.data(), c->d_hist + c->rep[r].hist_off, sizeof(unsigned) * h.size(), hipMemcpyDeviceToHost));
+    if (gr1d)
+        for (int i = 0; i < BINSR; ++i) gr1d[i] = h[i];
+    if (gr2d)
+        for (int i = 0; i < BINSR * BINST; ++i) gr2d[i] = h[BINSR + i];
+    return 0;
+}
+
+int pimc_get_work(pimc_ctx* c, double* pair_evals, double* bead_moves) {
+    std::vector<Mut> m(c->nrep);
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(m.data(), c->d_mut, sizeof(Mut) * c->nrep, hipMemcpyDeviceToHost));
+    double pe = 0;
+    for (auto& x : m) pe += (double)x.pair_evals;
+    if (pair_evals) *pair_evals = pe;
+    if (bead_moves) *bead_moves = c->bead_moves;
+    return 0;
+}
+
+int pimc_pot_energy_slice(pimc_ctx* c, int nq, const int* qr, const int* qa, const int* qit, const double* qpos,
+                          double* out) {
+    if (!c->committed) return fail(c, "pimc_commit first");
+    for (int q = 0; q < nq; ++q) {
+        if (qr[q] < 0 || qr[q] >= c->nrep) return fail(c, "query replica out of range");
+        const Rep& p = c->rep[qr[q]];
+        if (qa[q] < 0 || qa[q] >= p.natoms || qit[q] < 0 || qit[q] >= p.P) return fail(c, "query out of range");
+    }
+    int *d_r = dalloc<int>(nq), *d_a = dalloc<int>(nq), *d_i = dalloc<int>(nq);
+    double *d_p = dalloc<double>(3 * (size_t)nq), *d_o = dalloc<double>(nq);
+    int rc = 0;
+    if (!d_r || !d_a || !d_i || !d_p || !d_o) rc = fail(c, "hipMalloc failed");
+    if (!rc) {
+        (void)hipMemcpy(d_r, qr, sizeof(int) * nq, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d_a, qa, sizeof(int) * nq, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d_i, qit, sizeof(int) * nq, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d_p, qpos, sizeof(double) * 3 * nq, hipMemcpyHostToDevice);
+        if (c->fp64)
+            hipLaunchKernelGGL(k_pot_slice<double>, dim3((nq + 127) / 128), dim3(128), 0, c->stream, c->d_rep,
+                               c->d_beads, c->d_rotb, c->tables(), nq, d_r, d_a, d_i, d_p, d_o);
+        else
+            hipLaunchKernelGGL(k_pot_slice<float>, dim3((nq + 127) / 128), dim3(128), 0, c->stream, c->d_rep,
+                               c->d_beads, c->d_rotb, c->tables(), nq, d_r, d_a, d_i, d_p, d_o);
+        if (hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, "pot_energy_slice failed");
+        else (void)hipMemcpy(out, d_o, sizeof(double) * nq, hipMemcpyDeviceToHost);
+    }
+    void* ps[] = {d_r, d_a, d_i, d_p, d_o};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
+int pimc_estimate(pimc_ctx* c, double* out) {
+    if (!c->committed) return fail(c, "pimc_commit first");
+    if (c->fp64) launch_estimate<double>(c, c->d_est);
+    else launch_estimate<float>(c, c->d_est);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(out, c->d_est, sizeof(double) * 8 * c->nrep, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pimc_kernel_time(pimc_ctx* c, const char* family, double* ms, long* launches) {
+    auto it = c->fam.find(family);
+    if (it == c->fam.end()) {
+        *ms = 0;
+        *launches = 0;
+        return 0;
+    }
+    *ms = it->second.ms;
+    *launches = it->second.n;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,223 @@
+"""ctypes binding of libmoribs_pimc.so (include/moribs_pimc.h).
+
+The library is the only compute path: if it is missing or no HIP device is present the
+constructor raises; nothing here falls back to a CPU implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmoribs_pimc.so")
+
+FLAG_FP64 = 0x1
+FLAG_REF_ORDER = 0x2
+FLAG_PROFILE = 0x4
+MAX_TYPES = 4
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_u = ctypes.c_uint
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class ReplicaDesc(ctypes.Structure):
+    _fields_ = [
+        ("ntypes", _i),
+        ("numb", _i * MAX_TYPES),
+        ("molecule", _i * MAX_TYPES),
+        ("levels", _i * MAX_TYPES),
+        ("mcstep", _d * MAX_TYPES),
+        ("mass", _d * MAX_TYPES),
+        ("pot1d", _i * MAX_TYPES),
+        ("pot2d", _i * MAX_TYPES),
+        ("rotdens", _i),
+        ("P", _i),
+        ("Prot", _i),
+        ("T", _d),
+        ("rtstep", _d),
+    ]
+
+
+# every entry point declared in include/moribs_pimc.h, with its ctypes signature
+SIGNATURES = {
+    "pimc_create": (_i, [_i, _u, _u, ctypes.POINTER(ctypes.c_void_p)]),
+    "pimc_destroy": (None, [ctypes.c_void_p]),
+    "pimc_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "pimc_device_name": (_i, [ctypes.c_char_p, _i]),
+    "pimc_add_pot1d": (_i, [ctypes.c_void_p, _i, _dp, _dp]),
+    "pimc_add_pot2d": (_i, [ctypes.c_void_p, _i, _i, _d, _d, _dp, _dp, _dp]),
+    "pimc_add_rotdens": (_i, [ctypes.c_void_p, _i, _dp, _dp, _dp, _dp]),
+    "pimc_set_replica": (_i, [ctypes.c_void_p, _i, ctypes.POINTER(ReplicaDesc)]),
+    "pimc_commit": (_i, [ctypes.c_void_p]),
+    "pimc_set_state": (_i, [ctypes.c_void_p, _i, _dp, _dp]),
+    "pimc_get_state": (_i, [ctypes.c_void_p, _i, _dp, _dp, _dp]),
+    "pimc_run": (_i, [ctypes.c_void_p, _i, _i]),
+    "pimc_get_samples": (_i, [ctypes.c_void_p, _i, _dp, _i]),
+    "pimc_clear_samples": (_i, [ctypes.c_void_p]),
+    "pimc_get_counters": (_i, [ctypes.c_void_p, _i, _dp, _dp]),
+    "pimc_get_histograms": (_i, [ctypes.c_void_p, _i, _dp, _dp]),
+    "pimc_get_work": (_i, [ctypes.c_void_p, _dp, _dp]),
+    "pimc_pot_energy_slice": (_i, [ctypes.c_void_p, _i, _ip, _ip, _ip, _dp, _dp]),
+    "pimc_estimate": (_i, [ctypes.c_void_p, _dp]),
+    "pimc_kernel_time": (_i, [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_long)]),
+    "pimc_synchronize": (_i, [ctypes.c_void_p]),
+}
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class PimcError(RuntimeError):
+    pass
+
+
+class Context:
+    """A set of replicas sampled on the GPU through the C ABI."""
+
+    def __init__(self, n_replicas, flags=0, seed=985456376, lib=None):
+        self.lib = lib or load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.pimc_create(int(n_replicas), int(flags), int(seed) & 0xFFFFFFFF, ctypes.byref(h))
+        if rc != 0:
+            raise PimcError(f"pimc_create failed ({rc}): no usable HIP device")
+        self.h = h
+        self.n = int(n_replicas)
+        self.descs = [None] * self.n
+
+    def close(self):
+        if self.h:
+            self.lib.pimc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc < 0:
+            raise PimcError(self.lib.pimc_last_error(self.h).decode())
+        return rc
+
+    # tables
+    def add_pot1d(self, r, v):
+        r, v = _f64(r), _f64(v)
+        return self._check(self.lib.pimc_add_pot1d(self.h, len(r), _dptr(r), _dptr(v)))
+
+    def add_pot2d(self, rgrid, cgrid, dr, dc, v):
+        rg, cg, vv = _f64(rgrid), _f64(cgrid), _f64(v).reshape(-1)
+        return self._check(self.lib.pimc_add_pot2d(self.h, len(rg), len(cg), float(dr), float(dc), _dptr(rg), _dptr(cg), _dptr(vv)))
+
+    def add_rotdens(self, cosg, rho, erot, esq):
+        a = [_f64(x) for x in (cosg, rho, erot, esq)]
+        return self._check(self.lib.pimc_add_rotdens(self.h, len(a[0]), *[_dptr(x) for x in a]))
+
+    # replicas
+    def set_replica(self, r, ntypes, numb, molecule, levels, mcstep, mass, P, Prot, T, rtstep, pot1d=None, pot2d=None,
+                    rotdens=-1):
+        d = ReplicaDesc()
+        d.ntypes = ntypes
+        for t in range(MAX_TYPES):
+            d.numb[t] = numb[t] if t < ntypes else 0
+            d.molecule[t] = molecule[t] if t < ntypes else 0
+            d.levels[t] = levels[t] if t < ntypes else 0
+            d.mcstep[t] = mcstep[t] if t < ntypes else 0.0
+            d.mass[t] = mass[t] if t < ntypes else 1.0
+            d.pot1d[t] = (pot1d[t] if pot1d is not None and t < ntypes else -1)
+            d.pot2d[t] = (pot2d[t] if pot2d is not None and t < ntypes else -1)
+        d.rotdens = rotdens
+        d.P, d.Prot, d.T, d.rtstep = P, Prot, T, rtstep
+        self.descs[r] = d
+        return self._check(self.lib.pimc_set_replica(self.h, r, ctypes.byref(d)))
+
+    def commit(self):
+        return self._check(self.lib.pimc_commit(self.h))
+
+    def natoms(self, r):
+        d = self.descs[r]
+        return sum(d.numb[t] for t in range(d.ntypes))
+
+    def set_state(self, r, coords, angles=None):
+        c = _f64(coords)
+        a = _f64(angles) if angles is not None else None
+        return self._check(self.lib.pimc_set_state(self.h, r, _dptr(c), _dptr(a) if a is not None else None))
+
+    def get_state(self, r):
+        n = self.natoms(r) * self.descs[r].P
+        c = np.empty(3 * n)
+        a = np.empty(3 * n)
+        cs = np.empty(3 * n)
+        self._check(self.lib.pimc_get_state(self.h, r, _dptr(c), _dptr(a), _dptr(cs)))
+        return c.reshape(3, n), a.reshape(3, n), cs.reshape(3, n)
+
+    def run(self, n_passes, sample_every=0):
+        return self._check(self.lib.pimc_run(self.h, int(n_passes), int(sample_every)))
+
+    def samples(self, r, max_samples=1 << 16):
+        out = np.empty(8 * max_samples)
+        n = self._check(self.lib.pimc_get_samples(self.h, r, _dptr(out), max_samples))
+        return out[: 8 * n].reshape(n, 8)
+
+    def clear_samples(self):
+        return self._check(self.lib.pimc_clear_samples(self.h))
+
+    def counters(self, r):
+        nt = self.descs[r].ntypes
+        tot = np.zeros(3 * nt)
+        acc = np.zeros(3 * nt)
+        self._check(self.lib.pimc_get_counters(self.h, r, _dptr(tot), _dptr(acc)))
+        return tot.reshape(nt, 3), acc.reshape(nt, 3)
+
+    def histograms(self, r):
+        g1 = np.zeros(300)
+        g2 = np.zeros(300 * 50)
+        self._check(self.lib.pimc_get_histograms(self.h, r, _dptr(g1), _dptr(g2)))
+        return g1, g2.reshape(300, 50)
+
+    def work(self):
+        pe, bm = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.pimc_get_work(self.h, ctypes.byref(pe), ctypes.byref(bm)))
+        return pe.value, bm.value
+
+    def pot_energy_slice(self, reps, atoms, its, pos):
+        reps = np.ascontiguousarray(reps, dtype=np.int32)
+        atoms = np.ascontiguousarray(atoms, dtype=np.int32)
+        its = np.ascontiguousarray(its, dtype=np.int32)
+        pos = _f64(pos).reshape(-1)
+        out = np.empty(len(reps))
+        ip = lambda a: a.ctypes.data_as(_ip)
+        self._check(self.lib.pimc_pot_energy_slice(self.h, len(reps), ip(reps), ip(atoms), ip(its), _dptr(pos), _dptr(out)))
+        return out
+
+    def estimate(self):
+        out = np.empty(8 * self.n)
+        self._check(self.lib.pimc_estimate(self.h, _dptr(out)))
+        return out.reshape(self.n, 8)
+
+    def kernel_time(self, family):
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        self._check(self.lib.pimc_kernel_time(self.h, family.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def synchronize(self):
+        return self._check(self.lib.pimc_synchronize(self.h))

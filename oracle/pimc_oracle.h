@@ -1,0 +1,132 @@
+/*
+ * pimc_oracle.h -- FP64 CPU restatement of the MoRiBS-PIMC sampling hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product (moribs-pimc_amd/) never links or calls it.
+ *
+ * Every function names the reference routine it restates (file:line in the
+ * upstream MoRiBS-PIMC tree).  Parity status of each function is recorded in
+ * the header comment of pimc_oracle.c.
+ */
+#ifndef PIMC_ORACLE_H
+#define PIMC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAX_TYPES 6
+#define OR_NDIM 3
+
+/* move kinds, mc_setup.h:161-164 */
+#define OR_MCMOLEC 0
+#define OR_MCMULTI 1
+#define OR_MCROTAT 2
+
+/* schedule of one pass */
+#define OR_ORDER_REFERENCE 0   /* mc_main.cc:348-381 + PIMCPass mc_main.cc:510-522 */
+#define OR_ORDER_SEGMENTS  1   /* same moves, same random numbers, segment-parallel order */
+
+typedef struct or_sys or_sys;
+
+/* ---------- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) ---------- */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* ---------- stand-alone table/geometry routines ---------- */
+void   or_spline(const double* x, const double* y, int n, double yp1, double ypn, double* y2);
+double or_splint(const double* xa, const double* ya, const double* y2a, int n, double x);
+void   or_init_spline(const double* grid, const double* data, double* sdata, int n);
+void   or_matpre(const double eul[3], double rotmat[9]);          /* column-major (Fortran) */
+void   or_deleul(const double e1[3], const double e2[3], double erel[3], int* istop);
+void   or_rotpro(double chi, double phi, double theta, double* rho, double* erot, double* esq,
+                 const double* rhoprp, const double* erotpr, const double* erotsq, int* jstop);
+void   or_rotden(const double e1[3], const double e2[3], double erel[3], double* rho, double* erot,
+                 double* esq, const double* rhoprp, const double* erotpr, const double* erotsq, int* istop);
+void   or_vcalc(double r, double theta, double chi, double r0, double rmax, double rstep, int nrgrd,
+                int nthgrd, int nchgrd, const double* vtable, double* vpes);
+void   or_vcord(const double eulang[3], const double rcom[3], const double rpt[3], const double* vtable,
+                int nrgrd, int nthgrd, int nchgrd, double rvmax, double rvmin, double rvstep,
+                double* vpot, double* radret, double* theret, double* chiret,
+                double hatx[3], double haty[3], double hatz[3], int ivcord);
+void   or_caleng(const double com1[3], const double com2[3], double* e, const double eul1[3],
+                 const double eul2[3]);
+double or_linden_rho(double cost, double tau, double bconst, int lmax, int iodevn, int nslice,
+                     double* erot, double* erotsq);
+int    or_linden_lmax(double tau, double bconst);
+
+/* ---------- system ---------- */
+or_sys* or_create(int ntypes, const int* numb, const int* molecule, const int* levels,
+                  const double* mcstep, const double* mass, int P, int Prot, double T,
+                  double rtstep, int rotden_type);
+void    or_destroy(or_sys* s);
+const char* or_last_error(or_sys* s);
+
+int  or_set_pot1d(or_sys* s, int type, int n, const double* x, const double* y);
+int  or_set_pot2d(or_sys* s, int type, int nr, int nc, double dr, double dc, const double* rg,
+                  const double* cg, const double* v);
+int  or_set_rotdens(or_sys* s, int type, int n, const double* x, const double* rho,
+                    const double* erot, const double* esq);
+/* 3D tables are referenced, not copied (the density tables are 3 x 189 MB) */
+int  or_set_pot3d(or_sys* s, int nr, int nth, int nch, double rmin, double rmax, const double* v);
+int  or_set_rot3d(or_sys* s, const double* rho, const double* erot, const double* esq);
+void or_set_minimage(or_sys* s, int on, const double box[3]);
+
+/* state, [3][N*P] slice-fastest like MCCoords/MCAngles (mc_setup.cc:139-145) */
+void or_set_state(or_sys* s, const double* coords, const double* angles);
+void or_get_state(or_sys* s, double* coords, double* angles, double* cosines);
+void or_get_counters(or_sys* s, double* total, double* accep); /* [ntypes][3] */
+void or_reset_counters(or_sys* s);
+int  or_natoms(or_sys* s);
+int  or_imtype(or_sys* s);
+void or_get_pot1d_extrap(or_sys* s, int type, double* alpha, double* unode, double* c6, double* y2);
+
+/* ---------- reference functions on the state ---------- */
+double or_spot1d(or_sys* s, double r, int type);
+double or_lpot2d(or_sys* s, double r, double cost, int type);
+double or_srotdens(or_sys* s, double g, int type);
+double or_srotdens_deriv(or_sys* s, double g, int type);
+double or_srotdens_esqrt(or_sys* s, double g, int type);
+/* pos == NULL means MCCoords */
+double or_pot_energy_slice(or_sys* s, int atom0, const double* pos, int it);
+double or_pot_energy_path(or_sys* s, int atom0, const double* pos);
+double or_pot_rot_energy(or_sys* s, int atom0, const double* cosine, int it);
+double or_pot_rot_e3d(or_sys* s, int atom0, const double eul[3], int it);
+
+/* ---------- estimators (mc_estim.cc) ---------- */
+double or_kin_energy(or_sys* s);
+double or_pot_energy_densities(or_sys* s, double* gr1d, double* gr2d);  /* bins may be NULL */
+double or_rot_energy(or_sys* s, double* erotsq, double* erot_termsq);
+double or_rot_e3d(or_sys* s, double* erotsq, double* erot_termsq);
+/* one MCGetAverage sample (mc_main.cc:551-616): out[0..7] =
+   kin, pot, rot, rotsq, Cv, Cv_trans, Cv_rot, rot_termsq */
+void   or_sample(or_sys* s, double out[8]);
+
+/* ---------- moves and schedule ---------- */
+/* counter-based draws: key = (seed, replica); see pimc_oracle.c for the counter map */
+void or_set_rng(or_sys* s, uint32_t seed, uint32_t replica);
+int  or_molecular_move(or_sys* s, int type, uint32_t pass);
+int  or_bisection_move(or_sys* s, int type, int time, uint32_t pass);
+int  or_bisection_atom(or_sys* s, int type, int atom, int time, uint32_t pass);
+int  or_rotations_move(or_sys* s, int type, uint32_t pass, uint32_t sweep);
+int  or_rotations_3d(or_sys* s, int type, uint32_t pass, uint32_t sweep);
+int  or_pass(or_sys* s, uint32_t pass, int order);
+/* samples every `every` passes after each pass; out [npasses/every][8] (may be NULL) */
+int  or_run(or_sys* s, uint32_t pass0, int npasses, int order, int every, double* out);
+
+/* tape mode: draws taken from per-stream tapes in the reference's call order
+   (1 rnd1, 2 rnd2, 3 rnd3, 8 gauss r1, 9 gauss r2, 15 runif) */
+int  or_set_tape(or_sys* s, int stream, const double* v, long n);
+long or_tape_pos(or_sys* s, int stream);
+int  or_tape_short(or_sys* s);
+void or_tape_off(or_sys* s);
+
+/* draw helpers exposed for tests */
+double or_uniform(uint32_t seed, uint32_t replica, const uint32_t ctr[4], int which);
+double or_gauss(uint32_t seed, uint32_t replica, const uint32_t ctr[4], double alpha);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
