@@ -79,6 +79,13 @@ def workload(name, seed=985456376, n_replicas=256):
         reps.append(replica(64, 512, 0.37, seed))
     elif name == "tiny":
         reps.append(replica(3, 16, 0.5, seed))
+    elif name == "mini":  # a small uneven batch for replica-level parity checks
+        rng = np.random.default_rng(seed)
+        for r in range(n_replicas if n_replicas < 256 else 6):
+            n = int(rng.integers(1, 13))
+            P = int(rng.choice([16, 32, 64]))
+            T = float(fmt_temperature(float(rng.uniform(0.3, 2.0))))
+            reps.append(replica(n, P, T, seed + 1 + r))
     elif name == "c2":
         rng = np.random.default_rng(seed)
         for r in range(n_replicas):
