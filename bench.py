@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Throughput of the PIMC sampling hot path on one MI355X.
+
+A step is one pass (mc_main.cc:348-381) over every replica of the workload, inputs resident in
+HBM, with one estimator sample (MCGetAverage) per replica per pass.  Default workload: 256
+independent replicas (N ~ U{1..100} He around one linear rotor, P in {64,128,256,512},
+T ~ U(0.3, 2) K) in one context.  Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "moribs-pimc_amd"))
+
+import pimc_configs as pc  # noqa: E402
+import pimc_hip as ph  # noqa: E402
+import pimc_tables as pt  # noqa: E402
+
+METRIC = "bead-moves/sec & pair-evals/sec on 1 MI355X; surrogate-DA vs plain at equal σ_E"
+FP32_PEAK_TFLOPS = 157.3
+# algorithmic FP32 FLOP per pair evaluation: displacement, distance and interpolation
+FLOP_1D = 28.0   # clamped cubic spline on the 1D pair table
+FLOP_2D = 36.0   # bilinear (r, cos theta) on the rotor table
+DESC = {
+    "c2": "256 replicas, N~U{1..100} He + 1 linear rotor (B=0.39 cm^-1), P in {64,128,256,512}, T~U(0.3,2) K",
+    "c1": "64 He + 1 linear rotor, P=Prot=512, T=0.37 K",
+    "c0": "8 He + 1 linear rotor, P=Prot=128, T=0.37 K",
+}
+
+
+def flop_mix(wl):
+    """Mean FLOP per pair evaluation of the bisection kernel's pair mix, per replica
+    weighted by its pair count: He movers see N-1 He (1D) + 1 rotor (2D), the rotor N He (2D)."""
+    n1 = n2 = 0.0
+    for rp in wl["replicas"]:
+        n = rp["numb"][0]
+        n1 += n * (n - 1) * rp["P"]
+        n2 += 2 * n * rp["P"]
+    return (n1 * FLOP_1D + n2 * FLOP_2D) / (n1 + n2)
+
+
+# ----------------------------------------------------------------------------- CPU baseline
+def write_deck(d, rp, tables, passes):
+    """qmc.input, tables and xyz.init in the reference's formats for one replica."""
+    r, v = tables["pot1d"][0]
+    pt.write_pot1d(os.path.join(d, "He.pot"), r, v)
+    rg, cg, dr, dc, V = tables["pot2d"][0]
+    pt.write_pot2d(os.path.join(d, "CO2He.pot"), rg, cg, V, dr, dc)
+    c, rho, er, es = tables["rot"][rp["rotdens"]]
+    pt.write_rot(os.path.join(d, pc.rot_file_name(rp)), c, rho, er, es)
+    P = rp["P"]
+    with open(os.path.join(d, "qmc.input"), "w") as f:
+        f.write(f"""MASTERDIR ./
+OUTPUTDIR ./
+FILENAMEPREFIX gr
+DIMENSION 3
+DENSITY 0.02
+TEMPERATURE {pt.fmt_temperature(rp['T'])}
+ATOM He4 {rp['numb'][0]} BOLTZMANN {rp['mcstep'][0]} {rp['levels'][0]} He PRIMITIVE
+MOLECULE CO2 1 BOLTZMANN {rp['mcstep'][1]} {rp['levels'][1]} CO2He PRIMITIVE
+ROTATION CO2 {rp['rtstep']} {rp['Prot']}
+NUMBEROFSLICES {P}
+NUMBEROFPASSES {passes}
+NUMBEROFBLOCKS 1 0
+MCSKIP_RATIO 1000000000
+MCSKIP_TOTAL 1000000000
+MCSKIP_AVERG {P}
+READMCCOORDS
+""")
+    co, an = rp["coords"], rp["angles"]
+    n = co.shape[1]
+    with open(os.path.join(d, "xyz.init"), "w") as f:
+        f.write("0\ncomment\n")
+        for i in range(n):
+            f.write(f"X {co[0,i]:.17g} {an[0,i]:.17g} {co[1,i]:.17g} {an[1,i]:.17g} {co[2,i]:.17g} {an[2,i]:.17g}\n")
+
+
+def cpu_baseline(wl, budget_s=12.0):
+    """The reference program (oracle/_ref/pimc, built from the upstream sources) on a sample of
+    the workload's replicas, one single-threaded process per host core; per-pass time is the
+    difference of two runs with different pass counts (removes start-up and table input)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "pimc")
+    if not os.path.exists(exe):
+        return None
+    cores = max(1, min(16, os.cpu_count() or 1))
+    reps = wl["replicas"][:cores]
+    rate_guess = 2.5e5  # bead moves per second per core, only to size the runs
+    jobs = []
+    tmp = tempfile.mkdtemp(prefix="pimc_cpu_")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    for i, rp in enumerate(reps):
+        bm = pc.bead_moves_per_pass(rp)
+        n2 = max(2, int(budget_s * 0.25 * rate_guess / bm))
+        n1 = max(1, n2 // 4)
+        for n in (n1, n2):
+            d = os.path.join(tmp, f"r{i}_{n}")
+            os.makedirs(d)
+            write_deck(d, rp, wl["tables"], n)
+            jobs.append((i, n, d))
+    # run the short and long decks of every replica concurrently, one core each
+    times = {}
+    procs = []
+    for i, n, d in jobs:
+        procs.append((i, n, time.perf_counter(), subprocess.Popen([exe], cwd=d, env=env, stdout=subprocess.DEVNULL,
+                                                                  stderr=subprocess.DEVNULL), d))
+        if len(procs) >= cores:
+            _reap(procs, times)
+    while procs:
+        _reap(procs, times)
+    shutil.rmtree(tmp, ignore_errors=True)
+    bm_sum = t_sum = 0.0
+    for i, rp in enumerate(reps):
+        (n1, t1), (n2, t2) = sorted((n, t) for (j, n), t in times.items() if j == i)
+        per_pass = (t2 - t1) / (n2 - n1)
+        bm_sum += pc.bead_moves_per_pass(rp)
+        t_sum += max(per_pass, 1e-9)
+    return dict(value=cores * bm_sum / t_sum, unit="bead-moves/s", cores=cores, kind="reference",
+                sample=f"first {len(reps)} replicas of the workload, upstream pimc (-Ofast, 1 thread each), "
+                       f"{cores} concurrent processes; per-pass time from two pass counts")
+
+
+def _reap(procs, times):
+    """The reference's main() returns 1 after a complete run (mc_main.cc:507); completion is
+    told by the block energies it writes at the end of the block."""
+    while True:
+        for k, (i, n, t0, p, d) in enumerate(procs):
+            if p.poll() is not None:
+                if p.returncode not in (0, 1) or not os.path.exists(os.path.join(d, "gr.eng")):
+                    raise RuntimeError(f"reference pimc failed on replica {i} ({p.returncode})")
+                times[(i, n)] = time.perf_counter() - t0
+                procs.pop(k)
+                return
+        time.sleep(0.01)
+
+
+# ----------------------------------------------------------------------------- GPU
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; replicas are independent chains, so N GPUs "
+                                   "would each run their own share with no communication (see DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    wl = pc.workload(args.workload)
+    ctx = pc.load_into_context(ph.Context(len(wl["replicas"]), flags=ph.FLAG_PROFILE), wl)
+    bm_pass = float(sum(pc.bead_moves_per_pass(rp) for rp in wl["replicas"]))
+    ctx.run(args.warmup, sample_every=1)
+    ctx.synchronize()
+    fams = ["bisect", "centroid", "rotor_prep", "rotor", "estimate"]
+    efams = ["bisect", "centroid", "rotor", "estimate"]
+    k0 = {f: ctx.kernel_time(f) for f in fams}
+    e0 = {f: ctx.family_evals(f) for f in efams}
+    pe0, _ = ctx.work()
+    t0 = time.perf_counter()
+    ctx.run(args.steps, sample_every=1)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    pe1, _ = ctx.work()
+    k1 = {f: ctx.kernel_time(f) for f in fams}
+    e1 = {f: ctx.family_evals(f) for f in efams}
+    dt = t1 - t0
+    kt = {f: (k1[f][0] - k0[f][0], k1[f][1] - k0[f][1]) for f in fams}
+    ev = {f: e1[f] - e0[f] for f in efams}
+    dom = max(efams, key=lambda f: kt[f][0])
+    ms, nl = kt[dom]
+    flop_per_eval = {"bisect": flop_mix(wl), "centroid": flop_mix(wl), "rotor": FLOP_2D, "estimate": flop_mix(wl)}[dom]
+    flop_per_launch = ev[dom] * flop_per_eval / max(nl, 1)
+    achieved = flop_per_launch / (ms / max(nl, 1) * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": round(args.steps * bm_pass / dt, 1),
+        "unit": "bead-moves/s",
+        "pair_evals_per_s": round((pe1 - pe0) / dt, 1),
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": DESC[args.workload], "replicas": len(wl["replicas"]),
+                   "bead_moves_per_step": bm_pass, "pair_terms": "fp32, fp64 accumulation",
+                   "estimators": "one MCGetAverage sample per replica per pass"},
+        "kernels_ms_per_step": {f: round(kt[f][0] / args.steps, 4) for f in fams},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": None, "kernel": dom,
+                     "note": "FP32 vector peak; the path has no matrix contraction, FLOPs are algorithmic "
+                             f"({FLOP_1D:g}/1D and {FLOP_2D:g}/2D pair evaluation)"},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(wl)
+        if cb is not None:
+            out["cpu_baseline"] = cb
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -118,6 +118,8 @@ int  pimc_estimate(pimc_ctx* ctx, double* out);
    set: "bisect", "centroid", "rotor", "estimate" */
 int  pimc_kernel_time(pimc_ctx* ctx, const char* family, double* ms, long* launches);
 int  pimc_synchronize(pimc_ctx* ctx);
+/* pair evaluations performed by one kernel family since creation */
+int  pimc_family_evals(pimc_ctx* ctx, const char* family, double* evals);
 
 #ifdef __cplusplus
 }

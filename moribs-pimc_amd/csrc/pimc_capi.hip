@@ -109,12 +109,22 @@ struct pimc_ctx {
     int nrotreps = 0;
     double* d_est = nullptr;
     size_t lds_bisect = 0, lds_rotor = 0;
-    int Smax = 1, Pmax = 1, max_types = 1;
-    int rotor_block = 256;
+    int Smax = 1, Pmax = 1, Nmax = 1, max_types = 1;
+    int rotor_block = 256, bisect_block = 64;
+    int2* d_rows = nullptr;     // (replica, atom) rows of the estimator pair sum
+    double2* d_rowbuf = nullptr;
+    int nrows = 0;
+    void* d_prep = nullptr;     // rotor-partner displacements, float4 or double4
+    int prep_items_max = 0;
     long long nbeads = 0, nrotb = 0, nhist = 0;
     hipGraphExec_t graph_exec = nullptr;
     bool graph_ok = false;
     std::map<std::string, Family> fam;
+    struct Pending {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
     double bead_moves_per_pass = 0;
     double bead_moves = 0;
     Tables tables() const {
@@ -177,10 +187,14 @@ int pimc_create(int n_replicas, unsigned flags, unsigned seed, pimc_ctx** out) {
 
 void pimc_destroy(pimc_ctx* c) {
     if (!c) return;
+    for (auto& p : c->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     void* ptrs[] = {c->d_rep, c->d_mut, c->d_beads, c->d_rotb, c->d_hist, c->d_samples, c->d_k1f, c->d_k1d,
                     c->d_t1, c->d_v2f, c->d_v2d, c->d_g2d, c->d_g2f, c->d_t2, c->d_rk, c->d_re, c->d_tr,
-                    c->d_work, c->d_all, c->d_rotreps, c->d_est};
+                    c->d_work, c->d_all, c->d_rotreps, c->d_est, c->d_rows, c->d_rowbuf, c->d_prep};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -257,9 +271,11 @@ int pimc_set_replica(pimc_ctx* c, int r, const pimc_replica_desc* d) {
     if (r < 0 || r >= c->nrep) return fail(c, "replica index out of range");
     if (d->ntypes < 1 || d->ntypes > PIMC_MAX_TYPES) return fail(c, "ntypes out of range");
     if (d->P < 2 || d->P > 65535) return fail(c, "P out of range");
-    int natoms = 0, nmolt = 0;
+    int natoms = 0, nmolt = 0, natt = 0;
     for (int t = 0; t < d->ntypes; ++t) {
         if (d->numb[t] < 1) return fail(c, "every type needs at least one atom");
+        if (d->molecule[t] == 0 && ++natt > 1)
+            return fail(c, "No more then one atom/molecule type: densities and potential energy");
         if (d->molecule[t] != 0 && t != d->ntypes - 1) return fail(c, "Molecules should follow atoms in input file");
         if (d->molecule[t] == 2) return fail(c, "non-linear rotors are not supported by this build");
         if (d->molecule[t] == 1) {
@@ -284,8 +300,9 @@ int pimc_commit(pimc_ctx* c) {
     const double lam = 100.0 * (kHBAR * kHBAR) / (kAMU * kKB);
     c->rep.resize(c->nrep);
     long long boff = 0, roff = 0, hoff = 0;
-    std::vector<int2> work;
+    std::vector<int2> work, rows;
     std::vector<int> rotreps;
+    long long rowoff = 0, prepoff = 0;
     size_t lds_b = 0, lds_r = 0;
     c->bead_moves_per_pass = 0;
     for (int r = 0; r < c->nrep; ++r) {
@@ -303,6 +320,9 @@ int pimc_commit(pimc_ctx* c) {
         p.imtype = -1;
         p.rotkind = 0;
         p.rot_atom = -1;
+        p.lin_atom = -1;
+        p.pot1d_id = -1;
+        p.pot2d_id = -1;
         p.key = r;
         p.rotdens = d.rotdens;
         int g = 0, S = 1;
@@ -316,6 +336,11 @@ int pimc_commit(pimc_ctx* c) {
             p.pot1d[t] = d.pot1d[t];
             p.pot2d[t] = d.pot2d[t];
             S = std::max(S, 1 << d.levels[t]);
+            if (d.molecule[t] == 0) p.pot1d_id = d.pot1d[t];
+            if (d.molecule[t] == 1) {
+                p.lin_atom = g;
+                p.pot2d_id = d.pot2d[t];
+            }
             if (d.molecule[t] == 1 && d.Prot > 0) {
                 p.imtype = t;
                 p.rotkind = 1;
@@ -338,7 +363,7 @@ int pimc_commit(pimc_ctx* c) {
             if (d.rotdens < 0 || d.rotdens >= (int)c->tr.size()) return fail(c, "linear rotor without a .rot table");
             c->bead_moves_per_pass += (double)p.P * p.Prot;
             rotreps.push_back(r);
-            lds_r = std::max(lds_r, (size_t)5 * p.Prot * sizeof(double));
+            lds_r = std::max(lds_r, (size_t)6 * p.Prot * sizeof(double));
         }
         p.bead_off = boff;
         boff += (long long)p.natoms * p.P;
@@ -346,10 +371,20 @@ int pimc_commit(pimc_ctx* c) {
         roff += (p.rotkind ? (long long)p.tnumb[p.imtype] * p.Prot : 0);
         p.hist_off = hoff;
         hoff += BINSR + BINSR * BINST;
+        p.row_off = rowoff;
+        for (int a = 0; a < p.natoms; ++a) rows.push_back(make_int2(r, a));
+        rowoff += p.natoms;
+        p.prep_off = prepoff;
+        if (p.rotkind == 1) {
+            prepoff += (long long)p.P * (p.natoms - 1);
+            c->prep_items_max = std::max(c->prep_items_max, p.P * (p.natoms - 1));
+        }
         int W = S + 1;
-        lds_b = std::max(lds_b, (size_t)(3 * p.natoms * W + 6 * W) * sizeof(double));
+        size_t knots = p.pot1d_id >= 0 ? (size_t)c->t1[p.pot1d_id].n * (c->fp64 ? sizeof(double4) : sizeof(float4)) : 0;
+        lds_b = std::max(lds_b, (size_t)((3 * p.natoms * W + 6 * W + 3) & ~3) * sizeof(double) + knots);
         int nseg = c->ref_order ? 1 : p.P / S;
         for (int k = 0; k < nseg; ++k) work.push_back(make_int2(r, k));
+        c->Nmax = std::max(c->Nmax, p.natoms);
         c->Smax = std::max(c->Smax, S);
         c->Pmax = std::max(c->Pmax, p.P);
         c->max_types = std::max(c->max_types, p.ntypes);
@@ -361,8 +396,19 @@ int pimc_commit(pimc_ctx* c) {
     c->nbeads = boff;
     c->nrotb = roff;
     c->nhist = hoff;
+    // largest windows first: the longest segment sequences start while the grid fills
+    std::stable_sort(work.begin(), work.end(), [&](const int2& x, const int2& y) {
+        const Rep &a = c->rep[x.x], &b = c->rep[y.x];
+        return (long long)a.natoms * a.natoms * a.S > (long long)b.natoms * b.natoms * b.S;
+    });
     c->nwork = (int)work.size();
     c->nrotreps = (int)rotreps.size();
+    c->nrows = (int)rows.size();
+    c->d_rows = dalloc<int2>(rows.size());
+    c->d_rowbuf = dalloc<double2>(rows.size());
+    c->d_prep = dalloc<char>((size_t)std::max<long long>(prepoff, 1) * (c->fp64 ? sizeof(double4) : sizeof(float4)));
+    if (!c->d_rows || !c->d_rowbuf || !c->d_prep) return fail(c, "hipMalloc failed");
+    HIPCHECK(c, hipMemcpy(c->d_rows, rows.data(), sizeof(int2) * rows.size(), hipMemcpyHostToDevice));
     // device allocations
     c->d_rep = dalloc<Rep>(c->nrep);
     c->d_mut = dalloc<Mut>(c->nrep);
@@ -425,11 +471,13 @@ int pimc_commit(pimc_ctx* c) {
         up(c->d_rk, c->rk.data(), c->rk.size() * sizeof(double4)) ||
         up(c->d_re, c->re.data(), c->re.size() * sizeof(double4)) || up(c->d_tr, c->tr.data(), c->tr.size() * sizeof(TabRot)))
         return fail(c, "table upload failed");
-    // rotor block: enough lanes for one group per slice move of the largest chain
+    // rotor block: up to 64 lanes per slice move of the largest chain
     int pr = 1;
     for (auto& p : c->rep)
         if (p.rotkind) pr = std::max(pr, p.Prot);
-    c->rotor_block = std::min(1024, std::max(64, ((pr / 2 + 63) / 64) * 64));
+    c->rotor_block = std::min(1024, std::max(64, ((pr + 1) / 2) * 64));
+    // bisection block: enough lanes for the widest level of the largest window
+    c->bisect_block = c->Nmax <= 24 ? 64 : (c->Nmax <= 64 ? 128 : 256);
     c->committed = true;
     return 0;
 }
@@ -517,34 +565,41 @@ struct Timed {
             (void)hipEventRecord(a, c->stream);
         }
     }
-    ~Timed() {
+    ~Timed() {  // events are resolved later, so timing does not serialise the stream
         if (c->profile) {
             (void)hipEventRecord(b, c->stream);
-            (void)hipEventSynchronize(b);
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, a, b);
-            Family& f = c->fam[name];
-            f.ms += ms;
-            f.n += 1;
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
+            c->pending.push_back({name, a, b});
         }
     }
 };
+
+void resolve_events(pimc_ctx* c) {
+    for (auto& p : c->pending) {
+        (void)hipEventSynchronize(p.b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, p.a, p.b);
+        Family& f = c->fam[p.name];
+        f.ms += ms;
+        f.n += 1;
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    c->pending.clear();
+}
 
 template <typename R>
 void launch_bisect(pimc_ctx* c, int u, int tb, int te) {
     Run run{c->seed, u, c->ref_order ? 1 : 0, tb, te};
     Timed t(c, "bisect");
-    hipLaunchKernelGGL(k_bisect<R>, dim3(c->nwork), dim3(64), c->lds_bisect, c->stream, c->d_rep, c->d_mut,
-                       c->d_beads, c->d_rotb, c->tables(), c->d_work, run);
+    hipLaunchKernelGGL(k_bisect<R>, dim3(c->nwork), dim3(c->bisect_block), c->lds_bisect, c->stream, c->d_rep,
+                       c->d_mut, c->d_beads, c->d_rotb, c->tables(), c->d_work, run);
 }
 
 template <typename R>
 void launch_centroid(pimc_ctx* c, int type) {
     Run run{c->seed, 0, c->ref_order ? 1 : 0, type, type + 1};
     Timed t(c, "centroid");
-    hipLaunchKernelGGL(k_centroid<R>, dim3(c->nrep), dim3(256), 0, c->stream, c->d_rep, c->d_mut, c->d_beads,
+    hipLaunchKernelGGL(k_centroid<R>, dim3(c->nrep), dim3(1024), 0, c->stream, c->d_rep, c->d_mut, c->d_beads,
                        c->d_rotb, c->tables(), c->d_all, run);
 }
 
@@ -552,16 +607,24 @@ template <typename R>
 void launch_rotor(pimc_ctx* c, int u) {
     if (c->nrotreps == 0) return;
     Run run{c->seed, u, c->ref_order ? 1 : 0, 0, 0};
+    vec4_t<R>* prep = static_cast<vec4_t<R>*>(c->d_prep);
+    {
+        Timed t(c, "rotor_prep");
+        hipLaunchKernelGGL(k_rotor_prep<R>, dim3((c->prep_items_max + 255) / 256, c->nrotreps), dim3(256), 0,
+                           c->stream, c->d_rep, c->d_beads, prep, c->d_rotreps);
+    }
     Timed t(c, "rotor");
     hipLaunchKernelGGL(k_rotor<R>, dim3(c->nrotreps), dim3(c->rotor_block), c->lds_rotor, c->stream, c->d_rep,
-                       c->d_mut, c->d_beads, c->d_rotb, c->tables(), c->d_rotreps, run);
+                       c->d_mut, prep, c->d_rotb, c->tables(), c->d_rotreps, run);
 }
 
 template <typename R>
 void launch_estimate(pimc_ctx* c, double* out) {
     Timed t(c, "estimate");
-    hipLaunchKernelGGL(k_estimate<R>, dim3(c->nrep), dim3(256), 0, c->stream, c->d_rep, c->d_mut, c->d_beads,
-                       c->d_rotb, c->tables(), c->d_all, out ? nullptr : c->d_hist, c->d_samples, c->sample_cap, out);
+    hipLaunchKernelGGL(k_est_rows<R>, dim3(c->nrows), dim3(256), 0, c->stream, c->d_rep, c->d_beads, c->d_rotb,
+                       c->tables(), c->d_rows, out ? nullptr : c->d_hist, c->d_rowbuf);
+    hipLaunchKernelGGL(k_est_final, dim3(c->nrep), dim3(256), 0, c->stream, c->d_rep, c->d_mut, c->d_rotb, c->tables(),
+                       c->d_all, c->d_rowbuf, c->d_samples, c->sample_cap, out);
 }
 
 // one pass in the order described at the top of this file
@@ -687,7 +750,8 @@ int pimc_get_work(pimc_ctx* c, double* pair_evals, double* bead_moves) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     HIPCHECK(c, hipMemcpy(m.data(), c->d_mut, sizeof(Mut) * c->nrep, hipMemcpyDeviceToHost));
     double pe = 0;
-    for (auto& x : m) pe += (double)x.pair_evals;
+    for (auto& x : m)
+        for (int k = 0; k < 4; ++k) pe += (double)x.pair_evals[k];
     if (pair_evals) *pair_evals = pe;
     if (bead_moves) *bead_moves = c->bead_moves;
     return 0;
@@ -735,7 +799,23 @@ int pimc_estimate(pimc_ctx* c, double* out) {
     return 0;
 }
 
+int pimc_family_evals(pimc_ctx* c, const char* family, double* evals) {
+    static const char* names[4] = {"bisect", "centroid", "rotor", "estimate"};
+    int k = -1;
+    for (int i = 0; i < 4; ++i)
+        if (std::strcmp(names[i], family) == 0) k = i;
+    if (k < 0) return fail(c, "unknown kernel family");
+    std::vector<Mut> m(c->nrep);
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(m.data(), c->d_mut, sizeof(Mut) * c->nrep, hipMemcpyDeviceToHost));
+    double e = 0;
+    for (auto& x : m) e += (double)x.pair_evals[k];
+    *evals = e;
+    return 0;
+}
+
 int pimc_kernel_time(pimc_ctx* c, const char* family, double* ms, long* launches) {
+    resolve_events(c);
     auto it = c->fam.find(family);
     if (it == c->fam.end()) {
         *ms = 0;

@@ -12,6 +12,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace pimc {
 
@@ -28,10 +29,13 @@ struct Rep {
     int tnumb[MAXT], tmol[MAXT], tlev[MAXT], tbegin[MAXT];
     int pot1d[MAXT], pot2d[MAXT], rotdens;
     int rot_atom;  // global index of the (first) rotor atom, -1 if none
+    int lin_atom;  // global index of the linear molecule, -1 if none
+    int pot1d_id;  // 1D table of the atom type, -1 if none
+    int pot2d_id;  // 2D table of the linear-molecule type, -1 if none
     int key;       // replica index, second word of the generator key
     double lambda[MAXT], mcstep[MAXT];
     double T, beta, tau, rtstep;
-    long long bead_off, rot_off, hist_off, sample_off;
+    long long bead_off, rot_off, hist_off, sample_off, row_off, prep_off;
 };
 
 struct Mut {
@@ -40,7 +44,7 @@ struct Mut {
     unsigned nsamples, pad;
     unsigned long long total[MAXT][3];
     unsigned long long accep[MAXT][3];
-    unsigned long long pair_evals;
+    unsigned long long pair_evals[4];  // bisect, centroid, rotor, estimate
 };
 
 struct RotBead {
@@ -113,58 +117,118 @@ __device__ __forceinline__ double gauss_from(uint4 w, double alpha) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Interpolation.  Cell search: a guess from the mean spacing, then local correction to the
-// cell a bisection search would return (largest k with x[k] <= r, capped at n-2).
-__device__ __forceinline__ float4 knot1(const Tables& T, int i, float) { return T.k1f[i]; }
-__device__ __forceinline__ double4 knot1(const Tables& T, int i, double) { return T.k1d[i]; }
+// Interpolation.  Table descriptors are resolved once per kernel, so a lookup costs one
+// round trip to the table.  Cell search: a guess from the mean spacing, then local
+// correction to the cell a bisection search would return (largest k with x[k] <= r,
+// capped at n-2).
+template <typename R>
+using vec4_t = typename std::conditional<std::is_same<R, float>::value, float4, double4>::type;
 
 template <typename R>
-__device__ __forceinline__ R spot1d(const Tables& T, int tid, R r) {
+struct Lut1D {
+    const vec4_t<R>* k;  // knots (x, y, y'', -)
+    int n;
+    R x0, xmax, inv_h, alpha, unode, c6;
+};
+
+template <typename R>
+struct Lut2D {
+    const R* v;  // V[r][cos]
+    const R* g;  // r grid, then cos grid
+    int nr, nc;
+    R rmin, cmin, dr, dc;
+};
+
+__device__ __forceinline__ const float4* knots_ptr(const Tables& T, float) { return T.k1f; }
+__device__ __forceinline__ const double4* knots_ptr(const Tables& T, double) { return T.k1d; }
+__device__ __forceinline__ const float* values_ptr(const Tables& T, float) { return T.v2f; }
+__device__ __forceinline__ const double* values_ptr(const Tables& T, double) { return T.v2d; }
+__device__ __forceinline__ const float* grid_ptr(const Tables& T, float) { return T.g2f; }
+__device__ __forceinline__ const double* grid_ptr(const Tables& T, double) { return T.g2d; }
+
+template <typename R>
+__device__ __forceinline__ Lut1D<R> lut1d(const Tables& T, int tid) {
+    Lut1D<R> L;
+    if (tid < 0) {
+        L.k = nullptr;
+        L.n = 0;
+        L.x0 = L.xmax = L.inv_h = L.alpha = L.unode = L.c6 = (R)0;
+        return L;
+    }
     const Tab1D t = T.t1[tid];
-    if (r >= (R)t.xmax) {
+    L.k = knots_ptr(T, R()) + t.off;
+    L.n = t.n;
+    L.x0 = (R)t.x0;
+    L.xmax = (R)t.xmax;
+    L.inv_h = (R)t.inv_h;
+    L.alpha = (R)t.alpha;
+    L.unode = (R)t.unode;
+    L.c6 = (R)t.c6;
+    return L;
+}
+
+template <typename R>
+__device__ __forceinline__ Lut2D<R> lut2d(const Tables& T, int tid) {
+    Lut2D<R> L;
+    if (tid < 0) {
+        L.v = nullptr;
+        L.g = nullptr;
+        L.nr = L.nc = 2;
+        L.rmin = L.cmin = L.dr = L.dc = (R)1;
+        return L;
+    }
+    const Tab2D t = T.t2[tid];
+    L.v = values_ptr(T, R()) + t.off;
+    L.g = grid_ptr(T, R()) + t.goff;
+    L.nr = t.nr;
+    L.nc = t.nc;
+    L.rmin = L.g[0];
+    L.cmin = L.g[t.nr];
+    L.dr = (R)t.dr;
+    L.dc = (R)t.dc;
+    return L;
+}
+
+// SPot1D: spline inside the grid, U0 exp(-alpha r) below it, -C6/r^6 above it.
+// `k` is the knot array (LDS or global copy of L.k).
+template <typename R>
+__device__ __forceinline__ R spot1d(const Lut1D<R>& L, const vec4_t<R>* __restrict__ k, R r) {
+    if (r >= L.xmax) {
         R r2 = r * r;
-        return -(R)t.c6 / (r2 * r2 * r2);
+        return -L.c6 / (r2 * r2 * r2);
     }
-    if (r <= (R)t.x0) return (R)t.unode * exp(-(R)t.alpha * r);
-    R f = floor((r - (R)t.x0) * (R)t.inv_h);
-    f = fmin(fmax(f, (R)0), (R)(t.n - 2));
-    int k = (int)f;
-    auto a = knot1(T, t.off + k, R());
-    auto b = knot1(T, t.off + k + 1, R());
-    while (k > 0 && a.x > r) {
-        --k;
+    if (r <= L.x0) return L.unode * exp(-L.alpha * r);
+    R f = floor((r - L.x0) * L.inv_h);
+    f = fmin(fmax(f, (R)0), (R)(L.n - 2));
+    int i = (int)f;
+    vec4_t<R> a = k[i], b = k[i + 1];
+    while (i > 0 && a.x > r) {
+        --i;
         b = a;
-        a = knot1(T, t.off + k, R());
+        a = k[i];
     }
-    while (k < t.n - 2 && b.x <= r) {
-        ++k;
+    while (i < L.n - 2 && b.x <= r) {
+        ++i;
         a = b;
-        b = knot1(T, t.off + k + 1, R());
+        b = k[i + 1];
     }
     R h = b.x - a.x;
     R A = (b.x - r) / h, B = (r - a.x) / h;
     return A * a.y + B * b.y + ((A * A * A - A) * a.z + (B * B * B - B) * b.z) * (h * h) / (R)6;
 }
 
-__device__ __forceinline__ float v2(const Tables& T, long long i, float) { return T.v2f[i]; }
-__device__ __forceinline__ double v2(const Tables& T, long long i, double) { return T.v2d[i]; }
-__device__ __forceinline__ float g2(const Tables& T, int i, float) { return T.g2f[i]; }
-__device__ __forceinline__ double g2(const Tables& T, int i, double) { return T.g2d[i]; }
-
 // LPot2D: cells clamped to the grid, weights extrapolate linearly off-grid.
 template <typename R>
-__device__ __forceinline__ R lpot2d(const Tables& T, int tid, R r, R c) {
-    const Tab2D t = T.t2[tid];
-    R rmin = g2(T, t.goff, R()), cmin = g2(T, t.goff + t.nr, R());
-    R fr = floor((r - rmin) / (R)t.dr), fc = floor((c - cmin) / (R)t.dc);
-    fr = fmin(fmax(fr, (R)0), (R)(t.nr - 2));
-    fc = fmin(fmax(fc, (R)0), (R)(t.nc - 2));
+__device__ __forceinline__ R lpot2d(const Lut2D<R>& L, R r, R c) {
+    R fr = floor((r - L.rmin) / L.dr), fc = floor((c - L.cmin) / L.dc);
+    fr = fmin(fmax(fr, (R)0), (R)(L.nr - 2));
+    fc = fmin(fmax(fc, (R)0), (R)(L.nc - 2));
     int ir = (int)fr, ic = (int)fc;
-    long long base = t.off + (long long)ir * t.nc + ic;
-    R y1 = v2(T, base, R()), y4 = v2(T, base + 1, R());
-    R y2 = v2(T, base + t.nc, R()), y3 = v2(T, base + t.nc + 1, R());
-    R r1 = g2(T, t.goff + ir, R()), r2 = g2(T, t.goff + ir + 1, R());
-    R c1 = g2(T, t.goff + t.nr + ic, R()), c2 = g2(T, t.goff + t.nr + ic + 1, R());
+    const R* v = L.v + (long long)ir * L.nc + ic;
+    R y1 = v[0], y4 = v[1];
+    R y2 = v[L.nc], y3 = v[L.nc + 1];
+    R r1 = L.g[ir], r2 = L.g[ir + 1];
+    R c1 = L.g[L.nr + ic], c2 = L.g[L.nr + ic + 1];
     R dr = (r - r1) / (r2 - r1);
     R dc = (c - c1) / (c2 - c1);
     return ((R)1 - dr) * ((R)1 - dc) * y1 + dr * ((R)1 - dc) * y2 + dr * dc * y3 + ((R)1 - dr) * dc * y4;
@@ -217,21 +281,22 @@ __device__ __forceinline__ double rot_spline(const Tables& T, int tid, double g,
 }
 
 // ---------------------------------------------------------------------------------------
-// Pair potential of a mover (type ta) and a partner (type tj) at one slice; `nh` is the
-// linear rotor's axis at that slice.  Dispatch of PotEnergy(int,double**,int).
+// Pair potential of a mover and a partner at one slice (dispatch of PotEnergy(int,double**,int),
+// mc_piqmc.cc:1847-1958, for one atom type and one linear-molecule type per system,
+// mc_input.cc:386-387).  rot_a / rot_j: the mover / partner is the linear molecule; (nx,ny,nz)
+// is its axis at that slice.
 template <typename R>
-__device__ __forceinline__ R pair_v(const Tables& T, const Rep& rp, int ta, int tj, double xa, double ya,
-                                    double za, double xj, double yj, double zj, double nx, double ny, double nz) {
+__device__ __forceinline__ R pair_v(const Lut1D<R>& l1, const vec4_t<R>* __restrict__ k1, const Lut2D<R>& l2,
+                                    bool rot_a, bool rot_j, double xa, double ya, double za, double xj, double yj,
+                                    double zj, double nx, double ny, double nz) {
     R dx = (R)(xa - xj), dy = (R)(ya - yj), dz = (R)(za - zj);
     R r = sqrt(dx * dx + dy * dy + dz * dz);
-    int ma = rp.tmol[ta], mj = rp.tmol[tj];
-    if (ma == 1 || mj == 1) {
-        int typep = (ma == 1) ? ta : tj;
+    if (rot_a || rot_j) {
         R cost = ((R)nx * dx + (R)ny * dy + (R)nz * dz) / r;
-        if (ma == 1) cost = -cost;
-        return lpot2d<R>(T, rp.pot2d[typep], r, cost);
+        if (rot_a) cost = -cost;
+        return lpot2d<R>(l2, r, cost);
     }
-    return spot1d<R>(T, rp.pot1d[tj], r);
+    return spot1d<R>(l1, k1, r);
 }
 
 template <typename T>

@@ -1,19 +1,25 @@
 // pimc_kernels.hpp -- gfx950 kernels of the sampling hot path.
 //
-//   k_bisect    one wavefront per (replica, imaginary-time segment).  The segment window
-//               (all atoms' beads on the S+1 slices of the segment, plus the linear rotor's
-//               axis there) is staged in LDS once and every atom of the selected types is
-//               moved in turn, as MCBisectionMove does (mc_piqmc.cc:194-294).  Segments of
-//               one offset share only fixed end points, so they run concurrently.
-//   k_centroid  one workgroup per replica, atoms in turn, Delta V over all slices x partners
-//               (MCMolecularMove mc_piqmc.cc:54-102).
-//   k_rotor     one workgroup per replica: the linear rotor's chain lives in LDS; each half of
-//               the even/odd checkerboard (mc_piqmc.cc:490-512) is a parallel step over slices,
-//               a lane group per slice sums Delta V over partners (MCRotLinStep :781-936).
-//   k_estimate  one workgroup per replica: K, V (+ g(r), g(r,theta)), E_rot and the Cv terms of
-//               MCGetAverage (mc_main.cc:551-616).
-//   k_pot_slice batched PotEnergy(atom, pos, it) (mc_piqmc.cc:1796-1965).
+//   k_bisect     one workgroup per (replica, imaginary-time segment).  The segment window
+//                (all atoms' beads on the S+1 slices of the segment, plus the linear rotor's
+//                axis there) is staged in LDS once and every atom of the selected types is
+//                moved in turn, as MCBisectionMove does (mc_piqmc.cc:194-294).  Segments of
+//                one offset share only fixed end points, so they run concurrently.
+//   k_centroid   one workgroup per replica, atoms in turn, Delta V over all slices x partners
+//                (MCMolecularMove mc_piqmc.cc:54-102).
+//   k_rotor_prep rotor-partner displacements (dr, |dr|) of every translational slice, once per
+//                launch of k_rotor: translations do not change during its sweeps.
+//   k_rotor      one workgroup per replica: the linear rotor's chain lives in LDS; each half of
+//                the even/odd checkerboard (mc_piqmc.cc:490-512) is a parallel step over slices,
+//                a lane group per slice sums Delta V over partners (MCRotLinStep :781-936).
+//   k_est_rows   one workgroup per (replica, atom a0): the a0 row of the pair sum of
+//                GetPotEnergy_Densities (mc_estim.cc:500-687, pairs a0 < a1, all slices, with
+//                the g(r) / g(r,theta) bins) and a0's term of GetKinEnergy (mc_estim.cc:876-937).
+//   k_est_final  one workgroup per replica: rows summed in order (deterministic), GetRotEnergy
+//                (mc_estim.cc:939-987) and the Cv terms of MCGetAverage (mc_main.cc:551-616).
+//   k_pot_slice  batched PotEnergy(atom, pos, it) (mc_piqmc.cc:1796-1965).
 #pragma once
+#include <type_traits>
 #include "pimc_device.hpp"
 
 namespace pimc {
@@ -29,148 +35,12 @@ __device__ __forceinline__ double4 bead(const double4* beads, const Rep& rp, int
     return beads[rp.bead_off + (long long)a * rp.P + it];
 }
 
-// ------------------------------------------------------------------------- bisection
-// LDS: wx,wy,wz [natoms*(S+1)], nh [3*(S+1)], nw [3*(S+1)]
-template <typename R>
-__global__ __launch_bounds__(64) void k_bisect(const Rep* __restrict__ reps, Mut* __restrict__ mut,
-                                               double4* __restrict__ beads, const RotBead* __restrict__ rotb,
-                                               Tables T, const int2* __restrict__ work, Run run) {
-    extern __shared__ double smem[];
-    const int2 wk = work[blockIdx.x];
-    const Rep& rp = reps[wk.x];
-    const int lane = threadIdx.x;
-    const int S = rp.S, P = rp.P, N = rp.natoms;
-    int s0;
-    if (run.ref_order) {
-        if (run.u >= P) return;
-        s0 = run.u;
-    } else {
-        if (run.u >= S) return;
-        s0 = run.u + wk.y * S;
-    }
-    const int W = S + 1;
-    double* wx = smem;
-    double* wy = wx + N * W;
-    double* wz = wy + N * W;
-    double* nh = wz + N * W;
-    double* nw = nh + 3 * W;
-    for (int i = lane; i < N * W; i += 64) {
-        int a = i / W, w = i - a * W;
-        double4 b = bead(beads, rp, a, (s0 + w) % P);
-        wx[i] = b.x;
-        wy[i] = b.y;
-        wz[i] = b.z;
-    }
-    const bool linrot = rp.rotkind == 1 && rp.rot_atom >= 0;
-    for (int w = lane; w < W; w += 64) {
-        double x = 0, y = 0, z = 1;
-        if (linrot) {
-            int ar = rp.rot_atom - rp.tbegin[rp.imtype];
-            RotBead rb = rotb[rp.rot_off + (long long)ar * rp.Prot + ((s0 + w) % P) / rp.RotRatio];
-            x = rb.nx; y = rb.ny; z = rb.nz;
-        }
-        nh[3 * w] = x;
-        nh[3 * w + 1] = y;
-        nh[3 * w + 2] = z;
-    }
-    __syncthreads();
-
-    const Mut* m = mut + wk.x;
-    const uint32_t pass = m->pass;
-    unsigned long long evals = 0;
-    unsigned long long tot[MAXT] = {0, 0, 0, 0}, acc_n[MAXT] = {0, 0, 0, 0};
-    const int tend = run.type_end < rp.ntypes ? run.type_end : rp.ntypes;
-
-    for (int ty = run.type_begin; ty < tend; ++ty) {
-        const int L = rp.tlev[ty];
-        const int St = 1 << L;
-        const double bnorm = 1.0 / (rp.lambda[ty] * rp.tau);
-        for (int ia = 0; ia < rp.tnumb[ty]; ++ia) {
-            const int a = rp.tbegin[ty] + ia;
-            if (lane < 3) {  // end points of the moving atom's segment
-                double* src = lane == 0 ? wx : (lane == 1 ? wy : wz);
-                nw[lane] = src[a * W];
-                nw[3 * St + lane] = src[a * W + St];
-            }
-            __syncthreads();
-            double cum = 0.0;
-            bool accepted = false;
-            for (int level = 0; level < L; ++level) {
-                const int lss = 1 << (L - level);
-                const int half = lss >> 1;
-                const int nmid = 1 << level;
-                const double bkin = bnorm / (double)lss;
-                // new midpoints: slot = half + q*lss, from the current new end points
-                for (int i = lane; i < 3 * nmid; i += 64) {
-                    int q = i / 3, id = i - 3 * q;
-                    int slot = half + q * lss;
-                    uint4 w4 = draw(run.seed, rp.key, pass, K_BISECT, 0, (uint32_t)a, (uint32_t)s0,
-                                    ((uint32_t)level << 24) | ((uint32_t)q << 8) | (uint32_t)id);
-                    nw[3 * slot + id] = 0.5 * (nw[3 * (slot - half) + id] + nw[3 * (slot + half) + id]) +
-                                        gauss_from(w4, bkin);
-                }
-                __syncthreads();
-                // Delta V at the new midpoints: items (midpoint, partner)
-                const int np = N - 1;
-                const int items = nmid * np;
-                double acc = 0.0;
-                for (int i = lane; i < items; i += 64) {
-                    int q = i / np, jp = i - q * np;
-                    int j = jp + (jp >= a);
-                    int slot = half + q * lss;
-                    int tj = 0;
-                    while (tj + 1 < rp.ntypes && j >= rp.tbegin[tj + 1]) ++tj;
-                    double xj = wx[j * W + slot], yj = wy[j * W + slot], zj = wz[j * W + slot];
-                    double nx = nh[3 * slot], ny = nh[3 * slot + 1], nz = nh[3 * slot + 2];
-                    R vn = pair_v<R>(T, rp, ty, tj, nw[3 * slot], nw[3 * slot + 1], nw[3 * slot + 2], xj, yj, zj, nx, ny,
-                                     nz);
-                    R vo = pair_v<R>(T, rp, ty, tj, wx[a * W + slot], wy[a * W + slot], wz[a * W + slot], xj, yj, zj,
-                                     nx, ny, nz);
-                    acc += (double)vn - (double)vo;
-                }
-                acc = wave_sum(acc);
-                evals += 2ull * (unsigned long long)items;
-                // sum over all points up to this level; level acceptance of mc_piqmc.cc:263-273
-                const double prev = cum;
-                cum += acc;
-                double deltav = (cum - 2.0 * prev) * (rp.tau * (double)half);
-                accepted = false;
-                if (deltav < 0.0) accepted = true;
-                else {
-                    uint4 w4 = draw(run.seed, rp.key, pass, K_BISECT, 0, (uint32_t)a, (uint32_t)s0,
-                                    ((uint32_t)level << 24) | (0xFFFFu << 8));
-                    if (exp(-deltav) > u53(w4.x, w4.y)) accepted = true;
-                }
-                if (!accepted) break;
-            }
-            tot[ty] += 1;
-            if (accepted) {
-                acc_n[ty] += 1;
-                for (int i = lane; i < 3 * (St - 1); i += 64) {
-                    int slot = 1 + i / 3, id = i % 3;
-                    double* dst = id == 0 ? wx : (id == 1 ? wy : wz);
-                    dst[a * W + slot] = nw[3 * slot + id];
-                }
-            }
-            __syncthreads();
-        }
-    }
-    // interior slots back to HBM (end points were never modified)
-    for (int i = lane; i < N * (S - 1); i += 64) {
-        int a = i / (S - 1), w = 1 + i - a * (S - 1);
-        int idx = a * W + w;
-        beads[rp.bead_off + (long long)a * P + (s0 + w) % P] = make_double4(wx[idx], wy[idx], wz[idx], 0.0);
-    }
-    if (lane == 0) {
-        for (int ty = run.type_begin; ty < tend; ++ty) {
-            atomicAdd(&mut[wk.x].total[ty][1], tot[ty]);
-            atomicAdd(&mut[wk.x].accep[ty][1], acc_n[ty]);
-        }
-        atomicAdd(&mut[wk.x].pair_evals, evals);
-    }
+__device__ __forceinline__ int type_of(const Rep& rp, int a) {
+    int t = 0;
+    while (t + 1 < rp.ntypes && a >= rp.tbegin[t + 1]) ++t;
+    return t;
 }
 
-// ------------------------------------------------------------------------- centroid
 template <typename T>
 __device__ __forceinline__ T block_sum(T v, T* red) {
     v = wave_sum(v);
@@ -194,10 +64,155 @@ __device__ __forceinline__ void rotor_axis(const RotBead* rotb, const Rep& rp, i
     }
 }
 
+// ------------------------------------------------------------------------- bisection
+// LDS: wx,wy,wz [natoms*(S+1)], nh [3*(S+1)], nw [3*(S+1)]; blockDim a multiple of 64.
 template <typename R>
-__global__ __launch_bounds__(256) void k_centroid(const Rep* __restrict__ reps, Mut* __restrict__ mut,
-                                                  double4* __restrict__ beads, const RotBead* __restrict__ rotb,
-                                                  Tables T, const int* __restrict__ replist, Run run) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bisect(const Rep* __restrict__ reps, Mut* __restrict__ mut,
+                                                double4* __restrict__ beads, const RotBead* __restrict__ rotb,
+                                                Tables T, const int2* __restrict__ work, Run run) {
+    extern __shared__ double smem[];
+    __shared__ double red[16];
+    const int2 wk = work[blockIdx.x];
+    const Rep& rp = reps[wk.x];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int S = rp.S, P = rp.P, N = rp.natoms;
+    int s0;
+    if (run.ref_order) {
+        if (run.u >= P) return;
+        s0 = run.u;
+    } else {
+        if (run.u >= S) return;
+        s0 = run.u + wk.y * S;
+    }
+    const int W = S + 1;
+    double* wx = smem;
+    double* wy = wx + N * W;
+    double* wz = wy + N * W;
+    double* nh = wz + N * W;
+    double* nw = nh + 3 * W;
+    // the 1D spline knots follow the window (16-byte aligned)
+    vec4_t<R>* kn = reinterpret_cast<vec4_t<R>*>(smem + ((3 * N * W + 6 * W + 3) & ~3));
+    const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
+    const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
+    for (int i = tid; i < l1.n; i += nth) kn[i] = l1.k[i];
+    for (int i = tid; i < N * W; i += nth) {
+        int a = i / W, w = i - a * W;
+        double4 b = bead(beads, rp, a, (s0 + w) % P);
+        wx[i] = b.x;
+        wy[i] = b.y;
+        wz[i] = b.z;
+    }
+    for (int w = tid; w < W; w += nth) {
+        double x, y, z;
+        rotor_axis(rotb, rp, (s0 + w) % P, x, y, z);
+        nh[3 * w] = x;
+        nh[3 * w + 1] = y;
+        nh[3 * w + 2] = z;
+    }
+    __syncthreads();
+
+    const uint32_t pass = mut[wk.x].pass;
+    unsigned long long evals = 0;
+    unsigned long long tot[MAXT] = {0, 0, 0, 0}, acc_n[MAXT] = {0, 0, 0, 0};
+    const int tend = run.type_end < rp.ntypes ? run.type_end : rp.ntypes;
+
+    for (int ty = run.type_begin; ty < tend; ++ty) {
+        const int L = rp.tlev[ty];
+        const int St = 1 << L;
+        const double bnorm = 1.0 / (rp.lambda[ty] * rp.tau);
+        for (int ia = 0; ia < rp.tnumb[ty]; ++ia) {
+            const int a = rp.tbegin[ty] + ia;
+            if (tid < 3) {  // end points of the moving atom's segment
+                double* src = tid == 0 ? wx : (tid == 1 ? wy : wz);
+                nw[tid] = src[a * W];
+                nw[3 * St + tid] = src[a * W + St];
+            }
+            __syncthreads();
+            double cum = 0.0;
+            bool accepted = false;
+            for (int level = 0; level < L; ++level) {
+                const int lss = 1 << (L - level);
+                const int half = lss >> 1;
+                const int nmid = 1 << level;
+                const double bkin = bnorm / (double)lss;
+                // new midpoints: slot = half + q*lss, from the current new end points
+                for (int i = tid; i < 3 * nmid; i += nth) {
+                    int q = i / 3, id = i - 3 * q;
+                    int slot = half + q * lss;
+                    uint4 w4 = draw(run.seed, rp.key, pass, K_BISECT, 0, (uint32_t)a, (uint32_t)s0,
+                                    ((uint32_t)level << 24) | ((uint32_t)q << 8) | (uint32_t)id);
+                    nw[3 * slot + id] = 0.5 * (nw[3 * (slot - half) + id] + nw[3 * (slot + half) + id]) +
+                                        gauss_from(w4, bkin);
+                }
+                __syncthreads();
+                // Delta V at the new midpoints: items (midpoint, partner)
+                const int np = N - 1;
+                const int items = nmid * np;
+                double acc = 0.0;
+                const bool rot_a = (a == rp.lin_atom);
+                for (int i = tid; i < items; i += nth) {
+                    int q = i / np, jp = i - q * np;
+                    int j = jp + (jp >= a);
+                    int slot = half + q * lss;
+                    const bool rot_j = (j == rp.lin_atom);
+                    double xj = wx[j * W + slot], yj = wy[j * W + slot], zj = wz[j * W + slot];
+                    double nx = nh[3 * slot], ny = nh[3 * slot + 1], nz = nh[3 * slot + 2];
+#pragma unroll 1
+                    for (int no = 0; no < 2; ++no) {  // new position, then old
+                        double xa = no ? wx[a * W + slot] : nw[3 * slot];
+                        double ya = no ? wy[a * W + slot] : nw[3 * slot + 1];
+                        double za = no ? wz[a * W + slot] : nw[3 * slot + 2];
+                        R v = pair_v<R>(l1, kn, l2, rot_a, rot_j, xa, ya, za, xj, yj, zj, nx, ny, nz);
+                        acc += no ? -(double)v : (double)v;
+                    }
+                }
+                acc = block_sum(acc, red);
+                evals += 2ull * (unsigned long long)items;
+                // sum over all points up to this level; level acceptance of mc_piqmc.cc:263-273
+                const double prev = cum;
+                cum += acc;
+                double deltav = (cum - 2.0 * prev) * (rp.tau * (double)half);
+                accepted = false;
+                if (deltav < 0.0) accepted = true;
+                else {
+                    uint4 w4 = draw(run.seed, rp.key, pass, K_BISECT, 0, (uint32_t)a, (uint32_t)s0,
+                                    ((uint32_t)level << 24) | (0xFFFFu << 8));
+                    if (exp(-deltav) > u53(w4.x, w4.y)) accepted = true;
+                }
+                if (!accepted) break;
+            }
+            tot[ty] += 1;
+            if (accepted) {
+                acc_n[ty] += 1;
+                for (int i = tid; i < 3 * (St - 1); i += nth) {
+                    int slot = 1 + i / 3, id = i % 3;
+                    double* dst = id == 0 ? wx : (id == 1 ? wy : wz);
+                    dst[a * W + slot] = nw[3 * slot + id];
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // interior slots back to HBM (end points were never modified)
+    for (int i = tid; i < N * (S - 1); i += nth) {
+        int a = i / (S - 1), w = 1 + i - a * (S - 1);
+        int idx = a * W + w;
+        beads[rp.bead_off + (long long)a * P + (s0 + w) % P] = make_double4(wx[idx], wy[idx], wz[idx], 0.0);
+    }
+    if (tid == 0) {
+        for (int ty = run.type_begin; ty < tend; ++ty) {
+            atomicAdd(&mut[wk.x].total[ty][1], tot[ty]);
+            atomicAdd(&mut[wk.x].accep[ty][1], acc_n[ty]);
+        }
+        atomicAdd(&mut[wk.x].pair_evals[0], evals);
+    }
+}
+
+// ------------------------------------------------------------------------- centroid
+template <typename R>
+__global__ __launch_bounds__(1024) void k_centroid(const Rep* __restrict__ reps, Mut* __restrict__ mut,
+                                                   double4* __restrict__ beads, const RotBead* __restrict__ rotb,
+                                                   Tables T, const int* __restrict__ replist, Run run) {
     __shared__ double red[16];
     const int r = replist[blockIdx.x];
     const Rep& rp = reps[r];
@@ -205,6 +220,8 @@ __global__ __launch_bounds__(256) void k_centroid(const Rep* __restrict__ reps, 
     if (ty >= rp.ntypes) return;
     const uint32_t pass = mut[r].pass;
     const int N = rp.natoms, P = rp.P, np = N - 1;
+    const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
+    const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
     unsigned long long acc_n = 0;
     for (int ia = 0; ia < rp.tnumb[ty]; ++ia) {
         const int a = rp.tbegin[ty] + ia;
@@ -215,16 +232,16 @@ __global__ __launch_bounds__(256) void k_centroid(const Rep* __restrict__ reps, 
         const double dz = rp.mcstep[ty] * (u53(w1.x, w1.y) - 0.5);
         double acc = 0.0;
         const int items = P * np;
+        const bool rot_a = (a == rp.lin_atom);
         for (int i = threadIdx.x; i < items; i += blockDim.x) {
             int it = i / np, jp = i - it * np;
             int j = jp + (jp >= a);
-            int tj = 0;
-            while (tj + 1 < rp.ntypes && j >= rp.tbegin[tj + 1]) ++tj;
+            const bool rot_j = (j == rp.lin_atom);
             double4 xa = bead(beads, rp, a, it), xj = bead(beads, rp, j, it);
             double nx, ny, nz;
             rotor_axis(rotb, rp, it, nx, ny, nz);
-            R vn = pair_v<R>(T, rp, ty, tj, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, nx, ny, nz);
-            R vo = pair_v<R>(T, rp, ty, tj, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz);
+            R vn = pair_v<R>(l1, l1.k, l2, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, nx, ny, nz);
+            R vo = pair_v<R>(l1, l1.k, l2, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz);
             acc += (double)vn - (double)vo;
         }
         const double deltav = block_sum(acc, red);
@@ -244,15 +261,63 @@ __global__ __launch_bounds__(256) void k_centroid(const Rep* __restrict__ reps, 
     if (threadIdx.x == 0) {
         atomicAdd(&mut[r].total[ty][0], (unsigned long long)rp.tnumb[ty]);
         atomicAdd(&mut[r].accep[ty][0], acc_n);
-        atomicAdd(&mut[r].pair_evals, 2ull * (unsigned long long)rp.tnumb[ty] * P * (N - 1));
+        atomicAdd(&mut[r].pair_evals[1], 2ull * (unsigned long long)rp.tnumb[ty] * P * (N - 1));
     }
 }
 
 // ------------------------------------------------------------------------- linear rotor
-// LDS: phi[Prot], cth[Prot], n[3*Prot] (double)
+// prep[prep_off + it*(N-1) + jp] = (x_rotor - x_j at slice it, |.|), partners jp in atom order
+template <typename R>
+__global__ __launch_bounds__(256) void k_rotor_prep(const Rep* __restrict__ reps, const double4* __restrict__ beads,
+                                                    vec4_t<R>* __restrict__ prep, const int* __restrict__ replist) {
+    const int r = replist[blockIdx.y];
+    const Rep& rp = reps[r];
+    const int np = rp.natoms - 1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rp.P * np) return;
+    const int it = i / np, jp = i - it * np;
+    const int ar = rp.rot_atom;
+    const int j = jp + (jp >= ar);
+    double4 xa = bead(beads, rp, ar, it), xj = bead(beads, rp, j, it);
+    R dx = (R)(xa.x - xj.x), dy = (R)(xa.y - xj.y), dz = (R)(xa.z - xj.z);
+    R rr = sqrt(dx * dx + dy * dy + dz * dz);
+    vec4_t<R> v;
+    v.x = dx; v.y = dy; v.z = dz; v.w = rr;
+    prep[rp.prep_off + i] = v;
+}
+
+// Sum over a slice's partners of V(r, cos theta) with the rotor axis n: cos = -(n . dr)/r
+// (PotRotEnergy mc_piqmc.cc:1967-2044).  Items i = gl, gl+G, ... of the slice's prep block,
+// loaded in batches of four so their table lookups overlap.
+template <typename R>
+__device__ __forceinline__ double slice_energy(const Lut2D<R>& l2, const vec4_t<R>* __restrict__ pk, int items, int gl,
+                                               int G, double nx, double ny, double nz) {
+    double acc = 0.0;
+    const R fx = (R)nx, fy = (R)ny, fz = (R)nz;
+    for (int base = gl; base < items; base += 4 * G) {
+        vec4_t<R> d[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            int i = base + b * G;
+            if (i < items) d[b] = pk[i];
+            else { d[b].x = d[b].y = d[b].z = (R)0; d[b].w = (R)1; }
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            R c = -((fx * d[b].x + fy * d[b].y + fz * d[b].z) / d[b].w);
+            R v = lpot2d<R>(l2, d[b].w, c);
+            if (base + b * G < items) acc += (double)v;
+        }
+    }
+    for (int o = G >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
+    return acc;
+}
+
+// LDS: phi[Prot], cth[Prot], n[3*Prot], E[Prot] (double).  E[k] is the partner energy of slice k
+// at its current axis: the "old" sum of a move is the "new" sum of the last accepted one.
 template <typename R>
 __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mut* __restrict__ mut,
-                                                const double4* __restrict__ beads, RotBead* __restrict__ rotb,
+                                                const vec4_t<R>* __restrict__ prep, RotBead* __restrict__ rotb,
                                                 Tables T, const int* __restrict__ replist, Run run) {
     extern __shared__ double smem[];
     __shared__ unsigned long long red_acc[16];
@@ -271,6 +336,7 @@ __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mu
     double* phi = smem;
     double* cth = phi + Prot;
     double* nv = cth + Prot;
+    double* E = nv + 3 * Prot;
     const int ar = rp.rot_atom;
     const long long rbase = rp.rot_off + (long long)(ar - rp.tbegin[rp.imtype]) * Prot;
     for (int k = threadIdx.x; k < Prot; k += blockDim.x) {
@@ -281,18 +347,26 @@ __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mu
         nv[3 * k + 1] = b.ny;
         nv[3 * k + 2] = b.nz;
     }
-    __syncthreads();
     const uint32_t pass = mut[r].pass;
     const int rtype = rp.imtype;
+    const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
     const int np = N - 1;
     const int items = RR * np;
+    const vec4_t<R>* pr = prep + rp.prep_off;
     // lane groups of G lanes (power of two <= 64) per slice move
     const int maxmoves = (Prot + 1) / 2;
     int G = 1;
     while (G < 64 && G * 2 * maxmoves <= (int)blockDim.x) G *= 2;
     const int groups = blockDim.x / G;
     const int g = threadIdx.x / G, gl = threadIdx.x % G;
-    unsigned long long nacc = 0;
+    __syncthreads();
+    // energies of every slice at its current axis (translations are fixed for this launch)
+    for (int k = g; k < Prot; k += groups) {
+        double e = slice_energy<R>(l2, pr + (long long)k * RR * np, items, gl, G, nv[3 * k], nv[3 * k + 1], nv[3 * k + 2]);
+        if (gl == 0) E[k] = e;
+    }
+    __syncthreads();
+    unsigned long long nacc = 0, nev = (threadIdx.x == 0) ? (unsigned long long)Prot * items : 0ull;
     int err = 0;
     for (int sw = 0; sw < nsweep; ++sw) {
         const uint32_t jsweep = (uint32_t)(j0 + sw * jstride);
@@ -304,28 +378,16 @@ __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mu
                 if (k0 < 0) k0 += Prot;
                 if (k2 >= Prot) k2 -= Prot;
                 uint4 w0 = draw(run.seed, rp.key, pass, K_ROTLIN, jsweep, (uint32_t)ar, (uint32_t)k, 0);
-                uint4 w1 = draw(run.seed, rp.key, pass, K_ROTLIN, jsweep, (uint32_t)ar, (uint32_t)k, 1);
                 double cost = cth[k] + rp.rtstep * (u53(w0.x, w0.y) - 0.5);
                 double ph = phi[k] + rp.rtstep * (u53(w0.z, w0.w) - 0.5);
                 if (cost > 1.0) cost = 2.0 - cost;
                 if (cost < -1.0) cost = -2.0 - cost;
                 double sint = sqrt(1.0 - cost * cost);
                 double nnx = sint * cos(ph), nny = sint * sin(ph), nnz = cost;
-                double onx = nv[3 * k], ony = nv[3 * k + 1], onz = nv[3 * k + 2];
-                double acc = 0.0;
-                for (int i = gl; i < items; i += G) {
-                    int q = i / np, jp = i - q * np;
-                    int j = jp + (jp >= ar);
-                    int it = k * RR + q;
-                    double4 xa = bead(beads, rp, ar, it), xj = bead(beads, rp, j, it);
-                    R dx = (R)(xa.x - xj.x), dy = (R)(xa.y - xj.y), dz = (R)(xa.z - xj.z);
-                    R rr = sqrt(dx * dx + dy * dy + dz * dz);
-                    R co = -(((R)onx * dx + (R)ony * dy + (R)onz * dz) / rr);
-                    R cn = -(((R)nnx * dx + (R)nny * dy + (R)nnz * dz) / rr);
-                    acc += (double)lpot2d<R>(T, rp.pot2d[rtype], rr, cn) - (double)lpot2d<R>(T, rp.pot2d[rtype], rr, co);
-                }
-                for (int o = G >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
+                double enew = slice_energy<R>(l2, pr + (long long)k * RR * np, items, gl, G, nnx, nny, nnz);
                 if (gl == 0) {
+                    nev += (unsigned long long)items;
+                    double onx = nv[3 * k], ony = nv[3 * k + 1], onz = nv[3 * k + 2];
                     double p0 = nv[3 * k0] * onx + nv[3 * k0 + 1] * ony + nv[3 * k0 + 2] * onz;
                     double p1 = onx * nv[3 * k2] + ony * nv[3 * k2 + 1] + onz * nv[3 * k2 + 2];
                     double dold = rot_spline(T, rp.rotdens, p0, 0) * rot_spline(T, rp.rotdens, p1, 0);
@@ -338,8 +400,12 @@ __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mu
                         err = 1;  // "Negative rot density" (mc_piqmc.cc:850,886): the move is not taken
                     } else {
                         double rd = (dold > RZERO) ? dnew / dold : 1.0;
-                        rd *= exp(-rp.tau * acc);
-                        bool accepted = (rd > 1.0) || (rd > u53(w1.x, w1.y));
+                        rd *= exp(-rp.tau * (enew - E[k]));
+                        bool accepted = rd > 1.0;
+                        if (!accepted) {
+                            uint4 w1 = draw(run.seed, rp.key, pass, K_ROTLIN, jsweep, (uint32_t)ar, (uint32_t)k, 1);
+                            accepted = rd > u53(w1.x, w1.y);
+                        }
                         if (accepted) {
                             ++nacc;
                             cth[k] = cost;
@@ -347,6 +413,7 @@ __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mu
                             nv[3 * k] = nnx;
                             nv[3 * k + 1] = nny;
                             nv[3 * k + 2] = nnz;
+                            E[k] = enew;
                         }
                     }
                 }
@@ -364,11 +431,12 @@ __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mu
         rotb[rbase + k] = b;
     }
     unsigned long long s = block_sum(nacc, red_acc);
+    unsigned long long ne = block_sum(nev, red_acc);
     if (err) atomicOr(&mut[r].err, 1);
     if (threadIdx.x == 0) {
         atomicAdd(&mut[r].total[rtype][2], (unsigned long long)nsweep * Prot);
         atomicAdd(&mut[r].accep[rtype][2], s);
-        atomicAdd(&mut[r].pair_evals, 2ull * nsweep * Prot * (unsigned long long)items);
+        atomicAdd(&mut[r].pair_evals[2], ne);
     }
 }
 
@@ -379,60 +447,49 @@ __global__ void k_pass_end(Mut* mut, int n) {
 }
 
 // ------------------------------------------------------------------------- estimators
-// out (if not null) receives 8 doubles per replica; otherwise the sample ring.
+// rows[row_off + a0] = (pair potential of row a0 summed over slices, kinetic term of a0)
 template <typename R>
-__global__ __launch_bounds__(256) void k_estimate(const Rep* __restrict__ reps, Mut* __restrict__ mut,
-                                                  const double4* __restrict__ beads, const RotBead* __restrict__ rotb,
-                                                  Tables T, const int* __restrict__ replist, unsigned* __restrict__ hist,
-                                                  double* __restrict__ samples, int sample_cap, double* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_est_rows(const Rep* __restrict__ reps, const double4* __restrict__ beads,
+                                                  const RotBead* __restrict__ rotb, Tables T,
+                                                  const int2* __restrict__ rowlist, unsigned* __restrict__ hist,
+                                                  double2* __restrict__ rows) {
     __shared__ double red[16];
-    const int r = replist[blockIdx.x];
-    const Rep& rp = reps[r];
+    const int2 rw = rowlist[blockIdx.x];
+    const Rep& rp = reps[rw.x];
+    const int a0 = rw.y;
     const int N = rp.natoms, P = rp.P;
+    const int t0 = type_of(rp, a0);
+    const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
+    const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
     // kinetic, GetKinEnergy mc_estim.cc:876-937
     double kin = 0.0;
-    for (int i = threadIdx.x; i < N * P; i += blockDim.x) {
-        int a = i / P, it = i - a * P;
-        int ty = 0;
-        while (ty + 1 < rp.ntypes && a >= rp.tbegin[ty + 1]) ++ty;
-        double4 b0 = bead(beads, rp, a, it), b1 = bead(beads, rp, a, (it + 1) % P);
+    for (int it = threadIdx.x; it < P; it += blockDim.x) {
+        double4 b0 = bead(beads, rp, a0, it), b1 = bead(beads, rp, a0, (it + 1) % P);
         double dx = b0.x - b1.x, dy = b0.y - b1.y, dz = b0.z - b1.z;
-        kin += (dx * dx + dy * dy + dz * dz) / (4.0 * rp.beta * rp.lambda[ty]);
+        kin += (dx * dx + dy * dy + dz * dz);
     }
-    double r2avr = block_sum(kin, red);
-    double skin = (double)P * rp.T * (0.5 * (double)(3 * N) - r2avr);
     // potential and densities, GetPotEnergy_Densities mc_estim.cc:500-687
     const double drad = MAX_RADIUS / (double)BINSR;
     const double dth = 3.14159265358979323846 / (double)(BINST - 1);
     unsigned* g1 = hist ? hist + rp.hist_off : nullptr;
     unsigned* g2h = g1 ? g1 + BINSR : nullptr;
     double pot = 0.0;
-    const long long npair = (long long)N * (N - 1) / 2;
-    for (long long i = threadIdx.x; i < npair * P; i += blockDim.x) {
-        int it = (int)(i % P);
-        long long pi = i / P;
-        // pair index -> (a0 < a1), row-major over a0
-        int a0 = 0;
-        long long rem = pi;
-        while (rem >= N - 1 - a0) {
-            rem -= N - 1 - a0;
-            ++a0;
-        }
-        int a1 = a0 + 1 + (int)rem;
-        int t0 = 0, t1 = 0;
-        while (t0 + 1 < rp.ntypes && a0 >= rp.tbegin[t0 + 1]) ++t0;
-        while (t1 + 1 < rp.ntypes && a1 >= rp.tbegin[t1 + 1]) ++t1;
+    const int nb = N - 1 - a0;
+    const int m0 = rp.tmol[t0];
+    for (int i = threadIdx.x; i < nb * P; i += blockDim.x) {
+        int b = i / P, it = i - b * P;
+        int a1 = a0 + 1 + b;
+        int t1 = type_of(rp, a1);
+        int m1 = rp.tmol[t1];
         double4 b0 = bead(beads, rp, a0, it), b1 = bead(beads, rp, a1, it);
-        int m0 = rp.tmol[t0], m1 = rp.tmol[t1];
         R dx = (R)(b0.x - b1.x), dy = (R)(b0.y - b1.y), dz = (R)(b0.z - b1.z);
         R rr = sqrt(dx * dx + dy * dy + dz * dz);
         if (m0 == 1 || m1 == 1) {
             double nx, ny, nz;
             rotor_axis(rotb, rp, it, nx, ny, nz);
-            int typep = (m0 == 1) ? t0 : t1;
             R cost = ((R)nx * dx + (R)ny * dy + (R)nz * dz) / rr;
             if (m0 == 1) cost = -cost;
-            pot += (double)lpot2d<R>(T, rp.pot2d[typep], rr, cost);
+            pot += (double)lpot2d<R>(l2, rr, cost);
             if (g2h) {
                 int br = (int)floor((double)rr / drad);
                 if (br < BINSR && br >= 0) {
@@ -441,14 +498,27 @@ __global__ __launch_bounds__(256) void k_estimate(const Rep* __restrict__ reps, 
                 }
             }
         } else if (t0 == t1 && m0 == 0) {
-            pot += (double)spot1d<R>(T, rp.pot1d[t1], rr);
+            pot += (double)spot1d<R>(l1, l1.k, rr);
             if (g1) {
                 int br = (int)floor((double)rr / drad);
                 if (br < BINSR && br >= 0) atomicAdd(&g1[br], 1u);
             }
         }
     }
-    double spot = block_sum(pot, red) / (double)P;
+    double sp = block_sum(pot, red);
+    double sk = block_sum(kin, red) / (4.0 * rp.beta * rp.lambda[t0]);
+    if (threadIdx.x == 0) rows[rp.row_off + a0] = make_double2(sp, sk);
+}
+
+// out (if not null) receives 8 doubles per replica; otherwise the sample ring.
+__global__ __launch_bounds__(256) void k_est_final(const Rep* __restrict__ reps, Mut* __restrict__ mut,
+                                                   const RotBead* __restrict__ rotb, Tables T,
+                                                   const int* __restrict__ replist, const double2* __restrict__ rows,
+                                                   double* __restrict__ samples, int sample_cap, double* __restrict__ out) {
+    __shared__ double red[16];
+    const int r = replist[blockIdx.x];
+    const Rep& rp = reps[r];
+    const int N = rp.natoms, P = rp.P;
     // rotational energy, GetRotEnergy mc_estim.cc:939-987
     double srot = 0.0, esq = 0.0, term = 0.0;
     if (rp.rotkind == 1 && rp.rot_atom >= 0 && rp.rotdens >= 0) {
@@ -468,6 +538,14 @@ __global__ __launch_bounds__(256) void k_estimate(const Rep* __restrict__ reps, 
         esq = block_sum(a3, red);
     }
     if (threadIdx.x == 0) {
+        double r2avr = 0.0, spot = 0.0;
+        for (int a = 0; a < N; ++a) {
+            double2 v = rows[rp.row_off + a];
+            spot += v.x;
+            r2avr += v.y;
+        }
+        spot /= (double)P;
+        double skin = (double)P * rp.T * (0.5 * (double)(3 * N) - r2avr);
         double n3 = (double)(3 * N);
         double a = n3 * 0.5 / rp.tau - skin - spot - srot;
         double sCv = -0.5 * n3 / (rp.beta * rp.tau) - a * a + (2.0 / rp.beta) * (0.5 * n3 / rp.tau - skin) + term - esq;
@@ -484,6 +562,7 @@ __global__ __launch_bounds__(256) void k_estimate(const Rep* __restrict__ reps, 
         }
         if (dst)
             for (int q = 0; q < 8; ++q) dst[q] = rec[q];
+        atomicAdd(&mut[r].pair_evals[3], (unsigned long long)N * (N - 1) / 2 * P);
     }
 }
 
@@ -497,18 +576,16 @@ __global__ void k_pot_slice(const Rep* __restrict__ reps, const double4* __restr
     if (q >= nq) return;
     const Rep& rp = reps[qrep[q]];
     int a = qatom[q], it = qit[q];
-    int ta = 0;
-    while (ta + 1 < rp.ntypes && a >= rp.tbegin[ta + 1]) ++ta;
+    const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
+    const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
     double nx, ny, nz;
     rotor_axis(rotb, rp, it, nx, ny, nz);
     double s = 0.0;
     for (int j = 0; j < rp.natoms; ++j) {
         if (j == a) continue;
-        int tj = 0;
-        while (tj + 1 < rp.ntypes && j >= rp.tbegin[tj + 1]) ++tj;
         double4 xj = bead(beads, rp, j, it);
-        s += (double)pair_v<R>(T, rp, ta, tj, qpos[3 * q], qpos[3 * q + 1], qpos[3 * q + 2], xj.x, xj.y, xj.z, nx, ny,
-                               nz);
+        s += (double)pair_v<R>(l1, l1.k, l2, a == rp.lin_atom, j == rp.lin_atom, qpos[3 * q], qpos[3 * q + 1],
+                               qpos[3 * q + 2], xj.x, xj.y, xj.z, nx, ny, nz);
     }
     out[q] = s;
 }

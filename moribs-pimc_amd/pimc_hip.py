@@ -64,6 +64,7 @@ SIGNATURES = {
     "pimc_estimate": (_i, [ctypes.c_void_p, _dp]),
     "pimc_kernel_time": (_i, [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_long)]),
     "pimc_synchronize": (_i, [ctypes.c_void_p]),
+    "pimc_family_evals": (_i, [ctypes.c_void_p, ctypes.c_char_p, _dp]),
 }
 
 
@@ -218,6 +219,11 @@ class Context:
         ms, n = ctypes.c_double(), ctypes.c_long()
         self._check(self.lib.pimc_kernel_time(self.h, family.encode(), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def family_evals(self, family):
+        e = ctypes.c_double()
+        self._check(self.lib.pimc_family_evals(self.h, family.encode(), ctypes.byref(e)))
+        return e.value
 
     def synchronize(self):
         return self._check(self.lib.pimc_synchronize(self.h))
