@@ -221,9 +221,22 @@ int pimc_add_pot1d(pimc_ctx* c, int n, const double* r, const double* v) {
     t.unode = y[0] * std::exp(t.alpha * x[0]);
     double r0 = std::pow(x[n - 2], 6.0), r1 = std::pow(x[n - 1], 6.0);
     t.c6 = (y[n - 1] - y[n - 2]) / (1.0 / r0 - 1.0 / r1);
-    for (int i = 0; i < n; ++i) c->k1.push_back(make_double4(x[i], y[i], y2[i], 0.0));
+    // the fourth knot component is the inverse width of the cell the knot opens
+    for (int i = 0; i < n; ++i)
+        c->k1.push_back(make_double4(x[i], y[i], y2[i], i + 1 < n ? 1.0 / (x[i + 1] - x[i]) : 0.0));
     c->t1.push_back(t);
     return (int)c->t1.size() - 1;
+}
+
+// a grid counts as uniform when every point lies within 1e-6 (relative to the span) of the
+// straight line through its end points
+static bool uniform_grid(const double* g, int n, double& g0, double& step) {
+    g0 = g[0];
+    step = (g[n - 1] - g[0]) / (double)(n - 1);
+    double span = std::fabs(g[n - 1] - g[0]);
+    for (int i = 0; i < n; ++i)
+        if (std::fabs(g[i] - (g0 + i * step)) > 1e-6 * span) return false;
+    return step > 0;
 }
 
 int pimc_add_pot2d(pimc_ctx* c, int nr, int nc, double dr, double dc, const double* rg, const double* cg,
@@ -235,9 +248,10 @@ int pimc_add_pot2d(pimc_ctx* c, int nr, int nc, double dr, double dc, const doub
     t.nc = nc;
     t.off = (long long)c->v2.size();
     t.goff = (int)c->g2.size();
-    t.pad = 0;
     t.dr = dr;
     t.dc = dc;
+    bool ur = uniform_grid(rg, nr, t.gr0, t.gdr), uc = uniform_grid(cg, nc, t.gc0, t.gdc);
+    t.uniform = (ur && uc && dr > 0 && dc > 0) ? 1 : 0;
     c->g2.insert(c->g2.end(), rg, rg + nr);
     c->g2.insert(c->g2.end(), cg, cg + nc);
     c->v2.insert(c->v2.end(), v, v + (size_t)nr * nc);
@@ -288,6 +302,7 @@ int pimc_set_replica(pimc_ctx* c, int r, const pimc_replica_desc* d) {
     }
     if (natoms > 4096) return fail(c, "too many atoms");
     if (d->Prot > 0 && d->P % d->Prot) return fail(c, "NumbTimes is not proportional to NumbRotTimes");
+    if (d->Prot == 1) return fail(c, "rotational sampling needs at least two rotational slices");
     c->desc[r] = *d;
     c->desc_set[r] = 1;
     return 0;
@@ -363,7 +378,7 @@ int pimc_commit(pimc_ctx* c) {
             if (d.rotdens < 0 || d.rotdens >= (int)c->tr.size()) return fail(c, "linear rotor without a .rot table");
             c->bead_moves_per_pass += (double)p.P * p.Prot;
             rotreps.push_back(r);
-            lds_r = std::max(lds_r, (size_t)6 * p.Prot * sizeof(double));
+            lds_r = std::max(lds_r, (size_t)7 * p.Prot * sizeof(double));
         }
         p.bead_off = boff;
         boff += (long long)p.natoms * p.P;
@@ -381,7 +396,7 @@ int pimc_commit(pimc_ctx* c) {
         }
         int W = S + 1;
         size_t knots = p.pot1d_id >= 0 ? (size_t)c->t1[p.pot1d_id].n * (c->fp64 ? sizeof(double4) : sizeof(float4)) : 0;
-        lds_b = std::max(lds_b, (size_t)((3 * p.natoms * W + 6 * W + 3) & ~3) * sizeof(double) + knots);
+        lds_b = std::max(lds_b, (size_t)((3 * p.natoms * W + 9 * W + 16 + 3) & ~3) * sizeof(double) + knots);
         int nseg = c->ref_order ? 1 : p.P / S;
         for (int k = 0; k < nseg; ++k) work.push_back(make_int2(r, k));
         c->Nmax = std::max(c->Nmax, p.natoms);
@@ -446,7 +461,7 @@ int pimc_commit(pimc_ctx* c) {
     // tables
     std::vector<float4> k1f(c->k1.size());
     for (size_t i = 0; i < c->k1.size(); ++i)
-        k1f[i] = make_float4((float)c->k1[i].x, (float)c->k1[i].y, (float)c->k1[i].z, 0.f);
+        k1f[i] = make_float4((float)c->k1[i].x, (float)c->k1[i].y, (float)c->k1[i].z, (float)c->k1[i].w);
     std::vector<float> v2f(c->v2.begin(), c->v2.end()), g2f(c->g2.begin(), c->g2.end());
     c->d_k1f = dalloc<float4>(k1f.size());
     c->d_k1d = dalloc<double4>(c->k1.size());

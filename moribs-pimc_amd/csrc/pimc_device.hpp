@@ -60,8 +60,9 @@ struct Tab2D {
     int nr, nc;
     long long off;  // into values
     int goff;       // into grids: r grid at goff, cos grid at goff + nr
-    int pad;
-    double dr, dc;
+    int uniform;    // both grids uniform to 1e-6 relative: FP32 lookups use grid arithmetic
+    double dr, dc;  // steps of the file header (cell index)
+    double gr0, gdr, gc0, gdc;  // first point and spacing of the actual grids
 };
 struct TabRot {
     int n, off;
@@ -126,7 +127,7 @@ using vec4_t = typename std::conditional<std::is_same<R, float>::value, float4, 
 
 template <typename R>
 struct Lut1D {
-    const vec4_t<R>* k;  // knots (x, y, y'', -)
+    const vec4_t<R>* k;  // knots (x, y, y'', 1/(x[i+1]-x[i]))
     int n;
     R x0, xmax, inv_h, alpha, unode, c6;
 };
@@ -135,8 +136,9 @@ template <typename R>
 struct Lut2D {
     const R* v;  // V[r][cos]
     const R* g;  // r grid, then cos grid
-    int nr, nc;
-    R rmin, cmin, dr, dc;
+    int nr, nc, uniform;
+    R rmin, cmin, dr, dc, inv_dr, inv_dc;  // grid origins and file-header steps
+    R gdr, gdc, inv_gdr, inv_gdc;          // actual grid spacings (uniform grids)
 };
 
 __device__ __forceinline__ const float4* knots_ptr(const Tables& T, float) { return T.k1f; }
@@ -174,7 +176,9 @@ __device__ __forceinline__ Lut2D<R> lut2d(const Tables& T, int tid) {
         L.v = nullptr;
         L.g = nullptr;
         L.nr = L.nc = 2;
-        L.rmin = L.cmin = L.dr = L.dc = (R)1;
+        L.uniform = 0;
+        L.rmin = L.cmin = L.dr = L.dc = L.inv_dr = L.inv_dc = (R)1;
+        L.gdr = L.gdc = L.inv_gdr = L.inv_gdc = (R)1;
         return L;
     }
     const Tab2D t = T.t2[tid];
@@ -182,15 +186,28 @@ __device__ __forceinline__ Lut2D<R> lut2d(const Tables& T, int tid) {
     L.g = grid_ptr(T, R()) + t.goff;
     L.nr = t.nr;
     L.nc = t.nc;
+    L.uniform = t.uniform;
     L.rmin = L.g[0];
     L.cmin = L.g[t.nr];
     L.dr = (R)t.dr;
     L.dc = (R)t.dc;
+    L.inv_dr = (R)(1.0 / t.dr);
+    L.inv_dc = (R)(1.0 / t.dc);
+    L.gdr = (R)t.gdr;
+    L.gdc = (R)t.gdc;
+    L.inv_gdr = (R)(1.0 / t.gdr);
+    L.inv_gdc = (R)(1.0 / t.gdc);
     return L;
 }
 
+template <typename R>
+constexpr bool is_fp32() {
+    return std::is_same<R, float>::value;
+}
+
 // SPot1D: spline inside the grid, U0 exp(-alpha r) below it, -C6/r^6 above it.
-// `k` is the knot array (LDS or global copy of L.k).
+// `k` is the knot array (LDS or global copy of L.k).  FP64 keeps the reference's
+// arithmetic; FP32 multiplies by the stored inverse cell width instead of dividing.
 template <typename R>
 __device__ __forceinline__ R spot1d(const Lut1D<R>& L, const vec4_t<R>* __restrict__ k, R r) {
     if (r >= L.xmax) {
@@ -213,13 +230,33 @@ __device__ __forceinline__ R spot1d(const Lut1D<R>& L, const vec4_t<R>* __restri
         b = k[i + 1];
     }
     R h = b.x - a.x;
-    R A = (b.x - r) / h, B = (r - a.x) / h;
-    return A * a.y + B * b.y + ((A * A * A - A) * a.z + (B * B * B - B) * b.z) * (h * h) / (R)6;
+    if constexpr (is_fp32<R>()) {
+        R A = (b.x - r) * a.w, B = (r - a.x) * a.w;
+        return A * a.y + B * b.y + ((A * A * A - A) * a.z + (B * B * B - B) * b.z) * (h * h) * (R)(1.0 / 6.0);
+    } else {
+        R A = (b.x - r) / h, B = (r - a.x) / h;
+        return A * a.y + B * b.y + ((A * A * A - A) * a.z + (B * B * B - B) * b.z) * (h * h) / (R)6;
+    }
 }
 
 // LPot2D: cells clamped to the grid, weights extrapolate linearly off-grid.
 template <typename R>
 __device__ __forceinline__ R lpot2d(const Lut2D<R>& L, R r, R c) {
+    if constexpr (is_fp32<R>()) {
+        if (L.uniform) {  // grid positions by arithmetic, reciprocal multiplies, lerp form
+            R fr = floor((r - L.rmin) * L.inv_dr), fc = floor((c - L.cmin) * L.inv_dc);
+            fr = fmin(fmax(fr, (R)0), (R)(L.nr - 2));
+            fc = fmin(fmax(fc, (R)0), (R)(L.nc - 2));
+            int ir = (int)fr, ic = (int)fc;
+            const R* v = L.v + ir * L.nc + ic;
+            R y1 = v[0], y4 = v[1];
+            R y2 = v[L.nc], y3 = v[L.nc + 1];
+            R wr = (r - (L.rmin + fr * L.gdr)) * L.inv_gdr;
+            R wc = (c - (L.cmin + fc * L.gdc)) * L.inv_gdc;
+            R lo = y1 + wc * (y4 - y1), hi = y2 + wc * (y3 - y2);
+            return lo + wr * (hi - lo);
+        }
+    }
     R fr = floor((r - L.rmin) / L.dr), fc = floor((c - L.cmin) / L.dc);
     fr = fmin(fmax(fr, (R)0), (R)(L.nr - 2));
     fc = fmin(fmax(fc, (R)0), (R)(L.nc - 2));
@@ -232,6 +269,36 @@ __device__ __forceinline__ R lpot2d(const Lut2D<R>& L, R r, R c) {
     R dr = (r - r1) / (r2 - r1);
     R dc = (c - c1) / (c2 - c1);
     return ((R)1 - dr) * ((R)1 - dc) * y1 + dr * ((R)1 - dc) * y2 + dr * dc * y3 + ((R)1 - dr) * dc * y4;
+}
+
+// SRotDens (mc_poten.cc:548-570) with the table descriptor already resolved: the last value
+// above the grid, the line through the first two knots below it, the spline inside.
+__device__ __forceinline__ double rot_rho(const Tables& T, const TabRot& t, double g) {
+    const double4* k = T.rk + t.off;
+    if (g > t.xmax) return k[t.n - 1].y;
+    if (g < t.x0) {
+        double4 k0 = k[0], k1 = k[1];
+        double sa = (k1.y - k0.y) / (k1.x - k0.x);
+        double sb = (k0.y * k1.x - k1.y * k0.x) / (k1.x - k0.x);
+        return sa * g + sb;
+    }
+    double f = floor((g - t.x0) * t.inv_h);
+    f = fmin(fmax(f, 0.0), (double)(t.n - 2));
+    int i = (int)f;
+    double4 a = k[i], b = k[i + 1];
+    while (i > 0 && a.x > g) {
+        --i;
+        b = a;
+        a = k[i];
+    }
+    while (i < t.n - 2 && b.x <= g) {
+        ++i;
+        a = b;
+        b = k[i + 1];
+    }
+    double h = b.x - a.x;
+    double A = (b.x - g) / h, B = (g - a.x) / h;
+    return A * a.y + B * b.y + ((A * A * A - A) * a.z + (B * B * B - B) * b.z) * (h * h) / 6.;
 }
 
 // .rot spline in FP64 (x, f, f'') at column `col`: 0 rho, 1 erot, 2 esq.
@@ -292,7 +359,9 @@ __device__ __forceinline__ R pair_v(const Lut1D<R>& l1, const vec4_t<R>* __restr
     R dx = (R)(xa - xj), dy = (R)(ya - yj), dz = (R)(za - zj);
     R r = sqrt(dx * dx + dy * dy + dz * dz);
     if (rot_a || rot_j) {
-        R cost = ((R)nx * dx + (R)ny * dy + (R)nz * dz) / r;
+        R cost;
+        if constexpr (is_fp32<R>()) cost = __fdividef((R)nx * dx + (R)ny * dy + (R)nz * dz, r);
+        else cost = ((R)nx * dx + (R)ny * dy + (R)nz * dz) / r;
         if (rot_a) cost = -cost;
         return lpot2d<R>(l2, r, cost);
     }

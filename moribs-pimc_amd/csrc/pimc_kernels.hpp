@@ -90,8 +90,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     double* wz = wy + N * W;
     double* nh = wz + N * W;
     double* nw = nh + 3 * W;
-    // the 1D spline knots follow the window (16-byte aligned)
-    vec4_t<R>* kn = reinterpret_cast<vec4_t<R>*>(smem + ((3 * N * W + 6 * W + 3) & ~3));
+    double* gw = nw + 3 * W;  // draws of the moving atom: 3 per midpoint, then one per level
+    // the 1D spline knots follow (16-byte aligned)
+    vec4_t<R>* kn = reinterpret_cast<vec4_t<R>*>(smem + ((3 * N * W + 9 * W + 16 + 3) & ~3));
     const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
     const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
     for (int i = tid; i < l1.n; i += nth) kn[i] = l1.k[i];
@@ -127,6 +128,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 nw[tid] = src[a * W];
                 nw[3 * St + tid] = src[a * W + St];
             }
+            // every draw of this move at once: Gaussians of midpoint m (level-major order)
+            // and the level acceptance uniforms
+            const int nm = St - 1;
+            for (int i = tid; i < 3 * nm + L; i += nth) {
+                if (i < 3 * nm) {
+                    int m = i / 3, id = i - 3 * m;
+                    int level = 31 - __clz(m + 1);
+                    int q = m + 1 - (1 << level);
+                    uint4 w4 = draw(run.seed, rp.key, pass, K_BISECT, 0, (uint32_t)a, (uint32_t)s0,
+                                    ((uint32_t)level << 24) | ((uint32_t)q << 8) | (uint32_t)id);
+                    gw[i] = gauss_from(w4, bnorm / (double)(1 << (L - level)));
+                } else {
+                    int level = i - 3 * nm;
+                    uint4 w4 = draw(run.seed, rp.key, pass, K_BISECT, 0, (uint32_t)a, (uint32_t)s0,
+                                    ((uint32_t)level << 24) | (0xFFFFu << 8));
+                    gw[i] = u53(w4.x, w4.y);
+                }
+            }
             __syncthreads();
             double cum = 0.0;
             bool accepted = false;
@@ -134,15 +153,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 const int lss = 1 << (L - level);
                 const int half = lss >> 1;
                 const int nmid = 1 << level;
-                const double bkin = bnorm / (double)lss;
                 // new midpoints: slot = half + q*lss, from the current new end points
                 for (int i = tid; i < 3 * nmid; i += nth) {
                     int q = i / 3, id = i - 3 * q;
                     int slot = half + q * lss;
-                    uint4 w4 = draw(run.seed, rp.key, pass, K_BISECT, 0, (uint32_t)a, (uint32_t)s0,
-                                    ((uint32_t)level << 24) | ((uint32_t)q << 8) | (uint32_t)id);
                     nw[3 * slot + id] = 0.5 * (nw[3 * (slot - half) + id] + nw[3 * (slot + half) + id]) +
-                                        gauss_from(w4, bkin);
+                                        gw[3 * (nmid - 1 + q) + id];
                 }
                 __syncthreads();
                 // Delta V at the new midpoints: items (midpoint, partner)
@@ -174,11 +190,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 double deltav = (cum - 2.0 * prev) * (rp.tau * (double)half);
                 accepted = false;
                 if (deltav < 0.0) accepted = true;
-                else {
-                    uint4 w4 = draw(run.seed, rp.key, pass, K_BISECT, 0, (uint32_t)a, (uint32_t)s0,
-                                    ((uint32_t)level << 24) | (0xFFFFu << 8));
-                    if (exp(-deltav) > u53(w4.x, w4.y)) accepted = true;
-                }
+                else if (exp(-deltav) > gw[3 * nm + level]) accepted = true;
                 if (!accepted) break;
             }
             tot[ty] += 1;
@@ -282,29 +294,38 @@ __global__ __launch_bounds__(256) void k_rotor_prep(const Rep* __restrict__ reps
     R dx = (R)(xa.x - xj.x), dy = (R)(xa.y - xj.y), dz = (R)(xa.z - xj.z);
     R rr = sqrt(dx * dx + dy * dy + dz * dz);
     vec4_t<R> v;
-    v.x = dx; v.y = dy; v.z = dz; v.w = rr;
+    if constexpr (is_fp32<R>()) {  // unit vector: a move's cos(theta) is then one dot product
+        R ir = (R)1 / rr;
+        v.x = dx * ir; v.y = dy * ir; v.z = dz * ir;
+    } else {
+        v.x = dx; v.y = dy; v.z = dz;
+    }
+    v.w = rr;
     prep[rp.prep_off + i] = v;
 }
 
 // Sum over a slice's partners of V(r, cos theta) with the rotor axis n: cos = -(n . dr)/r
 // (PotRotEnergy mc_piqmc.cc:1967-2044).  Items i = gl, gl+G, ... of the slice's prep block,
-// loaded in batches of four so their table lookups overlap.
+// loaded in batches of eight so their table lookups overlap.
 template <typename R>
 __device__ __forceinline__ double slice_energy(const Lut2D<R>& l2, const vec4_t<R>* __restrict__ pk, int items, int gl,
                                                int G, double nx, double ny, double nz) {
+    constexpr int B = 8;
     double acc = 0.0;
     const R fx = (R)nx, fy = (R)ny, fz = (R)nz;
-    for (int base = gl; base < items; base += 4 * G) {
-        vec4_t<R> d[4];
+    for (int base = gl; base < items; base += B * G) {
+        vec4_t<R> d[B];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
+        for (int b = 0; b < B; ++b) {
             int i = base + b * G;
             if (i < items) d[b] = pk[i];
             else { d[b].x = d[b].y = d[b].z = (R)0; d[b].w = (R)1; }
         }
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            R c = -((fx * d[b].x + fy * d[b].y + fz * d[b].z) / d[b].w);
+        for (int b = 0; b < B; ++b) {
+            R c;
+            if constexpr (is_fp32<R>()) c = -(fx * d[b].x + fy * d[b].y + fz * d[b].z);
+            else c = -((fx * d[b].x + fy * d[b].y + fz * d[b].z) / d[b].w);
             R v = lpot2d<R>(l2, d[b].w, c);
             if (base + b * G < items) acc += (double)v;
         }
@@ -313,8 +334,10 @@ __device__ __forceinline__ double slice_energy(const Lut2D<R>& l2, const vec4_t<
     return acc;
 }
 
-// LDS: phi[Prot], cth[Prot], n[3*Prot], E[Prot] (double).  E[k] is the partner energy of slice k
-// at its current axis: the "old" sum of a move is the "new" sum of the last accepted one.
+// LDS: phi[Prot], cth[Prot], n[3*Prot], E[Prot], Lk[Prot] (double).  E[k] is the partner energy of
+// slice k at its current axis and Lk[k] the density factor rho(n_k . n_k+1) of the link k -> k+1:
+// the "old" values of a move are the "new" values of the last accepted move that changed them,
+// computed from the same operands in the same order, so a move evaluates only its new sums.
 template <typename R>
 __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mut* __restrict__ mut,
                                                 const vec4_t<R>* __restrict__ prep, RotBead* __restrict__ rotb,
@@ -337,6 +360,7 @@ __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mu
     double* cth = phi + Prot;
     double* nv = cth + Prot;
     double* E = nv + 3 * Prot;
+    double* Lk = E + Prot;
     const int ar = rp.rot_atom;
     const long long rbase = rp.rot_off + (long long)(ar - rp.tbegin[rp.imtype]) * Prot;
     for (int k = threadIdx.x; k < Prot; k += blockDim.x) {
@@ -359,11 +383,17 @@ __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mu
     while (G < 64 && G * 2 * maxmoves <= (int)blockDim.x) G *= 2;
     const int groups = blockDim.x / G;
     const int g = threadIdx.x / G, gl = threadIdx.x % G;
+    const TabRot trt = T.tr[rp.rotdens];
     __syncthreads();
     // energies of every slice at its current axis (translations are fixed for this launch)
     for (int k = g; k < Prot; k += groups) {
         double e = slice_energy<R>(l2, pr + (long long)k * RR * np, items, gl, G, nv[3 * k], nv[3 * k + 1], nv[3 * k + 2]);
         if (gl == 0) E[k] = e;
+    }
+    for (int k = threadIdx.x; k < Prot; k += blockDim.x) {
+        int k2 = k + 1 < Prot ? k + 1 : 0;
+        double p = nv[3 * k] * nv[3 * k2] + nv[3 * k + 1] * nv[3 * k2 + 1] + nv[3 * k + 2] * nv[3 * k2 + 2];
+        Lk[k] = rot_rho(T, trt, p);
     }
     __syncthreads();
     unsigned long long nacc = 0, nev = (threadIdx.x == 0) ? (unsigned long long)Prot * items : 0ull;
@@ -384,17 +414,25 @@ __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mu
                 if (cost < -1.0) cost = -2.0 - cost;
                 double sint = sqrt(1.0 - cost * cost);
                 double nnx = sint * cos(ph), nny = sint * sin(ph), nnz = cost;
+                // new link factors rho(n_k-1 . n') and rho(n' . n_k+1), on two lanes when G >= 2
+                double rho0 = 0.0, rho1 = 0.0;
+                if (gl < 2) {
+                    double p = (gl == 0) ? nv[3 * k0] * nnx + nv[3 * k0 + 1] * nny + nv[3 * k0 + 2] * nnz
+                                         : nnx * nv[3 * k2] + nny * nv[3 * k2 + 1] + nnz * nv[3 * k2 + 2];
+                    if (gl == 0) rho0 = rot_rho(T, trt, p);
+                    else rho1 = rot_rho(T, trt, p);
+                }
+                if (G == 1) {
+                    double p = nnx * nv[3 * k2] + nny * nv[3 * k2 + 1] + nnz * nv[3 * k2 + 2];
+                    rho1 = rot_rho(T, trt, p);
+                }
                 double enew = slice_energy<R>(l2, pr + (long long)k * RR * np, items, gl, G, nnx, nny, nnz);
+                if (G > 1) rho1 = __shfl_xor(rho1, 1, G);
                 if (gl == 0) {
                     nev += (unsigned long long)items;
-                    double onx = nv[3 * k], ony = nv[3 * k + 1], onz = nv[3 * k + 2];
-                    double p0 = nv[3 * k0] * onx + nv[3 * k0 + 1] * ony + nv[3 * k0 + 2] * onz;
-                    double p1 = onx * nv[3 * k2] + ony * nv[3 * k2 + 1] + onz * nv[3 * k2 + 2];
-                    double dold = rot_spline(T, rp.rotdens, p0, 0) * rot_spline(T, rp.rotdens, p1, 0);
+                    double dold = Lk[k0] * Lk[k];
                     if (fabs(dold) < RZERO) dold = 0.0;
-                    p0 = nv[3 * k0] * nnx + nv[3 * k0 + 1] * nny + nv[3 * k0 + 2] * nnz;
-                    p1 = nnx * nv[3 * k2] + nny * nv[3 * k2 + 1] + nnz * nv[3 * k2 + 2];
-                    double dnew = rot_spline(T, rp.rotdens, p0, 0) * rot_spline(T, rp.rotdens, p1, 0);
+                    double dnew = rho0 * rho1;
                     if (fabs(dnew) < RZERO) dnew = 0.0;
                     if (dold < 0.0 || dnew < 0.0) {
                         err = 1;  // "Negative rot density" (mc_piqmc.cc:850,886): the move is not taken
@@ -414,6 +452,8 @@ __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mu
                             nv[3 * k + 1] = nny;
                             nv[3 * k + 2] = nnz;
                             E[k] = enew;
+                            Lk[k0] = rho0;
+                            Lk[k] = rho1;
                         }
                     }
                 }
