@@ -139,6 +139,16 @@ class System:
                 self.L.or_set_pot2d(self.h, t, len(rg), len(cg), dr, dc, P(rg), P(cg), P(np.asarray(V).reshape(-1)))
                 c, rho, er, es = tables["rot"][rp["rotdens"]]
                 self.L.or_set_rotdens(self.h, t, len(c), P(c), P(rho), P(er), P(es))
+            if rp["molecule"][t] == 2:
+                # the oracle borrows 3D tables: keep them alive with the system
+                if tables.get("pot3d") is not None:
+                    nr, nth, nch, rmin, rmax, v = tables["pot3d"]
+                    v = np.ascontiguousarray(v, dtype=np.float64)
+                    self._keep.append(v)
+                    self.L.or_set_pot3d(self.h, nr, nth, nch, rmin, rmax, v.ctypes.data_as(_dp))
+                rho, er, es = [np.ascontiguousarray(x, dtype=np.float64) for x in tables["rot3d"]]
+                self._keep += [rho, er, es]
+                self.L.or_set_rot3d(self.h, rho.ctypes.data_as(_dp), er.ctypes.data_as(_dp), es.ctypes.data_as(_dp))
         self.L.or_set_rng(self.h, seed & 0xFFFFFFFF, replica)
         if "coords" in rp:
             self.set_state(rp["coords"], rp["angles"])

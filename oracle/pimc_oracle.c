@@ -766,7 +766,9 @@ void or_vcord(const double eulang[3], const double rcom[3], const double rpt[3],
 void or_caleng(const double com1[3], const double com2[3], double* e, const double eul1[3],
                const double eul2[3]) {
     const double qm = -1.04, qh = 0.520;
-    const double br2ang = 0.52917721092, hr2k = 3.1577465e5, kcal2k = 503.218978939;
+    /* kcal2k is declared under implicit double precision(a-h,o-z) (caleng_tip4p_gg.f:22,34-38):
+       its name starts with k, so the parameter is INTEGER and 503.218978939d0 truncates to 503 */
+    const double br2ang = 0.52917721092, hr2k = 3.1577465e5, kcal2k = 503.0;
     static const double ROwf[3] = {0.0, 0.0, 0.06562}, RH1wf[3] = {0.7557, 0.0, -0.5223},
                         RH2wf[3] = {-0.7557, 0.0, -0.5223}, RMwf[3] = {0.0, 0.0, -0.08438};
     double m1[9], m2[9];

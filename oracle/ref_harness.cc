@@ -121,7 +121,9 @@ int h_setup(int ntypes, const char* names, const int* numb, const int* molecule,
     MCInit();
     MCConfigInit();
     InitPotentials();
-    if (ROTATION) InitRotDensity();
+    // linear rotor: the .rot file in workdir (init_rotdens).  Non-linear rotor: the three
+    // 181x361x361 tables come from h_set_rot3d instead of init_rot3D's text files.
+    if (ROTATION && MCAtom[IMTYPE].molecule == 1) InitRotDensity();
     densities_init();
     densities_malloc();
     densities_reset(MC_TOTAL);
@@ -167,6 +169,17 @@ void h_set_rot3d(double* rho, double* eng, double* esq) {
     rhoprp = rho;
     erotpr = eng;
     erotsq = esq;
+}
+
+// the globals init_pot3D (mc_poten.cc:254-303) fills from a file
+void h_set_pot3d(int nr, int nth, int nch, double rmin, double rmax, double* v) {
+    Rgrd = nr;
+    THgrd = nth;
+    CHgrd = nch;
+    Rvmin = rmin;
+    Rvmax = rmax;
+    Rvstep = (Rvmax - Rvmin) / (double)(Rgrd - 1);
+    vtable = v;
 }
 
 double h_spot1d(double r, int type) { return SPot1D(r, type); }
