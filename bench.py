@@ -142,6 +142,25 @@ def _reap(procs, times):
         time.sleep(0.01)
 
 
+def traffic_per_launch(workload_tag, kernel):
+    """HBM bytes per dispatch of `kernel` from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in
+    profiles/ (counters scaled by the hbm_copy calibration), or None when not profiled."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload_tag}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            tok = line.split()
+            if tok and not tok[0].startswith("#") and tok[0] == kernel:
+                return float(tok[-1])
+    return None
+
+
+KERNEL_NAMES = {"bisect": "k_bisect<float>", "centroid": "k_centroid<float>", "rotor": "k_rotor<float>",
+                "estimate": "k_est_rows<float>"}
+WORKLOAD_TAG = {"c2": "c2_256replicas", "c1": "c1_64he", "c0": "c0_8he"}
+
+
 # ----------------------------------------------------------------------------- GPU
 def main():
     ap = argparse.ArgumentParser()
@@ -162,7 +181,7 @@ def main():
     bm_pass = float(sum(pc.bead_moves_per_pass(rp) for rp in wl["replicas"]))
     ctx.run(args.warmup, sample_every=1)
     ctx.synchronize()
-    fams = ["bisect", "centroid", "rotor_prep", "rotor", "estimate"]
+    fams = ["bisect", "centroid", "rotor", "estimate"]
     efams = ["bisect", "centroid", "rotor", "estimate"]
     k0 = {f: ctx.kernel_time(f) for f in fams}
     e0 = {f: ctx.family_evals(f) for f in efams}
@@ -201,7 +220,9 @@ def main():
                    "estimators": "one MCGetAverage sample per replica per pass"},
         "kernels_ms_per_step": {f: round(kt[f][0] / args.steps, 4) for f in fams},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": None, "kernel": dom,
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
+                     "traffic": traffic_per_launch(WORKLOAD_TAG[args.workload], KERNEL_NAMES[dom]),
+                     "kernel": dom,
                      "note": "FP32 vector peak; the path has no matrix contraction, FLOPs are algorithmic "
                              f"({FLOP_1D:g}/1D and {FLOP_2D:g}/2D pair evaluation)"},
     }

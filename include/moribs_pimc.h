@@ -84,6 +84,12 @@ int  pimc_add_rotdens(pimc_ctx* ctx, int n, const double* cosg, const double* rh
 /* ---- replicas and state ---- */
 int  pimc_set_replica(pimc_ctx* ctx, int replica, const pimc_replica_desc* desc);
 int  pimc_commit(pimc_ctx* ctx);   /* allocates device state after all replicas are set */
+/* Launch-shape options, set before pimc_commit (no counterpart in the reference, whose
+   OpenMP loops have no tunable shape):
+     "rotor_part_slices"  most rotor slices per workgroup of the rotor sweep (even, >= 2);
+                          0 (default) sizes parts from the LDS budget.  Results do not
+                          depend on it: every draw is addressed by (sweep, slice). */
+int  pimc_set_option(pimc_ctx* ctx, const char* name, long value);
 /* coords/angles [3][natoms*P] like MCCoords/MCAngles; cosines follow from angles
    (mc_main.cc:186-197).  angles may be NULL (phi=0, cos(theta)=1, chi=0, mc_setup.cc:476-489). */
 int  pimc_set_state(pimc_ctx* ctx, int replica, const double* coords, const double* angles);

@@ -110,12 +110,14 @@ struct pimc_ctx {
     double* d_est = nullptr;
     size_t lds_bisect = 0, lds_rotor = 0;
     int Smax = 1, Pmax = 1, Nmax = 1, max_types = 1;
-    int rotor_block = 256, bisect_block = 64;
+    int bisect_block = 64;
     int2* d_rows = nullptr;     // (replica, atom) rows of the estimator pair sum
     double2* d_rowbuf = nullptr;
     int nrows = 0;
-    void* d_prep = nullptr;     // rotor-partner displacements, float4 or double4
-    int prep_items_max = 0;
+    RotPart* d_parts = nullptr;  // rotor chains split into slice ranges, one workgroup each
+    unsigned* d_flags = nullptr;  // half-sweeps completed per part
+    int nparts = 0;
+    int opt_rotor_part_slices = 0;  // most rotor slices per part, 0: from the LDS budget
     long long nbeads = 0, nrotb = 0, nhist = 0;
     hipGraphExec_t graph_exec = nullptr;
     bool graph_ok = false;
@@ -194,7 +196,7 @@ void pimc_destroy(pimc_ctx* c) {
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     void* ptrs[] = {c->d_rep, c->d_mut, c->d_beads, c->d_rotb, c->d_hist, c->d_samples, c->d_k1f, c->d_k1d,
                     c->d_t1, c->d_v2f, c->d_v2d, c->d_g2d, c->d_g2f, c->d_t2, c->d_rk, c->d_re, c->d_tr,
-                    c->d_work, c->d_all, c->d_rotreps, c->d_est, c->d_rows, c->d_rowbuf, c->d_prep};
+                    c->d_work, c->d_all, c->d_rotreps, c->d_est, c->d_rows, c->d_rowbuf, c->d_parts, c->d_flags};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -308,6 +310,17 @@ int pimc_set_replica(pimc_ctx* c, int r, const pimc_replica_desc* d) {
     return 0;
 }
 
+int pimc_set_option(pimc_ctx* c, const char* name, long value) {
+    if (!c || !name) return fail(c, "pimc_set_option: null argument");
+    if (c->committed) return fail(c, "options are set before pimc_commit");
+    if (std::strcmp(name, "rotor_part_slices") == 0) {
+        if (value < 0 || value > (1 << 20)) return fail(c, "rotor_part_slices out of range");
+        c->opt_rotor_part_slices = (int)value;
+        return 0;
+    }
+    return fail(c, std::string("unknown option ") + name);
+}
+
 int pimc_commit(pimc_ctx* c) {
     if (c->committed) return fail(c, "already committed");
     for (int r = 0; r < c->nrep; ++r)
@@ -317,8 +330,8 @@ int pimc_commit(pimc_ctx* c) {
     long long boff = 0, roff = 0, hoff = 0;
     std::vector<int2> work, rows;
     std::vector<int> rotreps;
-    long long rowoff = 0, prepoff = 0;
-    size_t lds_b = 0, lds_r = 0;
+    long long rowoff = 0;
+    size_t lds_b = 0;
     c->bead_moves_per_pass = 0;
     for (int r = 0; r < c->nrep; ++r) {
         const pimc_replica_desc& d = c->desc[r];
@@ -376,9 +389,9 @@ int pimc_commit(pimc_ctx* c) {
         }
         if (p.rotkind == 1) {
             if (d.rotdens < 0 || d.rotdens >= (int)c->tr.size()) return fail(c, "linear rotor without a .rot table");
+            if (p.Prot % 2) return fail(c, "the checkerboard rotor sweep needs an even number of rotor slices");
             c->bead_moves_per_pass += (double)p.P * p.Prot;
             rotreps.push_back(r);
-            lds_r = std::max(lds_r, (size_t)7 * p.Prot * sizeof(double));
         }
         p.bead_off = boff;
         boff += (long long)p.natoms * p.P;
@@ -389,11 +402,6 @@ int pimc_commit(pimc_ctx* c) {
         p.row_off = rowoff;
         for (int a = 0; a < p.natoms; ++a) rows.push_back(make_int2(r, a));
         rowoff += p.natoms;
-        p.prep_off = prepoff;
-        if (p.rotkind == 1) {
-            prepoff += (long long)p.P * (p.natoms - 1);
-            c->prep_items_max = std::max(c->prep_items_max, p.P * (p.natoms - 1));
-        }
         int W = S + 1;
         size_t knots = p.pot1d_id >= 0 ? (size_t)c->t1[p.pot1d_id].n * (c->fp64 ? sizeof(double4) : sizeof(float4)) : 0;
         lds_b = std::max(lds_b, (size_t)((3 * p.natoms * W + 9 * W + 16 + 3) & ~3) * sizeof(double) + knots);
@@ -405,9 +413,51 @@ int pimc_commit(pimc_ctx* c) {
         c->max_types = std::max(c->max_types, p.ntypes);
     }
     if (lds_b > 160 * 1024) return fail(c, "segment window exceeds LDS (too many atoms for this segment size)");
-    if (lds_r > 150 * 1024) return fail(c, "rotor chain exceeds LDS");
     c->lds_bisect = lds_b;
+    // rotor parts: slice ranges whose partner geometry (vec4 per translational slice and partner)
+    // stays within a fixed LDS budget; the heaviest chains first, the parts of a chain adjacent
+    const size_t geo_bytes = c->fp64 ? sizeof(double4) : sizeof(float4);
+    const long long budget = 52 * 1024;  // three parts per CU
+    std::stable_sort(rotreps.begin(), rotreps.end(), [&](int x, int y) {
+        const Rep &a = c->rep[x], &b = c->rep[y];
+        return (long long)a.P * a.Prot * a.natoms > (long long)b.P * b.Prot * b.natoms;
+    });
+    std::vector<RotPart> parts;
+    size_t lds_r = 0;
+    for (int r : rotreps) {
+        const Rep& p = c->rep[r];
+        const long long per_slice = (long long)p.RotRatio * (p.natoms - 1) * (long long)geo_bytes;
+        int ns = (int)std::max<long long>(2, (budget - 16 * (long long)sizeof(double)) /
+                                                 (7 * (long long)sizeof(double) + per_slice)) & ~1;
+        if (c->opt_rotor_part_slices > 0) ns = std::max(2, std::min(ns, c->opt_rotor_part_slices & ~1));
+        if (ns >= p.Prot) ns = p.Prot;
+        int W = (p.Prot + ns - 1) / ns;
+        ns = ((p.Prot + W - 1) / W + 1) & ~1;  // even, balanced
+        W = (p.Prot + ns - 1) / ns;
+        const int fbase = (int)parts.size();
+        for (int w = 0; w < W; ++w) {
+            RotPart q{};
+            q.r = r;
+            q.k0 = w * ns;
+            q.k1 = std::min(p.Prot, (w + 1) * ns);
+            q.w = w;
+            q.W = W;
+            q.fbase = fbase;
+            parts.push_back(q);
+            const int n = q.k1 - q.k0;
+            lds_r = std::max(lds_r, (size_t)((7 * (n + 2) + 3) & ~3) * sizeof(double) + (size_t)(n * per_slice));
+        }
+    }
+    if (lds_r > 160 * 1024) return fail(c, "rotor chain exceeds LDS");
     c->lds_rotor = lds_r;
+    c->nparts = (int)parts.size();
+    c->d_parts = dalloc<RotPart>(parts.size());
+    c->d_flags = dalloc<unsigned>(parts.size());
+    if (!c->d_parts || !c->d_flags) return fail(c, "hipMalloc failed");
+    if (!parts.empty()) {
+        HIPCHECK(c, hipMemcpy(c->d_parts, parts.data(), sizeof(RotPart) * parts.size(), hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemset(c->d_flags, 0, sizeof(unsigned) * parts.size()));
+    }
     c->nbeads = boff;
     c->nrotb = roff;
     c->nhist = hoff;
@@ -421,8 +471,7 @@ int pimc_commit(pimc_ctx* c) {
     c->nrows = (int)rows.size();
     c->d_rows = dalloc<int2>(rows.size());
     c->d_rowbuf = dalloc<double2>(rows.size());
-    c->d_prep = dalloc<char>((size_t)std::max<long long>(prepoff, 1) * (c->fp64 ? sizeof(double4) : sizeof(float4)));
-    if (!c->d_rows || !c->d_rowbuf || !c->d_prep) return fail(c, "hipMalloc failed");
+    if (!c->d_rows || !c->d_rowbuf) return fail(c, "hipMalloc failed");
     HIPCHECK(c, hipMemcpy(c->d_rows, rows.data(), sizeof(int2) * rows.size(), hipMemcpyHostToDevice));
     // device allocations
     c->d_rep = dalloc<Rep>(c->nrep);
@@ -486,11 +535,6 @@ int pimc_commit(pimc_ctx* c) {
         up(c->d_rk, c->rk.data(), c->rk.size() * sizeof(double4)) ||
         up(c->d_re, c->re.data(), c->re.size() * sizeof(double4)) || up(c->d_tr, c->tr.data(), c->tr.size() * sizeof(TabRot)))
         return fail(c, "table upload failed");
-    // rotor block: up to 64 lanes per slice move of the largest chain
-    int pr = 1;
-    for (auto& p : c->rep)
-        if (p.rotkind) pr = std::max(pr, p.Prot);
-    c->rotor_block = std::min(1024, std::max(64, ((pr + 1) / 2) * 64));
     // bisection block: enough lanes for the widest level of the largest window
     c->bisect_block = c->Nmax <= 24 ? 64 : (c->Nmax <= 64 ? 128 : 256);
     c->committed = true;
@@ -620,17 +664,11 @@ void launch_centroid(pimc_ctx* c, int type) {
 
 template <typename R>
 void launch_rotor(pimc_ctx* c, int u) {
-    if (c->nrotreps == 0) return;
+    if (c->nparts == 0) return;
     Run run{c->seed, u, c->ref_order ? 1 : 0, 0, 0};
-    vec4_t<R>* prep = static_cast<vec4_t<R>*>(c->d_prep);
-    {
-        Timed t(c, "rotor_prep");
-        hipLaunchKernelGGL(k_rotor_prep<R>, dim3((c->prep_items_max + 255) / 256, c->nrotreps), dim3(256), 0,
-                           c->stream, c->d_rep, c->d_beads, prep, c->d_rotreps);
-    }
     Timed t(c, "rotor");
-    hipLaunchKernelGGL(k_rotor<R>, dim3(c->nrotreps), dim3(c->rotor_block), c->lds_rotor, c->stream, c->d_rep,
-                       c->d_mut, prep, c->d_rotb, c->tables(), c->d_rotreps, run);
+    hipLaunchKernelGGL(k_rotor<R>, dim3(c->nparts), dim3(256), c->lds_rotor, c->stream, c->d_rep, c->d_mut,
+                       c->d_beads, c->d_rotb, c->tables(), c->d_parts, c->d_flags, run);
 }
 
 template <typename R>
@@ -673,8 +711,10 @@ int check_errors(pimc_ctx* c) {
     std::vector<Mut> m(c->nrep);
     HIPCHECK(c, hipMemcpyAsync(m.data(), c->d_mut, sizeof(Mut) * c->nrep, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    for (int r = 0; r < c->nrep; ++r)
+    for (int r = 0; r < c->nrep; ++r) {
+        if (m[r].err & 2) return fail(c, "rotor sweep: a neighbouring part did not arrive (replica " + std::to_string(r) + ")");
         if (m[r].err) return fail(c, "Rotational Moves: Negative rot density (replica " + std::to_string(r) + ")");
+    }
     return 0;
 }
 

@@ -35,7 +35,7 @@ struct Rep {
     int key;       // replica index, second word of the generator key
     double lambda[MAXT], mcstep[MAXT];
     double T, beta, tau, rtstep;
-    long long bead_off, rot_off, hist_off, sample_off, row_off, prep_off;
+    long long bead_off, rot_off, hist_off, sample_off, row_off;
 };
 
 struct Mut {

@@ -7,11 +7,11 @@
 //                one offset share only fixed end points, so they run concurrently.
 //   k_centroid   one workgroup per replica, atoms in turn, Delta V over all slices x partners
 //                (MCMolecularMove mc_piqmc.cc:54-102).
-//   k_rotor_prep rotor-partner displacements (dr, |dr|) of every translational slice, once per
-//                launch of k_rotor: translations do not change during its sweeps.
-//   k_rotor      one workgroup per replica: the linear rotor's chain lives in LDS; each half of
-//                the even/odd checkerboard (mc_piqmc.cc:490-512) is a parallel step over slices,
-//                a lane group per slice sums Delta V over partners (MCRotLinStep :781-936).
+//   k_rotor      one workgroup per part of a linear rotor's chain (a range of slices): the part's
+//                orientations and its rotor-partner geometry live in LDS for all sweeps of a
+//                launch; each half of the even/odd checkerboard (mc_piqmc.cc:490-512) is a
+//                parallel step over slices, a lane group per slice sums Delta V over partners
+//                (MCRotLinStep :781-936); parts trade one boundary slice per half-sweep.
 //   k_est_rows   one workgroup per (replica, atom a0): the a0 row of the pair sum of
 //                GetPotEnergy_Densities (mc_estim.cc:500-687, pairs a0 < a1, all slices, with
 //                the g(r) / g(r,theta) bins) and a0's term of GetKinEnergy (mc_estim.cc:876-937).
@@ -278,38 +278,61 @@ __global__ __launch_bounds__(1024) void k_centroid(const Rep* __restrict__ reps,
 }
 
 // ------------------------------------------------------------------------- linear rotor
-// prep[prep_off + it*(N-1) + jp] = (x_rotor - x_j at slice it, |.|), partners jp in atom order
-template <typename R>
-__global__ __launch_bounds__(256) void k_rotor_prep(const Rep* __restrict__ reps, const double4* __restrict__ beads,
-                                                    vec4_t<R>* __restrict__ prep, const int* __restrict__ replist) {
-    const int r = replist[blockIdx.y];
-    const Rep& rp = reps[r];
-    const int np = rp.natoms - 1;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rp.P * np) return;
-    const int it = i / np, jp = i - it * np;
-    const int ar = rp.rot_atom;
-    const int j = jp + (jp >= ar);
-    double4 xa = bead(beads, rp, ar, it), xj = bead(beads, rp, j, it);
-    R dx = (R)(xa.x - xj.x), dy = (R)(xa.y - xj.y), dz = (R)(xa.z - xj.z);
-    R rr = sqrt(dx * dx + dy * dy + dz * dz);
-    vec4_t<R> v;
-    if constexpr (is_fp32<R>()) {  // unit vector: a move's cos(theta) is then one dot product
-        R ir = (R)1 / rr;
-        v.x = dx * ir; v.y = dy * ir; v.z = dz * ir;
-    } else {
-        v.x = dx; v.y = dy; v.z = dz;
+// The rotor chain of a replica is split into parts of contiguous slices [k0, k1) (even
+// length), one workgroup each.  A part builds the rotor-partner geometry of its translational
+// slices in LDS once per launch and keeps it for every sweep of the launch (translations do
+// not move during rotor sweeps).  Between half-sweeps a part needs one orientation from each
+// neighbour: slice k0-1 (odd, moved by the part before) before its even half and slice k1
+// (even, moved by the part after) before its odd half.  Each part publishes the boundary slice
+// it just moved to rotb and then raises its flag (count of half-sweeps done); the reader spins
+// on the flag with a bounded count, so every wave exits.  Boundary values and flags are
+// written and read only with device-coherent atomics, and the writer waits for its value
+// stores to complete before it stores the flag: the exchange needs no cache write-back or
+// invalidation, which on a multi-XCD part would flush the L2 holding everyone's tables.
+// Results do not depend on the split: draws are counter-based per (sweep, slice).
+struct RotPart {
+    int r, k0, k1, w, W, fbase, pad0, pad1;
+};
+
+constexpr unsigned ROT_SPIN_LIMIT = 1u << 22;
+
+// spin until *f >= target (device-coherent loads; the loads that follow are issued after the
+// flag value has returned, and read device-coherent memory themselves)
+__device__ __forceinline__ bool wait_count(unsigned* f, unsigned target) {
+    unsigned spins = 0;
+    while ((int)(__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+        if (++spins > ROT_SPIN_LIMIT) return false;
+        __builtin_amdgcn_s_sleep(2);
     }
-    v.w = rr;
-    prep[rp.prep_off + i] = v;
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    return true;
+}
+
+// store the flag after every earlier store of this lane has completed
+__device__ __forceinline__ void raise_count(unsigned* f, unsigned v) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double load_agent(const double* p) {
+    unsigned long long v =
+        __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)v);
+}
+
+__device__ __forceinline__ void store_agent(double* p, double x) {
+    __hip_atomic_store((unsigned long long*)p, (unsigned long long)__double_as_longlong(x), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Sum over a slice's partners of V(r, cos theta) with the rotor axis n: cos = -(n . dr)/r
-// (PotRotEnergy mc_piqmc.cc:1967-2044).  Items i = gl, gl+G, ... of the slice's prep block,
+// (PotRotEnergy mc_piqmc.cc:1967-2044).  Items i = gl, gl+G, ... of the slice's geometry block,
 // loaded in batches of eight so their table lookups overlap.
 template <typename R>
-__device__ __forceinline__ double slice_energy(const Lut2D<R>& l2, const vec4_t<R>* __restrict__ pk, int items, int gl,
-                                               int G, double nx, double ny, double nz) {
+__device__ __forceinline__ double slice_energy(const Lut2D<R>& l2, const vec4_t<R>* pk, int items, int gl, int G,
+                                               double nx, double ny, double nz) {
     constexpr int B = 8;
     double acc = 0.0;
     const R fx = (R)nx, fy = (R)ny, fz = (R)nz;
@@ -334,17 +357,22 @@ __device__ __forceinline__ double slice_energy(const Lut2D<R>& l2, const vec4_t<
     return acc;
 }
 
-// LDS: phi[Prot], cth[Prot], n[3*Prot], E[Prot], Lk[Prot] (double).  E[k] is the partner energy of
-// slice k at its current axis and Lk[k] the density factor rho(n_k . n_k+1) of the link k -> k+1:
-// the "old" values of a move are the "new" values of the last accepted move that changed them,
-// computed from the same operands in the same order, so a move evaluates only its new sums.
+// LDS (local index kl = k - k0 + 1; kl = 0 and ns+1 are the neighbours' boundary slices):
+// phi, cth, nv[3], E, Lk over ns+2 entries (double), then the geometry block
+// geo[(it - k0*RR) * np + jp] (vec4).  E[kl] is the partner energy of slice kl at its current
+// axis and Lk[kl] the density factor rho(n_kl . n_kl+1) of the link kl -> kl+1: the "old"
+// values of a move are the "new" values of the last accepted move that changed them, or are
+// recomputed from the same operands when a neighbour's slice changes.
 template <typename R>
-__global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mut* __restrict__ mut,
-                                                const vec4_t<R>* __restrict__ prep, RotBead* __restrict__ rotb,
-                                                Tables T, const int* __restrict__ replist, Run run) {
+__global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut* __restrict__ mut,
+                                               const double4* __restrict__ beads, RotBead* __restrict__ rotb,
+                                               Tables T, const RotPart* __restrict__ parts,
+                                               unsigned* __restrict__ flags, Run run) {
     extern __shared__ double smem[];
-    __shared__ unsigned long long red_acc[16];
-    const int r = replist[blockIdx.x];
+    __shared__ unsigned long long red_acc[4];
+    __shared__ int s_err;
+    const RotPart pt = parts[blockIdx.x];
+    const int r = pt.r;
     const Rep& rp = reps[r];
     if (rp.rotkind != 1 || rp.rot_atom < 0) return;
     const int Prot = rp.Prot, RR = rp.RotRatio, P = rp.P, N = rp.natoms;
@@ -356,60 +384,112 @@ __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mu
         if (run.u >= rp.S) return;
         nsweep = P / rp.S; j0 = run.u; jstride = rp.S;
     }
+    const int k0 = pt.k0, ns = pt.k1 - pt.k0, W = pt.W;
+    const int np = N - 1;
+    const int items = RR * np;
     double* phi = smem;
-    double* cth = phi + Prot;
-    double* nv = cth + Prot;
-    double* E = nv + 3 * Prot;
-    double* Lk = E + Prot;
+    double* cth = phi + (ns + 2);
+    double* nv = cth + (ns + 2);
+    double* E = nv + 3 * (ns + 2);
+    double* Lk = E + (ns + 2);
+    vec4_t<R>* geo = reinterpret_cast<vec4_t<R>*>(smem + ((7 * (ns + 2) + 3) & ~3));
     const int ar = rp.rot_atom;
     const long long rbase = rp.rot_off + (long long)(ar - rp.tbegin[rp.imtype]) * Prot;
-    for (int k = threadIdx.x; k < Prot; k += blockDim.x) {
+    unsigned* myflag = flags + pt.fbase + pt.w;
+    unsigned* prevflag = flags + pt.fbase + (pt.w + W - 1) % W;
+    unsigned* nextflag = flags + pt.fbase + (pt.w + 1) % W;
+    if (threadIdx.x == 0) s_err = 0;
+    // own slices and both neighbours' boundary slices
+    for (int kl = threadIdx.x; kl < ns + 2; kl += blockDim.x) {
+        int k = k0 + kl - 1;
+        if (k < 0) k += Prot;
+        if (k >= Prot) k -= Prot;
         RotBead b = rotb[rbase + k];
-        phi[k] = b.phi;
-        cth[k] = b.cth;
-        nv[3 * k] = b.nx;
-        nv[3 * k + 1] = b.ny;
-        nv[3 * k + 2] = b.nz;
+        phi[kl] = b.phi;
+        cth[kl] = b.cth;
+        nv[3 * kl] = b.nx;
+        nv[3 * kl + 1] = b.ny;
+        nv[3 * kl + 2] = b.nz;
+    }
+    // rotor-partner geometry of the part's translational slices (beads of one atom are
+    // contiguous in slices, so consecutive lanes take consecutive slices)
+    const int nit = ns * RR, it0 = k0 * RR;
+    for (int i = threadIdx.x; i < nit * np; i += blockDim.x) {
+        const int jp = i / nit, itl = i - jp * nit;
+        const int j = jp + (jp >= ar);
+        double4 xa = bead(beads, rp, ar, it0 + itl), xj = bead(beads, rp, j, it0 + itl);
+        R dx = (R)(xa.x - xj.x), dy = (R)(xa.y - xj.y), dz = (R)(xa.z - xj.z);
+        R rr = sqrt(dx * dx + dy * dy + dz * dz);
+        vec4_t<R> v;
+        if constexpr (is_fp32<R>()) {  // unit vector: a move's cos(theta) is then one dot product
+            R ir = (R)1 / rr;
+            v.x = dx * ir; v.y = dy * ir; v.z = dz * ir;
+        } else {
+            v.x = dx; v.y = dy; v.z = dz;
+        }
+        v.w = rr;
+        geo[itl * np + jp] = v;
     }
     const uint32_t pass = mut[r].pass;
     const int rtype = rp.imtype;
     const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
-    const int np = N - 1;
-    const int items = RR * np;
-    const vec4_t<R>* pr = prep + rp.prep_off;
+    const TabRot trt = T.tr[rp.rotdens];
     // lane groups of G lanes (power of two <= 64) per slice move
-    const int maxmoves = (Prot + 1) / 2;
+    const int maxmoves = (ns + 1) / 2;
     int G = 1;
     while (G < 64 && G * 2 * maxmoves <= (int)blockDim.x) G *= 2;
     const int groups = blockDim.x / G;
     const int g = threadIdx.x / G, gl = threadIdx.x % G;
-    const TabRot trt = T.tr[rp.rotdens];
+    const unsigned base = __hip_atomic_load(myflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    // energies of every slice at its current axis (translations are fixed for this launch)
-    for (int k = g; k < Prot; k += groups) {
-        double e = slice_energy<R>(l2, pr + (long long)k * RR * np, items, gl, G, nv[3 * k], nv[3 * k + 1], nv[3 * k + 2]);
-        if (gl == 0) E[k] = e;
+    for (int kl = 1 + g; kl <= ns; kl += groups) {
+        double e = slice_energy<R>(l2, geo + (long long)(kl - 1) * items, items, gl, G, nv[3 * kl], nv[3 * kl + 1],
+                                   nv[3 * kl + 2]);
+        if (gl == 0) E[kl] = e;
     }
-    for (int k = threadIdx.x; k < Prot; k += blockDim.x) {
-        int k2 = k + 1 < Prot ? k + 1 : 0;
-        double p = nv[3 * k] * nv[3 * k2] + nv[3 * k + 1] * nv[3 * k2 + 1] + nv[3 * k + 2] * nv[3 * k2 + 2];
-        Lk[k] = rot_rho(T, trt, p);
+    for (int kl = threadIdx.x; kl <= ns; kl += blockDim.x) {
+        double p = nv[3 * kl] * nv[3 * kl + 3] + nv[3 * kl + 1] * nv[3 * kl + 4] + nv[3 * kl + 2] * nv[3 * kl + 5];
+        Lk[kl] = rot_rho(T, trt, p);
     }
     __syncthreads();
-    unsigned long long nacc = 0, nev = (threadIdx.x == 0) ? (unsigned long long)Prot * items : 0ull;
+    unsigned long long nacc = 0, nev = (threadIdx.x == 0) ? (unsigned long long)ns * items : 0ull;
     int err = 0;
     for (int sw = 0; sw < nsweep; ++sw) {
         const uint32_t jsweep = (uint32_t)(j0 + sw * jstride);
         for (int parity = 0; parity < 2; ++parity) {
-            const int nmoves = (Prot - parity + 1) / 2;
+            const unsigned h = 2u * (unsigned)sw + (unsigned)parity;  // half-sweeps done before this one
+            // the neighbour's boundary slice this half reads: k0-1 (even half), k1 (odd half)
+            if (threadIdx.x == 0 && !(sw == 0 && parity == 0)) {
+                const int kl = parity == 0 ? 0 : ns + 1;
+                if (W == 1) {
+                    const int src = parity == 0 ? ns : 1;
+                    nv[3 * kl] = nv[3 * src];
+                    nv[3 * kl + 1] = nv[3 * src + 1];
+                    nv[3 * kl + 2] = nv[3 * src + 2];
+                } else {
+                    // the even half waits for the previous part's odd half of the last sweep,
+                    // the odd half for the next part's even half of this sweep
+                    if (!wait_count(parity == 0 ? prevflag : nextflag, base + h)) s_err = 2;
+                    int k = parity == 0 ? k0 - 1 : k0 + ns;
+                    if (k < 0) k += Prot;
+                    if (k >= Prot) k -= Prot;
+                    RotBead* b = rotb + rbase + k;
+                    nv[3 * kl] = load_agent(&b->nx);
+                    nv[3 * kl + 1] = load_agent(&b->ny);
+                    nv[3 * kl + 2] = load_agent(&b->nz);
+                }
+                const int lk = parity == 0 ? 0 : ns;
+                double p = nv[3 * lk] * nv[3 * lk + 3] + nv[3 * lk + 1] * nv[3 * lk + 4] + nv[3 * lk + 2] * nv[3 * lk + 5];
+                Lk[lk] = rot_rho(T, trt, p);
+            }
+            __syncthreads();
+            const int nmoves = (ns - parity + 1) / 2;  // k0 is even, so local slice kl has parity (kl - 1) & 1
             for (int mv = g; mv < nmoves; mv += groups) {
-                const int k = 2 * mv + parity;
-                int k0 = k - 1, k2 = k + 1;
-                if (k0 < 0) k0 += Prot;
-                if (k2 >= Prot) k2 -= Prot;
+                const int kl = 2 * mv + parity + 1;
+                const int k = k0 + kl - 1;
                 uint4 w0 = draw(run.seed, rp.key, pass, K_ROTLIN, jsweep, (uint32_t)ar, (uint32_t)k, 0);
-                double cost = cth[k] + rp.rtstep * (u53(w0.x, w0.y) - 0.5);
-                double ph = phi[k] + rp.rtstep * (u53(w0.z, w0.w) - 0.5);
+                double cost = cth[kl] + rp.rtstep * (u53(w0.x, w0.y) - 0.5);
+                double ph = phi[kl] + rp.rtstep * (u53(w0.z, w0.w) - 0.5);
                 if (cost > 1.0) cost = 2.0 - cost;
                 if (cost < -1.0) cost = -2.0 - cost;
                 double sint = sqrt(1.0 - cost * cost);
@@ -417,20 +497,21 @@ __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mu
                 // new link factors rho(n_k-1 . n') and rho(n' . n_k+1), on two lanes when G >= 2
                 double rho0 = 0.0, rho1 = 0.0;
                 if (gl < 2) {
-                    double p = (gl == 0) ? nv[3 * k0] * nnx + nv[3 * k0 + 1] * nny + nv[3 * k0 + 2] * nnz
-                                         : nnx * nv[3 * k2] + nny * nv[3 * k2 + 1] + nnz * nv[3 * k2 + 2];
+                    const int kn = gl == 0 ? kl - 1 : kl + 1;
+                    double p = gl == 0 ? nv[3 * kn] * nnx + nv[3 * kn + 1] * nny + nv[3 * kn + 2] * nnz
+                                       : nnx * nv[3 * kn] + nny * nv[3 * kn + 1] + nnz * nv[3 * kn + 2];
                     if (gl == 0) rho0 = rot_rho(T, trt, p);
                     else rho1 = rot_rho(T, trt, p);
                 }
                 if (G == 1) {
-                    double p = nnx * nv[3 * k2] + nny * nv[3 * k2 + 1] + nnz * nv[3 * k2 + 2];
+                    double p = nnx * nv[3 * kl + 3] + nny * nv[3 * kl + 4] + nnz * nv[3 * kl + 5];
                     rho1 = rot_rho(T, trt, p);
                 }
-                double enew = slice_energy<R>(l2, pr + (long long)k * RR * np, items, gl, G, nnx, nny, nnz);
+                double enew = slice_energy<R>(l2, geo + (long long)(kl - 1) * items, items, gl, G, nnx, nny, nnz);
                 if (G > 1) rho1 = __shfl_xor(rho1, 1, G);
                 if (gl == 0) {
                     nev += (unsigned long long)items;
-                    double dold = Lk[k0] * Lk[k];
+                    double dold = Lk[kl - 1] * Lk[kl];
                     if (fabs(dold) < RZERO) dold = 0.0;
                     double dnew = rho0 * rho1;
                     if (fabs(dnew) < RZERO) dnew = 0.0;
@@ -438,7 +519,7 @@ __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mu
                         err = 1;  // "Negative rot density" (mc_piqmc.cc:850,886): the move is not taken
                     } else {
                         double rd = (dold > RZERO) ? dnew / dold : 1.0;
-                        rd *= exp(-rp.tau * (enew - E[k]));
+                        rd *= exp(-rp.tau * (enew - E[kl]));
                         bool accepted = rd > 1.0;
                         if (!accepted) {
                             uint4 w1 = draw(run.seed, rp.key, pass, K_ROTLIN, jsweep, (uint32_t)ar, (uint32_t)k, 1);
@@ -446,35 +527,48 @@ __global__ __launch_bounds__(1024) void k_rotor(const Rep* __restrict__ reps, Mu
                         }
                         if (accepted) {
                             ++nacc;
-                            cth[k] = cost;
-                            phi[k] = ph;
-                            nv[3 * k] = nnx;
-                            nv[3 * k + 1] = nny;
-                            nv[3 * k + 2] = nnz;
-                            E[k] = enew;
-                            Lk[k0] = rho0;
-                            Lk[k] = rho1;
+                            cth[kl] = cost;
+                            phi[kl] = ph;
+                            nv[3 * kl] = nnx;
+                            nv[3 * kl + 1] = nny;
+                            nv[3 * kl + 2] = nnz;
+                            E[kl] = enew;
+                            Lk[kl - 1] = rho0;
+                            Lk[kl] = rho1;
                         }
                     }
                 }
             }
             __syncthreads();
+            // publish the boundary slice this half moved (k0 after the even half, k1-1 after
+            // the odd half), then the count of half-sweeps done
+            if (W > 1 && threadIdx.x == 0) {
+                const int kl = parity == 0 ? 1 : ns;
+                RotBead* b = rotb + rbase + (k0 + kl - 1);
+                store_agent(&b->phi, phi[kl]);
+                store_agent(&b->cth, cth[kl]);
+                store_agent(&b->nx, nv[3 * kl]);
+                store_agent(&b->ny, nv[3 * kl + 1]);
+                store_agent(&b->nz, nv[3 * kl + 2]);
+                raise_count(myflag, base + h + 1u);
+            }
         }
     }
-    for (int k = threadIdx.x; k < Prot; k += blockDim.x) {
-        RotBead b = rotb[rbase + k];
-        b.phi = phi[k];
-        b.cth = cth[k];
-        b.nx = nv[3 * k];
-        b.ny = nv[3 * k + 1];
-        b.nz = nv[3 * k + 2];
-        rotb[rbase + k] = b;
+    for (int kl = 1 + threadIdx.x; kl <= ns; kl += blockDim.x) {
+        RotBead b = rotb[rbase + k0 + kl - 1];
+        b.phi = phi[kl];
+        b.cth = cth[kl];
+        b.nx = nv[3 * kl];
+        b.ny = nv[3 * kl + 1];
+        b.nz = nv[3 * kl + 2];
+        rotb[rbase + k0 + kl - 1] = b;
     }
     unsigned long long s = block_sum(nacc, red_acc);
     unsigned long long ne = block_sum(nev, red_acc);
     if (err) atomicOr(&mut[r].err, 1);
     if (threadIdx.x == 0) {
-        atomicAdd(&mut[r].total[rtype][2], (unsigned long long)nsweep * Prot);
+        if (s_err) atomicOr(&mut[r].err, s_err);
+        atomicAdd(&mut[r].total[rtype][2], (unsigned long long)nsweep * ns);
         atomicAdd(&mut[r].accep[rtype][2], s);
         atomicAdd(&mut[r].pair_evals[2], ne);
     }

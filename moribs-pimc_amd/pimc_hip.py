@@ -52,6 +52,7 @@ SIGNATURES = {
     "pimc_add_rotdens": (_i, [ctypes.c_void_p, _i, _dp, _dp, _dp, _dp]),
     "pimc_set_replica": (_i, [ctypes.c_void_p, _i, ctypes.POINTER(ReplicaDesc)]),
     "pimc_commit": (_i, [ctypes.c_void_p]),
+    "pimc_set_option": (_i, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_long]),
     "pimc_set_state": (_i, [ctypes.c_void_p, _i, _dp, _dp]),
     "pimc_get_state": (_i, [ctypes.c_void_p, _i, _dp, _dp, _dp]),
     "pimc_run": (_i, [ctypes.c_void_p, _i, _i]),
@@ -150,6 +151,9 @@ class Context:
         d.P, d.Prot, d.T, d.rtstep = P, Prot, T, rtstep
         self.descs[r] = d
         return self._check(self.lib.pimc_set_replica(self.h, r, ctypes.byref(d)))
+
+    def set_option(self, name, value):
+        self._check(self.lib.pimc_set_option(self.h, name.encode(), int(value)))
 
     def commit(self):
         return self._check(self.lib.pimc_commit(self.h))

@@ -26,8 +26,11 @@ def perturbed(rp, seed, spread=0.25):
     return c, a
 
 
-def gpu_ctx(wl, flags=0, seed=SEED):
-    return pc.load_into_context(ph.Context(len(wl["replicas"]), flags=flags, seed=seed), wl)
+def gpu_ctx(wl, flags=0, seed=SEED, **options):
+    ctx = ph.Context(len(wl["replicas"]), flags=flags, seed=seed)
+    for k, v in options.items():
+        ctx.set_option(k, v)
+    return pc.load_into_context(ctx, wl)
 
 
 @pytest.mark.parametrize("fp64", [True, False])
@@ -111,6 +114,39 @@ def test_batch_trajectory_fp64():
         assert np.array_equal(ctx.counters(r)[1], S.counters()[1]), r
         assert np.abs(cg - co).max() < 1e-9, r
         assert np.abs(ag[:2] - ao[:2]).max() < 1e-9, r
+
+
+@pytest.mark.parametrize("name,slices,ref_order", [("tiny", 4, False), ("tiny", 6, True), ("c0", 6, False),
+                                                   ("mini", 4, False)])
+def test_trajectory_fp64_rotor_parts(name, slices, ref_order):
+    """Rotor chains split over several workgroups (boundary slices traded through memory
+    between half-sweeps) follow the oracle's chain like the single-workgroup sweep."""
+    wl = pc.workload(name)
+    flags = ph.FLAG_FP64 | (ph.FLAG_REF_ORDER if ref_order else 0)
+    ctx = gpu_ctx(wl, flags, rotor_part_slices=slices)
+    ctx.run(2)
+    for r, rp in enumerate(wl["replicas"]):
+        S = O.System(rp, wl["tables"], seed=SEED, replica=r)
+        S.run(0, 2, order=O.ORDER_REFERENCE if ref_order else O.ORDER_SEGMENTS)
+        cg, ag, _ = ctx.get_state(r)
+        co, ao, _ = S.state()
+        assert np.array_equal(ctx.counters(r)[0], S.counters()[0]), r
+        assert np.array_equal(ctx.counters(r)[1], S.counters()[1]), r
+        assert np.abs(cg - co).max() < 1e-9, r
+        assert np.abs(ag[:2] - ao[:2]).max() < 1e-9, r
+
+
+def test_rotor_parts_fp32_match_single_part():
+    """Production arithmetic: the split changes nothing but the launch shape."""
+    wl = pc.workload("c0")
+    a = gpu_ctx(wl, 0)
+    b = gpu_ctx(wl, 0, rotor_part_slices=8)
+    a.run(3)
+    b.run(3)
+    ca, aa, _ = a.get_state(0)
+    cb, ab, _ = b.get_state(0)
+    assert np.array_equal(a.counters(0)[1], b.counters(0)[1])
+    assert np.array_equal(aa, ab) and np.array_equal(ca, cb)
 
 
 def test_trajectory_fp32_decisions():
