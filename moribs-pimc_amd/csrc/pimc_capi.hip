@@ -116,7 +116,7 @@ struct pimc_ctx {
     int nrows = 0;
     RotPart* d_parts = nullptr;  // rotor chains split into slice ranges, one workgroup each
     unsigned* d_flags = nullptr;  // half-sweeps completed per part
-    int nparts = 0;
+    int nparts = 0, nflags = 0;
     int opt_rotor_part_slices = 0;  // most rotor slices per part, 0: from the LDS budget
     long long nbeads = 0, nrotb = 0, nhist = 0;
     hipGraphExec_t graph_exec = nullptr;
@@ -379,6 +379,8 @@ int pimc_commit(pimc_ctx* c) {
         }
         p.natoms = g;
         p.S = S;
+        for (int t = 0; t < d.ntypes; ++t)
+            if (d.levels[t] < 1 || d.levels[t] > 8) return fail(c, "bisection levels must be 1..8");
         if (!c->ref_order && d.P % S) return fail(c, "segment order needs P divisible by the segment size");
         // table checks
         for (int t = 0; t < d.ntypes; ++t) {
@@ -450,13 +452,41 @@ int pimc_commit(pimc_ctx* c) {
     }
     if (lds_r > 160 * 1024) return fail(c, "rotor chain exceeds LDS");
     c->lds_rotor = lds_r;
+    // Workgroup i of a launch runs on XCD i mod 8.  All parts of a chain go to one XCD, so its
+    // boundary exchange and its .rot table stay in one L2; chains are spread over the XCDs by
+    // work (heaviest first, to the least loaded), and each XCD takes its parts in that order.
+    {
+        constexpr int NXCD = 8;
+        std::vector<std::vector<RotPart>> bins(NXCD);
+        std::vector<double> load(NXCD, 0.0);
+        for (size_t i = 0; i < parts.size();) {
+            const Rep& p = c->rep[parts[i].r];
+            const int W = parts[i].W;
+            int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[x] += (double)p.P * p.Prot * p.natoms;
+            for (int w = 0; w < W; ++w) bins[x].push_back(parts[i + w]);
+            i += W;
+        }
+        size_t len = 0;
+        for (auto& b : bins) len = std::max(len, b.size());
+        std::vector<RotPart> grid(len * NXCD);
+        for (size_t j = 0; j < len; ++j)
+            for (int x = 0; x < NXCD; ++x) {
+                RotPart q{};
+                q.r = -1;
+                if (j < bins[x].size()) q = bins[x][j];
+                grid[j * NXCD + x] = q;
+            }
+        c->nflags = (int)parts.size();
+        parts.swap(grid);
+    }
     c->nparts = (int)parts.size();
     c->d_parts = dalloc<RotPart>(parts.size());
-    c->d_flags = dalloc<unsigned>(parts.size());
+    c->d_flags = dalloc<unsigned>(c->nflags);
     if (!c->d_parts || !c->d_flags) return fail(c, "hipMalloc failed");
     if (!parts.empty()) {
         HIPCHECK(c, hipMemcpy(c->d_parts, parts.data(), sizeof(RotPart) * parts.size(), hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemset(c->d_flags, 0, sizeof(unsigned) * parts.size()));
+        HIPCHECK(c, hipMemset(c->d_flags, 0, sizeof(unsigned) * c->nflags));
     }
     c->nbeads = boff;
     c->nrotb = roff;

@@ -372,6 +372,7 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
     __shared__ unsigned long long red_acc[4];
     __shared__ int s_err;
     const RotPart pt = parts[blockIdx.x];
+    if (pt.r < 0) return;  // padding of the XCD-ordered grid
     const int r = pt.r;
     const Rep& rp = reps[r];
     if (rp.rotkind != 1 || rp.rot_atom < 0) return;
