@@ -61,6 +61,8 @@ typedef struct {
     int    Prot;                     /* rotational slices (ROTATION line), 0 = no rotation */
     double T;                        /* TEMPERATURE, Kelvin */
     double rtstep;                   /* rotational step (ROTATION line) */
+    int    pot3d;                    /* 3D table of the atom / non-linear rotor pairs, or -1 */
+    int    rot3d;                    /* rho/E/E^2 tables of the non-linear rotor, or -1 */
 } pimc_replica_desc;
 
 /* ---- lifetime ---- */
@@ -80,6 +82,12 @@ int  pimc_add_pot2d(pimc_ctx* ctx, int nr, int nc, double dr, double dc, const d
 /* init_rotdens mc_poten.cc:508-546: the four .rot columns */
 int  pimc_add_rotdens(pimc_ctx* ctx, int n, const double* cosg, const double* rho,
                       const double* erot, const double* esq);
+/* init_pot3D mc_poten.cc:254-303: header Rgrd THgrd CHgrd Rvmin Rvmax (R in bohr), then
+   V[(ir*THgrd + ith)*CHgrd + ich] in the file's order */
+int  pimc_add_pot3d(pimc_ctx* ctx, int nr, int nth, int nch, double rmin, double rmax, const double* v);
+/* init_rot3D mc_poten.cc:440-506: the .rho, .eng and .esq tables, 181*361*361 values each in
+   rotpro's order (rotpro_sub.f:7-26); copied to the device at once (566 MB) */
+int  pimc_add_rot3d(pimc_ctx* ctx, const double* rho, const double* erot, const double* esq);
 
 /* ---- replicas and state ---- */
 int  pimc_set_replica(pimc_ctx* ctx, int replica, const pimc_replica_desc* desc);
@@ -91,7 +99,8 @@ int  pimc_commit(pimc_ctx* ctx);   /* allocates device state after all replicas 
                           depend on it: every draw is addressed by (sweep, slice). */
 int  pimc_set_option(pimc_ctx* ctx, const char* name, long value);
 /* coords/angles [3][natoms*P] like MCCoords/MCAngles; cosines follow from angles
-   (mc_main.cc:186-197).  angles may be NULL (phi=0, cos(theta)=1, chi=0, mc_setup.cc:476-489). */
+   (mc_main.cc:186-197).  angles may be NULL (phi=0, cos(theta)=1, chi=0, mc_setup.cc:476-489).
+   Only rotor entries of angles are kept; pimc_get_state returns those defaults for atoms. */
 int  pimc_set_state(pimc_ctx* ctx, int replica, const double* coords, const double* angles);
 int  pimc_get_state(pimc_ctx* ctx, int replica, double* coords, double* angles, double* cosines);
 

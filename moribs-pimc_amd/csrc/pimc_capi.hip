@@ -82,6 +82,10 @@ struct pimc_ctx {
     std::vector<double> v2, g2;
     std::vector<TabRot> tr;
     std::vector<double4> rk, re;
+    std::vector<Tab3D> t3;
+    std::vector<double> v3;
+    double* d_r3 = nullptr;  // rho, E, E^2 of every non-linear density table, 3 * ROT3D_SIZE each
+    int nr3 = 0;
     // device
     hipStream_t stream = nullptr;
     Rep* d_rep = nullptr;
@@ -102,6 +106,20 @@ struct pimc_ctx {
     double4* d_rk = nullptr;
     double4* d_re = nullptr;
     TabRot* d_tr = nullptr;
+    double* d_v3d = nullptr;
+    float* d_v3f = nullptr;
+    Tab3D* d_t3 = nullptr;
+    Frame* d_frames = nullptr;     // rotation matrix of every non-linear rotor slice
+    int2* d_work1 = nullptr;       // bisection work items of the non-linear family
+    int nwork1 = 0;
+    int* d_all0 = nullptr;         // replicas of the linear family
+    int* d_all1 = nullptr;         // replicas of the non-linear family
+    int nall0 = 0, nall1 = 0;
+    int* d_rot3 = nullptr;         // replicas with rotating non-linear rotors
+    int nrot3 = 0, nrotmax = 0;
+    int2* d_rows1 = nullptr;
+    int nrows1 = 0;
+    size_t lds_bisect1 = 0;
     int2* d_work = nullptr;
     int nwork = 0;
     int* d_all = nullptr;
@@ -142,6 +160,14 @@ struct pimc_ctx {
         T.rk = d_rk;
         T.re = d_re;
         T.tr = d_tr;
+        return T;
+    }
+    Tables3 tables3() const {
+        Tables3 T;
+        T.v3d = d_v3d;
+        T.v3f = d_v3f;
+        T.t3 = d_t3;
+        T.r3 = d_r3;
         return T;
     }
 };
@@ -196,7 +222,8 @@ void pimc_destroy(pimc_ctx* c) {
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     void* ptrs[] = {c->d_rep, c->d_mut, c->d_beads, c->d_rotb, c->d_hist, c->d_samples, c->d_k1f, c->d_k1d,
                     c->d_t1, c->d_v2f, c->d_v2d, c->d_g2d, c->d_g2f, c->d_t2, c->d_rk, c->d_re, c->d_tr,
-                    c->d_work, c->d_all, c->d_rotreps, c->d_est, c->d_rows, c->d_rowbuf, c->d_parts, c->d_flags};
+                    c->d_work, c->d_all, c->d_rotreps, c->d_est, c->d_rows, c->d_rowbuf, c->d_parts, c->d_flags,
+                    c->d_r3, c->d_v3d, c->d_v3f, c->d_t3, c->d_frames, c->d_work1, c->d_all0, c->d_all1, c->d_rot3, c->d_rows1};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -282,6 +309,42 @@ int pimc_add_rotdens(pimc_ctx* c, int n, const double* x, const double* rho, con
     return (int)c->tr.size() - 1;
 }
 
+int pimc_add_pot3d(pimc_ctx* c, int nr, int nth, int nch, double rmin, double rmax, const double* v) {
+    if (c->committed) return fail(c, "tables must be added before pimc_commit");
+    if (nr < 2 || nth < 2 || nch < 2 || !(rmax > rmin) || !v) return fail(c, "init_pot3D: bad 3D grid");
+    if (nch != 361 && nch != 181 && nch != 91 && nth != 181)
+        return fail(c, "init_pot3D: Degree wise THETA or CHI grid is violated");
+    Tab3D t{};
+    t.nr = nr;
+    t.nth = nth;
+    t.nch = nch;
+    t.rmin = rmin;
+    t.rmax = rmax;
+    t.rstep = (rmax - rmin) / (double)(nr - 1);
+    t.off = (long long)c->v3.size();
+    c->v3.insert(c->v3.end(), v, v + (size_t)nr * nth * nch);
+    c->t3.push_back(t);
+    return (int)c->t3.size() - 1;
+}
+
+int pimc_add_rot3d(pimc_ctx* c, const double* rho, const double* erot, const double* esq) {
+    if (c->committed) return fail(c, "tables must be added before pimc_commit");
+    if (!rho || !erot || !esq) return fail(c, "init_rot3D: null table");
+    const size_t one = (size_t)ROT3D_SIZE * sizeof(double);
+    double* buf = nullptr;
+    if (hipMalloc(&buf, 3 * one * (size_t)(c->nr3 + 1)) != hipSuccess) return fail(c, "hipMalloc failed (rot3d)");
+    if (c->nr3 > 0) {
+        HIPCHECK(c, hipMemcpy(buf, c->d_r3, 3 * one * (size_t)c->nr3, hipMemcpyDeviceToDevice));
+        (void)hipFree(c->d_r3);
+    }
+    double* dst = buf + (size_t)3 * ROT3D_SIZE * c->nr3;
+    HIPCHECK(c, hipMemcpy(dst, rho, one, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(dst + ROT3D_SIZE, erot, one, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(dst + 2 * ROT3D_SIZE, esq, one, hipMemcpyHostToDevice));
+    c->d_r3 = buf;
+    return c->nr3++;
+}
+
 int pimc_set_replica(pimc_ctx* c, int r, const pimc_replica_desc* d) {
     if (c->committed) return fail(c, "replicas must be set before pimc_commit");
     if (r < 0 || r >= c->nrep) return fail(c, "replica index out of range");
@@ -293,7 +356,13 @@ int pimc_set_replica(pimc_ctx* c, int r, const pimc_replica_desc* d) {
         if (d->molecule[t] == 0 && ++natt > 1)
             return fail(c, "No more then one atom/molecule type: densities and potential energy");
         if (d->molecule[t] != 0 && t != d->ntypes - 1) return fail(c, "Molecules should follow atoms in input file");
-        if (d->molecule[t] == 2) return fail(c, "non-linear rotors are not supported by this build");
+        if (d->molecule[t] == 2) {
+            ++nmolt;
+            if (d->numb[t] > 2) return fail(c, "GetRotE3D supports at most two non-linear rotors");
+            if (d->ntypes > 1 && d->pot3d < 0) return fail(c, "non-linear rotor without a 3D potential table");
+            if (d->Prot > 0 && d->rot3d < 0) return fail(c, "rotating non-linear rotor without rho/E/E^2 tables");
+        }
+        if (d->molecule[t] < 0 || d->molecule[t] > 2) return fail(c, "unknown molecule kind");
         if (d->molecule[t] == 1) {
             ++nmolt;
             if (d->numb[t] != 1) return fail(c, "No more than one linear dopant molecule so far");
@@ -303,6 +372,7 @@ int pimc_set_replica(pimc_ctx* c, int r, const pimc_replica_desc* d) {
         natoms += d->numb[t];
     }
     if (natoms > 4096) return fail(c, "too many atoms");
+    if (nmolt > 1) return fail(c, "one molecule type per replica");
     if (d->Prot > 0 && d->P % d->Prot) return fail(c, "NumbTimes is not proportional to NumbRotTimes");
     if (d->Prot == 1) return fail(c, "rotational sampling needs at least two rotational slices");
     c->desc[r] = *d;
@@ -328,8 +398,9 @@ int pimc_commit(pimc_ctx* c) {
     const double lam = 100.0 * (kHBAR * kHBAR) / (kAMU * kKB);
     c->rep.resize(c->nrep);
     long long boff = 0, roff = 0, hoff = 0;
-    std::vector<int2> work, rows;
-    std::vector<int> rotreps;
+    std::vector<int2> work, rows, work1, rows1;
+    std::vector<int> rotreps, all0, all1, rot3;
+    size_t lds_b1 = 0;
     long long rowoff = 0;
     size_t lds_b = 0;
     c->bead_moves_per_pass = 0;
@@ -353,6 +424,11 @@ int pimc_commit(pimc_ctx* c) {
         p.pot2d_id = -1;
         p.key = r;
         p.rotdens = d.rotdens;
+        p.fam = 0;
+        p.nl_atom = -1;
+        p.nrot = 0;
+        p.pot3d_id = d.pot3d;
+        p.rot3d_id = d.rot3d;
         int g = 0, S = 1;
         for (int t = 0; t < d.ntypes; ++t) {
             p.tnumb[t] = d.numb[t];
@@ -374,6 +450,14 @@ int pimc_commit(pimc_ctx* c) {
                 p.rotkind = 1;
                 p.rot_atom = g;
             }
+            if (d.molecule[t] == 2) {
+                p.fam = 1;
+                p.imtype = t;
+                p.nl_atom = g;
+                p.rot_atom = g;
+                p.nrot = d.numb[t];
+                if (d.Prot > 0) p.rotkind = 2;
+            }
             g += d.numb[t];
             c->bead_moves_per_pass += (double)d.numb[t] * d.P * ((1 << d.levels[t]) - 1) + (double)d.numb[t] * d.P;
         }
@@ -389,6 +473,17 @@ int pimc_commit(pimc_ctx* c) {
             if (d.molecule[t] == 1 && (d.pot2d[t] < 0 || d.pot2d[t] >= (int)c->t2.size()))
                 return fail(c, "linear rotor without a 2D potential table");
         }
+        if (p.fam == 1) {
+            if (p.pot3d_id >= (int)c->t3.size() || (p.natoms > p.nrot && p.pot3d_id < 0))
+                return fail(c, "non-linear rotor without a 3D potential table");
+            if (p.rotkind == 2 && (p.rot3d_id < 0 || p.rot3d_id >= c->nr3))
+                return fail(c, "rotating non-linear rotor without rho/E/E^2 tables");
+            if (p.rotkind == 2) {
+                c->bead_moves_per_pass += (double)p.P * p.Prot * p.nrot;
+                rot3.push_back(r);
+                c->nrotmax = std::max(c->nrotmax, p.nrot);
+            }
+        }
         if (p.rotkind == 1) {
             if (d.rotdens < 0 || d.rotdens >= (int)c->tr.size()) return fail(c, "linear rotor without a .rot table");
             if (p.Prot % 2) return fail(c, "the checkerboard rotor sweep needs an even number of rotor slices");
@@ -398,24 +493,28 @@ int pimc_commit(pimc_ctx* c) {
         p.bead_off = boff;
         boff += (long long)p.natoms * p.P;
         p.rot_off = roff;
-        roff += (p.rotkind ? (long long)p.tnumb[p.imtype] * p.Prot : 0);
+        roff += ((p.rotkind || p.fam == 1) ? (long long)p.tnumb[p.imtype] * p.Prot : 0);
         p.hist_off = hoff;
         hoff += BINSR + BINSR * BINST;
         p.row_off = rowoff;
-        for (int a = 0; a < p.natoms; ++a) rows.push_back(make_int2(r, a));
+        for (int a = 0; a < p.natoms; ++a) (p.fam ? rows1 : rows).push_back(make_int2(r, a));
+        (p.fam ? all1 : all0).push_back(r);
         rowoff += p.natoms;
         int W = S + 1;
         size_t knots = p.pot1d_id >= 0 ? (size_t)c->t1[p.pot1d_id].n * (c->fp64 ? sizeof(double4) : sizeof(float4)) : 0;
-        lds_b = std::max(lds_b, (size_t)((3 * p.natoms * W + 9 * W + 16 + 3) & ~3) * sizeof(double) + knots);
+        const size_t lds_w = (size_t)(((3 * p.natoms * W + 9 * W + 16 + 3) & ~3) + 12 * p.nrot * W) * sizeof(double) + knots;
+        (p.fam ? lds_b1 : lds_b) = std::max(p.fam ? lds_b1 : lds_b, lds_w);
         int nseg = c->ref_order ? 1 : p.P / S;
-        for (int k = 0; k < nseg; ++k) work.push_back(make_int2(r, k));
+        for (int k = 0; k < nseg; ++k) (p.fam ? work1 : work).push_back(make_int2(r, k));
         c->Nmax = std::max(c->Nmax, p.natoms);
         c->Smax = std::max(c->Smax, S);
         c->Pmax = std::max(c->Pmax, p.P);
         c->max_types = std::max(c->max_types, p.ntypes);
     }
-    if (lds_b > 160 * 1024) return fail(c, "segment window exceeds LDS (too many atoms for this segment size)");
+    if (std::max(lds_b, lds_b1) > 160 * 1024)
+        return fail(c, "segment window exceeds LDS (too many atoms for this segment size)");
     c->lds_bisect = lds_b;
+    c->lds_bisect1 = lds_b1;
     // rotor parts: slice ranges whose partner geometry (vec4 per translational slice and partner)
     // stays within a fixed LDS budget; the heaviest chains first, the parts of a chain adjacent
     const size_t geo_bytes = c->fp64 ? sizeof(double4) : sizeof(float4);
@@ -492,15 +591,34 @@ int pimc_commit(pimc_ctx* c) {
     c->nrotb = roff;
     c->nhist = hoff;
     // largest windows first: the longest segment sequences start while the grid fills
-    std::stable_sort(work.begin(), work.end(), [&](const int2& x, const int2& y) {
+    auto heavier = [&](const int2& x, const int2& y) {
         const Rep &a = c->rep[x.x], &b = c->rep[y.x];
         return (long long)a.natoms * a.natoms * a.S > (long long)b.natoms * b.natoms * b.S;
-    });
+    };
+    std::stable_sort(work.begin(), work.end(), heavier);
+    std::stable_sort(work1.begin(), work1.end(), heavier);
     c->nwork = (int)work.size();
+    c->nwork1 = (int)work1.size();
+    c->nall0 = (int)all0.size();
+    c->nall1 = (int)all1.size();
+    c->nrot3 = (int)rot3.size();
+    c->nrows1 = (int)rows1.size();
+    c->d_work1 = dalloc<int2>(work1.size());
+    c->d_all1 = dalloc<int>(all1.size());
+    c->d_rot3 = dalloc<int>(rot3.size());
+    c->d_rows1 = dalloc<int2>(rows1.size());
+    if (!c->d_work1 || !c->d_all1 || !c->d_rot3 || !c->d_rows1) return fail(c, "hipMalloc failed");
+    auto upv = [&](void* d, const void* h, size_t bytes) -> int {
+        if (bytes == 0) return 0;
+        return hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+    };
+    if (upv(c->d_work1, work1.data(), work1.size() * sizeof(int2)) || upv(c->d_all1, all1.data(), all1.size() * sizeof(int)) ||
+        upv(c->d_rot3, rot3.data(), rot3.size() * sizeof(int)) || upv(c->d_rows1, rows1.data(), rows1.size() * sizeof(int2)))
+        return fail(c, "upload failed");
     c->nrotreps = (int)rotreps.size();
     c->nrows = (int)rows.size();
     c->d_rows = dalloc<int2>(rows.size());
-    c->d_rowbuf = dalloc<double2>(rows.size());
+    c->d_rowbuf = dalloc<double2>((size_t)rowoff);
     if (!c->d_rows || !c->d_rowbuf) return fail(c, "hipMalloc failed");
     HIPCHECK(c, hipMemcpy(c->d_rows, rows.data(), sizeof(int2) * rows.size(), hipMemcpyHostToDevice));
     // device allocations
@@ -526,6 +644,26 @@ int pimc_commit(pimc_ctx* c) {
     std::vector<int> all(c->nrep);
     for (int r = 0; r < c->nrep; ++r) all[r] = r;
     HIPCHECK(c, hipMemcpy(c->d_all, all.data(), sizeof(int) * c->nrep, hipMemcpyHostToDevice));
+    c->d_all0 = dalloc<int>(all0.size());
+    c->d_frames = dalloc<Frame>(std::max<long long>(roff, 1));
+    if (!c->d_all0 || !c->d_frames) return fail(c, "hipMalloc failed");
+    if (!all0.empty()) HIPCHECK(c, hipMemcpy(c->d_all0, all0.data(), sizeof(int) * all0.size(), hipMemcpyHostToDevice));
+    {  // frames of the default orientation (phi = 0, cos theta = 1, chi = 0)
+        std::vector<Frame> fr(std::max<long long>(roff, 1), make_frame(0.0, 1.0, 0.0));
+        HIPCHECK(c, hipMemcpy(c->d_frames, fr.data(), sizeof(Frame) * fr.size(), hipMemcpyHostToDevice));
+    }
+    {  // 3D potential tables, FP64 and FP32
+        std::vector<float> v3f(c->v3.begin(), c->v3.end());
+        c->d_v3d = dalloc<double>(c->v3.size());
+        c->d_v3f = dalloc<float>(v3f.size());
+        c->d_t3 = dalloc<Tab3D>(c->t3.size());
+        if (!c->d_v3d || !c->d_v3f || !c->d_t3) return fail(c, "hipMalloc failed");
+        if (!c->v3.empty()) {
+            HIPCHECK(c, hipMemcpy(c->d_v3d, c->v3.data(), sizeof(double) * c->v3.size(), hipMemcpyHostToDevice));
+            HIPCHECK(c, hipMemcpy(c->d_v3f, v3f.data(), sizeof(float) * v3f.size(), hipMemcpyHostToDevice));
+            HIPCHECK(c, hipMemcpy(c->d_t3, c->t3.data(), sizeof(Tab3D) * c->t3.size(), hipMemcpyHostToDevice));
+        }
+    }
     if (!rotreps.empty())
         HIPCHECK(c, hipMemcpy(c->d_rotreps, rotreps.data(), sizeof(int) * rotreps.size(), hipMemcpyHostToDevice));
     // default state: every bead at the origin of its own lattice site is the caller's job;
@@ -581,9 +719,10 @@ int pimc_set_state(pimc_ctx* c, int r, const double* coords, const double* angle
         for (int i = 0; i < n; ++i) b[i] = make_double4(coords[i], coords[n + i], coords[2 * n + i], 0.0);
         HIPCHECK(c, hipMemcpy(c->d_beads + p.bead_off, b.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
     }
-    if (p.rotkind && angles) {
+    if ((p.rotkind || p.fam == 1) && angles) {
         const int nr = p.tnumb[p.imtype];
         std::vector<RotBead> rb((size_t)nr * p.Prot);
+        std::vector<Frame> fr((size_t)nr * p.Prot);
         for (int a = 0; a < nr; ++a)
             for (int k = 0; k < p.Prot; ++k) {
                 int i = (p.tbegin[p.imtype] + a) * p.P + k;  // rotor slice k lives at offset + k
@@ -596,8 +735,11 @@ int pimc_set_state(pimc_ctx* c, int r, const double* coords, const double* angle
                 b.ny = sint * std::sin(b.phi);
                 b.nz = b.cth;
                 b.pad0 = b.pad1 = 0;
+                fr[(size_t)a * p.Prot + k] = make_frame(b.phi, b.cth, b.chi);
             }
         HIPCHECK(c, hipMemcpy(c->d_rotb + p.rot_off, rb.data(), sizeof(RotBead) * rb.size(), hipMemcpyHostToDevice));
+        if (p.fam == 1)
+            HIPCHECK(c, hipMemcpy(c->d_frames + p.rot_off, fr.data(), sizeof(Frame) * fr.size(), hipMemcpyHostToDevice));
     }
     return 0;
 }
@@ -622,7 +764,7 @@ int pimc_get_state(pimc_ctx* c, int r, double* coords, double* angles, double* c
             if (angles) { angles[i] = 0; angles[n + i] = 1; angles[2 * n + i] = 0; }
             if (cosines) { cosines[i] = 0; cosines[n + i] = 0; cosines[2 * n + i] = 1; }
         }
-        if (p.rotkind) {
+        if (p.rotkind || p.fam == 1) {
             const int nr = p.tnumb[p.imtype];
             std::vector<RotBead> rb((size_t)nr * p.Prot);
             HIPCHECK(c, hipMemcpy(rb.data(), c->d_rotb + p.rot_off, sizeof(RotBead) * rb.size(), hipMemcpyDeviceToHost));
@@ -680,33 +822,51 @@ template <typename R>
 void launch_bisect(pimc_ctx* c, int u, int tb, int te) {
     Run run{c->seed, u, c->ref_order ? 1 : 0, tb, te};
     Timed t(c, "bisect");
-    hipLaunchKernelGGL(k_bisect<R>, dim3(c->nwork), dim3(c->bisect_block), c->lds_bisect, c->stream, c->d_rep,
-                       c->d_mut, c->d_beads, c->d_rotb, c->tables(), c->d_work, run);
+    if (c->nwork)
+        hipLaunchKernelGGL(k_bisect<R>, dim3(c->nwork), dim3(c->bisect_block), c->lds_bisect, c->stream, c->d_rep,
+                           c->d_mut, c->d_beads, c->d_rotb, c->tables(), c->d_work, run);
+    if (c->nwork1)
+        hipLaunchKernelGGL(k_bisect_nl<R>, dim3(c->nwork1), dim3(c->bisect_block), c->lds_bisect1, c->stream,
+                           c->d_rep, c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_work1, run);
 }
 
 template <typename R>
 void launch_centroid(pimc_ctx* c, int type) {
     Run run{c->seed, 0, c->ref_order ? 1 : 0, type, type + 1};
     Timed t(c, "centroid");
-    hipLaunchKernelGGL(k_centroid<R>, dim3(c->nrep), dim3(1024), 0, c->stream, c->d_rep, c->d_mut, c->d_beads,
-                       c->d_rotb, c->tables(), c->d_all, run);
+    if (c->nall0)
+        hipLaunchKernelGGL((k_centroid<R, 0>), dim3(c->nall0), dim3(1024), 0, c->stream, c->d_rep, c->d_mut,
+                           c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_all0, run);
+    if (c->nall1)
+        hipLaunchKernelGGL((k_centroid<R, 1>), dim3(c->nall1), dim3(1024), 0, c->stream, c->d_rep, c->d_mut,
+                           c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_all1, run);
 }
 
 template <typename R>
 void launch_rotor(pimc_ctx* c, int u) {
-    if (c->nparts == 0) return;
+    if (c->nparts == 0 && c->nrot3 == 0) return;
     Run run{c->seed, u, c->ref_order ? 1 : 0, 0, 0};
     Timed t(c, "rotor");
-    hipLaunchKernelGGL(k_rotor<R>, dim3(c->nparts), dim3(256), c->lds_rotor, c->stream, c->d_rep, c->d_mut,
-                       c->d_beads, c->d_rotb, c->tables(), c->d_parts, c->d_flags, run);
+    if (c->nparts)
+        hipLaunchKernelGGL(k_rotor<R>, dim3(c->nparts), dim3(256), c->lds_rotor, c->stream, c->d_rep, c->d_mut,
+                           c->d_beads, c->d_rotb, c->tables(), c->tables3(), c->d_parts, c->d_flags, run);
+    if (c->nrot3)  // group frames in LDS: (rotors + proposal) per lane group, at most 256 groups
+        hipLaunchKernelGGL(k_rotor3d<R>, dim3(c->nrot3), dim3(256), (size_t)256 * (c->nrotmax + 1) * sizeof(Frame),
+                           c->stream, c->d_rep, c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rot3,
+                           run);
 }
 
 template <typename R>
 void launch_estimate(pimc_ctx* c, double* out) {
     Timed t(c, "estimate");
-    hipLaunchKernelGGL(k_est_rows<R>, dim3(c->nrows), dim3(256), 0, c->stream, c->d_rep, c->d_beads, c->d_rotb,
-                       c->tables(), c->d_rows, out ? nullptr : c->d_hist, c->d_rowbuf);
-    hipLaunchKernelGGL(k_est_final, dim3(c->nrep), dim3(256), 0, c->stream, c->d_rep, c->d_mut, c->d_rotb, c->tables(),
+    if (c->nrows)
+        hipLaunchKernelGGL((k_est_rows<R, 0>), dim3(c->nrows), dim3(256), 0, c->stream, c->d_rep, c->d_beads,
+                           c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rows, out ? nullptr : c->d_hist, c->d_rowbuf);
+    if (c->nrows1)
+        hipLaunchKernelGGL((k_est_rows<R, 1>), dim3(c->nrows1), dim3(256), 0, c->stream, c->d_rep, c->d_beads,
+                           c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rows1, out ? nullptr : c->d_hist, c->d_rowbuf);
+    hipLaunchKernelGGL(k_est_final, dim3(c->nrep), dim3(256), 0, c->stream, c->d_rep, c->d_mut, c->d_rotb,
+                       c->d_frames, c->tables(), c->tables3(),
                        c->d_all, c->d_rowbuf, c->d_samples, c->sample_cap, out);
 }
 
@@ -743,6 +903,7 @@ int check_errors(pimc_ctx* c) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     for (int r = 0; r < c->nrep; ++r) {
         if (m[r].err & 2) return fail(c, "rotor sweep: a neighbouring part did not arrive (replica " + std::to_string(r) + ")");
+        if (m[r].err & 4) return fail(c, "large matrix test error (relative Euler angle off the rotden grid, replica " + std::to_string(r) + ")");
         if (m[r].err) return fail(c, "Rotational Moves: Negative rot density (replica " + std::to_string(r) + ")");
     }
     return 0;
@@ -861,10 +1022,10 @@ int pimc_pot_energy_slice(pimc_ctx* c, int nq, const int* qr, const int* qa, con
         (void)hipMemcpy(d_p, qpos, sizeof(double) * 3 * nq, hipMemcpyHostToDevice);
         if (c->fp64)
             hipLaunchKernelGGL(k_pot_slice<double>, dim3((nq + 127) / 128), dim3(128), 0, c->stream, c->d_rep,
-                               c->d_beads, c->d_rotb, c->tables(), nq, d_r, d_a, d_i, d_p, d_o);
+                               c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), nq, d_r, d_a, d_i, d_p, d_o);
         else
             hipLaunchKernelGGL(k_pot_slice<float>, dim3((nq + 127) / 128), dim3(128), 0, c->stream, c->d_rep,
-                               c->d_beads, c->d_rotb, c->tables(), nq, d_r, d_a, d_i, d_p, d_o);
+                               c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), nq, d_r, d_a, d_i, d_p, d_o);
         if (hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, "pot_energy_slice failed");
         else (void)hipMemcpy(out, d_o, sizeof(double) * nq, hipMemcpyDeviceToHost);
     }

@@ -32,6 +32,11 @@ struct Rep {
     int lin_atom;  // global index of the linear molecule, -1 if none
     int pot1d_id;  // 1D table of the atom type, -1 if none
     int pot2d_id;  // 2D table of the linear-molecule type, -1 if none
+    int fam;       // 0: atoms and at most one linear molecule; 1: atoms and non-linear rotors
+    int nl_atom;   // global index of the first non-linear rotor (the rotor type is last), -1 if none
+    int nrot;      // number of non-linear rotors
+    int pot3d_id;  // 3D table of the atom / non-linear rotor pairs, -1 if none
+    int rot3d_id;  // rho/E/E^2 tables of the non-linear rotor, -1 if none
     int key;       // replica index, second word of the generator key
     double lambda[MAXT], mcstep[MAXT];
     double T, beta, tau, rtstep;
@@ -69,6 +74,13 @@ struct TabRot {
     double x0, xmax, inv_h;
 };
 
+// 3D potential table of vcalc.f:1-65: V[(ir*nth + ith)*nch + ich], R in bohr on a uniform grid
+struct Tab3D {
+    int nr, nth, nch, pad;
+    double rmin, rmax, rstep;
+    long long off;
+};
+
 struct Tables {
     const float4* k1f;
     const double4* k1d;
@@ -82,6 +94,25 @@ struct Tables {
     const double4* re;  // erot, erot'', esq, esq''
     const TabRot* tr;
 };
+
+// tables of the non-linear rotors, a kernel argument of its own so kernels of the linear
+// family pay no registers for them
+struct Tables3 {
+    const double* v3d;  // 3D potential values (vtable order), FP64
+    const float* v3f;   // the same in FP32
+    const Tab3D* t3;
+    const double* r3;   // rho, E, E^2 of each non-linear rotor density table, 3 * ROT3D_SIZE each
+};
+
+// rotation matrix of the Euler angles of one rotor slice (matpre, rotden.f:136-163), Fortran
+// column order m[(j-1)*3 + (i-1)] = rotmat(i, j), so the body axes are its three columns;
+// zn = |z axis| as vcord's dotang takes it (vcord.f:176-188)
+struct Frame {
+    double m[9];
+    double zn, pad0, pad1;
+};
+
+constexpr long long ROT3D_SIZE = 181LL * 361 * 361;  // rotpro_sub.f:7-26, mc_setup.h:116
 
 // ---------------------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Counter map:

@@ -21,6 +21,7 @@
 #pragma once
 #include <type_traits>
 #include "pimc_device.hpp"
+#include "pimc_nl.hpp"
 
 namespace pimc {
 
@@ -64,12 +65,32 @@ __device__ __forceinline__ void rotor_axis(const RotBead* rotb, const Rep& rp, i
     }
 }
 
+// PotEnergy's dispatch (mc_piqmc.cc:1847-1958) for the non-linear family: pairs of the atom
+// type by SPot1D, atom / non-linear rotor pairs by vcord with the rotor's frame at that slice
+// (rcom = the rotor, rpt = the atom), rotor / rotor pairs by caleng.
+template <typename R>
+__device__ __forceinline__ double pair_nl(const Lut1D<R>& l1, const vec4_t<R>* __restrict__ k1, const Lut3D<R>& l3,
+                                          bool rot_a, bool rot_j, double xa, double ya, double za, double xj,
+                                          double yj, double zj, const Frame& fa, const Frame& fj) {
+    if (rot_a && rot_j) return caleng_v(xa, ya, za, xj, yj, zj, fa, fj);
+    if (rot_a) return (double)vcord_v<R>(l3, fa, xa, ya, za, xj, yj, zj);
+    if (rot_j) return (double)vcord_v<R>(l3, fj, xj, yj, zj, xa, ya, za);
+    const R dx = (R)(xa - xj), dy = (R)(ya - yj), dz = (R)(za - zj);
+    return (double)spot1d<R>(l1, k1, sqrt(dx * dx + dy * dy + dz * dz));
+}
+
+__device__ __forceinline__ const Frame& frame_at(const Frame* __restrict__ frames, const Rep& rp, int a, int it) {
+    return frames[rp.rot_off + (long long)(a - rp.nl_atom) * rp.Prot + it / rp.RotRatio];
+}
+
 // ------------------------------------------------------------------------- bisection
 // LDS: wx,wy,wz [natoms*(S+1)], nh [3*(S+1)], nw [3*(S+1)]; blockDim a multiple of 64.
-template <typename R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bisect(const Rep* __restrict__ reps, Mut* __restrict__ mut,
-                                                double4* __restrict__ beads, const RotBead* __restrict__ rotb,
-                                                Tables T, const int2* __restrict__ work, Run run) {
+// FAM 1 (non-linear rotors) adds the rotors' frames at the window's slots: fr [nrot*(S+1)].
+template <typename R, int FAM>
+__device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* __restrict__ mut,
+                                            double4* __restrict__ beads, const RotBead* __restrict__ rotb,
+                                            const Frame* __restrict__ frames, Tables T, Tables3 T3,
+                                            const int2* __restrict__ work, Run run) {
     extern __shared__ double smem[];
     __shared__ double red[16];
     const int2 wk = work[blockIdx.x];
@@ -91,11 +112,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     double* nh = wz + N * W;
     double* nw = nh + 3 * W;
     double* gw = nw + 3 * W;  // draws of the moving atom: 3 per midpoint, then one per level
-    // the 1D spline knots follow (16-byte aligned)
-    vec4_t<R>* kn = reinterpret_cast<vec4_t<R>*>(smem + ((3 * N * W + 9 * W + 16 + 3) & ~3));
+    // rotor frames (FAM 1), then the 1D spline knots (16-byte aligned)
+    Frame* fr = reinterpret_cast<Frame*>(smem + ((3 * N * W + 9 * W + 16 + 3) & ~3));
+    const int nfr = FAM == 1 ? rp.nrot * W : 0;
+    vec4_t<R>* kn = reinterpret_cast<vec4_t<R>*>(smem + ((3 * N * W + 9 * W + 16 + 3) & ~3) + 12 * nfr);
     const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
     const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
+    const Lut3D<R> l3 = lut3d<R>(T3, FAM == 1 ? rp.pot3d_id : -1);
     for (int i = tid; i < l1.n; i += nth) kn[i] = l1.k[i];
+    if constexpr (FAM == 1)
+        for (int i = tid; i < nfr; i += nth) {
+            const int q = i / W, w = i - q * W;
+            fr[i] = frame_at(frames, rp, rp.nl_atom + q, (s0 + w) % P);
+        }
     for (int i = tid; i < N * W; i += nth) {
         int a = i / W, w = i - a * W;
         double4 b = bead(beads, rp, a, (s0 + w) % P);
@@ -165,12 +194,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 const int np = N - 1;
                 const int items = nmid * np;
                 double acc = 0.0;
-                const bool rot_a = (a == rp.lin_atom);
+                const bool rot_a = FAM == 1 ? (rp.nl_atom >= 0 && a >= rp.nl_atom) : (a == rp.lin_atom);
                 for (int i = tid; i < items; i += nth) {
                     int q = i / np, jp = i - q * np;
                     int j = jp + (jp >= a);
                     int slot = half + q * lss;
-                    const bool rot_j = (j == rp.lin_atom);
+                    const bool rot_j = FAM == 1 ? (rp.nl_atom >= 0 && j >= rp.nl_atom) : (j == rp.lin_atom);
                     double xj = wx[j * W + slot], yj = wy[j * W + slot], zj = wz[j * W + slot];
                     double nx = nh[3 * slot], ny = nh[3 * slot + 1], nz = nh[3 * slot + 2];
 #pragma unroll 1
@@ -178,8 +207,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                         double xa = no ? wx[a * W + slot] : nw[3 * slot];
                         double ya = no ? wy[a * W + slot] : nw[3 * slot + 1];
                         double za = no ? wz[a * W + slot] : nw[3 * slot + 2];
-                        R v = pair_v<R>(l1, kn, l2, rot_a, rot_j, xa, ya, za, xj, yj, zj, nx, ny, nz);
-                        acc += no ? -(double)v : (double)v;
+                        if constexpr (FAM == 1) {
+                            double v = pair_nl<R>(l1, kn, l3, rot_a, rot_j, xa, ya, za, xj, yj, zj,
+                                                  fr[(rot_a ? a - rp.nl_atom : 0) * W + slot],
+                                                  fr[(rot_j ? j - rp.nl_atom : 0) * W + slot]);
+                            acc += no ? -v : v;
+                        } else {
+                            R v = pair_v<R>(l1, kn, l2, rot_a, rot_j, xa, ya, za, xj, yj, zj, nx, ny, nz);
+                            acc += no ? -(double)v : (double)v;
+                        }
                     }
                 }
                 acc = block_sum(acc, red);
@@ -220,11 +256,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
 }
 
-// ------------------------------------------------------------------------- centroid
+// the linear family keeps four waves per SIMD; the non-linear family needs more registers
 template <typename R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bisect(
+    const Rep* __restrict__ reps, Mut* __restrict__ mut, double4* __restrict__ beads, const RotBead* __restrict__ rotb,
+    Tables T, const int2* __restrict__ work, Run run) {
+    bisect_body<R, 0>(reps, mut, beads, rotb, nullptr, T, Tables3{}, work, run);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_bisect_nl(const Rep* __restrict__ reps, Mut* __restrict__ mut,
+                                                   double4* __restrict__ beads, const RotBead* __restrict__ rotb,
+                                                   const Frame* __restrict__ frames, Tables T, Tables3 T3,
+                                                   const int2* __restrict__ work, Run run) {
+    bisect_body<R, 1>(reps, mut, beads, rotb, frames, T, T3, work, run);
+}
+
+// ------------------------------------------------------------------------- centroid
+template <typename R, int FAM>
 __global__ __launch_bounds__(1024) void k_centroid(const Rep* __restrict__ reps, Mut* __restrict__ mut,
                                                    double4* __restrict__ beads, const RotBead* __restrict__ rotb,
-                                                   Tables T, const int* __restrict__ replist, Run run) {
+                                                   const Frame* __restrict__ frames, Tables T, Tables3 T3,
+                                                   const int* __restrict__ replist, Run run) {
     __shared__ double red[16];
     const int r = replist[blockIdx.x];
     const Rep& rp = reps[r];
@@ -244,17 +297,28 @@ __global__ __launch_bounds__(1024) void k_centroid(const Rep* __restrict__ reps,
         const double dz = rp.mcstep[ty] * (u53(w1.x, w1.y) - 0.5);
         double acc = 0.0;
         const int items = P * np;
-        const bool rot_a = (a == rp.lin_atom);
+        const bool rot_a = FAM == 1 ? (rp.nl_atom >= 0 && a >= rp.nl_atom) : (a == rp.lin_atom);
         for (int i = threadIdx.x; i < items; i += blockDim.x) {
             int it = i / np, jp = i - it * np;
             int j = jp + (jp >= a);
-            const bool rot_j = (j == rp.lin_atom);
+            const bool rot_j = FAM == 1 ? (rp.nl_atom >= 0 && j >= rp.nl_atom) : (j == rp.lin_atom);
             double4 xa = bead(beads, rp, a, it), xj = bead(beads, rp, j, it);
-            double nx, ny, nz;
-            rotor_axis(rotb, rp, it, nx, ny, nz);
-            R vn = pair_v<R>(l1, l1.k, l2, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, nx, ny, nz);
-            R vo = pair_v<R>(l1, l1.k, l2, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz);
-            acc += (double)vn - (double)vo;
+            if constexpr (FAM == 1) {
+                const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
+                const Frame& fa = frame_at(frames, rp, rot_a ? a : rp.nl_atom, it);
+                const Frame& fj = frame_at(frames, rp, rot_j ? j : rp.nl_atom, it);
+                double vn = pair_nl<R>(l1, l1.k, l3, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z,
+                                       fa, fj);
+                double vo = pair_nl<R>(l1, l1.k, l3, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, fa, fj);
+                acc += vn - vo;
+            } else {
+                double nx, ny, nz;
+                rotor_axis(rotb, rp, it, nx, ny, nz);
+                R vn = pair_v<R>(l1, l1.k, l2, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, nx, ny,
+                                 nz);
+                R vo = pair_v<R>(l1, l1.k, l2, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz);
+                acc += (double)vn - (double)vo;
+            }
         }
         const double deltav = block_sum(acc, red);
         bool accepted = false;
@@ -366,7 +430,7 @@ __device__ __forceinline__ double slice_energy(const Lut2D<R>& l2, const vec4_t<
 template <typename R>
 __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut* __restrict__ mut,
                                                const double4* __restrict__ beads, RotBead* __restrict__ rotb,
-                                               Tables T, const RotPart* __restrict__ parts,
+                                               Tables T, Tables3 T3, const RotPart* __restrict__ parts,
                                                unsigned* __restrict__ flags, Run run) {
     extern __shared__ double smem[];
     __shared__ unsigned long long red_acc[4];
@@ -575,6 +639,149 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
     }
 }
 
+// ------------------------------------------------------------------------- non-linear rotors
+// MCRotations3D (mc_piqmc.cc:692-779) with MCRot3Dstep (:938-1199): even slices, then odd;
+// at each slice every rotor in turn.  One workgroup per replica and a group of G lanes per
+// slice: the density factors rho(t0,t1), rho(t1,t2) (old) and rho(t0,t1'), rho(t1',t2) (new)
+// on four lanes, the old and new sums of PotRotE3D (mc_piqmc.cc:2046-2151) over the slice's
+// translational slices and partners across the group.  Old sums are recomputed for every move
+// as the reference does: with two rotors a partner rotor may have moved since.  Orientations
+// live in rotb and their frames in `frames`; a group keeps the frames of all rotors at its
+// slice (and the proposal's) in LDS, so a rotor sees the moves of the rotors before it there.
+template <typename R>
+__global__ __launch_bounds__(256) void k_rotor3d(const Rep* __restrict__ reps, Mut* __restrict__ mut,
+                                                 const double4* __restrict__ beads, RotBead* __restrict__ rotb,
+                                                 Frame* __restrict__ frames, Tables T, Tables3 T3, const int* __restrict__ replist,
+                                                 Run run) {
+    extern __shared__ double smem[];
+    __shared__ unsigned long long red_acc[4];
+    __shared__ int s_err;
+    const int r = replist[blockIdx.x];
+    const Rep& rp = reps[r];
+    if (rp.rotkind != 2 || rp.rot3d_id < 0) return;
+    const int Prot = rp.Prot, RR = rp.RotRatio, P = rp.P, N = rp.natoms, nrot = rp.nrot;
+    int nsweep, j0, jstride;
+    if (run.ref_order) {
+        if (run.u >= P) return;
+        nsweep = 1; j0 = run.u; jstride = 0;
+    } else {
+        if (run.u >= rp.S) return;
+        nsweep = P / rp.S; j0 = run.u; jstride = rp.S;
+    }
+    const int np = N - 1;
+    const int items = RR * np;
+    const long long rb0 = rp.rot_off;  // rotor q, slice k at rb0 + q*Prot + k
+    const double* tab = T3.r3 + (long long)rp.rot3d_id * 3 * ROT3D_SIZE;
+    const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
+    const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
+    const int maxmoves = (Prot + 1) / 2;
+    int G = 1;
+    while (G < 64 && G * 2 * maxmoves <= (int)blockDim.x) G *= 2;
+    const int groups = blockDim.x / G;
+    const int g = threadIdx.x / G, gl = threadIdx.x % G;
+    Frame* gf = reinterpret_cast<Frame*>(smem) + (long long)g * (nrot + 1);  // rotors at the slice, then the proposal
+    const uint32_t pass = mut[r].pass;
+    const int rtype = rp.imtype;
+    const double twopi = 2.0 * PI_D;
+    if (threadIdx.x == 0) s_err = 0;
+    __syncthreads();
+    unsigned long long nacc = 0;
+    int err = 0;
+    for (int sw = 0; sw < nsweep; ++sw) {
+        const uint32_t jsweep = (uint32_t)(j0 + sw * jstride);
+        for (int parity = 0; parity < 2; ++parity) {
+            const int nmoves = (Prot - parity + 1) / 2;
+            for (int mv = g; mv < nmoves; mv += groups) {
+                const int k = 2 * mv + parity;
+                int k0 = k - 1, k2 = k + 1;
+                if (k0 < 0) k0 += Prot;
+                if (k2 >= Prot) k2 -= Prot;
+                for (int q = gl; q < nrot; q += G) gf[q] = frames[rb0 + (long long)q * Prot + k];
+                __builtin_amdgcn_wave_barrier();
+                for (int q = 0; q < nrot; ++q) {
+                    const int gatom = rp.nl_atom + q;
+                    const long long ik = rb0 + (long long)q * Prot + k;
+                    const RotBead cur = rotb[ik];
+                    const uint4 w0 = draw(run.seed, rp.key, pass, K_ROT3D, jsweep, (uint32_t)gatom, (uint32_t)k, 0);
+                    const uint4 w1 = draw(run.seed, rp.key, pass, K_ROT3D, jsweep, (uint32_t)gatom, (uint32_t)k, 1);
+                    const double rand1 = u53(w0.x, w0.y), rand2 = u53(w0.z, w0.w);
+                    const double rand3 = u53(w1.x, w1.y), rand4 = u53(w1.z, w1.w);
+                    double cost = cur.cth + rp.rtstep * (rand1 - 0.5);
+                    double phi = cur.phi + twopi * (rp.rtstep * (rand2 - 0.5));
+                    double chi = cur.chi + twopi * (rp.rtstep * (rand3 - 0.5));
+                    if (phi < 0.0) phi = twopi + phi;
+                    if (chi < 0.0) chi = twopi + chi;
+                    phi = fmod(phi, twopi);
+                    chi = fmod(chi, twopi);
+                    if (cost > 1.0) cost = 2.0 - cost;
+                    if (cost < -1.0) cost = -2.0 - cost;
+                    if (gl == 0) gf[nrot] = make_frame(phi, cost, chi);
+                    __builtin_amdgcn_wave_barrier();
+                    // density factors: 0 (t0,t1) and 1 (t1,t2) old, 2 (t0,t1') and 3 (t1',t2) new
+                    double d[4] = {0.0, 0.0, 0.0, 0.0};
+                    for (int c = gl; c < 4; c += G) {
+                        const Frame& fa = (c == 1) ? gf[q] : (c == 3 ? gf[nrot] : frames[rb0 + (long long)q * Prot + k0]);
+                        const Frame& fb = (c == 0) ? gf[q] : (c == 2 ? gf[nrot] : frames[rb0 + (long long)q * Prot + k2]);
+                        double er, es;
+                        if (rotden_frames(fa, fb, tab, d[c], er, es)) err = 1;
+                    }
+                    if (G > 1)
+                        for (int c = 0; c < 4; ++c) d[c] = __shfl(d[c], c % G, G);
+                    // PotRotE3D at the old and the proposed orientation
+                    double eo = 0.0, en = 0.0;
+                    for (int i = gl; i < items; i += G) {
+                        const int it = k * RR + i / np, jp = i % np;
+                        const int j = jp + (jp >= gatom);
+                        const bool rj = j >= rp.nl_atom;
+                        const double4 xa = bead(beads, rp, gatom, it), xj = bead(beads, rp, j, it);
+                        const Frame& fj = gf[rj ? j - rp.nl_atom : q];
+                        eo += pair_nl<R>(l1, l1.k, l3, true, rj, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, gf[q], fj);
+                        en += pair_nl<R>(l1, l1.k, l3, true, rj, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, gf[nrot], fj);
+                    }
+                    for (int o = G >> 1; o > 0; o >>= 1) {
+                        eo += __shfl_xor(eo, o, G);
+                        en += __shfl_xor(en, o, G);
+                    }
+                    double dold = d[0] * d[1];
+                    if (fabs(dold) < RZERO) dold = 0.0;
+                    if (dold < 0.0) dold = fabs(dold);
+                    double dnew = d[2] * d[3];
+                    if (fabs(dnew) < RZERO) dnew = 0.0;
+                    if (dnew < 0.0) dnew = fabs(dnew);
+                    double rd = (dold > RZERO) ? dnew / dold : 1.0;
+                    rd *= exp(-rp.tau * (en - eo));
+                    const bool accepted = rd > 1.0 || rd > rand4;
+                    if (accepted && gl == 0) {
+                        ++nacc;
+                        const double sint = sqrt(1.0 - cost * cost);
+                        RotBead b = cur;
+                        b.cth = cost;
+                        b.phi = phi;
+                        b.chi = chi;
+                        b.nx = sint * cos(phi);
+                        b.ny = sint * sin(phi);
+                        b.nz = cost;
+                        rotb[ik] = b;
+                        frames[ik] = gf[nrot];
+                        gf[q] = gf[nrot];
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (err) s_err = 1;
+    unsigned long long s = block_sum(nacc, red_acc);
+    if (threadIdx.x == 0) {
+        if (s_err) atomicOr(&mut[r].err, 4);
+        const unsigned long long moves = (unsigned long long)nsweep * Prot * nrot;
+        atomicAdd(&mut[r].total[rtype][2], moves);
+        atomicAdd(&mut[r].accep[rtype][2], s);
+        atomicAdd(&mut[r].pair_evals[2], 2ull * moves * (unsigned long long)items);
+    }
+}
+
 // ------------------------------------------------------------------------- end of pass
 __global__ void k_pass_end(Mut* mut, int n) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -583,11 +790,11 @@ __global__ void k_pass_end(Mut* mut, int n) {
 
 // ------------------------------------------------------------------------- estimators
 // rows[row_off + a0] = (pair potential of row a0 summed over slices, kinetic term of a0)
-template <typename R>
+template <typename R, int FAM>
 __global__ __launch_bounds__(256) void k_est_rows(const Rep* __restrict__ reps, const double4* __restrict__ beads,
-                                                  const RotBead* __restrict__ rotb, Tables T,
-                                                  const int2* __restrict__ rowlist, unsigned* __restrict__ hist,
-                                                  double2* __restrict__ rows) {
+                                                  const RotBead* __restrict__ rotb, const Frame* __restrict__ frames,
+                                                  Tables T, Tables3 T3, const int2* __restrict__ rowlist,
+                                                  unsigned* __restrict__ hist, double2* __restrict__ rows) {
     __shared__ double red[16];
     const int2 rw = rowlist[blockIdx.x];
     const Rep& rp = reps[rw.x];
@@ -619,6 +826,15 @@ __global__ __launch_bounds__(256) void k_est_rows(const Rep* __restrict__ reps, 
         double4 b0 = bead(beads, rp, a0, it), b1 = bead(beads, rp, a1, it);
         R dx = (R)(b0.x - b1.x), dy = (R)(b0.y - b1.y), dz = (R)(b0.z - b1.z);
         R rr = sqrt(dx * dx + dy * dy + dz * dz);
+        if constexpr (FAM == 1) {
+            if (m0 == 2 || m1 == 2) {  // mc_estim.cc:582-672: vcord or caleng, no density bins
+                const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
+                const Frame& f0 = frame_at(frames, rp, m0 == 2 ? a0 : rp.nl_atom, it);
+                const Frame& f1 = frame_at(frames, rp, m1 == 2 ? a1 : rp.nl_atom, it);
+                pot += pair_nl<R>(l1, l1.k, l3, m0 == 2, m1 == 2, b0.x, b0.y, b0.z, b1.x, b1.y, b1.z, f0, f1);
+                continue;
+            }
+        }
         if (m0 == 1 || m1 == 1) {
             double nx, ny, nz;
             rotor_axis(rotb, rp, it, nx, ny, nz);
@@ -647,7 +863,8 @@ __global__ __launch_bounds__(256) void k_est_rows(const Rep* __restrict__ reps, 
 
 // out (if not null) receives 8 doubles per replica; otherwise the sample ring.
 __global__ __launch_bounds__(256) void k_est_final(const Rep* __restrict__ reps, Mut* __restrict__ mut,
-                                                   const RotBead* __restrict__ rotb, Tables T,
+                                                   const RotBead* __restrict__ rotb, const Frame* __restrict__ frames,
+                                                   Tables T, Tables3 T3,
                                                    const int* __restrict__ replist, const double2* __restrict__ rows,
                                                    double* __restrict__ samples, int sample_cap, double* __restrict__ out) {
     __shared__ double red[16];
@@ -671,6 +888,24 @@ __global__ __launch_bounds__(256) void k_est_final(const Rep* __restrict__ reps,
         srot = block_sum(a1, red);
         term = block_sum(a2, red);
         esq = block_sum(a3, red);
+    }
+    if (rp.rotkind == 2 && rp.rot3d_id >= 0) {  // GetRotE3D mc_estim.cc:989-1096, per rotor
+        const double* tab = T3.r3 + (long long)rp.rot3d_id * 3 * ROT3D_SIZE;
+        for (int q = 0; q < rp.nrot; ++q) {
+            const long long fb = rp.rot_off + (long long)q * rp.Prot;
+            double a1 = 0, a2 = 0, a3 = 0;
+            for (int k = threadIdx.x; k < rp.Prot; k += blockDim.x) {
+                double rho, er, es;
+                rotden_frames(frames[fb + k], frames[fb + (k + 1) % rp.Prot], tab, rho, er, es);
+                a1 += er;
+                a2 += es;
+                a3 += er * er;
+            }
+            const double n = (double)rp.Prot;
+            srot += block_sum(a1, red) / n;
+            esq += block_sum(a2, red) / (n * n);
+            term += block_sum(a3, red) / (n * n);
+        }
     }
     if (threadIdx.x == 0) {
         double r2avr = 0.0, spot = 0.0;
@@ -704,7 +939,8 @@ __global__ __launch_bounds__(256) void k_est_final(const Rep* __restrict__ reps,
 // ------------------------------------------------------------------------- PotEnergy(atom,pos,it)
 template <typename R>
 __global__ void k_pot_slice(const Rep* __restrict__ reps, const double4* __restrict__ beads,
-                            const RotBead* __restrict__ rotb, Tables T, int nq, const int* __restrict__ qrep,
+                            const RotBead* __restrict__ rotb, const Frame* __restrict__ frames, Tables T, Tables3 T3, int nq,
+                            const int* __restrict__ qrep,
                             const int* __restrict__ qatom, const int* __restrict__ qit, const double* __restrict__ qpos,
                             double* __restrict__ out) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -719,8 +955,15 @@ __global__ void k_pot_slice(const Rep* __restrict__ reps, const double4* __restr
     for (int j = 0; j < rp.natoms; ++j) {
         if (j == a) continue;
         double4 xj = bead(beads, rp, j, it);
-        s += (double)pair_v<R>(l1, l1.k, l2, a == rp.lin_atom, j == rp.lin_atom, qpos[3 * q], qpos[3 * q + 1],
-                               qpos[3 * q + 2], xj.x, xj.y, xj.z, nx, ny, nz);
+        if (rp.fam == 1) {
+            const bool ra = rp.nl_atom >= 0 && a >= rp.nl_atom, rj = rp.nl_atom >= 0 && j >= rp.nl_atom;
+            const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
+            s += pair_nl<R>(l1, l1.k, l3, ra, rj, qpos[3 * q], qpos[3 * q + 1], qpos[3 * q + 2], xj.x, xj.y, xj.z,
+                            frame_at(frames, rp, ra ? a : rp.nl_atom, it), frame_at(frames, rp, rj ? j : rp.nl_atom, it));
+        } else {
+            s += (double)pair_v<R>(l1, l1.k, l2, a == rp.lin_atom, j == rp.lin_atom, qpos[3 * q], qpos[3 * q + 1],
+                                   qpos[3 * q + 2], xj.x, xj.y, xj.z, nx, ny, nz);
+        }
     }
     out[q] = s;
 }

@@ -113,12 +113,16 @@ def bead_moves_per_pass(rp):
     s = 0
     for t in range(rp["ntypes"]):
         s += rp["numb"][t] * rp["P"] * ((1 << rp["levels"][t]) - 1) + rp["numb"][t] * rp["P"]
-    s += rp["P"] * rp["Prot"]
+    nrot = sum(rp["numb"][t] for t in range(rp["ntypes"]) if rp["molecule"][t] != 0)
+    if rp["Prot"]:
+        s += rp["P"] * rp["Prot"] * nrot
     return s
 
 
 def load_into_context(ctx, wl):
-    """Upload tables, replicas and initial paths of a workload into a pimc_hip.Context."""
+    """Upload tables, replicas and initial paths of a workload into a pimc_hip.Context.
+    Optional tables: "pot3d" (list of (nr, nth, nch, rmin, rmax, v)) and "rot3d" (list of
+    (rho, erot, esq)); replicas then name them with "pot3d" / "rot3d" indices."""
     t = wl["tables"]
     for r, v in t["pot1d"]:
         ctx.add_pot1d(r, v)
@@ -126,9 +130,14 @@ def load_into_context(ctx, wl):
         ctx.add_pot2d(rg, cg, dr, dc, V)
     for c, rho, er, es in t["rot"]:
         ctx.add_rotdens(c, rho, er, es)
+    for tab in t.get("pot3d_list", []):
+        ctx.add_pot3d(*tab)
+    for tab in t.get("rot3d_list", []):
+        ctx.add_rot3d(*tab)
     for i, rp in enumerate(wl["replicas"]):
         ctx.set_replica(i, rp["ntypes"], rp["numb"], rp["molecule"], rp["levels"], rp["mcstep"], rp["mass"], rp["P"],
-                        rp["Prot"], rp["T"], rp["rtstep"], pot1d=rp["pot1d"], pot2d=rp["pot2d"], rotdens=rp["rotdens"])
+                        rp["Prot"], rp["T"], rp["rtstep"], pot1d=rp["pot1d"], pot2d=rp["pot2d"], rotdens=rp["rotdens"],
+                        pot3d=rp.get("pot3d", -1), rot3d=rp.get("rot3d", -1))
     ctx.commit()
     for i, rp in enumerate(wl["replicas"]):
         ctx.set_state(i, rp["coords"], rp["angles"])

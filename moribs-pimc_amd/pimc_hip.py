@@ -38,6 +38,8 @@ class ReplicaDesc(ctypes.Structure):
         ("Prot", _i),
         ("T", _d),
         ("rtstep", _d),
+        ("pot3d", _i),
+        ("rot3d", _i),
     ]
 
 
@@ -50,6 +52,8 @@ SIGNATURES = {
     "pimc_add_pot1d": (_i, [ctypes.c_void_p, _i, _dp, _dp]),
     "pimc_add_pot2d": (_i, [ctypes.c_void_p, _i, _i, _d, _d, _dp, _dp, _dp]),
     "pimc_add_rotdens": (_i, [ctypes.c_void_p, _i, _dp, _dp, _dp, _dp]),
+    "pimc_add_pot3d": (_i, [ctypes.c_void_p, _i, _i, _i, _d, _d, _dp]),
+    "pimc_add_rot3d": (_i, [ctypes.c_void_p, _dp, _dp, _dp]),
     "pimc_set_replica": (_i, [ctypes.c_void_p, _i, ctypes.POINTER(ReplicaDesc)]),
     "pimc_commit": (_i, [ctypes.c_void_p]),
     "pimc_set_option": (_i, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_long]),
@@ -134,9 +138,21 @@ class Context:
         a = [_f64(x) for x in (cosg, rho, erot, esq)]
         return self._check(self.lib.pimc_add_rotdens(self.h, len(a[0]), *[_dptr(x) for x in a]))
 
+    def add_pot3d(self, nr, nth, nch, rmin, rmax, v):
+        v = _f64(v).reshape(-1)
+        if v.size != nr * nth * nch:
+            raise PimcError("3D table size does not match its header")
+        return self._check(self.lib.pimc_add_pot3d(self.h, nr, nth, nch, float(rmin), float(rmax), _dptr(v)))
+
+    def add_rot3d(self, rho, erot, esq):
+        a = [_f64(x).reshape(-1) for x in (rho, erot, esq)]
+        if any(x.size != 181 * 361 * 361 for x in a):
+            raise PimcError("non-linear density tables hold 181*361*361 values each")
+        return self._check(self.lib.pimc_add_rot3d(self.h, *[_dptr(x) for x in a]))
+
     # replicas
     def set_replica(self, r, ntypes, numb, molecule, levels, mcstep, mass, P, Prot, T, rtstep, pot1d=None, pot2d=None,
-                    rotdens=-1):
+                    rotdens=-1, pot3d=-1, rot3d=-1):
         d = ReplicaDesc()
         d.ntypes = ntypes
         for t in range(MAX_TYPES):
@@ -148,6 +164,7 @@ class Context:
             d.pot1d[t] = (pot1d[t] if pot1d is not None and t < ntypes else -1)
             d.pot2d[t] = (pot2d[t] if pot2d is not None and t < ntypes else -1)
         d.rotdens = rotdens
+        d.pot3d, d.rot3d = pot3d, rot3d
         d.P, d.Prot, d.T, d.rtstep = P, Prot, T, rtstep
         self.descs[r] = d
         return self._check(self.lib.pimc_set_replica(self.h, r, ctypes.byref(d)))
