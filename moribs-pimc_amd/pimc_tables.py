@@ -106,6 +106,144 @@ def linden_exact_energy(T, bconst_cm, iodevn=-1, lmax=400):
     return e, (e2 - e * e) / T ** 2
 
 
+# ---------------------------------------------------------------- asymmetric-top density matrix
+BOLTZ_ASYM = 0.6950356  # cm^-1 per K in asymrho (asymrho.f:42)
+ROT3D_SHAPE = (181, 361, 361)  # theta, phi, chi in whole degrees (rotpro_sub.f:7-26)
+
+
+def _asym_block(J, ks, A, B, C):
+    """Rigid-rotor Hamiltonian of one k-parity block in the |J k> basis (rotmat, asymrho.f:792-813):
+    diagonal (A+C)/2 J(J+1) + (B - (A+C)/2) k^2, k -> k-2 coupling (A-C)/4 c+(J,k-2) c+(J,k-1)."""
+    def cplus(j, k):
+        return 0.0 if (k >= j or k < -j) else math.sqrt(j * (j + 1.0) - k * (k + 1.0))
+    n = len(ks)
+    H = np.zeros((n, n))
+    for i, k in enumerate(ks):
+        H[i, i] = 0.5 * (A + C) * J * (J + 1) + (B - 0.5 * (A + C)) * k * k
+        if i > 0:
+            H[i, i - 1] = H[i - 1, i] = 0.25 * (A - C) * cplus(J, k - 2) * cplus(J, k - 1)
+    return H
+
+
+def _wigner_d(J, theta):
+    """d^J_{mk}(theta) = <J m| exp(-i theta J_y) |J k> (the convention of Zare eq. 3.57 used by
+    wigd, asymrho.f:1006-1039), from the eigenvectors of J_y: stable at large J where the
+    alternating factorial sum loses its digits.  Rows m, columns k, both -J..J."""
+    ms = np.arange(-J, J + 1)
+    Jy = np.zeros((2 * J + 1, 2 * J + 1), complex)
+    for i, m in enumerate(ms[:-1]):
+        cp = math.sqrt(J * (J + 1.0) - m * (m + 1.0))
+        Jy[i + 1, i] += cp / 2j
+        Jy[i, i + 1] -= cp / 2j
+    lam, V = np.linalg.eigh(Jy)
+    return ((V * np.exp(-1j * theta * lam)) @ V.conj().T).real
+
+
+def asymrho_tables(T, nslice, A, B, C, maxj, iodevn=-1, thetas=None):
+    """Free asymmetric-top propagator, energy and energy-square estimators on the degree grid of
+    relative Euler angles (asymrho, asymrho.f:34-731; J. Chem. Phys. 134, 054117 (2011) eqs 15-16).
+
+    rho(theta, phi, chi) = sum_J (2J+1)/(8 pi^2) sum_s exp(-tau E_Js) sum_{m,k} c_m c_k d^J_mk(theta)
+    cos(m phi + k chi), the sums running over the eigenstates s of each k-parity block (even k
+    for iodevn 0, odd k for 1, both for -1); E and E^2 are the energy-weighted sums divided by rho
+    (cm^-1 and cm^-2; zero where |rho| < 1e-16).  Each theta is a product of complex matrices
+    Re[Phi M X^T] with Phi[phi, m] = e^{i m phi}, X[chi, k] = e^{i k chi}.  Values for chi beyond
+    the symmetry-unique range are copied as asymrho.f:665-709 does.
+
+    Returns rho, eng, esq of shape (len(thetas), 361, 361) and the list of thetas."""
+    if iodevn not in (-1, 0, 1):
+        raise ValueError("iodevn can only be -1 0 1")
+    thetas = list(range(181)) if thetas is None else list(thetas)
+    beta = 1.0 / (BOLTZ_ASYM * T)
+    tau = beta / nslice
+    eps = 1.0e-16
+    blocks = {0: [], 1: []}  # parity of k -> list of (J, ks, energies, vectors)
+    for J in range(maxj + 1):
+        kev = list(range(-J if J % 2 == 0 else -J + 1, J + 1, 2))
+        kod = list(range(-J + 1 if J % 2 == 0 else -J, J + 1, 2))
+        for par, ks in ((0, kev), (1, kod)):
+            if not ks:
+                continue
+            e, v = np.linalg.eigh(_asym_block(J, ks, A, B, C))
+            blocks[par].append((J, ks, e, v))
+    emax = max(e.max() for par in blocks for (_, _, e, _) in blocks[par])
+    if math.exp(-tau * emax) > 1.0e-8:
+        raise ValueError("too large contribution from emax: raise maxj")
+    ang = np.radians(np.arange(361.0))
+    n = 2 * maxj + 1
+    idx = np.arange(-maxj, maxj + 1)
+    E = np.exp(1j * np.outer(ang, idx))  # [angle, m] -> e^{i m angle}
+    out = [np.zeros((len(thetas), 361, 361)) for _ in range(3)]
+    pars = {-1: (0, 1), 0: (0,), 1: (1,)}[iodevn]
+    for ti, ith in enumerate(thetas):
+        th = math.radians(ith)
+        acc = {}
+        for par in pars:
+            M = [np.zeros((n, n)) for _ in range(3)]
+            for J, ks, e, v in blocks[par]:
+                pre = (2 * J + 1) / (8.0 * math.pi * math.pi)
+                d = _wigner_d(J, th)
+                kk = np.asarray(ks) + J
+                dsub = d[np.ix_(kk, kk)]
+                w = np.exp(-tau * e)
+                keep = w * pre >= eps
+                if not keep.any():
+                    continue
+                cw = v[:, keep]
+                for q, p in enumerate((0, 1, 2)):
+                    wt = w[keep] * e[keep] ** p
+                    G = (cw * wt) @ cw.T  # sum_s w_s E_s^p c_m c_k
+                    sl = np.asarray(ks) + maxj
+                    M[q][np.ix_(sl, sl)] += pre * G * dsub
+            acc[par] = [np.real(E @ Mq @ E.T) for Mq in M]  # [phi, chi]
+        if iodevn == -1:
+            r = acc[0][0] + acc[1][0]
+            with np.errstate(divide="ignore", invalid="ignore"):
+                # E of each parity is divided by its own rho first (zero where |rho| < 1e-16)
+                ee = np.where(np.abs(acc[0][0]) > eps, acc[0][1] / acc[0][0], 0.0)
+                eo = np.where(np.abs(acc[1][0]) > eps, acc[1][1] / acc[1][0], 0.0)
+                se = np.where(np.abs(acc[0][0]) > eps, acc[0][2] / acc[0][0], 0.0)
+                so = np.where(np.abs(acc[1][0]) > eps, acc[1][2] / acc[1][0], 0.0)
+                eng = (eo * acc[1][0] + ee * acc[0][0]) / r
+                esq = (so * acc[1][0] + se * acc[0][0]) / r
+            small = np.abs(r) < eps
+            eng[small] = 0.0
+            esq[small] = 0.0
+        else:
+            a = acc[pars[0]]
+            r = a[0]
+            with np.errstate(divide="ignore", invalid="ignore"):
+                eng = np.where(np.abs(r) > eps, a[1] / r, 0.0)
+                esq = np.where(np.abs(r) > eps, a[2] / r, 0.0)
+        for q, x in enumerate((r, eng, esq)):
+            out[q][ti] = _asym_symmetric_fill(x)
+    return out[0], out[1], out[2], thetas
+
+
+def _asym_symmetric_fill(x):
+    """Copy the symmetry-unique (phi, chi) triangle onto the rest of the grid (asymrho.f:665-709)."""
+    x = x.copy()
+    for iphi in range(90, 181):
+        for ichi in range(180 - iphi, iphi + 1):
+            x[iphi, ichi] = x[180 - ichi, 180 - iphi]
+    for iphi in range(180, 271):
+        for ichi in range(iphi - 180, 360 - iphi + 1):
+            x[iphi, ichi] = x[180 + ichi, iphi - 180]
+    for iphi in range(180, 361):
+        for ichi in range(360 - iphi, iphi + 1):
+            x[iphi, ichi] = x[360 - ichi, 360 - iphi]
+    for iphi in range(0, 361):
+        for ichi in range(iphi, 361):
+            x[iphi, ichi] = x[ichi, iphi]
+    return x
+
+
+def read_asym_column(path):
+    """One E15.8 column of an asymrho output file (rho.denXXX_rho/_eng/_esq)."""
+    with open(path) as f:
+        return np.array([float(t) for t in f.read().split()])
+
+
 # ---------------------------------------------------------------- file formats
 def fmt_temperature(T):
     """C++ default ostream formatting of a double (precision 6, %g), as used to build the

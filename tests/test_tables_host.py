@@ -169,3 +169,51 @@ def test_context_without_device_fails_loudly():
         pytest.skip("a device is present")
     with pytest.raises(ph.PimcError):
         ph.Context(1)
+
+
+# ----------------------------------------------------------------------------- asymmetric top
+def test_asymrho_reproduces_reference_fixture():
+    """The asymmetric-top propagator and estimators reproduce the table the reference's own
+    asymrho run left in nmv_prop/ (rho.den010_rho/_eng/_esq: T 0.37 K, 128 slices, no spin
+    symmetry, theta 10 deg, A B C 0.6666525 0.2306476 0.1769383 cm^-1, maxj 66; every 10th phi
+    row kept) to the precision of its E15.8 columns, wherever rho is above the summation noise."""
+    z = np.load(os.path.join(GOLD, "asymrho_theta010.npz"))
+    T, P, iod, th, A, B, C, maxj = z["params"]
+    rho, eng, esq, _ = pt.asymrho_tables(T, int(P), A, B, C, int(maxj), int(iod), [int(th)])
+    rows = z["phi"]
+    got = (rho[0][rows], eng[0][rows], esq[0][rows])
+    ref = (z["rho"], z["eng"], z["esq"])
+    mask = np.abs(ref[0]) > 1e-8 * np.abs(ref[0]).max()
+    assert mask.sum() > 3000
+    assert np.max(np.abs(got[0] - ref[0])[mask] / np.abs(ref[0][mask])) < 1e-7
+    for g, r in zip(got[1:], ref[1:]):
+        assert np.max(np.abs(g - r)[mask] / np.maximum(np.abs(r[mask]), 1.0)) < 1e-7
+
+
+def test_wigner_d_matches_zare_sum():
+    """d^J_mk(theta) from J_y eigenvectors equals the factorial sum of Zare eq. 3.57 at small J."""
+    from math import factorial, cos, sin, sqrt
+
+    def zare(j, m, k, t):
+        s = 0.0
+        for nu in range(max(0, k - m), min(j + k, j - m) + 1):
+            den = factorial(j - m - nu) * factorial(j + k - nu) * factorial(nu + m - k) * factorial(nu) * (-1) ** nu
+            s += sqrt(factorial(j + k) * factorial(j - k) * factorial(j + m) * factorial(j - m)) / den * \
+                cos(t / 2) ** (2 * j + k - m - 2 * nu) * (-sin(t / 2)) ** (m - k + 2 * nu)
+        return s
+    for j in (0, 1, 3, 6):
+        d = pt._wigner_d(j, 0.83)
+        for m in range(-j, j + 1):
+            for k in range(-j, j + 1):
+                assert abs(d[m + j, k + j] - zare(j, m, k, 0.83)) < 1e-12
+
+
+def test_asymmetric_top_normalisation():
+    """Only J = 0 survives the integral over SO(3): sum over the degree grid of
+    rho sin(theta) dtheta dphi dchi approaches 1."""
+    thetas = list(range(0, 181, 2))
+    rho, _, _, _ = pt.asymrho_tables(20.0, 4, 27.88, 14.51, 9.28, 14, -1, thetas)
+    d = np.radians(1.0)
+    w = np.sin(np.radians(thetas))[:, None, None]
+    integral = np.sum(rho[:, :-1, :-1] * w) * np.radians(2.0) * d * d
+    assert abs(integral - 1.0) < 2e-3
