@@ -34,7 +34,13 @@ DESC = {
     "c2": "256 replicas, N~U{1..100} He + 1 linear rotor (B=0.39 cm^-1), P in {64,128,256,512}, T~U(0.3,2) K",
     "c1": "64 He + 1 linear rotor, P=Prot=512, T=0.37 K",
     "c0": "8 He + 1 linear rotor, P=Prot=128, T=0.37 K",
+    "c3": "asymmetric top (A,B,C=27.88,14.51,9.28 cm^-1) in 32 para-H2, 100x181x91 Legendre-expansion table, "
+          "P=Prot=256, T=0.5 K, plain moves",
+    "c4": "128 replicas of two rigid TIP4P water rotors, P=Prot=2048, T=1 K, plain moves",
 }
+ROTDEN_BYTES = 4 * 8    # rotpro in a move: corner and three forward neighbours of rho (FP64)
+VCALC_BYTES = 4 * 4     # vcalc: corner and three forward neighbours of V (FP32)
+HBM_PEAK_GBS = 8000.0
 
 
 def flop_mix(wl):
@@ -142,6 +148,27 @@ def _reap(procs, times):
         time.sleep(0.01)
 
 
+def cpu_baseline_port(wl, budget_s=15.0):
+    """The oracle (FP64 C restatement, one core) on the first replica of a workload whose tables
+    the reference program would have to parse from 1 GB of text: passes until the budget is
+    spent, bead moves from the loop structure."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    rp = wl["replicas"][0]
+    t = wl["tables"]
+    tabs = dict(pot1d=t["pot1d"], pot2d=[], rot=[], pot3d=t["pot3d_list"][0] if t["pot3d_list"] else None,
+                rot3d=t["rot3d_list"][0])
+    S = O.System(rp, tabs)
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < budget_s or n == 0:
+        S.run(n, 1, order=O.ORDER_SEGMENTS)
+        n += 1
+    dt = time.perf_counter() - t0
+    return dict(value=n * pc.bead_moves_per_pass(rp) / dt, unit="bead-moves/s", cores=1, kind="port",
+                sample=f"{n} pass(es) of the first replica on the oracle (FP64, one host core)")
+
+
 def traffic_per_launch(workload_tag, kernel):
     """HBM bytes per dispatch of `kernel` from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in
     profiles/ (counters scaled by the hbm_copy calibration), or None when not profiled."""
@@ -156,9 +183,12 @@ def traffic_per_launch(workload_tag, kernel):
     return None
 
 
-KERNEL_NAMES = {"bisect": "k_bisect<float>", "centroid": "k_centroid<float>", "rotor": "k_rotor<float>",
-                "estimate": "k_est_rows<float>"}
-WORKLOAD_TAG = {"c2": "c2_256replicas", "c1": "c1_64he", "c0": "c0_8he"}
+KERNEL_NAMES = {"bisect": "k_bisect<float>", "centroid": "k_centroid<float,0>", "rotor": "k_rotor<float>",
+                "estimate": "k_est_rows<float,0>"}
+KERNEL_NAMES_NL = {"bisect": "k_bisect_nl<float>", "centroid": "k_centroid<float,1>", "rotor": "k_rotor3d<float>",
+                   "estimate": "k_est_rows<float,1>"}
+WORKLOAD_TAG = {"c2": "c2_256replicas", "c1": "c1_64he", "c0": "c0_8he", "c3": "c3_asymtop_32h2",
+                "c4": "c4_water2_128replicas"}
 
 
 # ----------------------------------------------------------------------------- GPU
@@ -167,7 +197,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2"])
+    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -176,7 +206,8 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
-    wl = pc.workload(args.workload)
+    nonlinear = args.workload in ("c3", "c4")
+    wl = pc.nl_workload(args.workload) if nonlinear else pc.workload(args.workload)
     ctx = pc.load_into_context(ph.Context(len(wl["replicas"]), flags=ph.FLAG_PROFILE), wl)
     bm_pass = float(sum(pc.bead_moves_per_pass(rp) for rp in wl["replicas"]))
     ctx.run(args.warmup, sample_every=1)
@@ -198,9 +229,25 @@ def main():
     ev = {f: e1[f] - e0[f] for f in efams}
     dom = max(efams, key=lambda f: kt[f][0])
     ms, nl = kt[dom]
-    flop_per_eval = {"bisect": flop_mix(wl), "centroid": flop_mix(wl), "rotor": FLOP_2D, "estimate": flop_mix(wl)}[dom]
-    flop_per_launch = ev[dom] * flop_per_eval / max(nl, 1)
-    achieved = flop_per_launch / (ms / max(nl, 1) * 1e-3) / 1e12
+    if nonlinear and dom == "rotor":
+        # density look-ups dominate: 4 rotden calls per move, four FP64 values of rho each, plus
+        # vcalc's four FP32 values per rotor-partner evaluation
+        moves = args.steps * sum(rp["P"] * rp["Prot"] * rp["numb"][-1] for rp in wl["replicas"])
+        bytes_per_launch = (4.0 * moves * ROTDEN_BYTES + ev[dom] * VCALC_BYTES) / max(nl, 1)
+        achieved = bytes_per_launch / (ms / max(nl, 1) * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "note": "algorithmic gather bytes of rotden and vcalc"}
+    else:
+        flop_per_eval = {"bisect": flop_mix(wl), "centroid": flop_mix(wl), "rotor": FLOP_2D,
+                         "estimate": flop_mix(wl)}[dom] if not nonlinear else FLOP_2D
+        flop_per_launch = ev[dom] * flop_per_eval / max(nl, 1)
+        achieved = flop_per_launch / (ms / max(nl, 1) * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
+                "note": "FP32 vector peak; the path has no matrix contraction, FLOPs are algorithmic "
+                        f"({FLOP_1D:g}/1D and {FLOP_2D:g}/2D pair evaluation)"}
+    roof["traffic"] = traffic_per_launch(WORKLOAD_TAG[args.workload], (KERNEL_NAMES_NL if nonlinear else KERNEL_NAMES)[dom])
+    roof["kernel"] = dom
     out = {
         "metric": METRIC,
         "value": round(args.steps * bm_pass / dt, 1),
@@ -219,15 +266,10 @@ def main():
                    "bead_moves_per_step": bm_pass, "pair_terms": "fp32, fp64 accumulation",
                    "estimators": "one MCGetAverage sample per replica per pass"},
         "kernels_ms_per_step": {f: round(kt[f][0] / args.steps, 4) for f in fams},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-                     "traffic": traffic_per_launch(WORKLOAD_TAG[args.workload], KERNEL_NAMES[dom]),
-                     "kernel": dom,
-                     "note": "FP32 vector peak; the path has no matrix contraction, FLOPs are algorithmic "
-                             f"({FLOP_1D:g}/1D and {FLOP_2D:g}/2D pair evaluation)"},
+        "roofline": roof,
     }
     if not args.no_cpu:
-        cb = cpu_baseline(wl)
+        cb = cpu_baseline_port(wl) if nonlinear else cpu_baseline(wl)
         if cb is not None:
             out["cpu_baseline"] = cb
     print(json.dumps(out))

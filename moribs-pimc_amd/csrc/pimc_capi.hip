@@ -84,7 +84,7 @@ struct pimc_ctx {
     std::vector<double4> rk, re;
     std::vector<Tab3D> t3;
     std::vector<double> v3;
-    double* d_r3 = nullptr;  // rho, E, E^2 of every non-linear density table, 3 * ROT3D_SIZE each
+    double4* d_r3 = nullptr;  // rho, E, E^2 of every non-linear density table, 3 * ROT3D_SIZE points each
     int nr3 = 0;
     // device
     hipStream_t stream = nullptr;
@@ -106,8 +106,8 @@ struct pimc_ctx {
     double4* d_rk = nullptr;
     double4* d_re = nullptr;
     TabRot* d_tr = nullptr;
-    double* d_v3d = nullptr;
-    float* d_v3f = nullptr;
+    double4* d_v3d = nullptr;
+    float4* d_v3f = nullptr;
     Tab3D* d_t3 = nullptr;
     Frame* d_frames = nullptr;     // rotation matrix of every non-linear rotor slice
     int2* d_work1 = nullptr;       // bisection work items of the non-linear family
@@ -117,6 +117,8 @@ struct pimc_ctx {
     int nall0 = 0, nall1 = 0;
     int* d_rot3 = nullptr;         // replicas with rotating non-linear rotors
     int nrot3 = 0, nrotmax = 0;
+    int rot3_block = 256;          // lanes of the non-linear rotor sweep
+    size_t lds_rot3 = 0;
     int2* d_rows1 = nullptr;
     int nrows1 = 0;
     size_t lds_bisect1 = 0;
@@ -327,21 +329,38 @@ int pimc_add_pot3d(pimc_ctx* c, int nr, int nth, int nch, double rmin, double rm
     return (int)c->t3.size() - 1;
 }
 
+// one table as (value, d_chi, d_phi, d_theta) per point of the 181 x 361 x 361 grid; the
+// differences are those rotpro forms (rotpro_sub.f:31-61), zero where it drops a term
+static void rot3d_points(const double* t, std::vector<double4>& out) {
+    out.resize((size_t)ROT3D_SIZE);
+    for (int ith = 0; ith <= 180; ++ith)
+        for (int iph = 0; iph <= 360; ++iph)
+            for (int ich = 0; ich <= 360; ++ich) {
+                const size_t i0 = ((size_t)ith * 361 + iph) * 361 + ich;
+                const double v = t[i0];
+                out[i0] = make_double4(v, ich != 360 ? t[i0 + 1] - v : 0.0, iph != 360 ? t[i0 + 361] - v : 0.0,
+                                       ith != 180 ? t[i0 + 361 * 361] - v : 0.0);
+            }
+}
+
 int pimc_add_rot3d(pimc_ctx* c, const double* rho, const double* erot, const double* esq) {
     if (c->committed) return fail(c, "tables must be added before pimc_commit");
     if (!rho || !erot || !esq) return fail(c, "init_rot3D: null table");
-    const size_t one = (size_t)ROT3D_SIZE * sizeof(double);
-    double* buf = nullptr;
+    const size_t one = (size_t)ROT3D_SIZE * sizeof(double4);
+    double4* buf = nullptr;
     if (hipMalloc(&buf, 3 * one * (size_t)(c->nr3 + 1)) != hipSuccess) return fail(c, "hipMalloc failed (rot3d)");
     if (c->nr3 > 0) {
         HIPCHECK(c, hipMemcpy(buf, c->d_r3, 3 * one * (size_t)c->nr3, hipMemcpyDeviceToDevice));
         (void)hipFree(c->d_r3);
     }
-    double* dst = buf + (size_t)3 * ROT3D_SIZE * c->nr3;
-    HIPCHECK(c, hipMemcpy(dst, rho, one, hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(dst + ROT3D_SIZE, erot, one, hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(dst + 2 * ROT3D_SIZE, esq, one, hipMemcpyHostToDevice));
     c->d_r3 = buf;
+    double4* dst = buf + (size_t)3 * ROT3D_SIZE * c->nr3;
+    std::vector<double4> pts;
+    const double* src[3] = {rho, erot, esq};
+    for (int q = 0; q < 3; ++q) {
+        rot3d_points(src[q], pts);
+        HIPCHECK(c, hipMemcpy(dst + (size_t)q * ROT3D_SIZE, pts.data(), one, hipMemcpyHostToDevice));
+    }
     return c->nr3++;
 }
 
@@ -652,15 +671,31 @@ int pimc_commit(pimc_ctx* c) {
         std::vector<Frame> fr(std::max<long long>(roff, 1), make_frame(0.0, 1.0, 0.0));
         HIPCHECK(c, hipMemcpy(c->d_frames, fr.data(), sizeof(Frame) * fr.size(), hipMemcpyHostToDevice));
     }
-    {  // 3D potential tables, FP64 and FP32
-        std::vector<float> v3f(c->v3.begin(), c->v3.end());
-        c->d_v3d = dalloc<double>(c->v3.size());
-        c->d_v3f = dalloc<float>(v3f.size());
+    {  // 3D potential tables: (V, dV_r, dV_theta, dV_chi) per point, FP64 and FP32, the
+       // differences those vcalc forms (vcalc.f:33-58), zero at an axis' last point
+        std::vector<double4> pd(c->v3.size());
+        std::vector<float4> pf(c->v3.size());
+        for (const Tab3D& t : c->t3)
+            for (int ir = 0; ir < t.nr; ++ir)
+                for (int ith = 0; ith < t.nth; ++ith)
+                    for (int ich = 0; ich < t.nch; ++ich) {
+                        const size_t i0 = (size_t)t.off + ((size_t)ir * t.nth + ith) * t.nch + ich;
+                        const size_t ir1 = i0 + (size_t)t.nth * t.nch, it1 = i0 + t.nch, ic1 = i0 + 1;
+                        const double* v = c->v3.data();
+                        const bool lr = ir == t.nr - 1, lt = ith == t.nth - 1, lc = ich == t.nch - 1;
+                        pd[i0] = make_double4(v[i0], lr ? 0.0 : v[ir1] - v[i0], lt ? 0.0 : v[it1] - v[i0],
+                                              lc ? 0.0 : v[ic1] - v[i0]);
+                        const float f0 = (float)v[i0];
+                        pf[i0] = make_float4(f0, lr ? 0.0f : (float)v[ir1] - f0, lt ? 0.0f : (float)v[it1] - f0,
+                                             lc ? 0.0f : (float)v[ic1] - f0);
+                    }
+        c->d_v3d = dalloc<double4>(pd.size());
+        c->d_v3f = dalloc<float4>(pf.size());
         c->d_t3 = dalloc<Tab3D>(c->t3.size());
         if (!c->d_v3d || !c->d_v3f || !c->d_t3) return fail(c, "hipMalloc failed");
         if (!c->v3.empty()) {
-            HIPCHECK(c, hipMemcpy(c->d_v3d, c->v3.data(), sizeof(double) * c->v3.size(), hipMemcpyHostToDevice));
-            HIPCHECK(c, hipMemcpy(c->d_v3f, v3f.data(), sizeof(float) * v3f.size(), hipMemcpyHostToDevice));
+            HIPCHECK(c, hipMemcpy(c->d_v3d, pd.data(), sizeof(double4) * pd.size(), hipMemcpyHostToDevice));
+            HIPCHECK(c, hipMemcpy(c->d_v3f, pf.data(), sizeof(float4) * pf.size(), hipMemcpyHostToDevice));
             HIPCHECK(c, hipMemcpy(c->d_t3, c->t3.data(), sizeof(Tab3D) * c->t3.size(), hipMemcpyHostToDevice));
         }
     }
@@ -703,6 +738,23 @@ int pimc_commit(pimc_ctx* c) {
         up(c->d_rk, c->rk.data(), c->rk.size() * sizeof(double4)) ||
         up(c->d_re, c->re.data(), c->re.size() * sizeof(double4)) || up(c->d_tr, c->tr.data(), c->tr.size() * sizeof(TabRot)))
         return fail(c, "table upload failed");
+    // non-linear rotor sweep: 512 lanes when the chains are short enough that the lane groups'
+    // frames and the link cache fit in LDS, else 256 (groups = lanes / G, G as the kernel picks it);
+    // 512 lanes keep a 256-register budget per lane
+    {
+        int protmax = 2;
+        for (int r : rot3) protmax = std::max(protmax, c->rep[r].Prot);
+        auto lds_for = [&](int block) {
+            int G = 1;
+            const int maxmoves = (protmax + 1) / 2;
+            while (G < 64 && G * 2 * maxmoves <= block) G *= 2;
+            return (size_t)((c->nrotmax * protmax + 1) & ~1) * sizeof(double) +
+                   (size_t)(block / G) * (c->nrotmax + 1) * sizeof(Frame);
+        };
+        c->rot3_block = lds_for(512) <= 64 * 1024 ? 512 : 256;
+        c->lds_rot3 = lds_for(c->rot3_block);
+        if (!rot3.empty() && c->lds_rot3 > 160 * 1024) return fail(c, "non-linear rotor chain exceeds LDS");
+    }
     // bisection block: enough lanes for the widest level of the largest window
     c->bisect_block = c->Nmax <= 24 ? 64 : (c->Nmax <= 64 ? 128 : 256);
     c->committed = true;
@@ -850,10 +902,12 @@ void launch_rotor(pimc_ctx* c, int u) {
     if (c->nparts)
         hipLaunchKernelGGL(k_rotor<R>, dim3(c->nparts), dim3(256), c->lds_rotor, c->stream, c->d_rep, c->d_mut,
                            c->d_beads, c->d_rotb, c->tables(), c->tables3(), c->d_parts, c->d_flags, run);
-    if (c->nrot3)  // group frames in LDS: (rotors + proposal) per lane group, at most 256 groups
-        hipLaunchKernelGGL(k_rotor3d<R>, dim3(c->nrot3), dim3(256), (size_t)256 * (c->nrotmax + 1) * sizeof(Frame),
-                           c->stream, c->d_rep, c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rot3,
-                           run);
+    if (c->nrot3 && c->rot3_block == 512)
+        hipLaunchKernelGGL((k_rotor3d<R, 512>), dim3(c->nrot3), dim3(512), c->lds_rot3, c->stream, c->d_rep,
+                           c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rot3, run);
+    else if (c->nrot3)
+        hipLaunchKernelGGL((k_rotor3d<R, 256>), dim3(c->nrot3), dim3(256), c->lds_rot3, c->stream, c->d_rep,
+                           c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rot3, run);
 }
 
 template <typename R>

@@ -97,11 +97,15 @@ struct Tables {
 
 // tables of the non-linear rotors, a kernel argument of its own so kernels of the linear
 // family pay no registers for them
+// A grid point of a 3D table holds its value and the forward differences to the next point
+// along each axis (zero at the last point of an axis), in the order of its look-up
+// (vcalc: r, theta, chi; rotpro: chi, phi, theta), so one look-up is one aligned load.
 struct Tables3 {
-    const double* v3d;  // 3D potential values (vtable order), FP64
-    const float* v3f;   // the same in FP32
+    const double4* v3d;  // 3D potential (vtable order): V, dV_r, dV_theta, dV_chi, FP64
+    const float4* v3f;   // the same in FP32 (differences taken in FP32)
     const Tab3D* t3;
-    const double* r3;   // rho, E, E^2 of each non-linear rotor density table, 3 * ROT3D_SIZE each
+    const double4* r3;   // per density table: rho, then E, then E^2 blocks of ROT3D_SIZE points,
+                         // each point (value, d_chi, d_phi, d_theta)
 };
 
 // rotation matrix of the Euler angles of one rotor slice (matpre, rotden.f:136-163), Fortran

@@ -42,6 +42,7 @@ __device__ __forceinline__ int type_of(const Rep& rp, int a) {
     return t;
 }
 
+// sum over the block; `red` holds one value per wave (16 for 1024 lanes)
 template <typename T>
 __device__ __forceinline__ T block_sum(T v, T* red) {
     v = wave_sum(v);
@@ -644,17 +645,21 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
 // at each slice every rotor in turn.  One workgroup per replica and a group of G lanes per
 // slice: the density factors rho(t0,t1), rho(t1,t2) (old) and rho(t0,t1'), rho(t1',t2) (new)
 // on four lanes, the old and new sums of PotRotE3D (mc_piqmc.cc:2046-2151) over the slice's
-// translational slices and partners across the group.  Old sums are recomputed for every move
-// as the reference does: with two rotors a partner rotor may have moved since.  Orientations
-// live in rotb and their frames in `frames`; a group keeps the frames of all rotors at its
-// slice (and the proposal's) in LDS, so a rotor sees the moves of the rotors before it there.
-template <typename R>
-__global__ __launch_bounds__(256) void k_rotor3d(const Rep* __restrict__ reps, Mut* __restrict__ mut,
+// translational slices and partners across the group.  Old energy sums are recomputed for
+// every move as the reference does: with two rotors a partner rotor may have moved since.  The
+// old density factors are cached per link (Lk[q][k] = rho(slice k, slice k+1) of rotor q): they
+// are the new factors of the last accepted move on that link, from the same frames, so a move
+// evaluates only its two new links.  Orientations live in rotb and their frames in `frames`; a
+// group keeps the frames of all rotors at its slice (and the proposal's) in LDS, so a rotor sees
+// the moves of the rotors before it there.
+// LDS: Lk [nrot*Prot] doubles, then the groups' frames [groups*(nrot+1)].
+template <typename R, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_rotor3d(const Rep* __restrict__ reps, Mut* __restrict__ mut,
                                                  const double4* __restrict__ beads, RotBead* __restrict__ rotb,
                                                  Frame* __restrict__ frames, Tables T, Tables3 T3, const int* __restrict__ replist,
                                                  Run run) {
     extern __shared__ double smem[];
-    __shared__ unsigned long long red_acc[4];
+    __shared__ unsigned long long red_acc[BLOCK / 64];  // one partial per wave
     __shared__ int s_err;
     const int r = replist[blockIdx.x];
     const Rep& rp = reps[r];
@@ -671,7 +676,7 @@ __global__ __launch_bounds__(256) void k_rotor3d(const Rep* __restrict__ reps, M
     const int np = N - 1;
     const int items = RR * np;
     const long long rb0 = rp.rot_off;  // rotor q, slice k at rb0 + q*Prot + k
-    const double* tab = T3.r3 + (long long)rp.rot3d_id * 3 * ROT3D_SIZE;
+    const double4* tab = T3.r3 + (long long)rp.rot3d_id * 3 * ROT3D_SIZE;
     const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
     const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
     const int maxmoves = (Prot + 1) / 2;
@@ -679,14 +684,21 @@ __global__ __launch_bounds__(256) void k_rotor3d(const Rep* __restrict__ reps, M
     while (G < 64 && G * 2 * maxmoves <= (int)blockDim.x) G *= 2;
     const int groups = blockDim.x / G;
     const int g = threadIdx.x / G, gl = threadIdx.x % G;
-    Frame* gf = reinterpret_cast<Frame*>(smem) + (long long)g * (nrot + 1);  // rotors at the slice, then the proposal
+    double* Lk = smem;
+    Frame* gf = reinterpret_cast<Frame*>(smem + ((nrot * Prot + 1) & ~1)) + (long long)g * (nrot + 1);
     const uint32_t pass = mut[r].pass;
     const int rtype = rp.imtype;
     const double twopi = 2.0 * PI_D;
     if (threadIdx.x == 0) s_err = 0;
-    __syncthreads();
     unsigned long long nacc = 0;
     int err = 0;
+    for (int i = threadIdx.x; i < nrot * Prot; i += blockDim.x) {
+        const int q = i / Prot, k = i - q * Prot;
+        const long long b = rb0 + (long long)q * Prot;
+        double er, es;
+        if (rotden_frames(frames[b + k], frames[b + (k + 1 == Prot ? 0 : k + 1)], tab, Lk[i], er, es)) err = 1;
+    }
+    __syncthreads();
     for (int sw = 0; sw < nsweep; ++sw) {
         const uint32_t jsweep = (uint32_t)(j0 + sw * jstride);
         for (int parity = 0; parity < 2; ++parity) {
@@ -717,16 +729,17 @@ __global__ __launch_bounds__(256) void k_rotor3d(const Rep* __restrict__ reps, M
                     if (cost < -1.0) cost = -2.0 - cost;
                     if (gl == 0) gf[nrot] = make_frame(phi, cost, chi);
                     __builtin_amdgcn_wave_barrier();
-                    // density factors: 0 (t0,t1) and 1 (t1,t2) old, 2 (t0,t1') and 3 (t1',t2) new
-                    double d[4] = {0.0, 0.0, 0.0, 0.0};
-                    for (int c = gl; c < 4; c += G) {
-                        const Frame& fa = (c == 1) ? gf[q] : (c == 3 ? gf[nrot] : frames[rb0 + (long long)q * Prot + k0]);
-                        const Frame& fb = (c == 0) ? gf[q] : (c == 2 ? gf[nrot] : frames[rb0 + (long long)q * Prot + k2]);
+                    // density factors: old (t0,t1) and (t1,t2) from the link cache, new (t0,t1')
+                    // and (t1',t2) on lanes 0 and 1
+                    double d[4] = {Lk[q * Prot + k0], Lk[q * Prot + k], 0.0, 0.0};
+                    for (int c = 2 + gl; c < 4; c += G) {
+                        const Frame& fa = (c == 3) ? gf[nrot] : frames[rb0 + (long long)q * Prot + k0];
+                        const Frame& fb = (c == 2) ? gf[nrot] : frames[rb0 + (long long)q * Prot + k2];
                         double er, es;
                         if (rotden_frames(fa, fb, tab, d[c], er, es)) err = 1;
                     }
                     if (G > 1)
-                        for (int c = 0; c < 4; ++c) d[c] = __shfl(d[c], c % G, G);
+                        for (int c = 2; c < 4; ++c) d[c] = __shfl(d[c], (c - 2) % G, G);
                     // PotRotE3D at the old and the proposed orientation
                     double eo = 0.0, en = 0.0;
                     for (int i = gl; i < items; i += G) {
@@ -764,6 +777,8 @@ __global__ __launch_bounds__(256) void k_rotor3d(const Rep* __restrict__ reps, M
                         rotb[ik] = b;
                         frames[ik] = gf[nrot];
                         gf[q] = gf[nrot];
+                        Lk[q * Prot + k0] = d[2];
+                        Lk[q * Prot + k] = d[3];
                     }
                     __builtin_amdgcn_wave_barrier();
                 }
@@ -890,7 +905,7 @@ __global__ __launch_bounds__(256) void k_est_final(const Rep* __restrict__ reps,
         esq = block_sum(a3, red);
     }
     if (rp.rotkind == 2 && rp.rot3d_id >= 0) {  // GetRotE3D mc_estim.cc:989-1096, per rotor
-        const double* tab = T3.r3 + (long long)rp.rot3d_id * 3 * ROT3D_SIZE;
+        const double4* tab = T3.r3 + (long long)rp.rot3d_id * 3 * ROT3D_SIZE;
         for (int q = 0; q < rp.nrot; ++q) {
             const long long fb = rp.rot_off + (long long)q * rp.Prot;
             double a1 = 0, a2 = 0, a3 = 0;

@@ -76,7 +76,7 @@ __device__ inline void deleul_frames(const Frame& f1, const Frame& f2, double& p
 // rotden (rotden.f:1-31) with rotpro (rotpro_sub.f:1-64): corner value plus one forward
 // difference per axis on the 1-degree grid, index (itheta*361 + iphi)*361 + ichi; E and E^2
 // converted from cm^-1 to Kelvin.  Returns 1 for an out-of-range angle (the reference's istop).
-__device__ inline int rotden_frames(const Frame& f1, const Frame& f2, const double* __restrict__ tab, double& rho,
+__device__ inline int rotden_frames(const Frame& f1, const Frame& f2, const double4* __restrict__ tab, double& rho,
                                     double& erot, double& esq) {
     const double wno2k = 0.6950356;
     double p2, t2, c2;
@@ -87,40 +87,26 @@ __device__ inline int rotden_frames(const Frame& f1, const Frame& f2, const doub
     if (ichi > 360 || ichi < 0) { ichi = 0; stop = 1; }
     if (iphi > 360 || iphi < 0) { iphi = 0; stop = 1; }
     if (ith > 180 || ith < 0) { ith = 0; stop = 1; }
-    const double* trho = tab;
-    const double* teng = tab + ROT3D_SIZE;
-    const double* tesq = tab + 2 * ROT3D_SIZE;
     const long long i0 = ((long long)ith * 361 + iphi) * 361 + ichi;
-    const double r0 = trho[i0], e0 = teng[i0], s0 = tesq[i0];
-    double dc1 = 0, dc2 = 0, dc3 = 0, dp1 = 0, dp2 = 0, dp3 = 0, dt1 = 0, dt2 = 0, dt3 = 0;
-    if (ichi != 360) {
-        const long long i = i0 + 1;
-        dc1 = trho[i] - r0; dc2 = teng[i] - e0; dc3 = tesq[i] - s0;
-    }
-    if (iphi != 360) {
-        const long long i = i0 + 361;
-        dp1 = trho[i] - r0; dp2 = teng[i] - e0; dp3 = tesq[i] - s0;
-    }
-    if (ith != 180) {
-        const long long i = i0 + 361 * 361;
-        dt1 = trho[i] - r0; dt2 = teng[i] - e0; dt3 = tesq[i] - s0;
-    }
     const double xc = chi - (double)ichi, xp = phi - (double)iphi, xt = theta - (double)ith;
-    rho = r0 + dc1 * xc + dp1 * xp + dt1 * xt;
-    erot = (e0 + dc2 * xc + dp2 * xp + dt2 * xt) / wno2k;
-    esq = (s0 + dc3 * xc + dp3 * xp + dt3 * xt) / (wno2k * wno2k);
+    const double4 r = tab[i0];
+    rho = r.x + r.y * xc + r.z * xp + r.w * xt;
+    const double4 e = tab[ROT3D_SIZE + i0];
+    erot = (e.x + e.y * xc + e.z * xp + e.w * xt) / wno2k;
+    const double4 q = tab[2 * ROT3D_SIZE + i0];
+    esq = (q.x + q.y * xc + q.z * xp + q.w * xt) / (wno2k * wno2k);
     return stop;
 }
 
 template <typename R>
 struct Lut3D {
-    const R* v;
+    const vec4_t<R>* v;
     int nr, nth, nch;
     R rmin, rmax, rstep, inv_rstep;
 };
 
-__device__ __forceinline__ const float* v3_ptr(const Tables3& T, float) { return T.v3f; }
-__device__ __forceinline__ const double* v3_ptr(const Tables3& T, double) { return T.v3d; }
+__device__ __forceinline__ const float4* v3_ptr(const Tables3& T, float) { return T.v3f; }
+__device__ __forceinline__ const double4* v3_ptr(const Tables3& T, double) { return T.v3d; }
 
 template <typename R>
 __device__ __forceinline__ Lut3D<R> lut3d(const Tables3& T, int tid) {
@@ -158,16 +144,9 @@ __device__ __forceinline__ R vcalc(const Lut3D<R>& L, R r, R theta, R chi) {
     if (ith > L.nth - 1) ith = L.nth - 1;
     if (ith < 0) ith = 0;
     if (ir > L.nr - 1) ir = L.nr - 1;  // r == rmax with rounding up
-    const long long i0 = ((long long)ir * L.nth + ith) * L.nch + ich;
-    const R v0 = L.v[i0];
-    R v = v0;
-    if (ir != L.nr - 1) {
-        const R grad = (L.v[i0 + (long long)L.nth * L.nch] - v0) / L.rstep;
-        v = v + grad * (r - (L.rmin + (R)ir * L.rstep));
-    }
-    if (ith != L.nth - 1) v = v + (L.v[i0 + L.nch] - v0) * (theta - (R)ith);
-    if (ich != L.nch - 1) v = v + (L.v[i0 + 1] - v0) * (chi - (R)ich);
-    return v;
+    const vec4_t<R> q = L.v[((long long)ir * L.nth + ith) * L.nch + ich];
+    // the differences are zero at an axis' last point, where vcalc drops that term
+    return q.x + (q.y / L.rstep) * (r - (L.rmin + (R)ir * L.rstep)) + q.z * (theta - (R)ith) + q.w * (chi - (R)ich);
 }
 
 // vcord (vcord.f:1-98): partner at rpt around the rotor centre rcom with body frame f

@@ -108,6 +108,67 @@ def workload(name, seed=985456376, n_replicas=256):
     return dict(name=name, tables=tabs, replicas=reps)
 
 
+# ---------------------------------------------------------------- non-linear rotor workloads
+MASS_H2 = 2.015650642                   # mc_const.h:45
+MASS_H2O = 2.0 * 1.0078 + 15.994915     # mc_setup.cc:303-306
+ABC_H2O = (27.88, 14.51, 9.28)          # asymmetric-top rotational constants, cm^-1
+
+
+def nl_initial_path(n_solvent, n_rotors, P, rng, shell=(5.0, 7.5), sep=3.0):
+    """Rotors near the origin (sep apart), solvent molecules on a shell around them; every bead of
+    a molecule at one point; random orientations."""
+    natoms = n_solvent + n_rotors
+    pts = []
+    while len(pts) < n_solvent:
+        x = rng.normal(size=3)
+        x *= rng.uniform(*shell) / np.linalg.norm(x)
+        if all(np.linalg.norm(x - p) > 2.9 for p in pts):
+            pts.append(x)
+    for q in range(n_rotors):
+        pts.append(np.array([sep * (q - 0.5 * (n_rotors - 1)), 0.0, 0.0]))
+    coords = np.zeros((3, natoms * P))
+    for a, p in enumerate(pts):
+        coords[:, a * P:(a + 1) * P] = p[:, None]
+    angles = np.zeros((3, natoms * P))
+    angles[0] = np.repeat(rng.uniform(0, 2 * np.pi, natoms), P)
+    angles[1] = np.repeat(rng.uniform(-1, 1, natoms), P)
+    angles[2] = np.repeat(rng.uniform(0, 2 * np.pi, natoms), P)
+    return coords, angles
+
+
+def nl_workload(name, seed=985456376, n_replicas=None):
+    """'c3': an asymmetric top (A, B, C = 27.88, 14.51, 9.28 cm^-1) in 32 para-H2 (12-6-8-9 pair
+    potential), P = P_rot = 256, T = 0.5 K, rotor-H2 potential from a seeded Legendre expansion on
+    a 100 x 181 x 91 grid, one replica.  'c4': two rigid TIP4P water rotors (caleng), T = 1 K,
+    P = P_rot = 2048, 128 replicas.  Density tables from asymrho_tables at (T, P_rot)."""
+    from pimc_tables import asymrho_tables, h2h2_table, legendre_3d_table
+    rng = np.random.default_rng(seed)
+    if name == "c3":
+        T, P, maxj, nrep = 0.5, 256, 20, n_replicas or 1
+        tabs = dict(pot1d=[h2h2_table()], pot2d=[], rot=[], pot3d_list=[legendre_3d_table()])
+        base = dict(ntypes=2, type_names=["pH2", "H2O"], numb=[32, 1], molecule=[0, 2], levels=[3, 3],
+                    mcstep=[0.3, 0.1], mass=[MASS_H2, MASS_H2O], P=P, Prot=P, T=T, rtstep=0.4,
+                    pot1d=[0, -1], pot2d=[-1, -1], rotdens=-1, pot3d=0, rot3d=0)
+        nsolv, nrot = 32, 1
+    elif name == "c4":
+        T, P, maxj, nrep = 1.0, 2048, 44, n_replicas or 128
+        tabs = dict(pot1d=[], pot2d=[], rot=[], pot3d_list=[])
+        base = dict(ntypes=1, type_names=["H2O"], numb=[2], molecule=[2], levels=[3], mcstep=[0.05],
+                    mass=[MASS_H2O], P=P, Prot=P, T=T, rtstep=0.3, pot1d=[-1], pot2d=[-1], rotdens=-1,
+                    pot3d=-1, rot3d=0)
+        nsolv, nrot = 0, 2
+    else:
+        raise ValueError(name)
+    rho, eng, esq, _ = asymrho_tables(T, P, *ABC_H2O, maxj, -1)
+    tabs["rot3d_list"] = [(rho.reshape(-1), eng.reshape(-1), esq.reshape(-1))]
+    reps = []
+    for r in range(nrep):
+        rp = dict(base)
+        rp["coords"], rp["angles"] = nl_initial_path(nsolv, nrot, P, rng, sep=3.0 if nsolv else 2.9)
+        reps.append(rp)
+    return dict(name=name, tables=tabs, replicas=reps)
+
+
 def bead_moves_per_pass(rp):
     """Loop-structure count: sum n P (2^L - 1) + sum n P + P Prot n_rot (mc_main.cc:348-381)."""
     s = 0

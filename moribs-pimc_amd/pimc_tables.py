@@ -238,6 +238,39 @@ def _asym_symmetric_fill(x):
     return x
 
 
+def legendre_3d_table(seed=20110201, nr=100, rmin=4.0, rmax=24.0, nch=91, depth=80.0):
+    """Rotor-particle potential V(R, theta, chi) on the vcord/vcalc grid: R in bohr (nr points),
+    theta 0..180 deg, chi 0..nch-1 deg (91: the C2v-folded quadrant), V[(ir*181 + ith)*nch + ich]
+    in Kelvin.  A seeded Legendre-type expansion, l <= 4 and even m <= min(l, 4):
+    V = sum_lm a_lm f_lm(R) P_l^m(cos theta) cos(m chi) with an isotropic Morse well of `depth`
+    (minimum near 6.3 bohr) and anisotropic terms of random sign and size on bounded Gaussian
+    radial shapes around the well, so the repulsive wall stays repulsive in every direction.
+    Returns (nr, 181, nch, rmin, rmax, v)."""
+    rng = np.random.default_rng(seed)
+    R = np.linspace(rmin, rmax, nr)[:, None, None]
+    th = np.radians(np.arange(181.0))[None, :, None]
+    ch = np.radians(np.arange(float(nch)))[None, None, :]
+    x = np.cos(th)
+    # associated Legendre P_l^m(x) for l <= 4 by explicit forms (no Condon-Shortley phase)
+    s = np.sqrt(np.maximum(1.0 - x * x, 0.0))
+    P = {(0, 0): np.ones_like(x), (1, 0): x, (2, 0): 0.5 * (3 * x * x - 1), (2, 2): 3 * s * s,
+         (3, 0): 0.5 * (5 * x ** 3 - 3 * x), (3, 2): 15 * x * s * s,
+         (4, 0): (35 * x ** 4 - 30 * x * x + 3) / 8, (4, 2): 7.5 * (7 * x * x - 1) * s * s, (4, 4): 105 * s ** 4}
+    norm = {(l, m): 1.0 / max(1.0, float(np.abs(P[(l, m)]).max())) for (l, m) in P}
+
+    def morse(re, a):
+        return (1.0 - np.exp(-a * (R - re))) ** 2 - 1.0
+    v = depth * morse(6.3, 0.9) * P[(0, 0)]
+    for (l, m) in P:
+        if (l, m) == (0, 0):
+            continue
+        amp = rng.uniform(0.05, 0.25) * depth * rng.choice([-1.0, 1.0])
+        shape = np.exp(-((R - rng.uniform(5.8, 7.0)) / rng.uniform(1.0, 2.0)) ** 2)
+        v = v + amp * shape * norm[(l, m)] * P[(l, m)] * np.cos(m * ch)
+    v = np.broadcast_to(v, (nr, 181, nch))
+    return nr, 181, nch, float(rmin), float(rmax), np.ascontiguousarray(v.reshape(-1))
+
+
 def read_asym_column(path):
     """One E15.8 column of an asymrho output file (rho.denXXX_rho/_eng/_esq)."""
     with open(path) as f:
