@@ -1,0 +1,583 @@
+// Host driver: a drop-in for the reference's `pimc` program on the sampling path.
+//
+// It reads the reference's qmc.input (keyword set of mc_input.cc:18-57 and IOReadParams
+// mc_input.cc:71-399), the table files named as InitPotentials / InitRotDensity name them
+// (mc_poten.cc:93-119,150-158,283-297,305-360,379-438,440-506,508-546), builds the initial
+// configuration (xyz.init as initconf.f reads it, else the lattice of initLattice_config
+// mc_setup.cc:491-650 and replInitial_config), runs blocks of passes through the C ABI of
+// moribs_pimc.h, and writes the reference's outputs:
+//   <OUTPUTDIR><FILENAMEPREFIX>.eng       one line per block after equilibration (SaveEnergy mc_main.cc:764-790)
+//   <OUTPUTDIR><FILENAMEPREFIX>_sum.eng   accumulated averages every MCSKIP_TOTAL steps (SaveSumEnergy :792-836)
+//   <OUTPUTDIR><FILENAMEPREFIX>.xyz       the configuration at the end of every block (IOxyz mc_input.cc:690-746)
+//   stdout                                the accept-ratio line every MCSKIP_RATIO steps (MCSaveAcceptRatio :879-943)
+//
+// Time steps and passes.  The reference counts "steps" (one time slice of one pass) and
+// samples when the step count is a multiple of MCSKIP_AVERG (mc_main.cc:340-484).  A pass on
+// the device moves every slice at once, so sampling, sum lines and ratio lines are taken at
+// pass granularity: every max(1, MCSKIP_AVERG / NUMBEROFSLICES) passes, and so on.
+//
+// Usage:  pimc [qmc.input] [--seed N] [--fp64] [--ref-order] [--replicas R] [--init-only]
+// --init-only writes the initial configuration to <OUTPUTDIR><FILENAMEPREFIX>.xyz and stops
+// before any device work.  With R > 1 the same deck runs as R independent chains in one context; chain r > 0 writes
+// its files under the prefix <OUTPUTDIR><FILENAMEPREFIX>_r<r>.
+#include "moribs_pimc.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+[[noreturn]] void die(const std::string& where, const std::string& what) {
+    std::cerr << "pimc: " << where << ": " << what << std::endl;
+    std::exit(1);
+}
+
+// ------------------------------------------------------------------------------- the deck
+struct Species {
+    std::string type, fpot;
+    int numb = 0, molecule = 0, levels = 0;
+    double mcstep = 0.0, mass = 0.0, rtstep = 0.0;
+};
+
+struct Deck {
+    std::string master = "./", outdir = "./", prefix = "pimc";
+    int ndim = 3;
+    double T = 0.0, density = 0.0;
+    std::vector<Species> types;
+    int P = 0, Prot = 0;
+    long passes = 0, blocks = 0, eqblocks = 0, startblock = 0;
+    bool rotation = false, read_coords = false;
+    std::string rot_type;
+    double rot_step = 0.0;
+    long skip_ratio = 100000, skip_total = 10000, skip_averg = 1;  // mc_setup.cc:91-93
+    int natoms() const { int n = 0; for (auto& s : types) n += s.numb; return n; }
+};
+
+// Masses by type name, MCInitParams mc_setup.cc:244-319 with the constants of mc_const.h:44-50.
+double mass_of(const std::string& t) {
+    const double H1 = 1.0078, H2 = 2.015650642, HE4 = 4.0026032497, C12 = 12.0, N14 = 14.003,
+                 O16 = 15.994915, S32 = 31.972;
+    if (t == "He4") return HE4;
+    if (t == "H2") return H2;
+    if (t == "OCS") return O16 + C12 + S32;
+    if (t == "N2O") return 2.0 * N14 + O16;
+    if (t == "CO2") return C12 + 2.0 * O16;
+    if (t == "CO") return C12 + O16;
+    if (t == "HCN") return H1 + C12 + N14;
+    if (t == "HCCCN") return H1 + 3.0 * C12 + N14;
+    if (t == "H2O") return 2.0 * H1 + O16;
+    if (t == "SO2") return 2.0 * O16 + S32;
+    if (t == "HCOOCH3") return 4.0 * H1 + 2.0 * O16 + 2.0 * C12;
+    die("MCInitParams", "Unknown atom/molecule type " + t);
+}
+
+// Keywords as IOReadParams reads them: one keyword and its fields per line, the rest of the
+// line is a comment.  Keywords that select methods outside the sampling path are refused.
+Deck read_deck(const std::string& path) {
+    std::ifstream in(path);
+    if (!in.good()) die("IOReadParams", "cannot open " + path);
+    Deck d;
+    std::string key, rest;
+    bool impurity = false;
+    int rtypes = 0;
+    auto need = [&](std::istream& s, const std::string& k) {
+        if (!s) die("IOReadParams", "missing or bad field after " + k);
+    };
+    while (in >> key) {
+        if (key == "RESTART") die("IOReadParams", "RESTART is not supported by this driver");
+        else if (key == "MASTERDIR") { in >> d.master; need(in, key); }
+        else if (key == "OUTPUTDIR") { in >> d.outdir; need(in, key); }
+        else if (key == "FILENAMEPREFIX") { in >> d.prefix; need(in, key); }
+        else if (key == "TEMPERATURE") { in >> d.T; need(in, key); }
+        else if (key == "DENSITY") { in >> d.density; need(in, key); }
+        else if (key == "DIMENSION") { in >> d.ndim; need(in, key); }
+        else if (key == "ATOM" || key == "MOLECULE" || key == "NONLINEAR") {
+            Species s;
+            std::string stat, model;
+            in >> s.type >> s.numb >> stat >> s.mcstep >> s.levels >> s.fpot >> model;
+            need(in, key);
+            if (s.numb < 0) die("IOReadParams", "spherical models (negative N) are not supported");
+            if (stat == "BOSE") die("IOReadParams", "Bose statistics (exchange moves) are not supported; use BOLTZMANN");
+            if (stat != "BOLTZMANN") die("IOReadParams", "Unknown statistics");
+            if (model == "EFFECTIVE") die("IOReadParams", "EFFECTIVE pair actions are not supported; use PRIMITIVE");
+            if (model != "PRIMITIVE") die("IOReadParams", "Unknown model of interaction");
+            s.molecule = key == "MOLECULE" ? 1 : key == "NONLINEAR" ? 2 : 0;
+            if (s.molecule == 0 && impurity) die("IOReadParams", "Molecules should follow atoms in input file");
+            if (s.numb > 0) {
+                if (s.molecule) impurity = true;
+                d.types.push_back(s);
+            }
+        }
+        else if (key == "NUMBEROFSLICES") { in >> d.P; need(in, key); }
+        else if (key == "NUMBEROFPASSES") { in >> d.passes; need(in, key); }
+        else if (key == "NUMBEROFBLOCKS") { in >> d.blocks >> d.eqblocks; need(in, key); }
+        else if (key == "INITIALBLOCK") { in >> d.startblock; need(in, key); }
+        else if (key == "ROTATION") {
+            in >> d.rot_type >> d.rot_step >> d.Prot;
+            need(in, key);
+            d.rotation = true;
+            ++rtypes;
+        }
+        else if (key == "ROTDENSI") {
+            int kind = 0;
+            in >> kind;
+            need(in, key);
+            if (kind != 0) die("IOReadParams", "ROTDENSI: only the tabulated density (0) is supported");
+        }
+        else if (key == "REFLECTX" || key == "REFLECTY" || key == "REFLECTZ") {
+            int on = 0;
+            in >> on;
+            need(in, key);
+            if (on) die("IOReadParams", key + " 1 is not supported");
+        }
+        else if (key == "ROTSYM") die("IOReadParams", "ROTSYM is not supported");
+        else if (key == "WORM") die("IOReadParams", "the worm algorithm is not supported");
+        else if (key == "MINIMAGE") die("IOReadParams", "periodic boundaries (MINIMAGE) are not supported");
+        else if (key == "CRYSTAL") die("IOReadParams", "crystal lattices are not supported");
+        else if (key == "READMCCOORDS") d.read_coords = true;
+        else if (key == "MCSKIP_RATIO") { in >> d.skip_ratio; need(in, key); }
+        else if (key == "MCSKIP_TOTAL") { in >> d.skip_total; need(in, key); }
+        else if (key == "MCSKIP_AVERG") { in >> d.skip_averg; need(in, key); }
+        std::getline(in, rest);  // the rest of the line is a comment
+    }
+    if (d.ndim != 3) die("IOReadParams", "DIMENSION must be 3");
+    if (d.types.empty()) die("IOReadParams", "no particles");
+    if ((int)d.types.size() > PIMC_MAX_TYPES) die("IOReadParams", "too many particle types");
+    int atypes = 0, mtypes = 0;
+    for (auto& s : d.types) (s.molecule ? mtypes : atypes)++;
+    if (atypes > 1 || mtypes > 1)
+        die("IOReadParams", "No more then one atom/molecule type: densities and potential energy");
+    if (d.rotation) {
+        if (rtypes != 1) die("IOReadParams", "Rotational degrees of freedom for one molecule's type only");
+        bool found = false;
+        for (auto& s : d.types)
+            if (s.type == d.rot_type) {
+                if (s.molecule == 0) die("IOReadParams", "Rotational degrees of freedom for molecules only");
+                s.rtstep = d.rot_step;
+                found = true;
+                break;
+            }
+        if (!found) die("IOReadParams", "Can't find a particle type to sample rotational degrees of freedom");
+    }
+    if (d.P <= 0 || d.passes <= 0 || d.blocks <= 0) die("IOReadParams", "NUMBEROFSLICES/PASSES/BLOCKS must be positive");
+    if (d.skip_ratio <= 0 || d.skip_total <= 0 || d.skip_averg <= 0) die("IOReadParams", "MCSKIP_* must be positive");
+    for (auto& s : d.types) s.mass = mass_of(s.type);
+    return d;
+}
+
+// ------------------------------------------------------------------------------ table files
+// Whitespace-separated numbers; '#' starts a comment running to the end of the line
+// (read_datafile mc_poten.cc:757-788 skips comment lines).
+std::vector<double> read_numbers(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f.good()) die("read_datafile", "cannot open " + path);
+    std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    std::vector<double> out;
+    const char* p = text.c_str();
+    const char* end = p + text.size();
+    while (p < end) {
+        if (*p == '#') {
+            while (p < end && *p != '\n') ++p;
+            continue;
+        }
+        if (std::isspace((unsigned char)*p)) { ++p; continue; }
+        char* q = nullptr;
+        double v = std::strtod(p, &q);
+        if (q == p) die("read_datafile", "bad number in " + path);
+        out.push_back(v);
+        p = q;
+    }
+    return out;
+}
+
+std::string fmt_default(double x) {  // ostream << double, precision 6 (file names, mc_poten.cc:443-448)
+    std::ostringstream s;
+    s << x;
+    return s.str();
+}
+
+struct Context {
+    pimc_ctx* ctx = nullptr;
+    void check(int rc, const char* where) const {
+        if (rc < 0) die(where, pimc_last_error(ctx));
+    }
+};
+
+// Reads the tables of every species and fills the replica description.
+pimc_replica_desc load_tables(const Deck& d, Context& c) {
+    pimc_replica_desc r;
+    std::memset(&r, 0, sizeof r);
+    r.ntypes = (int)d.types.size();
+    r.P = d.P;
+    r.Prot = d.rotation ? d.Prot : 0;
+    r.T = d.T;
+    r.rotdens = -1;
+    r.pot3d = -1;
+    r.rot3d = -1;
+    const std::string tag = "_T" + fmt_default(d.T) + "t" + std::to_string(d.Prot);
+    bool has_atoms = false;
+    for (auto& s : d.types) has_atoms |= s.molecule == 0;
+    for (int t = 0; t < r.ntypes; ++t) {
+        const Species& s = d.types[t];
+        r.numb[t] = s.numb;
+        r.molecule[t] = s.molecule;
+        r.levels[t] = s.levels;
+        r.mcstep[t] = s.mcstep;
+        r.mass[t] = s.mass;
+        r.pot1d[t] = r.pot2d[t] = -1;
+        if (s.molecule == 0) {  // init_pot1D: two columns, r (Angstrom) and V (Kelvin)
+            auto v = read_numbers(s.fpot + ".pot");
+            if (v.size() < 4 || v.size() % 2) die("init_pot1D", s.fpot + ".pot: expected two columns");
+            const int n = (int)v.size() / 2;
+            std::vector<double> rr(n), vv(n);
+            for (int i = 0; i < n; ++i) { rr[i] = v[2 * i]; vv[i] = v[2 * i + 1]; }
+            r.pot1d[t] = pimc_add_pot1d(c.ctx, n, rr.data(), vv.data());
+            c.check(r.pot1d[t], "init_pot1D");
+        } else if (s.molecule == 1) {
+            if (s.numb != 1) die("InitPotentials", "No more than one linear dopant molecule so far");
+            if (has_atoms) {  // init_pot2D: rsize csize dr dc r-grid c-grid V[r][c]
+                auto v = read_numbers(s.fpot + ".pot");
+                if (v.size() < 4) die("init_pot2D", s.fpot + ".pot: short header");
+                const int nr = (int)v[0], nc = (int)v[1];
+                if (nr < 2 || nc < 2 || v.size() != 4 + (size_t)nr + nc + (size_t)nr * nc)
+                    die("init_pot2D", s.fpot + ".pot: sizes do not match the data");
+                const double* g = v.data() + 4;
+                r.pot2d[t] = pimc_add_pot2d(c.ctx, nr, nc, v[2], v[3], g, g + nr, g + nr + nc);
+                c.check(r.pot2d[t], "init_pot2D");
+            }
+            if (d.rotation) {  // init_rotdens: TYPE_T<T>t<Prot>.rot, four columns
+                auto v = read_numbers(s.type + tag + ".rot");
+                if (v.size() < 8 || v.size() % 4) die("init_rotdens", s.type + tag + ".rot: expected four columns");
+                const int n = (int)v.size() / 4;
+                std::vector<double> col[4];
+                for (auto& x : col) x.resize(n);
+                for (int i = 0; i < n; ++i)
+                    for (int k = 0; k < 4; ++k) col[k][i] = v[4 * i + k];
+                r.rotdens = pimc_add_rotdens(c.ctx, n, col[0].data(), col[1].data(), col[2].data(), col[3].data());
+                c.check(r.rotdens, "init_rotdens");
+            }
+        } else {
+            if (r.ntypes > 1) {  // init_pot3D: Rgrd THgrd CHgrd Rvmin Rvmax, then V
+                auto v = read_numbers(s.fpot + ".pot");
+                if (v.size() < 5) die("init_pot3D", s.fpot + ".pot: short header");
+                const int nr = (int)v[0], nth = (int)v[1], nch = (int)v[2];
+                if (v.size() != 5 + (size_t)nr * nth * nch) die("init_pot3D", s.fpot + ".pot: sizes do not match the data");
+                r.pot3d = pimc_add_pot3d(c.ctx, nr, nth, nch, v[3], v[4], v.data() + 5);
+                c.check(r.pot3d, "init_pot3D");
+            }
+            if (d.rotation) {  // init_rot3D: TYPE_T<T>t<Prot>.rho/.eng/.esq, 181*361*361 values each
+                const size_t n = 181ull * 361 * 361;
+                auto rho = read_numbers(s.type + tag + ".rho");
+                auto eng = read_numbers(s.type + tag + ".eng");
+                auto esq = read_numbers(s.type + tag + ".esq");
+                if (rho.size() != n || eng.size() != n || esq.size() != n)
+                    die("init_rot3D", s.type + tag + ": each of .rho/.eng/.esq must hold 181*361*361 values");
+                r.rot3d = pimc_add_rot3d(c.ctx, rho.data(), eng.data(), esq.data());
+                c.check(r.rot3d, "init_rot3D");
+            }
+        }
+        if (s.type == d.rot_type) r.rtstep = s.rtstep;
+    }
+    return r;
+}
+
+// ------------------------------------------------------------------- initial configuration
+// [3][natoms*P] coordinates and angles (phi, cos(theta), chi), slice fastest.
+void initial_config(const Deck& d, std::vector<double>& coords, std::vector<double>& angles) {
+    const int N = d.natoms(), P = d.P, n = N * P;
+    coords.assign(3 * (size_t)n, 0.0);
+    angles.assign(3 * (size_t)n, 0.0);
+    for (int i = 0; i < n; ++i) angles[n + i] = 1.0;  // MCConfigInit: phi 0, cos 1, chi 0
+    if (d.read_coords) {
+        // xyz.init (initconf.f): a line with a count (and permutation indices), a comment line,
+        // then one line per bead: label x phi y cos(theta) z chi
+        std::ifstream f("xyz.init");
+        if (!f.good()) die("initconf", "cannot open xyz.init");
+        std::string line, label;
+        std::getline(f, line);
+        std::getline(f, line);
+        for (int i = 0; i < n; ++i) {
+            if (!std::getline(f, line)) die("initconf", "xyz.init holds fewer than natoms*P beads");
+            std::istringstream s(line);
+            s >> label;
+            for (int j = 0; j < 3; ++j) s >> coords[(size_t)j * n + i] >> angles[(size_t)j * n + i];
+            if (!s) die("initconf", "bad bead line in xyz.init");
+        }
+        return;
+    }
+    if (d.density <= 0.0) die("MCInit", "DENSITY must be positive to build the initial lattice");
+    int natoms = 0, nmolcs = 0;
+    for (auto& s : d.types) (s.molecule ? nmolcs : natoms) += s.numb;
+    const double box = std::pow((natoms + nmolcs) / d.density, 1.0 / 3.0);
+    int first = 0;
+    double shift[3];
+    // atoms: a simple cubic fill of the box, x fastest (initLattice_config)
+    const double abox = natoms ? box / std::pow((double)natoms, 1.0 / 3.0) : 0.0;
+    for (double& x : shift) x = 0.5 * abox;
+    for (auto& s : d.types) {
+        if (s.molecule == 0)
+            for (int a = first; a < first + s.numb; ++a) {
+                for (int id = 0; id < 3; ++id) coords[(size_t)id * n + a * P] = shift[id] - 0.5 * box;
+                shift[0] += abox;
+                for (int id = 0; id < 2; ++id)
+                    if (shift[id] > box) { shift[id] = 0.5 * abox; shift[id + 1] += abox; }
+            }
+        first += s.numb;
+    }
+    // molecules: their own cubic fill, wrapped before placing; moved out of the atoms' cube
+    // when the counts are equal
+    const double amol = nmolcs ? box / std::pow((double)nmolcs, 1.0 / 3.0) : 0.0;
+    for (double& x : shift) x = 0.5 * amol;
+    first = 0;
+    for (auto& s : d.types) {
+        if (s.molecule)
+            for (int a = first; a < first + s.numb; ++a) {
+                for (int id = 0; id < 2; ++id)
+                    if (shift[id] > box) { shift[id] = 0.5 * amol; shift[id + 1] += amol; }
+                for (int id = 0; id < 3; ++id)
+                    coords[(size_t)id * n + a * P] = shift[id] - 0.5 * box + (natoms == nmolcs ? box : 0.0);
+                shift[0] += amol;
+            }
+        first += s.numb;
+    }
+    for (int id = 0; id < 3; ++id)  // replInitial_config: every slice starts at slice 0
+        for (int a = 0; a < N; ++a)
+            for (int it = 1; it < P; ++it) coords[(size_t)id * n + a * P + it] = coords[(size_t)id * n + a * P];
+}
+
+// -------------------------------------------------------------------------------- outputs
+// One sample of MCGetAverage mc_main.cc:551-640: K, V, E_rot, E_rot^2, Cv, Cv_trans, Cv_rot, (E_rot term)^2.
+struct Sums {
+    double kin = 0, pot = 0, rot = 0, rotsq = 0, cv = 0, cvt = 0, cvr = 0, count = 0;
+    void add(const double* s) {
+        kin += s[0]; pot += s[1]; rot += s[2]; rotsq += s[3];
+        cv += s[4]; cvt += s[5]; cvr += s[6]; count += 1.0;
+    }
+};
+
+struct Chain {
+    std::string stem;
+    std::ofstream sum;
+    Sums block, total;
+    double sums_count = 0.0;
+    std::vector<double> tot0, acc0;  // counters at the start of the block
+};
+
+void set_sci(std::ostream& o) { o << std::setprecision(6) << std::scientific; }
+
+void save_energy(const Chain& c, long block) {  // SaveEnergy
+    std::ofstream f(c.stem + ".eng", std::ios::app);
+    if (!f.good()) die("SaveEnergy", "cannot open " + c.stem + ".eng");
+    set_sci(f);
+    const Sums& b = c.block;
+    const double n = b.count;
+    const char* sp = "   ";
+    f << std::setw(4) << block << sp << std::setw(14) << b.kin / n << sp << std::setw(14) << b.pot / n << sp
+      << std::setw(14) << (b.kin + b.pot) / n << sp << std::setw(14) << b.rot / n << sp << std::setw(14) << b.rotsq / n
+      << sp << std::setw(14) << (b.kin + b.pot + b.rot) / n << sp << std::setw(14) << b.cv / n << sp << std::setw(14)
+      << b.cvt / n << sp << std::setw(14) << b.cvr / n << sp << "\n";
+}
+
+void save_sum_energy(Chain& c, const Deck& d, int natoms) {  // SaveSumEnergy
+    const Sums& t = c.total;
+    const double n = t.count, T = d.T, beta = 1.0 / d.T;
+    const double e = (t.kin + t.pot + t.rot) / n;
+    double cv = 0.5 * 3 * natoms * d.P * T - e;
+    cv = -(cv * cv + t.cv / n) * beta / T;
+    double cvt = 0.5 * 3 * natoms * d.P * T - t.kin / n;
+    cvt = -(cvt * cvt + t.cvt / n) * beta / T;
+    double cvr = -t.rot / n;
+    cvr = -(cvr * cvr + t.cvr / n) * beta / T;
+    const char* sp = "   ";
+    c.sum << std::setw(4) << c.sums_count << sp << std::setw(14) << t.kin / n << sp << std::setw(14) << t.pot / n << sp
+          << std::setw(14) << (t.kin + t.pot) / n << sp << std::setw(14) << t.rot / n << sp << std::setw(14)
+          << t.rotsq / n << sp << std::setw(14) << e << sp << std::setw(14) << cv << sp << std::setw(14) << cvt << sp
+          << std::setw(14) << cvr << sp << "\n";
+    c.sum.flush();
+}
+
+void save_xyz(const Chain& c, const Deck& d, const std::vector<double>& coords, const std::vector<double>& cosines) {
+    std::ofstream f(c.stem + ".xyz");  // IOxyz: label, then (coordinate, direction cosine) per axis
+    if (!f.good()) die("IOxyz", "cannot open " + c.stem + ".xyz");
+    set_sci(f);
+    const int n = d.natoms() * d.P;
+    const char* sp = "   ";
+    f << n << "\n#" << sp << "xyz format:  [atom type]  x y z (Angstrom) \n";
+    int i = 0;
+    for (auto& s : d.types)
+        for (int a = 1; a <= s.numb; ++a)
+            for (int it = 0; it < d.P; ++it, ++i) {
+                f << std::setw(5) << (s.type + std::to_string(a)) << sp;
+                for (int id = 0; id < 3; ++id)
+                    f << std::setw(14) << coords[(size_t)id * n + i] << sp << std::setw(14) << cosines[(size_t)id * n + i] << sp;
+                f << "\n";
+            }
+}
+
+void print_ratios(const Deck& d, Context& c, Chain& ch, int r, long block, long pass, long step) {
+    const int nt = (int)d.types.size();
+    std::vector<double> tot(3 * nt), acc(3 * nt);
+    c.check(pimc_get_counters(c.ctx, r, tot.data(), acc.data()), "MCSaveAcceptRatio");
+    std::ostringstream o;  // MCSaveAcceptRatio: default float format, width 8
+    const char* sp = "   ";
+    o << "BLOCK:" << std::setw(8) << block << sp << "PASS:" << std::setw(8) << pass << sp << "STEP:" << std::setw(8)
+      << step << sp;
+    auto ratio = [&](int t, int m) {
+        const double T = tot[3 * t + m] - ch.tot0[3 * t + m], A = acc[3 * t + m] - ch.acc0[3 * t + m];
+        return A / T;
+    };
+    int imtype = -1;
+    for (int t = 0; t < nt; ++t) {
+        if (d.types[t].type == d.rot_type) imtype = t;
+        o << std::setw(8) << d.types[t].type << sp << std::setw(8) << ratio(t, PIMC_MCMOLEC) << sp << std::setw(8)
+          << ratio(t, PIMC_MCMULTI) << sp;
+    }
+    if (d.rotation && imtype >= 0) o << sp << "Rot: " << std::setw(8) << ratio(imtype, PIMC_MCROTAT) << sp;
+    if (r > 0) o << " [chain " << r << "]";
+    std::cout << o.str() << std::endl;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::string input = "qmc.input";
+    unsigned seed = 985456376u;  // mc_main.cc:151
+    unsigned flags = 0;
+    int nrep = 1;
+    bool init_only = false;
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        if (a == "--seed" && i + 1 < argc) seed = (unsigned)std::strtoul(argv[++i], nullptr, 10);
+        else if (a == "--fp64") flags |= PIMC_FLAG_FP64;
+        else if (a == "--ref-order") flags |= PIMC_FLAG_REF_ORDER;
+        else if (a == "--replicas" && i + 1 < argc) nrep = std::atoi(argv[++i]);
+        else if (a == "--init-only") init_only = true;
+        else if (a == "-h" || a == "--help") {
+            std::cout << "usage: pimc [qmc.input] [--seed N] [--fp64] [--ref-order] [--replicas R] [--init-only]\n";
+            return 0;
+        } else if (!a.empty() && a[0] != '-') input = a;
+        else die("main", "unknown option " + a);
+    }
+    if (nrep < 1) die("main", "--replicas must be positive");
+    Deck d = read_deck(input);
+    const int N = d.natoms(), n = N * d.P;
+
+    std::vector<double> coords, angles;
+    initial_config(d, coords, angles);
+    // direction cosines from (phi, cos(theta)) as MCConfigInit sets them; beads that carry no
+    // orientation on the device keep these
+    std::vector<double> cos0(3 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const double ph = angles[i], ct = angles[n + i], st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
+        cos0[i] = st * std::cos(ph);
+        cos0[n + i] = st * std::sin(ph);
+        cos0[2 * (size_t)n + i] = ct;
+    }
+    const std::string stem = d.outdir + d.prefix;
+    if (init_only) {
+        Chain ch;
+        ch.stem = stem;
+        save_xyz(ch, d, coords, cos0);
+        return 0;
+    }
+
+    Context c;
+    if (pimc_create(nrep, flags, seed, &c.ctx) != 0)
+        die("pimc_create", c.ctx ? pimc_last_error(c.ctx) : "no HIP device");
+    pimc_replica_desc desc = load_tables(d, c);
+    for (int r = 0; r < nrep; ++r) c.check(pimc_set_replica(c.ctx, r, &desc), "pimc_set_replica");
+    c.check(pimc_commit(c.ctx), "pimc_commit");
+    for (int r = 0; r < nrep; ++r) c.check(pimc_set_state(c.ctx, r, coords.data(), angles.data()), "pimc_set_state");
+
+    std::vector<Chain> chains(nrep);
+    for (int r = 0; r < nrep; ++r) {
+        Chain& ch = chains[r];
+        ch.stem = r ? stem + "_r" + std::to_string(r) : stem;
+        std::ofstream(ch.stem + ".eng", std::ios::trunc);
+        ch.sum.open(ch.stem + "_sum.eng", std::ios::trunc);
+        if (!ch.sum.good()) die("InitTotalAverage", "cannot open " + ch.stem + "_sum.eng");
+        set_sci(ch.sum);
+    }
+    std::cout << "pimc: " << input << ": " << N << " particles, " << d.P << " slices, T = " << d.T << " K, "
+              << nrep << " chain(s)" << std::endl;
+
+    // Pass-granular schedule of the step counters (steps = passes * slices).
+    const long P = d.P;
+    const long every = std::max(1L, d.skip_averg / P);
+    const long ratio_passes = std::max(1L, d.skip_ratio / P);
+    const long chunk = every * std::max(1L, std::min(1024L, ratio_passes / every));
+    auto crosses = [&](long q, long skip) { return (q * P) / skip > ((q - 1) * P) / skip; };
+
+    const int nt = (int)d.types.size();
+    std::vector<double> samples;
+    for (long block = d.startblock + 1; block <= d.startblock + d.blocks; ++block) {
+        const bool measured = block > d.eqblocks;
+        for (int r = 0; r < nrep; ++r) {
+            Chain& ch = chains[r];
+            ch.block = Sums();
+            ch.tot0.assign(3 * nt, 0.0);
+            ch.acc0.assign(3 * nt, 0.0);
+            c.check(pimc_get_counters(c.ctx, r, ch.tot0.data(), ch.acc0.data()), "ResetMCCounts");
+        }
+        c.check(pimc_clear_samples(c.ctx), "pimc_clear_samples");
+        for (long done = 0; done < d.passes;) {
+            const long m = std::min(chunk, d.passes - done);
+            c.check(pimc_run(c.ctx, (int)m, measured ? (int)every : 0), "pimc_run");
+            for (int r = 0; r < nrep; ++r) {
+                Chain& ch = chains[r];
+                int got = 0;
+                if (measured) {
+                    samples.resize(8 * (size_t)(m / every + 1));
+                    got = pimc_get_samples(c.ctx, r, samples.data(), (int)(m / every + 1));
+                    c.check(got, "pimc_get_samples");
+                }
+                int next = 0;
+                bool ratio_line = false;
+                for (long q = done + 1; q <= done + m; ++q) {
+                    ratio_line |= crosses(q, d.skip_ratio);
+                    if (measured && (q - done) % every == 0 && next < got) {
+                        ch.block.add(&samples[8 * (size_t)next]);
+                        ch.total.add(&samples[8 * (size_t)next]);
+                        ++next;
+                    }
+                    if (measured && crosses(q, d.skip_total) && ch.block.count > 0) {
+                        ch.sums_count += 1.0;
+                        save_sum_energy(ch, d, N);
+                    }
+                }
+                if (ratio_line) print_ratios(d, c, ch, r, block, done + m, (done + m) * P);
+            }
+            if (measured) c.check(pimc_clear_samples(c.ctx), "pimc_clear_samples");
+            done += m;
+        }
+        std::vector<double> cg(3 * (size_t)n), ag(3 * (size_t)n), sg(3 * (size_t)n);
+        for (int r = 0; r < nrep; ++r) {
+            Chain& ch = chains[r];
+            if (measured && ch.block.count > 0) save_energy(ch, block);
+            c.check(pimc_get_state(c.ctx, r, cg.data(), ag.data(), sg.data()), "pimc_get_state");
+            // beads without an orientation on the device keep their initial cosines
+            int first = 0;
+            for (auto& s : d.types) {
+                if (!(d.rotation && s.type == d.rot_type))
+                    for (int i = first * d.P; i < (first + s.numb) * d.P; ++i)
+                        for (int id = 0; id < 3; ++id) sg[(size_t)id * n + i] = cos0[(size_t)id * n + i];
+                first += s.numb;
+            }
+            save_xyz(ch, d, cg, sg);
+        }
+    }
+    double pe = 0, bm = 0;
+    c.check(pimc_get_work(c.ctx, &pe, &bm), "pimc_get_work");
+    std::cout << "pimc: done, " << bm << " bead moves, " << pe << " pair evaluations" << std::endl;
+    pimc_destroy(c.ctx);
+    return 0;
+}

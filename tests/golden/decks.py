@@ -1,0 +1,80 @@
+"""Small qmc.input decks with their table files, shared by the fixture generator
+(make_reference_runs.py, which runs the reference program on them) and the statistical parity
+tests of the host driver (which run moribs-pimc_amd/pimc on the same files).
+
+  co2_100K     the reference's own example examples/CO2_100K_4_4 (one CO2 rotor, 100 K,
+               P = Prot = 4) with fewer blocks; its .rot file is the fixture CO2_T100t4.rot
+  he4_co2      4 He + CO2 at 0.37 K, P = Prot = 64, lattice start; LJ He-He, Morse-Legendre
+               CO2-He and the linden CO2 density, all written in the reference's formats
+"""
+import os
+import shutil
+
+import numpy as np
+
+import pimc_configs as pc
+import pimc_tables as pt
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+DECKS = {
+    "co2_100K": """MASTERDIR        ./
+OUTPUTDIR        ./
+FILENAMEPREFIX   CO2_monomer
+DIMENSION     3
+DENSITY       0.0450   #  Angstrom^-3  Used to define the BoxLength
+TEMPERATURE   100       #  Kelvin
+MOLECULE     CO2    1   BOLTZMANN     0.18      1         CO2_fake  PRIMITIVE    #
+ROTATION   CO2   0.15  4
+ROTDENSI 0  -1 0.0 0.6666525 0.1769383 0.2306476 1
+NUMBEROFSLICES      4       #  number of time slices
+NUMBEROFPASSES     3000         #  number of passes per block
+NUMBEROFBLOCKS     {blocks}  5   #  number of blocks and number of equil blocks
+MCSKIP_RATIO     3000
+MCSKIP_TOTAL     3000
+MCSKIP_AVERG     4
+""",
+    "he4_co2": """MASTERDIR        ./
+OUTPUTDIR        ./
+FILENAMEPREFIX   he4co2
+DIMENSION     3
+DENSITY       0.02
+TEMPERATURE   0.37
+ATOM       He4  4    BOLTZMANN     0.30     3         He      PRIMITIVE
+MOLECULE   CO2  1    BOLTZMANN     0.08     3         CO2He   PRIMITIVE
+ROTATION   CO2   0.3  64
+NUMBEROFSLICES     64
+NUMBEROFPASSES     200
+NUMBEROFBLOCKS     {blocks}  10
+MCSKIP_RATIO     12800
+MCSKIP_TOTAL     12800
+MCSKIP_AVERG     64
+""",
+}
+BLOCKS = {"co2_100K": 205, "he4_co2": 1010}
+
+
+def materialise(name, d, blocks=None):
+    """Writes the deck and its table files into directory d."""
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "qmc.input"), "w") as f:
+        f.write(DECKS[name].format(blocks=blocks or BLOCKS[name]))
+    if name == "co2_100K":
+        shutil.copy(os.path.join(HERE, "CO2_T100t4.rot"), d)
+        with open(os.path.join(d, "CO2_fake.pot"), "w") as f:  # the example's empty 2D table
+            f.write("0 0\n0 0\n")
+    else:
+        r, v = pt.lj_table()
+        pt.write_pot1d(os.path.join(d, "He.pot"), r, v)
+        rg, cg, V = pt.morse_legendre_2d()
+        pt.write_pot2d(os.path.join(d, "CO2He.pot"), rg, cg, V)
+        c, rho, er, es = pc.rot_table(0.37, 64)
+        pt.write_rot(os.path.join(d, pt.rot_filename("CO2", 0.37, 64)), c, rho, er, es)
+
+
+ENG_COLUMNS = ("kin", "pot", "kin+pot", "rot", "rotsq", "total", "Cv", "Cv_trans", "Cv_rot")
+
+
+def read_eng(path):
+    """Block lines of a .eng file: [n_blocks, 10] (block number, then ENG_COLUMNS)."""
+    return np.loadtxt(path, ndmin=2)

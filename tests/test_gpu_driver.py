@@ -1,0 +1,88 @@
+"""The host driver on the GPU against the reference program, statistically.
+
+tests/golden/reference_runs.json holds the .eng block lines of the reference program (compiled
+from the upstream sources) on the decks of tests/golden/decks.py.  The driver runs the same
+deck files with several independent chains (default production path: segment order, FP32 pair
+terms); the pooled block means of K, V, E_rot and E must agree with the reference's within two
+combined standard errors of the mean.  Standard errors are blocking-corrected: the largest
+over bins of 1, 2, 4 and 8 consecutive blocks.  The output files keep the reference's layouts."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from golden import decks
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "moribs-pimc_amd", "pimc")
+with open(os.path.join(ROOT, "tests", "golden", "reference_runs.json")) as f:
+    REF = json.load(f)
+
+GPU_BLOCKS = {"co2_100K": 45, "he4_co2": 130}
+CHAINS = 8
+PREFIX = {"co2_100K": "CO2_monomer", "he4_co2": "he4co2"}
+
+
+def mean_se(chains):
+    """Mean and blocking-corrected standard error of block values from independent chains."""
+    x = np.concatenate(chains)
+    se = 0.0
+    for b in (1, 2, 4, 8):
+        bins = np.concatenate([c[: len(c) // b * b].reshape(-1, b).mean(axis=1) for c in chains])
+        if len(bins) >= 8:
+            se = max(se, bins.std(ddof=1) / np.sqrt(len(bins)))
+    return x.mean(), se
+
+
+@pytest.mark.parametrize("name", ["co2_100K", "he4_co2"])
+def test_driver_matches_reference_statistics(name, tmp_path):
+    d = str(tmp_path)
+    decks.materialise(name, d, blocks=GPU_BLOCKS[name])
+    r = subprocess.run([DRIVER, "--replicas", str(CHAINS)], cwd=d, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    neq = 5 if name == "co2_100K" else 10
+    engs = []
+    for c in range(CHAINS):
+        stem = os.path.join(d, PREFIX[name] + (f"_r{c}" if c else ""))
+        e = decks.read_eng(stem + ".eng")
+        assert e.shape == (GPU_BLOCKS[name] - neq, 10)
+        assert np.array_equal(e[:, 0], np.arange(neq + 1, GPU_BLOCKS[name] + 1))
+        engs.append(e)
+    ref = np.array(REF[name]["eng"])
+    cols = REF[name]["columns"]
+    report = []
+    for col in ("kin", "pot", "rot", "total"):
+        j = cols.index(col)
+        mg, sg = mean_se([e[:, j] for e in engs])
+        mr, sr = mean_se([ref[:, j]])
+        report.append((col, mg, sg, mr, sr))
+        if sr == 0 and sg == 0:
+            assert mg == mr == 0.0
+            continue
+        assert abs(mg - mr) <= 2.0 * np.hypot(sg, sr), report
+    print(name, report)
+
+
+def test_driver_output_layouts(tmp_path):
+    """.eng: block number and nine columns per measured block; _sum.eng: one line per
+    MCSKIP_TOTAL steps; .xyz: natoms*P bead lines; the accept-ratio line on stdout."""
+    d = str(tmp_path)
+    decks.materialise("he4_co2", d, blocks=12)
+    r = subprocess.run([DRIVER], cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    eng = decks.read_eng(os.path.join(d, "he4co2.eng"))
+    assert eng.shape == (2, 10)
+    assert np.allclose(eng[:, 3], eng[:, 1] + eng[:, 2], rtol=1e-5)          # kin + pot
+    assert np.allclose(eng[:, 6], eng[:, 1] + eng[:, 2] + eng[:, 4], rtol=1e-5)  # + rot
+    sums = np.loadtxt(os.path.join(d, "he4co2_sum.eng"), ndmin=2)
+    assert sums.shape == (2, 10) and np.array_equal(sums[:, 0], [1.0, 2.0])  # 12800 steps a block
+    assert np.allclose(sums[-1, 1:7], eng[:, 1:7].mean(axis=0), rtol=1e-5)
+    with open(os.path.join(d, "he4co2.xyz")) as f:
+        lines = f.read().splitlines()
+    assert int(lines[0]) == 5 * 64 and len(lines) == 2 + 5 * 64
+    assert lines[2].split()[0] == "He41" and lines[-1].split()[0] == "CO21"
+    ratio = [ln for ln in r.stdout.splitlines() if ln.startswith("BLOCK:")]
+    assert len(ratio) == 12 and "Rot:" in ratio[0] and "He4" in ratio[0]
