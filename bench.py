@@ -192,6 +192,83 @@ WORKLOAD_TAG = {"c2": "c2_256replicas", "c1": "c1_64he", "c0": "c0_8he", "c3": "
 
 
 # ----------------------------------------------------------------------------- GPU
+def blocked_se(x):
+    """Standard error of the mean of a correlated series: the plateau (largest value) of the
+    blocking estimate over block lengths 1, 2, 4, ... with at least 16 blocks."""
+    x = np.asarray(x, np.float64)
+    best, b = 0.0, 1
+    while len(x) // b >= 16:
+        m = x[: len(x) // b * b].reshape(-1, b).mean(axis=1)
+        best = max(best, m.std(ddof=1) / math.sqrt(len(m)))
+        b *= 2
+    return best
+
+
+def da_comparison(passes, rows_wanted=1_000_000, equil=100):
+    """Surrogate delayed acceptance against plain moves on the asymmetric-top workload: the
+    network is trained on exact pair energies of the run's own configurations (rows collected
+    pass by pass after equilibration), frozen, and both chains then run `passes` passes from the
+    same configuration with one E = K + V + E_rot sample per pass.  Efficiency is 1/(time *
+    sigma^2) with sigma the blocked standard error of <E>; the ratio compares the two at equal
+    sigma(<E>)."""
+    wl = pc.nl_workload("c3")
+    bm_pass = float(pc.bead_moves_per_pass(wl["replicas"][0]))
+    ctx = pc.load_into_context(ph.Context(1), wl)
+    ctx.run(equil)
+    xs, vs, n = [], [], 0
+    while n < rows_wanted:
+        ctx.run(1)
+        x, v = ctx.surrogate_collect(0)
+        xs.append(x)
+        vs.append(v)
+        n += len(v)
+    X, V = np.concatenate(xs), np.concatenate(vs)
+    hold = np.arange(len(V)) % 20 == 0
+    t0 = time.perf_counter()
+    loss = ctx.surrogate_train(X[~hold], V[~hold], epochs=8, batch=4096, lr=2e-3, seed=1)
+    t_train = time.perf_counter() - t0
+    theta, norm = ctx.surrogate_get()
+    rms = float(np.sqrt(np.mean((ctx.surrogate_eval(X[hold]) - V[hold]) ** 2)))
+    coords, angles, _ = ctx.get_state(0)
+
+    def chain(da):
+        c = ph.Context(1)
+        if da:
+            c.set_option("delayed_acceptance", 1)
+        pc.load_into_context(c, wl)
+        c.set_state(0, coords, angles)
+        if da:
+            c.surrogate_set(theta, norm)
+        c.run(5)
+        c.synchronize()
+        c.clear_samples()
+        t0 = time.perf_counter()
+        c.run(passes, sample_every=1)
+        c.synchronize()
+        dt = time.perf_counter() - t0
+        smp = c.samples(0)
+        e = smp[:, 0] + smp[:, 1] + smp[:, 2]
+        out = {"seconds": round(dt, 3), "bead_moves_per_s": round(passes * bm_pass / dt, 1),
+               "mean_E": round(float(e.mean()), 4), "sigma_E": round(blocked_se(e), 5)}
+        if da:
+            cnt = c.da_counters(0)
+            out.update(stage1_pass=round(cnt[1] / cnt[0], 4), stage2_accept=round(cnt[2] / max(cnt[1], 1), 4),
+                       accept=round(cnt[2] / cnt[0], 4))
+        tot, acc = c.counters(0)
+        out["rotor_accept"] = round(float(acc[-1][2] / tot[-1][2]), 4)
+        return out
+
+    plain, da = chain(False), chain(True)
+    eff = (plain["seconds"] * plain["sigma_E"] ** 2) / (da["seconds"] * da["sigma_E"] ** 2)
+    return {"workload": DESC["c3"].replace("plain moves", "delayed acceptance"), "passes": passes,
+            "training": {"rows": int((~hold).sum()), "epochs": len(loss), "seconds": round(t_train, 2),
+                         "loss_first_last": [round(float(loss[0]), 5), round(float(loss[-1]), 5)],
+                         "holdout_rms_K": round(rms, 4), "energy_spread_K": round(float(np.std(V)), 3)},
+            "plain": plain, "da": da,
+            "da_vs_plain_at_equal_sigma_E": round(eff, 4),
+            "E_difference_in_sigma": round(abs(plain["mean_E"] - da["mean_E"]) / math.hypot(plain["sigma_E"], da["sigma_E"]), 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +276,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--da", type=int, default=0, metavar="PASSES",
+                    help="compare surrogate delayed acceptance with plain moves on the asymmetric-top workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; replicas are independent chains, so N GPUs "
@@ -206,6 +285,12 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.da:
+        r = da_comparison(args.da)
+        print(json.dumps({"metric": METRIC, "value": r["da"]["bead_moves_per_s"], "unit": "bead-moves/s",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
+                          "config": {"workload": r["workload"]}, "surrogate_da": r}))
+        return
     nonlinear = args.workload in ("c3", "c4")
     wl = pc.nl_workload(args.workload) if nonlinear else pc.workload(args.workload)
     ctx = pc.load_into_context(ph.Context(len(wl["replicas"]), flags=ph.FLAG_PROFILE), wl)

@@ -96,7 +96,11 @@ int  pimc_commit(pimc_ctx* ctx);   /* allocates device state after all replicas 
    OpenMP loops have no tunable shape):
      "rotor_part_slices"  most rotor slices per workgroup of the rotor sweep (even, >= 2);
                           0 (default) sizes parts from the LDS budget.  Results do not
-                          depend on it: every draw is addressed by (sweep, slice). */
+                          depend on it: every draw is addressed by (sweep, slice).
+     "delayed_acceptance" 1: the non-linear rotor move (MCRot3Dstep mc_piqmc.cc:938-1199) runs
+                          as a two-stage test with the surrogate network as the first stage
+                          (see the surrogate section below); needs one non-linear rotor among
+                          atoms with a 3D potential, and a surrogate loaded before pimc_run. */
 int  pimc_set_option(pimc_ctx* ctx, const char* name, long value);
 /* coords/angles [3][natoms*P] like MCCoords/MCAngles; cosines follow from angles
    (mc_main.cc:186-197).  angles may be NULL (phi=0, cos(theta)=1, chi=0, mc_setup.cc:476-489).
@@ -135,6 +139,30 @@ int  pimc_kernel_time(pimc_ctx* ctx, const char* family, double* ms, long* launc
 int  pimc_synchronize(pimc_ctx* ctx);
 /* pair evaluations performed by one kernel family since creation */
 int  pimc_family_evals(pimc_ctx* ctx, const char* family, double* evals);
+
+/* ---- surrogate network and delayed acceptance ----
+   No counterpart in the reference: its rotor moves always evaluate PotRotE3D
+   (mc_piqmc.cc:2046-2151) in full.  The network (4 -> 128 -> 128 -> 128 -> 1, SiLU) maps the
+   body-frame geometry of an atom-rotor pair (R, ux, uy, uz) to the pair energy in Kelvin.
+   theta holds 33793 floats: W1[128][4] b1[128] W2[128][128] b2[128] W3[128][128] b3[128]
+   w4[128] b4; norm holds xm[4] xs[4] ym ys (input (x - xm)/xs, energy ys*net + ym). */
+/* training rows from one replica's current configuration: every (rotor slice, translational
+   slice, atom partner) pair, x[n][4] features and v[n] exact energies; returns n */
+int  pimc_surrogate_collect(pimc_ctx* ctx, int replica, float* x, float* v, int max_rows);
+/* normalises the rows, initialises the weights from seed and trains with Adam on minibatches
+   of `batch` rows for `epochs` epochs (learning rate decaying tenfold); loss[epochs] receives
+   the mean normalised loss of each epoch; the result becomes the sampler's surrogate */
+int  pimc_surrogate_train(pimc_ctx* ctx, const float* x, const float* v, int n, int epochs, int batch,
+                          double lr, unsigned seed, double* loss);
+/* loss (1/2n) sum (net(x) - y)^2 and its gradient (theta layout) of the raw network at theta */
+int  pimc_surrogate_grad(pimc_ctx* ctx, const float* theta, const float* x, const float* y, int n,
+                         float* grad, double* loss);
+int  pimc_surrogate_set(pimc_ctx* ctx, const float* theta, const float* norm);
+int  pimc_surrogate_get(pimc_ctx* ctx, float* theta, float* norm);
+/* the sampler's evaluation (BF16 matrix cores) on raw features: out[n] energies in Kelvin */
+int  pimc_surrogate_eval(pimc_ctx* ctx, const float* x, int n, float* out);
+/* out[4]: delayed-acceptance proposals, stage-1 passes, accepted moves, network evaluations */
+int  pimc_get_da_counters(pimc_ctx* ctx, int replica, double* out);
 
 #ifdef __cplusplus
 }

@@ -138,6 +138,12 @@ struct pimc_ctx {
     unsigned* d_flags = nullptr;  // half-sweeps completed per part
     int nparts = 0, nflags = 0;
     int opt_rotor_part_slices = 0;  // most rotor slices per part, 0: from the LDS budget
+    // surrogate network of the delayed-acceptance rotor move
+    bool opt_da = false;
+    MlpPacked* d_mlp = nullptr;     // the sampler's packed copy
+    bool mlp_ready = false;
+    std::vector<float> theta, norm;  // host copies: weights, then xm[4] xs[4] ym ys
+    size_t lds_rot3_da = 0;
     long long nbeads = 0, nrotb = 0, nhist = 0;
     hipGraphExec_t graph_exec = nullptr;
     bool graph_ok = false;
@@ -225,7 +231,7 @@ void pimc_destroy(pimc_ctx* c) {
     void* ptrs[] = {c->d_rep, c->d_mut, c->d_beads, c->d_rotb, c->d_hist, c->d_samples, c->d_k1f, c->d_k1d,
                     c->d_t1, c->d_v2f, c->d_v2d, c->d_g2d, c->d_g2f, c->d_t2, c->d_rk, c->d_re, c->d_tr,
                     c->d_work, c->d_all, c->d_rotreps, c->d_est, c->d_rows, c->d_rowbuf, c->d_parts, c->d_flags,
-                    c->d_r3, c->d_v3d, c->d_v3f, c->d_t3, c->d_frames, c->d_work1, c->d_all0, c->d_all1, c->d_rot3, c->d_rows1};
+                    c->d_r3, c->d_v3d, c->d_v3f, c->d_t3, c->d_frames, c->d_work1, c->d_all0, c->d_all1, c->d_rot3, c->d_rows1, c->d_mlp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -407,6 +413,11 @@ int pimc_set_option(pimc_ctx* c, const char* name, long value) {
         c->opt_rotor_part_slices = (int)value;
         return 0;
     }
+    if (std::strcmp(name, "delayed_acceptance") == 0) {
+        if (value != 0 && value != 1) return fail(c, "delayed_acceptance is 0 or 1");
+        c->opt_da = value == 1;
+        return 0;
+    }
     return fail(c, std::string("unknown option ") + name);
 }
 
@@ -486,10 +497,13 @@ int pimc_commit(pimc_ctx* c) {
             if (d.levels[t] < 1 || d.levels[t] > 8) return fail(c, "bisection levels must be 1..8");
         if (!c->ref_order && d.P % S) return fail(c, "segment order needs P divisible by the segment size");
         // table checks
+        int n_plain = 0;  // structureless atoms, the partners of the rotor-atom potentials
+        for (int t = 0; t < d.ntypes; ++t) n_plain += d.molecule[t] == 0 ? d.numb[t] : 0;
         for (int t = 0; t < d.ntypes; ++t) {
             if (d.molecule[t] == 0 && (d.pot1d[t] < 0 || d.pot1d[t] >= (int)c->t1.size()))
                 return fail(c, "atom type without a 1D potential table");
-            if (d.molecule[t] == 1 && (d.pot2d[t] < 0 || d.pot2d[t] >= (int)c->t2.size()))
+            // a lone rotor has no rotor-atom pairs and needs no 2D table
+            if (d.molecule[t] == 1 && (d.pot2d[t] >= (int)c->t2.size() || (n_plain > 0 && d.pot2d[t] < 0)))
                 return fail(c, "linear rotor without a 2D potential table");
         }
         if (p.fam == 1) {
@@ -497,6 +511,8 @@ int pimc_commit(pimc_ctx* c) {
                 return fail(c, "non-linear rotor without a 3D potential table");
             if (p.rotkind == 2 && (p.rot3d_id < 0 || p.rot3d_id >= c->nr3))
                 return fail(c, "rotating non-linear rotor without rho/E/E^2 tables");
+            if (p.rotkind == 2 && c->opt_da && (p.nrot != 1 || p.natoms < 2 || p.pot3d_id < 0))
+                return fail(c, "delayed acceptance needs one non-linear rotor among atoms with a 3D potential");
             if (p.rotkind == 2) {
                 c->bead_moves_per_pass += (double)p.P * p.Prot * p.nrot;
                 rot3.push_back(r);
@@ -753,6 +769,7 @@ int pimc_commit(pimc_ctx* c) {
         };
         c->rot3_block = lds_for(512) <= 64 * 1024 ? 512 : 256;
         c->lds_rot3 = lds_for(c->rot3_block);
+        c->lds_rot3_da = (size_t)((protmax + 1) & ~1) * sizeof(double) + 4 * sizeof(Frame);
         if (!rot3.empty() && c->lds_rot3 > 160 * 1024) return fail(c, "non-linear rotor chain exceeds LDS");
     }
     // bisection block: enough lanes for the widest level of the largest window
@@ -902,7 +919,10 @@ void launch_rotor(pimc_ctx* c, int u) {
     if (c->nparts)
         hipLaunchKernelGGL(k_rotor<R>, dim3(c->nparts), dim3(256), c->lds_rotor, c->stream, c->d_rep, c->d_mut,
                            c->d_beads, c->d_rotb, c->tables(), c->tables3(), c->d_parts, c->d_flags, run);
-    if (c->nrot3 && c->rot3_block == 512)
+    if (c->nrot3 && c->opt_da)
+        hipLaunchKernelGGL(k_rotor3d_da<R>, dim3(c->nrot3), dim3(256), c->lds_rot3_da, c->stream, c->d_rep, c->d_mut,
+                           c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rot3, run, c->d_mlp);
+    else if (c->nrot3 && c->rot3_block == 512)
         hipLaunchKernelGGL((k_rotor3d<R, 512>), dim3(c->nrot3), dim3(512), c->lds_rot3, c->stream, c->d_rep,
                            c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rot3, run);
     else if (c->nrot3)
@@ -969,6 +989,8 @@ extern "C" {
 
 int pimc_run(pimc_ctx* c, int n_passes, int sample_every) {
     if (!c->committed) return fail(c, "pimc_commit first");
+    if (c->opt_da && c->nrot3 && !c->mlp_ready)
+        return fail(c, "delayed acceptance needs a surrogate: pimc_surrogate_train or pimc_surrogate_set first");
     for (int p = 0; p < n_passes; ++p) {
         bool use_graph = !c->profile;
         if (use_graph && !c->graph_exec) {
@@ -1124,6 +1146,294 @@ int pimc_kernel_time(pimc_ctx* c, const char* family, double* ms, long* launches
     }
     *ms = it->second.ms;
     *launches = it->second.n;
+    return 0;
+}
+
+// ------------------------------------------------------------------------------ surrogate
+namespace {
+
+struct MlpBuffers {
+    float *x = nullptr, *y = nullptr, *Z1 = nullptr, *H1 = nullptr, *Z2 = nullptr, *H2 = nullptr, *Z3 = nullptr,
+          *H3 = nullptr, *out = nullptr, *dout = nullptr, *dA = nullptr, *dB = nullptr;
+    double* loss = nullptr;
+    int cap = 0;
+    bool alloc(int B) {
+        cap = B;
+        const size_t h = (size_t)B * MLP_H;
+        x = dalloc<float>((size_t)B * MLP_IN);
+        y = dalloc<float>(B);
+        Z1 = dalloc<float>(h); H1 = dalloc<float>(h);
+        Z2 = dalloc<float>(h); H2 = dalloc<float>(h);
+        Z3 = dalloc<float>(h); H3 = dalloc<float>(h);
+        out = dalloc<float>(B); dout = dalloc<float>(B);
+        dA = dalloc<float>(h); dB = dalloc<float>(h);
+        loss = dalloc<double>(1);
+        return x && y && Z1 && H1 && Z2 && H2 && Z3 && H3 && out && dout && dA && dB && loss;
+    }
+    ~MlpBuffers() {
+        void* p[] = {x, y, Z1, H1, Z2, H2, Z3, H3, out, dout, dA, dB, loss};
+        for (void* q : p)
+            if (q) (void)hipFree(q);
+    }
+};
+
+void mlp_gemm(hipStream_t s, int M, int N, int K, const float* A, int lda, int ta, const float* B, int ldb, int tb,
+              float* C, int ldc) {
+    hipLaunchKernelGGL(k_mlp_gemm, dim3((N + 31) / 32, (M + 31) / 32), dim3(64), 0, s, M, N, K, A, lda, ta, B, ldb,
+                       tb, C, ldc, 0);
+}
+
+void mlp_bias(hipStream_t s, float* Z, float* H, const float* b, int rows, int cols, int act) {
+    const long long n = (long long)rows * cols;
+    hipLaunchKernelGGL(k_mlp_bias_act, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, Z, H, b, rows, cols, act);
+}
+
+void mlp_back(hipStream_t s, float* dH, const float* Z, int rows) {
+    const long long n = (long long)rows * MLP_H;
+    hipLaunchKernelGGL(k_mlp_act_back, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dH, Z, n);
+}
+
+// Loss and gradient of the batch in m.x / m.y (B rows, normalised) at weights th; g gets the
+// gradient, m.loss accumulates the loss.
+void mlp_forward_backward(hipStream_t s, const float* th, float* g, MlpBuffers& m, int B) {
+    // forward: Z = H_prev W^T + b, H = SiLU(Z)
+    mlp_gemm(s, B, MLP_H, MLP_IN, m.x, MLP_IN, 0, th + MLP_OFF_W1, MLP_IN, 1, m.Z1, MLP_H);
+    mlp_bias(s, m.Z1, m.H1, th + MLP_OFF_B1, B, MLP_H, 1);
+    mlp_gemm(s, B, MLP_H, MLP_H, m.H1, MLP_H, 0, th + MLP_OFF_W2, MLP_H, 1, m.Z2, MLP_H);
+    mlp_bias(s, m.Z2, m.H2, th + MLP_OFF_B2, B, MLP_H, 1);
+    mlp_gemm(s, B, MLP_H, MLP_H, m.H2, MLP_H, 0, th + MLP_OFF_W3, MLP_H, 1, m.Z3, MLP_H);
+    mlp_bias(s, m.Z3, m.H3, th + MLP_OFF_B3, B, MLP_H, 1);
+    mlp_gemm(s, B, 1, MLP_H, m.H3, MLP_H, 0, th + MLP_OFF_W4, 1, 0, m.out, 1);
+    mlp_bias(s, m.out, m.out, th + MLP_OFF_B4, B, 1, 0);
+    hipLaunchKernelGGL(k_mlp_mse, dim3((B + 255) / 256), dim3(256), 0, s, m.out, m.y, B, m.dout, m.loss);
+    // backward
+    mlp_gemm(s, MLP_H, 1, B, m.H3, MLP_H, 1, m.dout, 1, 0, g + MLP_OFF_W4, 1);
+    hipLaunchKernelGGL(k_mlp_colsum, dim3(1), dim3(256), 0, s, m.dout, B, 1, g + MLP_OFF_B4);
+    mlp_gemm(s, B, MLP_H, 1, m.dout, 1, 0, th + MLP_OFF_W4, MLP_H, 0, m.dA, MLP_H);  // dH3
+    mlp_back(s, m.dA, m.Z3, B);                                                      // dZ3
+    mlp_gemm(s, MLP_H, MLP_H, B, m.dA, MLP_H, 1, m.H2, MLP_H, 0, g + MLP_OFF_W3, MLP_H);
+    hipLaunchKernelGGL(k_mlp_colsum, dim3(MLP_H), dim3(256), 0, s, m.dA, B, MLP_H, g + MLP_OFF_B3);
+    mlp_gemm(s, B, MLP_H, MLP_H, m.dA, MLP_H, 0, th + MLP_OFF_W3, MLP_H, 0, m.dB, MLP_H);  // dH2
+    mlp_back(s, m.dB, m.Z2, B);
+    mlp_gemm(s, MLP_H, MLP_H, B, m.dB, MLP_H, 1, m.H1, MLP_H, 0, g + MLP_OFF_W2, MLP_H);
+    hipLaunchKernelGGL(k_mlp_colsum, dim3(MLP_H), dim3(256), 0, s, m.dB, B, MLP_H, g + MLP_OFF_B2);
+    mlp_gemm(s, B, MLP_H, MLP_H, m.dB, MLP_H, 0, th + MLP_OFF_W2, MLP_H, 0, m.dA, MLP_H);  // dH1
+    mlp_back(s, m.dA, m.Z1, B);
+    mlp_gemm(s, MLP_H, MLP_IN, B, m.dA, MLP_H, 1, m.x, MLP_IN, 0, g + MLP_OFF_W1, MLP_IN);
+    hipLaunchKernelGGL(k_mlp_colsum, dim3(MLP_H), dim3(256), 0, s, m.dA, B, MLP_H, g + MLP_OFF_B1);
+}
+
+int mlp_upload(pimc_ctx* c) {
+    if (!c->d_mlp) c->d_mlp = dalloc<MlpPacked>(1);
+    float *th = dalloc<float>(MLP_NW), *nm = dalloc<float>(2 * MLP_IN + 2);
+    if (!c->d_mlp || !th || !nm) {
+        if (th) (void)hipFree(th);
+        if (nm) (void)hipFree(nm);
+        return fail(c, "surrogate: out of device memory");
+    }
+    hipMemcpy(th, c->theta.data(), sizeof(float) * MLP_NW, hipMemcpyHostToDevice);
+    hipMemcpy(nm, c->norm.data(), sizeof(float) * (2 * MLP_IN + 2), hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_mlp_pack, dim3((MLP_H * MLP_H + 255) / 256), dim3(256), 0, c->stream, th, nm, c->d_mlp);
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    (void)hipFree(th);
+    (void)hipFree(nm);
+    if (e != hipSuccess) return fail(c, std::string("surrogate pack: ") + hipGetErrorString(e));
+    c->mlp_ready = true;
+    return 0;
+}
+
+}  // namespace
+
+int pimc_surrogate_collect(pimc_ctx* c, int r, float* x, float* v, int max_rows) {
+    if (!c->committed) return fail(c, "pimc_commit first");
+    if (r < 0 || r >= c->nrep) return fail(c, "replica index out of range");
+    const Rep& p = c->rep[r];
+    if (p.fam != 1 || p.nrot != 1 || p.natoms < 2 || p.pot3d_id < 0)
+        return fail(c, "surrogate rows need one non-linear rotor among atoms with a 3D potential");
+    const long long n = (long long)p.Prot * p.RotRatio * (p.natoms - 1);
+    if (!x || !v || max_rows < n) return fail(c, "surrogate_collect: buffer smaller than " + std::to_string(n) + " rows");
+    float* dx = dalloc<float>((size_t)n * MLP_IN);
+    float* dv = dalloc<float>((size_t)n);
+    if (!dx || !dv) {
+        if (dx) (void)hipFree(dx);
+        if (dv) (void)hipFree(dv);
+        return fail(c, "surrogate_collect: out of device memory");
+    }
+    hipLaunchKernelGGL(k_surrogate_collect, dim3((unsigned)std::min<long long>((n + 255) / 256, 4096)), dim3(256), 0,
+                       c->stream, c->d_rep, c->d_beads, c->d_frames, c->tables3(), r, dx, dv, (int)n);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(x, dx, sizeof(float) * MLP_IN * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(v, dv, sizeof(float) * n, hipMemcpyDeviceToHost);
+    (void)hipFree(dx);
+    (void)hipFree(dv);
+    if (e != hipSuccess) return fail(c, std::string("surrogate_collect: ") + hipGetErrorString(e));
+    return (int)n;
+}
+
+int pimc_surrogate_grad(pimc_ctx* c, const float* theta, const float* x, const float* y, int n, float* grad,
+                        double* loss) {
+    if (!theta || !x || !y || !grad || !loss || n < 1 || n > (1 << 20)) return fail(c, "surrogate_grad: bad arguments");
+    MlpBuffers m;
+    float *th = dalloc<float>(MLP_NW), *g = dalloc<float>(MLP_NW);
+    int rc = 0;
+    if (!m.alloc(n) || !th || !g) rc = fail(c, "surrogate_grad: out of device memory");
+    if (rc == 0) {
+        hipMemcpy(th, theta, sizeof(float) * MLP_NW, hipMemcpyHostToDevice);
+        hipMemcpy(m.x, x, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice);
+        hipMemcpy(m.y, y, sizeof(float) * n, hipMemcpyHostToDevice);
+        hipMemset(m.loss, 0, sizeof(double));
+        mlp_forward_backward(c->stream, th, g, m, n);
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(grad, g, sizeof(float) * MLP_NW, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(loss, m.loss, sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(c, std::string("surrogate_grad: ") + hipGetErrorString(e));
+    }
+    if (th) (void)hipFree(th);
+    if (g) (void)hipFree(g);
+    return rc;
+}
+
+int pimc_surrogate_train(pimc_ctx* c, const float* x, const float* v, int n, int epochs, int batch, double lr,
+                         unsigned seed, double* loss) {
+    if (!x || !v || n < 64 || epochs < 1 || batch < 32 || batch > n || !(lr > 0.0))
+        return fail(c, "surrogate_train: bad arguments (n >= 64, 32 <= batch <= n, epochs >= 1, lr > 0)");
+    // normalisation from the training set
+    std::vector<double> mean(MLP_IN + 1, 0.0), var(MLP_IN + 1, 0.0);
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < MLP_IN; ++j) mean[j] += x[(size_t)i * MLP_IN + j];
+        mean[MLP_IN] += v[i];
+    }
+    for (double& m : mean) m /= n;
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < MLP_IN; ++j) var[j] += (x[(size_t)i * MLP_IN + j] - mean[j]) * (x[(size_t)i * MLP_IN + j] - mean[j]);
+        var[MLP_IN] += (v[i] - mean[MLP_IN]) * (v[i] - mean[MLP_IN]);
+    }
+    std::vector<float> norm(2 * MLP_IN + 2);
+    for (int j = 0; j <= MLP_IN; ++j) {
+        double sd = std::sqrt(var[j] / n);
+        if (!(sd > 1e-12)) sd = 1.0;
+        if (j < MLP_IN) {
+            norm[j] = (float)mean[j];
+            norm[MLP_IN + j] = (float)sd;
+        } else {
+            norm[2 * MLP_IN] = (float)mean[j];
+            norm[2 * MLP_IN + 1] = (float)sd;
+        }
+    }
+    std::vector<float> xn((size_t)n * MLP_IN), yn(n);
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < MLP_IN; ++j) xn[(size_t)i * MLP_IN + j] = (x[(size_t)i * MLP_IN + j] - norm[j]) / norm[MLP_IN + j];
+        yn[i] = (v[i] - norm[2 * MLP_IN]) / norm[2 * MLP_IN + 1];
+    }
+    // Glorot-uniform weights, zero biases
+    std::vector<float> th(MLP_NW, 0.0f);
+    uint64_t st = 0x9E3779B97F4A7C15ull ^ seed;
+    auto unif = [&]() {  // splitmix64 -> [0, 1)
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto fill = [&](int off, int fan_out, int fan_in) {
+        const double a = std::sqrt(6.0 / (fan_in + fan_out));
+        for (int i = 0; i < fan_out * fan_in; ++i) th[off + i] = (float)(a * (2.0 * unif() - 1.0));
+    };
+    fill(MLP_OFF_W1, MLP_H, MLP_IN);
+    fill(MLP_OFF_W2, MLP_H, MLP_H);
+    fill(MLP_OFF_W3, MLP_H, MLP_H);
+    fill(MLP_OFF_W4, 1, MLP_H);
+    MlpBuffers m;
+    float *dX = dalloc<float>((size_t)n * MLP_IN), *dY = dalloc<float>(n), *dth = dalloc<float>(MLP_NW),
+          *dg = dalloc<float>(MLP_NW), *dm = dalloc<float>(MLP_NW), *dv = dalloc<float>(MLP_NW);
+    int* dperm = dalloc<int>(n);
+    int rc = 0;
+    if (!m.alloc(batch) || !dX || !dY || !dth || !dg || !dm || !dv || !dperm) rc = fail(c, "surrogate_train: out of device memory");
+    if (rc == 0) {
+        hipMemcpy(dX, xn.data(), sizeof(float) * xn.size(), hipMemcpyHostToDevice);
+        hipMemcpy(dY, yn.data(), sizeof(float) * n, hipMemcpyHostToDevice);
+        hipMemcpy(dth, th.data(), sizeof(float) * MLP_NW, hipMemcpyHostToDevice);
+        hipMemset(dm, 0, sizeof(float) * MLP_NW);
+        hipMemset(dv, 0, sizeof(float) * MLP_NW);
+        std::vector<int> perm(n);
+        for (int i = 0; i < n; ++i) perm[i] = i;
+        const int steps = n / batch;
+        long t = 0;
+        for (int ep = 0; ep < epochs && rc == 0; ++ep) {
+            for (int i = n - 1; i > 0; --i) std::swap(perm[i], perm[(int)(unif() * (i + 1))]);
+            hipMemcpy(dperm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice);
+            hipMemset(m.loss, 0, sizeof(double));
+            const float lr_ep = (float)(lr * std::pow(0.1, (double)ep / epochs));  // decays tenfold over the run
+            for (int s = 0; s < steps; ++s) {
+                hipLaunchKernelGGL(k_mlp_gather, dim3((batch + 255) / 256), dim3(256), 0, c->stream, dX, dY, dperm,
+                                   s * batch, batch, m.x, m.y);
+                mlp_forward_backward(c->stream, dth, dg, m, batch);
+                ++t;
+                const float c1 = (float)(1.0 - std::pow(0.9, (double)t)), c2 = (float)(1.0 - std::pow(0.999, (double)t));
+                hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NW + 255) / 256), dim3(256), 0, c->stream, dth, dg, dm, dv,
+                                   MLP_NW, lr_ep, c1, c2);
+            }
+            double l = 0.0;
+            hipError_t e = hipStreamSynchronize(c->stream);
+            if (e == hipSuccess) e = hipMemcpy(&l, m.loss, sizeof(double), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = fail(c, std::string("surrogate_train: ") + hipGetErrorString(e));
+            if (loss) loss[ep] = l / steps;
+        }
+        if (rc == 0 && hipMemcpy(th.data(), dth, sizeof(float) * MLP_NW, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(c, "surrogate_train: copy back");
+    }
+    void* ptrs[] = {dX, dY, dth, dg, dm, dv, dperm};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    if (rc) return rc;
+    c->theta = th;
+    c->norm = norm;
+    return mlp_upload(c);
+}
+
+int pimc_surrogate_set(pimc_ctx* c, const float* theta, const float* norm) {
+    if (!theta || !norm) return fail(c, "surrogate_set: null argument");
+    for (int j = 0; j < MLP_IN; ++j)
+        if (!(norm[MLP_IN + j] != 0.0f)) return fail(c, "surrogate_set: zero input scale");
+    c->theta.assign(theta, theta + MLP_NW);
+    c->norm.assign(norm, norm + 2 * MLP_IN + 2);
+    return mlp_upload(c);
+}
+
+int pimc_surrogate_get(pimc_ctx* c, float* theta, float* norm) {
+    if (!c->mlp_ready) return fail(c, "no surrogate loaded");
+    if (theta) std::memcpy(theta, c->theta.data(), sizeof(float) * MLP_NW);
+    if (norm) std::memcpy(norm, c->norm.data(), sizeof(float) * (2 * MLP_IN + 2));
+    return 0;
+}
+
+int pimc_surrogate_eval(pimc_ctx* c, const float* x, int n, float* out) {
+    if (!c->mlp_ready) return fail(c, "no surrogate loaded");
+    if (!x || !out || n < 1) return fail(c, "surrogate_eval: bad arguments");
+    float *dx = dalloc<float>((size_t)n * MLP_IN), *dy = dalloc<float>(n);
+    int rc = 0;
+    if (!dx || !dy) rc = fail(c, "surrogate_eval: out of device memory");
+    if (rc == 0) {
+        hipMemcpy(dx, x, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice);
+        const int waves = (n + 31) / 32;
+        hipLaunchKernelGGL(k_mlp_infer, dim3((waves + 3) / 4), dim3(256), 0, c->stream, c->d_mlp, dx, n, dy);
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(out, dy, sizeof(float) * n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(c, std::string("surrogate_eval: ") + hipGetErrorString(e));
+    }
+    if (dx) (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    return rc;
+}
+
+int pimc_get_da_counters(pimc_ctx* c, int r, double* out) {
+    if (!c->committed || r < 0 || r >= c->nrep || !out) return fail(c, "bad replica");
+    Mut m;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(&m, c->d_mut + r, sizeof(Mut), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; ++k) out[k] = (double)m.da[k];
+    out[3] = (double)m.surrogate_evals;
     return 0;
 }
 

@@ -50,6 +50,8 @@ struct Mut {
     unsigned long long total[MAXT][3];
     unsigned long long accep[MAXT][3];
     unsigned long long pair_evals[4];  // bisect, centroid, rotor, estimate
+    unsigned long long da[3];          // delayed acceptance: proposals, stage-1 passes, accepted
+    unsigned long long surrogate_evals;
 };
 
 struct RotBead {

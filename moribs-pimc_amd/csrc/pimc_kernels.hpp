@@ -18,10 +18,12 @@
 //   k_est_final  one workgroup per replica: rows summed in order (deterministic), GetRotEnergy
 //                (mc_estim.cc:939-987) and the Cv terms of MCGetAverage (mc_main.cc:551-616).
 //   k_pot_slice  batched PotEnergy(atom, pos, it) (mc_piqmc.cc:1796-1965).
+//   k_rotor3d_da the non-linear rotor sweep with a surrogate first stage (delayed acceptance).
 #pragma once
 #include <type_traits>
 #include "pimc_device.hpp"
 #include "pimc_nl.hpp"
+#include "pimc_mlp.hpp"
 
 namespace pimc {
 
@@ -794,6 +796,213 @@ __global__ __launch_bounds__(BLOCK) void k_rotor3d(const Rep* __restrict__ reps,
         atomicAdd(&mut[r].total[rtype][2], moves);
         atomicAdd(&mut[r].accep[rtype][2], s);
         atomicAdd(&mut[r].pair_evals[2], 2ull * moves * (unsigned long long)items);
+    }
+}
+
+// ------------------------------------------------------------------------- delayed acceptance
+// MCRotations3D (mc_piqmc.cc:692-779) for one non-linear rotor among atoms, with the one-stage
+// test of MCRot3Dstep (:1166-1178) split in two (delayed acceptance, Christen & Fox 2005).
+// Stage 1 accepts with min(1, rho_new/rho_old * exp(-tau (Vs_new - Vs_old))), Vs the surrogate
+// network summed over the slice's rotor-atom pairs; only proposals that pass are charged the
+// exact PotRotE3D sums, and stage 2 accepts them with
+// min(1, exp(-tau ((V_new - V_old) - (Vs_new - Vs_old)))).  The product of the two tests is
+// in detailed balance with the exact target for any surrogate, so the stationary distribution is
+// that of the one-stage move.  Stage 1 uses the one-stage test's uniform (rand4); stage 2 draws
+// a fifth uniform (counter word 3 = 2).  Proposals and density factors are those of k_rotor3d.
+// One wave per slice move: the RotRatio x (N-1) pair rows at the old and the proposed
+// orientation go through the network 32 at a time on the matrix cores, and the exact pairs of a
+// proposal that passes spread over the 64 lanes.
+// LDS: Lk [Prot] doubles, the link density cache of k_rotor3d, then one proposal frame per wave.
+template <typename R>
+__global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps, Mut* __restrict__ mut,
+                                                    const double4* __restrict__ beads, RotBead* __restrict__ rotb,
+                                                    Frame* __restrict__ frames, Tables T, Tables3 T3,
+                                                    const int* __restrict__ replist, Run run,
+                                                    const MlpPacked* __restrict__ W) {
+    extern __shared__ double smem[];
+    __shared__ unsigned long long red_a[4], red_p[4];
+    __shared__ int s_err;
+    const int r = replist[blockIdx.x];
+    const Rep& rp = reps[r];
+    if (rp.rotkind != 2 || rp.rot3d_id < 0) return;
+    const int Prot = rp.Prot, RR = rp.RotRatio, P = rp.P, N = rp.natoms;
+    int nsweep, j0, jstride;
+    if (run.ref_order) {
+        if (run.u >= P) return;
+        nsweep = 1; j0 = run.u; jstride = 0;
+    } else {
+        if (run.u >= rp.S) return;
+        nsweep = P / rp.S; j0 = run.u; jstride = rp.S;
+    }
+    const int np = N - 1;
+    const int items = RR * np;
+    const int gatom = rp.nl_atom;  // the rotor, last
+    const long long rb0 = rp.rot_off;
+    const double4* tab = T3.r3 + (long long)rp.rot3d_id * 3 * ROT3D_SIZE;
+    const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
+    const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
+    double* Lk = smem;
+    Frame* wf = reinterpret_cast<Frame*>(smem + ((Prot + 1) & ~1)) + wv;
+    const uint32_t pass = mut[r].pass;
+    const int rtype = rp.imtype;
+    const double twopi = 2.0 * PI_D;
+    const double ys = (double)W->ys;
+    const int nbatch = (items + 31) / 32;
+    if (threadIdx.x == 0) s_err = 0;
+    unsigned long long nacc = 0, npass = 0;
+    int err = 0;
+    for (int i = threadIdx.x; i < Prot; i += blockDim.x) {
+        double er, es;
+        if (rotden_frames(frames[rb0 + i], frames[rb0 + (i + 1 == Prot ? 0 : i + 1)], tab, Lk[i], er, es)) err = 1;
+    }
+    __syncthreads();
+    for (int sw = 0; sw < nsweep; ++sw) {
+        const uint32_t jsweep = (uint32_t)(j0 + sw * jstride);
+        for (int parity = 0; parity < 2; ++parity) {
+            const int nmoves = (Prot - parity + 1) / 2;
+            for (int mv = wv; mv < nmoves; mv += nw) {
+                const int k = 2 * mv + parity;
+                int k0 = k - 1, k2 = k + 1;
+                if (k0 < 0) k0 += Prot;
+                if (k2 >= Prot) k2 -= Prot;
+                const long long ik = rb0 + k;
+                const RotBead cur = rotb[ik];
+                const uint4 w0 = draw(run.seed, rp.key, pass, K_ROT3D, jsweep, (uint32_t)gatom, (uint32_t)k, 0);
+                const uint4 w1 = draw(run.seed, rp.key, pass, K_ROT3D, jsweep, (uint32_t)gatom, (uint32_t)k, 1);
+                const uint4 w2 = draw(run.seed, rp.key, pass, K_ROT3D, jsweep, (uint32_t)gatom, (uint32_t)k, 2);
+                const double rand1 = u53(w0.x, w0.y), rand2 = u53(w0.z, w0.w);
+                const double rand3 = u53(w1.x, w1.y), rand4 = u53(w1.z, w1.w);
+                const double rand5 = u53(w2.x, w2.y);
+                double cost = cur.cth + rp.rtstep * (rand1 - 0.5);
+                double phi = cur.phi + twopi * (rp.rtstep * (rand2 - 0.5));
+                double chi = cur.chi + twopi * (rp.rtstep * (rand3 - 0.5));
+                if (phi < 0.0) phi = twopi + phi;
+                if (chi < 0.0) chi = twopi + chi;
+                phi = fmod(phi, twopi);
+                chi = fmod(chi, twopi);
+                if (cost > 1.0) cost = 2.0 - cost;
+                if (cost < -1.0) cost = -2.0 - cost;
+                const Frame& fo = frames[ik];
+                if (l == 0) *wf = make_frame(phi, cost, chi);
+                __builtin_amdgcn_wave_barrier();
+                const Frame& fn = *wf;
+                // new density factors (t0,t1') on lane 0 and (t1',t2) on lane 1
+                double dl = 0.0;
+                if (l < 2) {
+                    double er, es;
+                    if (rotden_frames(l == 0 ? frames[rb0 + k0] : fn, l == 0 ? fn : frames[rb0 + k2], tab, dl, er, es))
+                        err = 1;
+                }
+                const double d2 = __shfl(dl, 0), d3 = __shfl(dl, 1);
+                double dold = Lk[k0] * Lk[k];
+                if (fabs(dold) < RZERO) dold = 0.0;
+                if (dold < 0.0) dold = fabs(dold);
+                double dnew = d2 * d3;
+                if (fabs(dnew) < RZERO) dnew = 0.0;
+                if (dnew < 0.0) dnew = fabs(dnew);
+                const double ratio = (dold > RZERO) ? dnew / dold : 1.0;
+                // stage 1: surrogate sums at the old and the proposed orientation
+                float so = 0.0f, sn = 0.0f;
+#pragma unroll 1
+                for (int t = 0; t < 2 * nbatch; ++t) {  // (old, new) x batches of 32 rows
+                    const int which = t & 1, row = (t >> 1) * 32 + (l & 31);
+                    float f4[MLP_IN] = {0.0f, 0.0f, 0.0f, 0.0f};
+                    if (row < items) {
+                        const int it = k * RR + row / np, jp = row % np;
+                        const int j = jp + (jp >= gatom);
+                        const double4 xa = bead(beads, rp, gatom, it), xj = bead(beads, rp, j, it);
+                        const Frame& f = which ? fn : fo;
+                        mlp_pair_features(f.m, f.zn, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, f4);
+                        for (int q = 0; q < MLP_IN; ++q) f4[q] = (f4[q] - W->xm[q]) * W->xinv[q];
+                    }
+                    // an opaque copy of the weight pointer keeps the weight loads inside the
+                    // loop (hoisted out they would hold ~260 registers of weights per lane)
+                    const MlpPacked* Wt = W;
+                    __asm__ volatile("" : "+s"(Wt));
+                    const float o = mlp_rows32(Wt, f4);
+                    if (l < 32 && row < items) {
+                        if (which) sn += o;
+                        else so += o;
+                    }
+                }
+                so = wave_sum(so);
+                sn = wave_sum(sn);
+                const double dvs = ys * ((double)sn - (double)so);
+                const double rd1 = ratio * exp(-rp.tau * dvs);
+                if (rd1 > 1.0 || rd1 > rand4) {  // the same on every lane
+                    double eo = 0.0, en = 0.0;
+                    for (int i = l; i < 2 * items; i += 64) {
+                        const bool isnew = i >= items;
+                        const int ii = isnew ? i - items : i;
+                        const int it = k * RR + ii / np, jp = ii % np;
+                        const int j = jp + (jp >= gatom);
+                        const double4 xa = bead(beads, rp, gatom, it), xj = bead(beads, rp, j, it);
+                        const double v = pair_nl<R>(l1, l1.k, l3, true, false, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z,
+                                                    isnew ? fn : fo, fo);
+                        if (isnew) en += v;
+                        else eo += v;
+                    }
+                    eo = wave_sum(eo);
+                    en = wave_sum(en);
+                    const double rd2 = exp(-rp.tau * ((en - eo) - dvs));
+                    if (l == 0) ++npass;
+                    if ((rd2 > 1.0 || rd2 > rand5) && l == 0) {
+                        ++nacc;
+                        const double sint = sqrt(1.0 - cost * cost);
+                        RotBead b = cur;
+                        b.cth = cost;
+                        b.phi = phi;
+                        b.chi = chi;
+                        b.nx = sint * cos(phi);
+                        b.ny = sint * sin(phi);
+                        b.nz = cost;
+                        rotb[ik] = b;
+                        frames[ik] = *wf;
+                        Lk[k0] = d2;
+                        Lk[k] = d3;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            __syncthreads();
+        }
+    }
+    if (err) s_err = 1;
+    const unsigned long long sa = block_sum(nacc, red_a);
+    const unsigned long long sp = block_sum(npass, red_p);
+    if (threadIdx.x == 0) {
+        if (s_err) atomicOr(&mut[r].err, 4);
+        const unsigned long long moves = (unsigned long long)nsweep * Prot;
+        atomicAdd(&mut[r].total[rtype][2], moves);
+        atomicAdd(&mut[r].accep[rtype][2], sa);
+        atomicAdd(&mut[r].da[0], moves);
+        atomicAdd(&mut[r].da[1], sp);
+        atomicAdd(&mut[r].da[2], sa);
+        atomicAdd(&mut[r].pair_evals[2], 2ull * sp * (unsigned long long)items);
+        atomicAdd(&mut[r].surrogate_evals, 2ull * moves * (unsigned long long)items);
+    }
+}
+
+// Training rows of the surrogate from the current configuration of replica r: each (rotor
+// slice k, translational slice it of k, atom partner j) gives the body-frame features of the
+// pair and its exact energy (vcord on the 3D table in FP64, PotRotE3D's atom term).
+__global__ void k_surrogate_collect(const Rep* __restrict__ reps, const double4* __restrict__ beads,
+                                    const Frame* __restrict__ frames, Tables3 T3, int r, float* __restrict__ X,
+                                    float* __restrict__ y, int nrows) {
+    const Rep& rp = reps[r];
+    const int np = rp.natoms - 1, RR = rp.RotRatio, gatom = rp.nl_atom;
+    const Lut3D<double> l3 = lut3d<double>(T3, rp.pot3d_id);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nrows; i += gridDim.x * blockDim.x) {
+        const int k = i / (RR * np), rem = i - k * (RR * np);
+        const int it = k * RR + rem / np, jp = rem % np;
+        const int j = jp + (jp >= gatom);
+        const double4 xa = bead(beads, rp, gatom, it), xj = bead(beads, rp, j, it);
+        const Frame& f = frames[rp.rot_off + k];
+        float fe[MLP_IN];
+        mlp_pair_features(f.m, f.zn, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, fe);
+        for (int q = 0; q < MLP_IN; ++q) X[(long long)i * MLP_IN + q] = fe[q];
+        y[i] = (float)vcord_v<double>(l3, f, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z);
     }
 }
 

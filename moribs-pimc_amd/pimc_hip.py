@@ -21,6 +21,12 @@ _i = ctypes.c_int
 _u = ctypes.c_uint
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
+_fp = ctypes.POINTER(ctypes.c_float)
+
+# surrogate network (moribs-pimc_amd/csrc/pimc_mlp.hpp): 4 -> 128 -> 128 -> 128 -> 1, SiLU
+MLP_IN, MLP_H = 4, 128
+MLP_NW = MLP_H * MLP_IN + MLP_H + 2 * (MLP_H * MLP_H + MLP_H) + MLP_H + 1
+MLP_NNORM = 2 * MLP_IN + 2
 
 
 class ReplicaDesc(ctypes.Structure):
@@ -70,6 +76,13 @@ SIGNATURES = {
     "pimc_kernel_time": (_i, [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_long)]),
     "pimc_synchronize": (_i, [ctypes.c_void_p]),
     "pimc_family_evals": (_i, [ctypes.c_void_p, ctypes.c_char_p, _dp]),
+    "pimc_surrogate_collect": (_i, [ctypes.c_void_p, _i, _fp, _fp, _i]),
+    "pimc_surrogate_train": (_i, [ctypes.c_void_p, _fp, _fp, _i, _i, _i, _d, _u, _dp]),
+    "pimc_surrogate_grad": (_i, [ctypes.c_void_p, _fp, _fp, _fp, _i, _fp, _dp]),
+    "pimc_surrogate_set": (_i, [ctypes.c_void_p, _fp, _fp]),
+    "pimc_surrogate_get": (_i, [ctypes.c_void_p, _fp, _fp]),
+    "pimc_surrogate_eval": (_i, [ctypes.c_void_p, _fp, _i, _fp]),
+    "pimc_get_da_counters": (_i, [ctypes.c_void_p, _i, _dp]),
 }
 
 
@@ -86,6 +99,14 @@ def load_library(path=LIB_PATH):
 
 def _dptr(a):
     return a.ctypes.data_as(_dp)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_fp)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
 
 
 def _f64(a):
@@ -248,3 +269,51 @@ class Context:
 
     def synchronize(self):
         return self._check(self.lib.pimc_synchronize(self.h))
+
+    # surrogate network and delayed acceptance
+    def surrogate_collect(self, r):
+        """(features [n,4], exact pair energies [n]) of replica r's current configuration."""
+        d = self.descs[r]
+        n = d.Prot * (d.P // d.Prot) * (sum(d.numb[: d.ntypes]) - 1)
+        x = np.zeros((n, MLP_IN), np.float32)
+        v = np.zeros(n, np.float32)
+        got = self._check(self.lib.pimc_surrogate_collect(self.h, int(r), _fptr(x), _fptr(v), n))
+        return x[:got], v[:got]
+
+    def surrogate_train(self, x, v, epochs=20, batch=4096, lr=2e-3, seed=1):
+        x, v = _f32(x).reshape(-1, MLP_IN), _f32(v)
+        loss = np.zeros(epochs)
+        self._check(self.lib.pimc_surrogate_train(self.h, _fptr(x), _fptr(v), len(v), int(epochs), int(batch),
+                                                  float(lr), int(seed), _dptr(loss)))
+        return loss
+
+    def surrogate_grad(self, theta, x, y):
+        """Loss and gradient of the raw network (no normalisation) on rows x, targets y."""
+        theta, x, y = _f32(theta), _f32(x).reshape(-1, MLP_IN), _f32(y)
+        g = np.zeros(MLP_NW, np.float32)
+        loss = ctypes.c_double()
+        self._check(self.lib.pimc_surrogate_grad(self.h, _fptr(theta), _fptr(x), _fptr(y), len(y), _fptr(g),
+                                                 ctypes.byref(loss)))
+        return loss.value, g
+
+    def surrogate_set(self, theta, norm):
+        theta, norm = _f32(theta), _f32(norm)
+        assert theta.size == MLP_NW and norm.size == MLP_NNORM
+        self._check(self.lib.pimc_surrogate_set(self.h, _fptr(theta), _fptr(norm)))
+
+    def surrogate_get(self):
+        theta, norm = np.zeros(MLP_NW, np.float32), np.zeros(MLP_NNORM, np.float32)
+        self._check(self.lib.pimc_surrogate_get(self.h, _fptr(theta), _fptr(norm)))
+        return theta, norm
+
+    def surrogate_eval(self, x):
+        x = _f32(x).reshape(-1, MLP_IN)
+        out = np.zeros(len(x), np.float32)
+        self._check(self.lib.pimc_surrogate_eval(self.h, _fptr(x), len(x), _fptr(out)))
+        return out
+
+    def da_counters(self, r):
+        """Delayed acceptance: proposals, stage-1 passes, accepted moves, network evaluations."""
+        out = np.zeros(4)
+        self._check(self.lib.pimc_get_da_counters(self.h, int(r), _dptr(out)))
+        return out
