@@ -211,6 +211,9 @@ def da_comparison(passes, rows_wanted=1_000_000, equil=100):
     same configuration with one E = K + V + E_rot sample per pass.  Efficiency is 1/(time *
     sigma^2) with sigma the blocked standard error of <E>; the ratio compares the two at equal
     sigma(<E>)."""
+    def note(msg):
+        print(f"[da] {msg}", file=sys.stderr, flush=True)
+
     wl = pc.nl_workload("c3")
     bm_pass = float(pc.bead_moves_per_pass(wl["replicas"][0]))
     ctx = pc.load_into_context(ph.Context(1), wl)
@@ -223,6 +226,7 @@ def da_comparison(passes, rows_wanted=1_000_000, equil=100):
         vs.append(v)
         n += len(v)
     X, V = np.concatenate(xs), np.concatenate(vs)
+    note(f"{len(V)} rows collected")
     hold = np.arange(len(V)) % 20 == 0
     t0 = time.perf_counter()
     loss = ctx.surrogate_train(X[~hold], V[~hold], epochs=8, batch=4096, lr=2e-3, seed=1)
@@ -230,6 +234,7 @@ def da_comparison(passes, rows_wanted=1_000_000, equil=100):
     theta, norm = ctx.surrogate_get()
     rms = float(np.sqrt(np.mean((ctx.surrogate_eval(X[hold]) - V[hold]) ** 2)))
     coords, angles, _ = ctx.get_state(0)
+    note(f"trained in {t_train:.1f} s, hold-out rms {rms:.3f} K")
 
     def chain(da):
         c = ph.Context(1)
@@ -242,10 +247,14 @@ def da_comparison(passes, rows_wanted=1_000_000, equil=100):
         c.run(5)
         c.synchronize()
         c.clear_samples()
-        t0 = time.perf_counter()
-        c.run(passes, sample_every=1)
-        c.synchronize()
-        dt = time.perf_counter() - t0
+        dt = 0.0
+        for done in range(0, passes, 50):  # chunks with a progress line each
+            n = min(50, passes - done)
+            t0 = time.perf_counter()
+            c.run(n, sample_every=1)
+            c.synchronize()
+            dt += time.perf_counter() - t0
+            note(f"{'delayed acceptance' if da else 'plain'}: {done + n}/{passes} passes, {dt:.1f} s")
         smp = c.samples(0)
         e = smp[:, 0] + smp[:, 1] + smp[:, 2]
         out = {"seconds": round(dt, 3), "bead_moves_per_s": round(passes * bm_pass / dt, 1),

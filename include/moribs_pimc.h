@@ -121,6 +121,11 @@ int  pimc_get_counters(pimc_ctx* ctx, int replica, double* total, double* accep)
 /* accumulated g(r) (300 bins) and g(r,theta) (300x50) counts of one replica
    (bin_1Ddensity/bin_2Ddensity mc_estim.cc:1233-1286) */
 int  pimc_get_histograms(pimc_ctx* ctx, int replica, double* gr1d, double* gr2d);
+/* rotational correlation sums of one replica since pimc_clear_samples, out[10][Prot]:
+   row 0 sum of n(t0).n(t0+t), rows l = 1..9 sum of P_l of it, over time origins and samples
+   (GetRCF mc_estim.cc:1098-1139; the reference adds 1.0 in rows 1..9).  Returns Prot, or 0
+   when the replica does not rotate. */
+int  pimc_get_rcf(pimc_ctx* ctx, int replica, double* out, int max_values);
 /* pair evaluations performed and bead moves proposed since creation (all replicas) */
 int  pimc_get_work(pimc_ctx* ctx, double* pair_evals, double* bead_moves);
 

@@ -93,6 +93,10 @@ struct pimc_ctx {
     double4* d_beads = nullptr;
     RotBead* d_rotb = nullptr;
     unsigned* d_hist = nullptr;
+    double* d_rcf = nullptr;      // rotational correlation sums of the rotating replicas
+    int* d_rcfreps = nullptr;
+    int nrcf = 0;
+    long long nrcfvals = 0;
     double* d_samples = nullptr;
     int sample_cap = 0;
     float4* d_k1f = nullptr;
@@ -231,7 +235,8 @@ void pimc_destroy(pimc_ctx* c) {
     void* ptrs[] = {c->d_rep, c->d_mut, c->d_beads, c->d_rotb, c->d_hist, c->d_samples, c->d_k1f, c->d_k1d,
                     c->d_t1, c->d_v2f, c->d_v2d, c->d_g2d, c->d_g2f, c->d_t2, c->d_rk, c->d_re, c->d_tr,
                     c->d_work, c->d_all, c->d_rotreps, c->d_est, c->d_rows, c->d_rowbuf, c->d_parts, c->d_flags,
-                    c->d_r3, c->d_v3d, c->d_v3f, c->d_t3, c->d_frames, c->d_work1, c->d_all0, c->d_all1, c->d_rot3, c->d_rows1, c->d_mlp};
+                    c->d_r3, c->d_v3d, c->d_v3f, c->d_t3, c->d_frames, c->d_work1, c->d_all0, c->d_all1, c->d_rot3, c->d_rows1, c->d_mlp,
+                    c->d_rcf, c->d_rcfreps};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -427,7 +432,8 @@ int pimc_commit(pimc_ctx* c) {
         if (!c->desc_set[r]) return fail(c, "replica " + std::to_string(r) + " not set");
     const double lam = 100.0 * (kHBAR * kHBAR) / (kAMU * kKB);
     c->rep.resize(c->nrep);
-    long long boff = 0, roff = 0, hoff = 0;
+    long long boff = 0, roff = 0, hoff = 0, rcfoff = 0;
+    std::vector<int> rcfreps;
     std::vector<int2> work, rows, work1, rows1;
     std::vector<int> rotreps, all0, all1, rot3;
     size_t lds_b1 = 0;
@@ -531,6 +537,12 @@ int pimc_commit(pimc_ctx* c) {
         roff += ((p.rotkind || p.fam == 1) ? (long long)p.tnumb[p.imtype] * p.Prot : 0);
         p.hist_off = hoff;
         hoff += BINSR + BINSR * BINST;
+        p.rcf_off = -1;
+        if (p.rotkind) {  // GetRCF runs with ROTATION (mc_main.cc:571-587)
+            p.rcf_off = rcfoff;
+            rcfoff += (long long)RCF_L * p.Prot;
+            rcfreps.push_back(r);
+        }
         p.row_off = rowoff;
         for (int a = 0; a < p.natoms; ++a) (p.fam ? rows1 : rows).push_back(make_int2(r, a));
         (p.fam ? all1 : all0).push_back(r);
@@ -625,6 +637,14 @@ int pimc_commit(pimc_ctx* c) {
     c->nbeads = boff;
     c->nrotb = roff;
     c->nhist = hoff;
+    c->nrcf = (int)rcfreps.size();
+    c->nrcfvals = rcfoff;
+    c->d_rcf = dalloc<double>(std::max<long long>(rcfoff, 1));
+    c->d_rcfreps = dalloc<int>(rcfreps.size());
+    if (!c->d_rcf || !c->d_rcfreps) return fail(c, "hipMalloc failed");
+    HIPCHECK(c, hipMemset(c->d_rcf, 0, sizeof(double) * std::max<long long>(rcfoff, 1)));
+    if (!rcfreps.empty())
+        HIPCHECK(c, hipMemcpy(c->d_rcfreps, rcfreps.data(), sizeof(int) * rcfreps.size(), hipMemcpyHostToDevice));
     // largest windows first: the longest segment sequences start while the grid fills
     auto heavier = [&](const int2& x, const int2& y) {
         const Rep &a = c->rep[x.x], &b = c->rep[y.x];
@@ -769,7 +789,7 @@ int pimc_commit(pimc_ctx* c) {
         };
         c->rot3_block = lds_for(512) <= 64 * 1024 ? 512 : 256;
         c->lds_rot3 = lds_for(c->rot3_block);
-        c->lds_rot3_da = (size_t)((protmax + 1) & ~1) * sizeof(double) + 4 * sizeof(Frame);
+        c->lds_rot3_da = sizeof(MlpPacked) + (size_t)((protmax + 1) & ~1) * sizeof(double) + 4 * sizeof(Frame);
         if (!rot3.empty() && c->lds_rot3 > 160 * 1024) return fail(c, "non-linear rotor chain exceeds LDS");
     }
     // bisection block: enough lanes for the widest level of the largest window
@@ -939,6 +959,8 @@ void launch_estimate(pimc_ctx* c, double* out) {
     if (c->nrows1)
         hipLaunchKernelGGL((k_est_rows<R, 1>), dim3(c->nrows1), dim3(256), 0, c->stream, c->d_rep, c->d_beads,
                            c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rows1, out ? nullptr : c->d_hist, c->d_rowbuf);
+    if (!out && c->nrcf)
+        hipLaunchKernelGGL(k_rcf, dim3(c->nrcf), dim3(256), 0, c->stream, c->d_rep, c->d_rotb, c->d_rcfreps, c->d_rcf);
     hipLaunchKernelGGL(k_est_final, dim3(c->nrep), dim3(256), 0, c->stream, c->d_rep, c->d_mut, c->d_rotb,
                        c->d_frames, c->tables(), c->tables3(),
                        c->d_all, c->d_rowbuf, c->d_samples, c->sample_cap, out);
@@ -1039,6 +1061,7 @@ int pimc_clear_samples(pimc_ctx* c) {
     for (auto& x : m) x.nsamples = 0;
     HIPCHECK(c, hipMemcpy(c->d_mut, m.data(), sizeof(Mut) * c->nrep, hipMemcpyHostToDevice));
     HIPCHECK(c, hipMemset(c->d_hist, 0, sizeof(unsigned) * std::max<long long>(c->nhist, 1)));
+    HIPCHECK(c, hipMemset(c->d_rcf, 0, sizeof(double) * std::max<long long>(c->nrcfvals, 1)));
     return 0;
 }
 
@@ -1065,6 +1088,16 @@ int pimc_get_histograms(pimc_ctx* c, int r, double* gr1d, double* gr2d) {
     if (gr2d)
         for (int i = 0; i < BINSR * BINST; ++i) gr2d[i] = h[BINSR + i];
     return 0;
+}
+
+int pimc_get_rcf(pimc_ctx* c, int r, double* out, int max_values) {
+    if (!c->committed || r < 0 || r >= c->nrep) return fail(c, "bad replica");
+    const Rep& p = c->rep[r];
+    if (p.rcf_off < 0) return 0;
+    if (!out || max_values < RCF_L * p.Prot) return fail(c, "pimc_get_rcf: buffer smaller than 10*Prot");
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(out, c->d_rcf + p.rcf_off, sizeof(double) * RCF_L * p.Prot, hipMemcpyDeviceToHost));
+    return p.Prot;
 }
 
 int pimc_get_work(pimc_ctx* c, double* pair_evals, double* bead_moves) {
@@ -1417,7 +1450,8 @@ int pimc_surrogate_eval(pimc_ctx* c, const float* x, int n, float* out) {
     if (rc == 0) {
         hipMemcpy(dx, x, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice);
         const int waves = (n + 31) / 32;
-        hipLaunchKernelGGL(k_mlp_infer, dim3((waves + 3) / 4), dim3(256), 0, c->stream, c->d_mlp, dx, n, dy);
+        hipLaunchKernelGGL(k_mlp_infer, dim3((waves + 3) / 4), dim3(256), sizeof(MlpPacked), c->stream, c->d_mlp, dx,
+                           n, dy);
         hipError_t e = hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = hipMemcpy(out, dy, sizeof(float) * n, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = fail(c, std::string("surrogate_eval: ") + hipGetErrorString(e));

@@ -41,7 +41,9 @@ struct Rep {
     double lambda[MAXT], mcstep[MAXT];
     double T, beta, tau, rtstep;
     long long bead_off, rot_off, hist_off, sample_off, row_off;
+    long long rcf_off;  // [RCF_L][Prot] accumulators of the rotational correlation functions, -1 if none
 };
+constexpr int RCF_L = 10;  // NUMB_RCF mc_confg.h:58
 
 struct Mut {
     unsigned pass;

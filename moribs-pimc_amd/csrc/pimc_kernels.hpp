@@ -799,6 +799,39 @@ __global__ __launch_bounds__(BLOCK) void k_rotor3d(const Rep* __restrict__ reps,
     }
 }
 
+// ------------------------------------------------------------------------- rotational correlation
+// GetRCF (mc_estim.cc:1098-1139), one sample: for every time origin it0 and offset itc of the
+// first rotor of the rotating type, p = n(it0).n(it0+itc) with the unit axes of MCCosine;
+// rcf[0][itc] += p and rcf[l][itc] += P_l(p) for l = 1..9.  (The reference adds 1.0 for l >= 1:
+// its Legendre call is commented out; here P_l comes from the three-term recurrence.)
+// One workgroup per replica, a lane per offset, accumulated in FP64.
+__global__ __launch_bounds__(256) void k_rcf(const Rep* __restrict__ reps, const RotBead* __restrict__ rotb,
+                                             const int* __restrict__ list, double* __restrict__ rcf) {
+    const Rep& rp = reps[list[blockIdx.x]];
+    const int Prot = rp.Prot;
+    const RotBead* b = rotb + rp.rot_off;
+    double* out = rcf + rp.rcf_off;
+    for (int itc = threadIdx.x; itc < Prot; itc += blockDim.x) {
+        double acc[RCF_L];
+        for (int l = 0; l < RCF_L; ++l) acc[l] = 0.0;
+        int t1 = itc;
+        for (int it0 = 0; it0 < Prot; ++it0) {
+            const double p = b[it0].nx * b[t1].nx + b[it0].ny * b[t1].ny + b[it0].nz * b[t1].nz;
+            acc[0] += p;
+            double pm = 1.0, pc = p;  // P_0, P_1
+            acc[1] += pc;
+            for (int l = 1; l + 1 < RCF_L; ++l) {
+                const double pn = ((2 * l + 1) * p * pc - l * pm) / (l + 1);
+                pm = pc;
+                pc = pn;
+                acc[l + 1] += pc;
+            }
+            if (++t1 == Prot) t1 = 0;
+        }
+        for (int l = 0; l < RCF_L; ++l) out[(long long)l * Prot + itc] += acc[l];
+    }
+}
+
 // ------------------------------------------------------------------------- delayed acceptance
 // MCRotations3D (mc_piqmc.cc:692-779) for one non-linear rotor among atoms, with the one-stage
 // test of MCRot3Dstep (:1166-1178) split in two (delayed acceptance, Christen & Fox 2005).
@@ -812,14 +845,17 @@ __global__ __launch_bounds__(BLOCK) void k_rotor3d(const Rep* __restrict__ reps,
 // One wave per slice move: the RotRatio x (N-1) pair rows at the old and the proposed
 // orientation go through the network 32 at a time on the matrix cores, and the exact pairs of a
 // proposal that passes spread over the 64 lanes.
-// LDS: Lk [Prot] doubles, the link density cache of k_rotor3d, then one proposal frame per wave.
+// LDS: the packed network, Lk [Prot] doubles (the link density cache of k_rotor3d), then one
+// proposal frame per wave.
 template <typename R>
 __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps, Mut* __restrict__ mut,
                                                     const double4* __restrict__ beads, RotBead* __restrict__ rotb,
                                                     Frame* __restrict__ frames, Tables T, Tables3 T3,
                                                     const int* __restrict__ replist, Run run,
-                                                    const MlpPacked* __restrict__ W) {
-    extern __shared__ double smem[];
+                                                    const MlpPacked* __restrict__ Wg) {
+    extern __shared__ uint4 mlp_lds[];
+    MlpPacked* W = reinterpret_cast<MlpPacked*>(mlp_lds);
+    double* smem = reinterpret_cast<double*>(W + 1);
     __shared__ unsigned long long red_a[4], red_p[4];
     __shared__ int s_err;
     const int r = replist[blockIdx.x];
@@ -852,6 +888,7 @@ __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps
     if (threadIdx.x == 0) s_err = 0;
     unsigned long long nacc = 0, npass = 0;
     int err = 0;
+    mlp_stage(Wg, W);
     for (int i = threadIdx.x; i < Prot; i += blockDim.x) {
         double er, es;
         if (rotden_frames(frames[rb0 + i], frames[rb0 + (i + 1 == Prot ? 0 : i + 1)], tab, Lk[i], er, es)) err = 1;
@@ -916,11 +953,11 @@ __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps
                         mlp_pair_features(f.m, f.zn, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, f4);
                         for (int q = 0; q < MLP_IN; ++q) f4[q] = (f4[q] - W->xm[q]) * W->xinv[q];
                     }
-                    // an opaque copy of the weight pointer keeps the weight loads inside the
-                    // loop (hoisted out they would hold ~260 registers of weights per lane)
-                    const MlpPacked* Wt = W;
-                    __asm__ volatile("" : "+s"(Wt));
-                    const float o = mlp_rows32(Wt, f4);
+                    // an opaque zero offset keeps the weight reads inside the loop (hoisted out
+                    // they would hold ~130 registers of weights per lane)
+                    int z = 0;
+                    __asm__ volatile("" : "+s"(z));
+                    const float o = mlp_rows32(reinterpret_cast<const MlpPacked*>(reinterpret_cast<const char*>(W) + z), f4);
                     if (l < 32 && row < items) {
                         if (which) sn += o;
                         else so += o;

@@ -7,14 +7,18 @@
 // Master weights are FP32 in one flat vector `theta` (MLP_NW floats, layout MLP_OFF_*); the
 // gradient has the same layout.  Training runs batched GEMMs on the matrix cores
 // (v_mfma_f32_32x32x8bf16_1k: BF16 operands converted at load, FP32 accumulation) and small
-// elementwise kernels; the sampler reads a packed copy (MlpPacked: BF16 matrices, FP32 biases).
+// elementwise kernels; the sampler reads a packed copy (MlpPacked: BF16 matrices in the lane
+// order of the matrix instructions, FP32 biases), staged in LDS by the kernels that use it.
 //
 // Inference layout.  One wave evaluates 32 rows as H^T = W X^T, hidden units on the MFMA's M
-// axis and rows on its N axis.  With the 32x32x8 MFMA's lane layout (A: lane l holds
-// A[l%32][4*(l/32)+j]; B: B[4*(l/32)+j][l%32]; D: element i of lane l is
-// D[8*(i/4)+4*(l/32)+i%4][l%32]) the D registers of output tile mb, elements 4q..4q+3, are
-// exactly the B operand of k-block 4*mb+q of the next layer, so activations stay in registers
-// from layer to layer.
+// axis and rows on its N axis.  D layout of the 32x32 MFMAs: element i of lane l is
+// D[8*(i/4)+4*(l/32)+i%4][l%32].  Layer 1 (K = 4, padded to 8) uses the 32x32x8 MFMA
+// (A: lane l holds A[l%32][4*(l/32)+j], j < 4; B: B[4*(l/32)+j][l%32]).  The 128 -> 128 layers
+// use gfx950's 32x32x16 MFMA (A: A[l%32][8*(l/32)+j], j < 8; B: B[8*(l/32)+j][l%32]) with the
+// contraction index permuted: k-block kb takes, on lane l, elements 8*(kb%2)..8*(kb%2)+7 of the
+// previous layer's output tile kb/2, i.e. hidden units 16*kb + 8*(j/4) + 4*(l/32) + j%4.  The
+// packed weights follow the same permutation, so activations stay in registers from layer to
+// layer and every A operand is one 16-byte LDS read.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -33,17 +37,20 @@ constexpr int MLP_NW = MLP_OFF_B4 + 1;                       // 33793
 // FLOP of one evaluation: 2*(4*128 + 128*128 + 128*128 + 128)
 constexpr double MLP_FLOP = 2.0 * (MLP_IN * MLP_H + 2 * MLP_H * MLP_H + MLP_H);
 
-struct MlpPacked {
-    uint16_t w1[MLP_H * 8];      // BF16, inputs padded to 8 (columns 4..7 zero)
-    uint16_t w2[MLP_H * MLP_H];  // BF16 [out][in]
-    uint16_t w3[MLP_H * MLP_H];
+struct alignas(16) MlpPacked {
+    uint16_t w1[4 * 64 * 4];      // layer 1 A operands [tile][lane][4] (inputs 4..7 zero)
+    uint16_t w2[4 * 8 * 64 * 8];  // layer 2 A operands [tile][k-block][lane][8], permuted k
+    uint16_t w3[4 * 8 * 64 * 8];
     float b1[MLP_H], b2[MLP_H], b3[MLP_H], w4[MLP_H];
     float b4;
     float xm[MLP_IN], xinv[MLP_IN];  // normalised input = (x - xm) * xinv
     float ym, ys;                    // energy = ys * output + ym
 };
+static_assert(sizeof(MlpPacked) % 16 == 0, "staged with 16-byte copies");
 
 typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mlp_bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short mlp_u16x8 __attribute__((ext_vector_type(8)));
 typedef float mlp_f32x16 __attribute__((ext_vector_type(16)));
 
 __host__ __device__ inline uint16_t f2bf(float x) {  // round to nearest even
@@ -54,11 +61,21 @@ __host__ __device__ inline uint16_t f2bf(float x) {  // round to nearest even
     return (uint16_t)(u >> 16);
 }
 
-__device__ __forceinline__ float mlp_silu(float z) { return z / (1.0f + expf(-z)); }
+// SiLU z * sigmoid(z) from the hardware exponential and reciprocal (v_exp_f32, v_rcp_f32): the
+// activation is the bulk of the vector work per row, and BF16 operands bound the accuracy anyway.
+__device__ __forceinline__ float mlp_sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float mlp_silu(float z) { return z * mlp_sigmoid(z); }
 
 __device__ __forceinline__ mlp_f32x16 mlp_mfma(mlp_bf16x4 a, mlp_bf16x4 b, mlp_f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
 }
+
+__device__ __forceinline__ mlp_f32x16 mlp_mfma16(mlp_bf16x8 a, mlp_bf16x8 b, mlp_f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// unit of the output element i of lane half hi in tile mb
+__device__ __forceinline__ int mlp_unit(int mb, int i, int hi) { return mb * 32 + 8 * (i >> 2) + 4 * hi + (i & 3); }
 
 __device__ __forceinline__ mlp_bf16x4 mlp_load4(const uint16_t* p) {  // 4 BF16, 8-byte aligned
     const uint2 v = *reinterpret_cast<const uint2*>(p);
@@ -71,31 +88,39 @@ __device__ __forceinline__ mlp_bf16x4 mlp_load4(const uint16_t* p) {  // 4 BF16,
 }
 
 // One hidden 128 -> 128 layer on D-layout activations h (in place): bias and SiLU applied.
+// w: the layer's packed A operands (in LDS).
 __device__ __forceinline__ void mlp_hidden(const uint16_t* __restrict__ w, const float* __restrict__ b,
-                                           mlp_f32x16 h[4], int row, int hi) {
-    mlp_bf16x4 bin[16];
+                                           mlp_f32x16 h[4], int l) {
+    const int hi = l >> 5;
+    mlp_bf16x8 bin[8];
 #pragma unroll
-    for (int kb = 0; kb < 16; ++kb)
+    for (int kb = 0; kb < 8; ++kb) {
+        mlp_u16x8 t;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) bin[kb][j] = (short)f2bf(h[kb >> 2][4 * (kb & 3) + j]);
+        for (int j = 0; j < 8; ++j) t[j] = f2bf(h[kb >> 1][8 * (kb & 1) + j]);
+        bin[kb] = __builtin_bit_cast(mlp_bf16x8, t);
+    }
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
         mlp_f32x16 acc;
 #pragma unroll
         for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-        const uint16_t* wr = w + (mb * 32 + row) * MLP_H + hi * 4;
 #pragma unroll
-        for (int kb = 0; kb < 16; ++kb) acc = mlp_mfma(mlp_load4(wr + kb * 8), bin[kb], acc);
+        for (int kb = 0; kb < 8; ++kb) {
+            const uint4 a = *reinterpret_cast<const uint4*>(w + (((mb * 8 + kb) * 64 + l) << 3));
+            acc = mlp_mfma16(__builtin_bit_cast(mlp_bf16x8, a), bin[kb], acc);
+        }
 #pragma unroll
-        for (int i = 0; i < 16; ++i) acc[i] = mlp_silu(acc[i] + b[mb * 32 + 8 * (i >> 2) + 4 * hi + (i & 3)]);
+        for (int i = 0; i < 16; ++i) acc[i] = mlp_silu(acc[i] + b[mlp_unit(mb, i, hi)]);
         h[mb] = acc;
     }
 }
 
 // The whole wave evaluates 32 rows.  Lane l < 32 passes the normalised inputs of row l in
 // f[0..3] (lanes 32..63 pass anything).  Every lane returns the raw network output of row l%32.
+// W: the packed network staged in LDS.
 __device__ __forceinline__ float mlp_rows32(const MlpPacked* __restrict__ W, const float f[MLP_IN]) {
-    const int l = threadIdx.x & 63, hi = l >> 5, row = l & 31;
+    const int l = threadIdx.x & 63, hi = l >> 5;
     mlp_bf16x4 bx;
 #pragma unroll
     for (int j = 0; j < 4; ++j) bx[j] = hi ? (short)0 : (short)f2bf(f[j]);
@@ -105,20 +130,27 @@ __device__ __forceinline__ float mlp_rows32(const MlpPacked* __restrict__ W, con
         mlp_f32x16 acc;
 #pragma unroll
         for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-        acc = mlp_mfma(mlp_load4(W->w1 + (mb * 32 + row) * 8 + hi * 4), bx, acc);
+        acc = mlp_mfma(mlp_load4(W->w1 + ((mb * 64 + l) << 2)), bx, acc);
 #pragma unroll
-        for (int i = 0; i < 16; ++i) acc[i] = mlp_silu(acc[i] + W->b1[mb * 32 + 8 * (i >> 2) + 4 * hi + (i & 3)]);
+        for (int i = 0; i < 16; ++i) acc[i] = mlp_silu(acc[i] + W->b1[mlp_unit(mb, i, hi)]);
         h[mb] = acc;
     }
-    mlp_hidden(W->w2, W->b2, h, row, hi);
-    mlp_hidden(W->w3, W->b3, h, row, hi);
+    mlp_hidden(W->w2, W->b2, h, l);
+    mlp_hidden(W->w3, W->b3, h, l);
     float part = 0.0f;
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
-        for (int i = 0; i < 16; ++i) part += W->w4[mb * 32 + 8 * (i >> 2) + 4 * hi + (i & 3)] * h[mb][i];
+        for (int i = 0; i < 16; ++i) part += W->w4[mlp_unit(mb, i, hi)] * h[mb][i];
     part += __shfl_xor(part, 32);
     return part + W->b4;
+}
+
+// Copies the packed network into LDS (all lanes of the block take part; caller synchronises).
+__device__ __forceinline__ void mlp_stage(const MlpPacked* __restrict__ g, MlpPacked* s) {
+    const uint4* src = reinterpret_cast<const uint4*>(g);
+    uint4* dst = reinterpret_cast<uint4*>(s);
+    for (int i = threadIdx.x; i < (int)(sizeof(MlpPacked) / 16); i += blockDim.x) dst[i] = src[i];
 }
 
 // Pair geometry in the rotor's body frame: distance and the direction cosines of the partner
@@ -180,7 +212,7 @@ __global__ void k_mlp_bias_act(float* __restrict__ Z, float* __restrict__ H, con
 __global__ void k_mlp_act_back(float* __restrict__ dH, const float* __restrict__ Z, long long n) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float z = Z[i], s = 1.0f / (1.0f + expf(-z));
+    const float z = Z[i], s = mlp_sigmoid(z);
     dH[i] *= s * (1.0f + z * (1.0f - s));
 }
 
@@ -244,16 +276,20 @@ __global__ void k_mlp_gather(const float* __restrict__ X, const float* __restric
     by[b] = y[s];
 }
 
-// Packs theta and the normalisation into the sampler's copy.
+// Packs theta and the normalisation into the sampler's copy, weights in the lane order of the
+// matrix instructions (see the layout note at the top of this file).
 __global__ void k_mlp_pack(const float* __restrict__ th, const float* __restrict__ norm, MlpPacked* __restrict__ p) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < MLP_H * 8) {
-        const int o = i >> 3, k = i & 7;
+    if (i < 4 * 64 * 4) {  // [mb][lane][j]: W1[mb*32 + lane%32][4*(lane/32) + j]
+        const int j = i & 3, lane = (i >> 2) & 63, mb = i >> 8;
+        const int o = mb * 32 + (lane & 31), k = 4 * (lane >> 5) + j;
         p->w1[i] = k < MLP_IN ? f2bf(th[MLP_OFF_W1 + o * MLP_IN + k]) : (uint16_t)0;
     }
-    if (i < MLP_H * MLP_H) {
-        p->w2[i] = f2bf(th[MLP_OFF_W2 + i]);
-        p->w3[i] = f2bf(th[MLP_OFF_W3 + i]);
+    if (i < MLP_H * MLP_H) {  // [mb][kb][lane][j]: W[mb*32 + lane%32][16*kb + 8*(j/4) + 4*(lane/32) + j%4]
+        const int j = i & 7, lane = (i >> 3) & 63, kb = (i >> 9) & 7, mb = i >> 12;
+        const int o = mb * 32 + (lane & 31), u = 16 * kb + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        p->w2[i] = f2bf(th[MLP_OFF_W2 + o * MLP_H + u]);
+        p->w3[i] = f2bf(th[MLP_OFF_W3 + o * MLP_H + u]);
     }
     if (i < MLP_H) {
         p->b1[i] = th[MLP_OFF_B1 + i];
@@ -273,9 +309,13 @@ __global__ void k_mlp_pack(const float* __restrict__ th, const float* __restrict
 }
 
 // Batched inference on raw inputs (the sampler's code path): out = ys * net((x - xm)/xs) + ym.
-// One wave per 32 rows.
-__global__ __launch_bounds__(256) void k_mlp_infer(const MlpPacked* __restrict__ W, const float* __restrict__ X,
+// One wave per 32 rows; the network is staged in LDS (dynamic, sizeof(MlpPacked) bytes).
+__global__ __launch_bounds__(256) void k_mlp_infer(const MlpPacked* __restrict__ Wg, const float* __restrict__ X,
                                                    int n, float* __restrict__ out) {
+    extern __shared__ uint4 mlp_lds[];
+    MlpPacked* W = reinterpret_cast<MlpPacked*>(mlp_lds);
+    mlp_stage(Wg, W);
+    __syncthreads();
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int l = threadIdx.x & 63;
     const int row = wave * 32 + (l & 31);

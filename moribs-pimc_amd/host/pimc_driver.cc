@@ -9,7 +9,13 @@
 //   <OUTPUTDIR><FILENAMEPREFIX>.eng       one line per block after equilibration (SaveEnergy mc_main.cc:764-790)
 //   <OUTPUTDIR><FILENAMEPREFIX>_sum.eng   accumulated averages every MCSKIP_TOTAL steps (SaveSumEnergy :792-836)
 //   <OUTPUTDIR><FILENAMEPREFIX>.xyz       the configuration at the end of every block (IOxyz mc_input.cc:690-746)
+//   <...><block>.gra/.gri/.grt/.g2d/.rcf  block distributions (MCSaveBlockAverages mc_main.cc:703-735,
+//                                         SaveDensities1D/2D mc_estim.cc:1327-1444,1666-1727, SaveRCF :1141-1184)
+//   <...>_sum.gra/_sum.g2d/_sum.rcf       accumulated distributions after every block (mc_main.cc:446-462)
 //   stdout                                the accept-ratio line every MCSKIP_RATIO steps (MCSaveAcceptRatio :879-943)
+// The atom-atom g(r) and the atom density around a linear rotor are binned on the device
+// (300 radial bins to 15 A, 50 polar bins); the 3D densities around a non-linear rotor
+// (SaveDensities3D, SaveRhoThetaChi) are not.
 //
 // Time steps and passes.  The reference counts "steps" (one time slice of one pass) and
 // samples when the step count is a multiple of MCSKIP_AVERG (mc_main.cc:340-484).  A pass on
@@ -370,7 +376,95 @@ struct Chain {
     Sums block, total;
     double sums_count = 0.0;
     std::vector<double> tot0, acc0;  // counters at the start of the block
+    // bin counts and rotational correlation sums: this block, and all measured blocks
+    std::vector<double> gr1, gr2, rcf, gr1_sum, gr2_sum, rcf_sum;
 };
+
+constexpr int NBR = 300, NBT = 50, NRCF = 10;  // MC_BINSR, MC_BINST (mc_estim.cc:25-26), NUMB_RCF
+constexpr double DR = 15.0 / NBR;             // MAX_RADIUS / MC_BINSR, MIN_RADIUS = 0
+const double DTH = M_PI / (NBT - 1);
+
+std::ofstream open_out(const std::string& name, const char* who) {
+    std::ofstream f(name);
+    if (!f.good()) die(who, "cannot open " + name);
+    f << std::setprecision(6) << std::scientific;
+    return f;
+}
+
+// SaveDensities2D: theta, r and the density per line, theta outer (mc_estim.cc:1666-1727)
+void save_g2d(const std::vector<double>& g, const std::string& name, double acount, const Deck& d, int n_atoms) {
+    auto f = open_out(name, "SaveDensities2D");
+    const char* sp = "   ";
+    const double norm3 = DR * DTH * d.P * acount;
+    for (int it = 0; it < NBT; ++it) {
+        for (int ir = 0; ir < NBR; ++ir) {
+            if (n_atoms)
+                f << std::setw(14) << (it + 0.5) * DTH * 180.0 / M_PI << sp << std::setw(14) << (ir + 0.5) * DR << sp
+                  << std::setw(14) << g[ir * NBT + it] / norm3 << sp;
+            f << "\n";
+        }
+    }
+}
+
+// SaveRCF: <n(tau).n(0)> and <P_l(n(tau).n(0))>, l = 1..9, over tau = k * tau_rot, k = 0..Prot
+void save_rcf(const std::vector<double>& rcf, const std::string& name, double acount, const Deck& d) {
+    auto f = open_out(name, "SaveRCF");
+    const char* sp = "   ";
+    const int Prot = d.Prot;
+    const double norm = acount * Prot, tau = 1.0 / (d.T * Prot);
+    for (int it = 0; it <= Prot; ++it)
+        f << std::setw(14) << it * tau << sp << std::setw(14) << rcf[it % Prot] / norm << sp << "\n";
+    f << "\n\n#\n";
+    for (int it = 0; it <= Prot; ++it) {
+        f << std::setw(14) << it * tau << sp;
+        for (int l = 1; l < NRCF; ++l) f << std::setw(14) << rcf[(size_t)l * Prot + it % Prot] / norm << sp;
+        f << "\n";
+    }
+}
+
+// SaveDensities1D (.gra, and .gri/.grt around a linear rotor) and SaveDensities2D (.g2d) of
+// one block; n_atoms is the number of structureless atoms (one atom type).
+void save_block_densities(const Chain& c, const std::string& fname, double acount, const Deck& d, int n_atoms,
+                          bool linear) {
+    const char* sp = "   ";
+    const double box = std::pow((double)d.natoms() / d.density, 1.0 / 3.0);
+    {
+        auto f = open_out(fname + ".gra", "SaveDensities1D");
+        const double norm0 = 2.0 * M_PI * DR * acount * d.P / (box * box * box);
+        for (int ir = 0; ir < NBR; ++ir) {
+            const double r = (ir + 0.5) * DR;
+            f << std::setw(14) << r << sp;
+            if (n_atoms) f << std::setw(14) << c.gr1[ir] / (r * r * norm0 * n_atoms * (n_atoms - 1)) << sp;
+            f << "\n";
+        }
+    }
+    if (!linear) return;
+    {
+        auto f = open_out(fname + ".gri", "SaveDensities1D");
+        const double norm1 = DR * d.P * acount * 4.0 * M_PI;
+        for (int ir = 0; ir < NBR; ++ir) {
+            const double r = (ir + 0.5) * DR;
+            double dens = 0.0;
+            for (int it = 0; it < NBT; ++it) dens += c.gr2[ir * NBT + it];
+            f << std::setw(14) << r << sp;
+            if (n_atoms) f << std::setw(14) << dens / (norm1 * r * r) << sp;
+            f << "\n";
+        }
+    }
+    {
+        auto f = open_out(fname + ".grt", "SaveDensities1D");
+        const double norm2 = d.P * acount * DTH;
+        for (int it = 0; it < NBT; ++it) {
+            double dens = 0.0;
+            for (int ir = 0; ir < NBR; ++ir) dens += c.gr2[ir * NBT + it];
+            f << std::setw(14) << (it + 0.5) * DTH * 180.0 / M_PI << sp;
+            if (n_atoms) f << std::setw(14) << dens / (norm2 * n_atoms) << sp;
+            f << "\n";
+        }
+    }
+    save_g2d(c.gr2, fname + ".g2d", acount, d, n_atoms);
+}
+
 
 void set_sci(std::ostream& o) { o << std::setprecision(6) << std::scientific; }
 
@@ -518,6 +612,12 @@ int main(int argc, char** argv) {
     auto crosses = [&](long q, long skip) { return (q * P) / skip > ((q - 1) * P) / skip; };
 
     const int nt = (int)d.types.size();
+    int n_plain = 0;
+    bool linear = false;
+    for (auto& s : d.types) {
+        if (s.molecule == 0) n_plain += s.numb;
+        if (s.molecule == 1) linear = true;
+    }
     std::vector<double> samples;
     for (long block = d.startblock + 1; block <= d.startblock + d.blocks; ++block) {
         const bool measured = block > d.eqblocks;
@@ -529,6 +629,11 @@ int main(int argc, char** argv) {
             c.check(pimc_get_counters(c.ctx, r, ch.tot0.data(), ch.acc0.data()), "ResetMCCounts");
         }
         c.check(pimc_clear_samples(c.ctx), "pimc_clear_samples");
+        for (auto& ch : chains) {
+            ch.gr1.assign(NBR, 0.0);
+            ch.gr2.assign(NBR * NBT, 0.0);
+            ch.rcf.assign(d.rotation ? (size_t)NRCF * d.Prot : 0, 0.0);
+        }
         for (long done = 0; done < d.passes;) {
             const long m = std::min(chunk, d.passes - done);
             c.check(pimc_run(c.ctx, (int)m, measured ? (int)every : 0), "pimc_run");
@@ -555,6 +660,17 @@ int main(int argc, char** argv) {
                     }
                 }
                 if (ratio_line) print_ratios(d, c, ch, r, block, done + m, (done + m) * P);
+                if (measured && got > 0) {  // bins and correlation sums of this chunk's samples
+                    std::vector<double> g1(NBR), g2(NBR * NBT);
+                    c.check(pimc_get_histograms(c.ctx, r, g1.data(), g2.data()), "pimc_get_histograms");
+                    for (int i = 0; i < NBR; ++i) ch.gr1[i] += g1[i];
+                    for (int i = 0; i < NBR * NBT; ++i) ch.gr2[i] += g2[i];
+                    if (d.rotation) {
+                        std::vector<double> rc((size_t)NRCF * d.Prot);
+                        c.check(pimc_get_rcf(c.ctx, r, rc.data(), (int)rc.size()), "pimc_get_rcf");
+                        for (size_t i = 0; i < rc.size(); ++i) ch.rcf[i] += rc[i];
+                    }
+                }
             }
             if (measured) c.check(pimc_clear_samples(c.ctx), "pimc_clear_samples");
             done += m;
@@ -562,7 +678,35 @@ int main(int argc, char** argv) {
         std::vector<double> cg(3 * (size_t)n), ag(3 * (size_t)n), sg(3 * (size_t)n);
         for (int r = 0; r < nrep; ++r) {
             Chain& ch = chains[r];
-            if (measured && ch.block.count > 0) save_energy(ch, block);
+            if (measured && ch.block.count > 0) {
+                // MCSaveBlockAverages, then the accumulated files (mc_main.cc:446-462)
+                std::ostringstream bn;
+                bn << std::setw(3) << std::setfill('0') << block;
+                const std::string fname = ch.stem + bn.str();
+                save_block_densities(ch, fname, ch.block.count, d, n_plain, linear);
+                if (d.rotation) save_rcf(ch.rcf, fname + ".rcf", ch.block.count, d);
+                save_energy(ch, block);
+                if (ch.gr1_sum.empty()) {
+                    ch.gr1_sum.assign(NBR, 0.0);
+                    ch.gr2_sum.assign(NBR * NBT, 0.0);
+                    ch.rcf_sum.assign(ch.rcf.size(), 0.0);
+                }
+                for (int i = 0; i < NBR; ++i) ch.gr1_sum[i] += ch.gr1[i];
+                for (int i = 0; i < NBR * NBT; ++i) ch.gr2_sum[i] += ch.gr2[i];
+                for (size_t i = 0; i < ch.rcf.size(); ++i) ch.rcf_sum[i] += ch.rcf[i];
+                {  // SaveGraSum
+                    auto f = open_out(ch.stem + "_sum.gra", "SaveGraSum");
+                    const double norma = DR * ch.total.count * d.P;
+                    for (int ir = 0; ir < NBR; ++ir) {
+                        f << std::setw(14) << (ir + 0.5) * DR << "   ";
+                        if (n_plain)
+                            f << std::setw(14) << ch.gr1_sum[ir] / (norma * n_plain * (n_plain - 1) / 2.0) << "   ";
+                        f << "\n";
+                    }
+                }
+                if (linear) save_g2d(ch.gr2_sum, ch.stem + "_sum.g2d", ch.total.count, d, n_plain);
+                if (d.rotation) save_rcf(ch.rcf_sum, ch.stem + "_sum.rcf", ch.total.count, d);
+            }
             c.check(pimc_get_state(c.ctx, r, cg.data(), ag.data(), sg.data()), "pimc_get_state");
             // beads without an orientation on the device keep their initial cosines
             int first = 0;

@@ -71,6 +71,7 @@ SIGNATURES = {
     "pimc_get_counters": (_i, [ctypes.c_void_p, _i, _dp, _dp]),
     "pimc_get_histograms": (_i, [ctypes.c_void_p, _i, _dp, _dp]),
     "pimc_get_work": (_i, [ctypes.c_void_p, _dp, _dp]),
+    "pimc_get_rcf": (_i, [ctypes.c_void_p, _i, _dp, _i]),
     "pimc_pot_energy_slice": (_i, [ctypes.c_void_p, _i, _ip, _ip, _ip, _dp, _dp]),
     "pimc_estimate": (_i, [ctypes.c_void_p, _dp]),
     "pimc_kernel_time": (_i, [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_long)]),
@@ -236,6 +237,14 @@ class Context:
         g2 = np.zeros(300 * 50)
         self._check(self.lib.pimc_get_histograms(self.h, r, _dptr(g1), _dptr(g2)))
         return g1, g2.reshape(300, 50)
+
+    def rcf(self, r):
+        """[10][Prot] rotational correlation sums since the last clear_samples (None: no rotor)."""
+        d = self.descs[r]
+        prot = d.Prot if d.Prot > 0 else d.P
+        out = np.zeros((10, prot))
+        n = self._check(self.lib.pimc_get_rcf(self.h, int(r), _dptr(out), out.size))
+        return out if n else None
 
     def work(self):
         pe, bm = ctypes.c_double(), ctypes.c_double()

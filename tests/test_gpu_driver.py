@@ -86,3 +86,28 @@ def test_driver_output_layouts(tmp_path):
     assert lines[2].split()[0] == "He41" and lines[-1].split()[0] == "CO21"
     ratio = [ln for ln in r.stdout.splitlines() if ln.startswith("BLOCK:")]
     assert len(ratio) == 12 and "Rot:" in ratio[0] and "He4" in ratio[0]
+    # distributions of the measured blocks 11, 12 and the accumulated ones
+    for b in ("011", "012"):
+        for ext in (".gra", ".gri", ".grt", ".g2d", ".rcf"):
+            assert os.path.exists(os.path.join(d, f"he4co2{b}{ext}")), b + ext
+    gra = np.loadtxt(os.path.join(d, "he4co2_sum.gra"))
+    assert gra.shape == (300, 2) and np.allclose(gra[:, 0], (np.arange(300) + 0.5) * 0.05)
+    # sum over bins of the _sum.gra column times dr: each of the N(N-1)/2 pairs once per slice
+    # within 15 A (the cluster is bound, so all of them)
+    assert abs(gra[:, 1].sum() * 0.05 - 1.0) < 1e-6
+    gri = np.loadtxt(os.path.join(d, "he4co2012.gri"))
+    grt = np.loadtxt(os.path.join(d, "he4co2012.grt"))
+    r = gri[:, 0]
+    assert abs(np.sum(gri[:, 1] * 4 * np.pi * r * r * 0.05) - 4.0) < 1e-6   # 4 He around the rotor
+    assert abs(np.sum(grt[:, 1]) * np.pi / 49 - 1.0) < 1e-6
+    g2d = np.loadtxt(os.path.join(d, "he4co2_sum.g2d"))
+    assert g2d.shape == (300 * 50, 3)
+    with open(os.path.join(d, "he4co2_sum.rcf")) as f:
+        blocks = f.read().split("\n\n\n#\n")
+    c0 = np.loadtxt(blocks[0].splitlines())
+    cl = np.loadtxt(blocks[1].splitlines())
+    assert c0.shape == (65, 2) and cl.shape == (65, 10)
+    assert abs(c0[0, 1] - 1.0) < 1e-6 and np.allclose(cl[0, 1:], 1.0, atol=1e-6)   # n(0).n(0) = 1
+    assert np.allclose(c0[:, 1], cl[:, 1], atol=1e-9)                             # P_1(p) = p
+    assert np.allclose(c0[:, 1], c0[::-1, 1], atol=1e-9)                          # symmetric in tau
+    assert np.allclose(c0[:, 0], np.arange(65) / (0.37 * 64), rtol=1e-5)

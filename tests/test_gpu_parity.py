@@ -175,3 +175,28 @@ def test_negative_density_is_an_error():
     ctx.set_state(0, rp["coords"], a)
     with pytest.raises(ph.PimcError, match="Negative rot density"):
         ctx.run(1)
+
+
+def test_rotational_correlation_of_a_sample():
+    """GetRCF (mc_estim.cc:1098-1139) on the state after one sampled pass: row 0 sums
+    n(t0).n(t0+t) over time origins, rows l sum P_l of it (numpy's Legendre series here)."""
+    from numpy.polynomial import legendre as L
+    wl = pc.workload("tiny")
+    ctx = gpu_ctx(wl, ph.FLAG_FP64, SEED)
+    ctx.run(2)
+    ctx.clear_samples()
+    ctx.run(1, sample_every=1)
+    got = ctx.rcf(0)
+    rp = wl["replicas"][0]
+    _, _, cos = ctx.get_state(0)
+    P, Prot = rp["P"], rp["Prot"]
+    rot = sum(rp["numb"]) - 1
+    n = cos[:, rot * P: rot * P + Prot]
+    want = np.zeros((10, Prot))
+    for t in range(Prot):
+        p = np.sum(n * np.roll(n, -t, axis=1), axis=0)
+        want[0, t] = p.sum()
+        for l in range(1, 10):
+            want[l, t] = L.legval(p, [0] * l + [1]).sum()
+    assert np.allclose(got, want, atol=1e-10 * Prot), np.abs(got - want).max()
+    assert abs(got[0, 0] - Prot) < 1e-9
