@@ -107,6 +107,14 @@ int  pimc_set_option(pimc_ctx* ctx, const char* name, long value);
    Only rotor entries of angles are kept; pimc_get_state returns those defaults for atoms. */
 int  pimc_set_state(pimc_ctx* ctx, int replica, const double* coords, const double* angles);
 int  pimc_get_state(pimc_ctx* ctx, int replica, double* coords, double* angles, double* cosines);
+/* restart (ConfigIO mc_input.cc:560-609 and the generator streams of RandomIO mc_randg.cc:36-88):
+   the state as pimc_get_state returned it, cosines included, and the replica's pass counter
+   (the generator's whole state: every draw is addressed by pass).  Rotation matrices of
+   non-linear rotors are rebuilt on the device with the moves' own code, so a resumed run
+   continues exactly as the uninterrupted one. */
+int  pimc_restore_state(pimc_ctx* ctx, int replica, const double* coords, const double* angles,
+                        const double* cosines, unsigned pass);
+int  pimc_get_pass(pimc_ctx* ctx, int replica, unsigned* pass);
 
 /* ---- sampling ---- */
 /* n_passes passes of PIMCPass over all replicas; one MCGetAverage sample per replica

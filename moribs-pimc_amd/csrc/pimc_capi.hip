@@ -833,6 +833,56 @@ int pimc_set_state(pimc_ctx* c, int r, const double* coords, const double* angle
     return 0;
 }
 
+int pimc_restore_state(pimc_ctx* c, int r, const double* coords, const double* angles, const double* cosines,
+                       unsigned pass) {
+    if (!c->committed) return fail(c, "pimc_commit first");
+    if (r < 0 || r >= c->nrep) return fail(c, "replica index out of range");
+    if (!coords || !angles || !cosines) return fail(c, "pimc_restore_state: null argument");
+    const Rep& p = c->rep[r];
+    const int n = p.natoms * p.P;
+    std::vector<double4> b(n);
+    for (int i = 0; i < n; ++i) b[i] = make_double4(coords[i], coords[n + i], coords[2 * n + i], 0.0);
+    HIPCHECK(c, hipMemcpy(c->d_beads + p.bead_off, b.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    if (p.rotkind || p.fam == 1) {
+        const int nr = p.tnumb[p.imtype];
+        std::vector<RotBead> rb((size_t)nr * p.Prot);
+        for (int a = 0; a < nr; ++a)
+            for (int k = 0; k < p.Prot; ++k) {
+                const int i = (p.tbegin[p.imtype] + a) * p.P + k;
+                RotBead& q = rb[(size_t)a * p.Prot + k];
+                q.phi = angles[i];
+                q.cth = angles[n + i];
+                q.chi = angles[2 * n + i];
+                q.nx = cosines[i];
+                q.ny = cosines[n + i];
+                q.nz = cosines[2 * n + i];
+                q.pad0 = q.pad1 = 0;
+            }
+        HIPCHECK(c, hipMemcpy(c->d_rotb + p.rot_off, rb.data(), sizeof(RotBead) * rb.size(), hipMemcpyHostToDevice));
+        if (p.fam == 1) {
+            const int m = (int)rb.size();
+            hipLaunchKernelGGL(k_frames_from_angles, dim3((m + 255) / 256), dim3(256), 0, c->stream, c->d_rep,
+                               c->d_rotb, c->d_frames, r, m);
+            HIPCHECK(c, hipGetLastError());
+        }
+    }
+    Mut m;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(&m, c->d_mut + r, sizeof(Mut), hipMemcpyDeviceToHost));
+    m.pass = pass;
+    HIPCHECK(c, hipMemcpy(c->d_mut + r, &m, sizeof(Mut), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int pimc_get_pass(pimc_ctx* c, int r, unsigned* pass) {
+    if (!c->committed || r < 0 || r >= c->nrep || !pass) return fail(c, "bad replica");
+    Mut m;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(&m, c->d_mut + r, sizeof(Mut), hipMemcpyDeviceToHost));
+    *pass = m.pass;
+    return 0;
+}
+
 int pimc_get_state(pimc_ctx* c, int r, double* coords, double* angles, double* cosines) {
     if (!c->committed) return fail(c, "pimc_commit first");
     if (r < 0 || r >= c->nrep) return fail(c, "replica index out of range");
@@ -1257,6 +1307,8 @@ void mlp_forward_backward(hipStream_t s, const float* th, float* g, MlpBuffers& 
 }
 
 int mlp_upload(pimc_ctx* c) {
+    hipError_t e0 = hipSuccess;
+    auto ck = [&](hipError_t x) { if (e0 == hipSuccess) e0 = x; };
     if (!c->d_mlp) c->d_mlp = dalloc<MlpPacked>(1);
     float *th = dalloc<float>(MLP_NW), *nm = dalloc<float>(2 * MLP_IN + 2);
     if (!c->d_mlp || !th || !nm) {
@@ -1264,13 +1316,14 @@ int mlp_upload(pimc_ctx* c) {
         if (nm) (void)hipFree(nm);
         return fail(c, "surrogate: out of device memory");
     }
-    hipMemcpy(th, c->theta.data(), sizeof(float) * MLP_NW, hipMemcpyHostToDevice);
-    hipMemcpy(nm, c->norm.data(), sizeof(float) * (2 * MLP_IN + 2), hipMemcpyHostToDevice);
+    ck(hipMemcpy(th, c->theta.data(), sizeof(float) * MLP_NW, hipMemcpyHostToDevice));
+    ck(hipMemcpy(nm, c->norm.data(), sizeof(float) * (2 * MLP_IN + 2), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_mlp_pack, dim3((MLP_H * MLP_H + 255) / 256), dim3(256), 0, c->stream, th, nm, c->d_mlp);
-    const hipError_t e = hipStreamSynchronize(c->stream);
+    ck(hipGetLastError());
+    ck(hipStreamSynchronize(c->stream));
     (void)hipFree(th);
     (void)hipFree(nm);
-    if (e != hipSuccess) return fail(c, std::string("surrogate pack: ") + hipGetErrorString(e));
+    if (e0 != hipSuccess) return fail(c, std::string("surrogate pack: ") + hipGetErrorString(e0));
     c->mlp_ready = true;
     return 0;
 }
@@ -1309,14 +1362,17 @@ int pimc_surrogate_grad(pimc_ctx* c, const float* theta, const float* x, const f
     MlpBuffers m;
     float *th = dalloc<float>(MLP_NW), *g = dalloc<float>(MLP_NW);
     int rc = 0;
+    hipError_t e0 = hipSuccess;
+    auto ck = [&](hipError_t x) { if (e0 == hipSuccess) e0 = x; };
     if (!m.alloc(n) || !th || !g) rc = fail(c, "surrogate_grad: out of device memory");
     if (rc == 0) {
-        hipMemcpy(th, theta, sizeof(float) * MLP_NW, hipMemcpyHostToDevice);
-        hipMemcpy(m.x, x, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice);
-        hipMemcpy(m.y, y, sizeof(float) * n, hipMemcpyHostToDevice);
-        hipMemset(m.loss, 0, sizeof(double));
+        ck(hipMemcpy(th, theta, sizeof(float) * MLP_NW, hipMemcpyHostToDevice));
+        ck(hipMemcpy(m.x, x, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice));
+        ck(hipMemcpy(m.y, y, sizeof(float) * n, hipMemcpyHostToDevice));
+        ck(hipMemset(m.loss, 0, sizeof(double)));
         mlp_forward_backward(c->stream, th, g, m, n);
-        hipError_t e = hipStreamSynchronize(c->stream);
+        ck(hipGetLastError());
+        hipError_t e = e0 != hipSuccess ? e0 : hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = hipMemcpy(grad, g, sizeof(float) * MLP_NW, hipMemcpyDeviceToHost);
         if (e == hipSuccess) e = hipMemcpy(loss, m.loss, sizeof(double), hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = fail(c, std::string("surrogate_grad: ") + hipGetErrorString(e));
@@ -1382,21 +1438,23 @@ int pimc_surrogate_train(pimc_ctx* c, const float* x, const float* v, int n, int
           *dg = dalloc<float>(MLP_NW), *dm = dalloc<float>(MLP_NW), *dv = dalloc<float>(MLP_NW);
     int* dperm = dalloc<int>(n);
     int rc = 0;
+    hipError_t e0 = hipSuccess;
+    auto ck = [&](hipError_t x) { if (e0 == hipSuccess) e0 = x; };
     if (!m.alloc(batch) || !dX || !dY || !dth || !dg || !dm || !dv || !dperm) rc = fail(c, "surrogate_train: out of device memory");
     if (rc == 0) {
-        hipMemcpy(dX, xn.data(), sizeof(float) * xn.size(), hipMemcpyHostToDevice);
-        hipMemcpy(dY, yn.data(), sizeof(float) * n, hipMemcpyHostToDevice);
-        hipMemcpy(dth, th.data(), sizeof(float) * MLP_NW, hipMemcpyHostToDevice);
-        hipMemset(dm, 0, sizeof(float) * MLP_NW);
-        hipMemset(dv, 0, sizeof(float) * MLP_NW);
+        ck(hipMemcpy(dX, xn.data(), sizeof(float) * xn.size(), hipMemcpyHostToDevice));
+        ck(hipMemcpy(dY, yn.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        ck(hipMemcpy(dth, th.data(), sizeof(float) * MLP_NW, hipMemcpyHostToDevice));
+        ck(hipMemset(dm, 0, sizeof(float) * MLP_NW));
+        ck(hipMemset(dv, 0, sizeof(float) * MLP_NW));
         std::vector<int> perm(n);
         for (int i = 0; i < n; ++i) perm[i] = i;
         const int steps = n / batch;
         long t = 0;
         for (int ep = 0; ep < epochs && rc == 0; ++ep) {
             for (int i = n - 1; i > 0; --i) std::swap(perm[i], perm[(int)(unif() * (i + 1))]);
-            hipMemcpy(dperm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice);
-            hipMemset(m.loss, 0, sizeof(double));
+            ck(hipMemcpy(dperm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+            ck(hipMemset(m.loss, 0, sizeof(double)));
             const float lr_ep = (float)(lr * std::pow(0.1, (double)ep / epochs));  // decays tenfold over the run
             for (int s = 0; s < steps; ++s) {
                 hipLaunchKernelGGL(k_mlp_gather, dim3((batch + 255) / 256), dim3(256), 0, c->stream, dX, dY, dperm,
@@ -1408,7 +1466,8 @@ int pimc_surrogate_train(pimc_ctx* c, const float* x, const float* v, int n, int
                                    MLP_NW, lr_ep, c1, c2);
             }
             double l = 0.0;
-            hipError_t e = hipStreamSynchronize(c->stream);
+            ck(hipGetLastError());
+            hipError_t e = e0 != hipSuccess ? e0 : hipStreamSynchronize(c->stream);
             if (e == hipSuccess) e = hipMemcpy(&l, m.loss, sizeof(double), hipMemcpyDeviceToHost);
             if (e != hipSuccess) rc = fail(c, std::string("surrogate_train: ") + hipGetErrorString(e));
             if (loss) loss[ep] = l / steps;
@@ -1446,13 +1505,16 @@ int pimc_surrogate_eval(pimc_ctx* c, const float* x, int n, float* out) {
     if (!x || !out || n < 1) return fail(c, "surrogate_eval: bad arguments");
     float *dx = dalloc<float>((size_t)n * MLP_IN), *dy = dalloc<float>(n);
     int rc = 0;
+    hipError_t e0 = hipSuccess;
+    auto ck = [&](hipError_t x) { if (e0 == hipSuccess) e0 = x; };
     if (!dx || !dy) rc = fail(c, "surrogate_eval: out of device memory");
     if (rc == 0) {
-        hipMemcpy(dx, x, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice);
+        ck(hipMemcpy(dx, x, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice));
         const int waves = (n + 31) / 32;
         hipLaunchKernelGGL(k_mlp_infer, dim3((waves + 3) / 4), dim3(256), sizeof(MlpPacked), c->stream, c->d_mlp, dx,
                            n, dy);
-        hipError_t e = hipStreamSynchronize(c->stream);
+        ck(hipGetLastError());
+        hipError_t e = e0 != hipSuccess ? e0 : hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = hipMemcpy(out, dy, sizeof(float) * n, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = fail(c, std::string("surrogate_eval: ") + hipGetErrorString(e));
     }

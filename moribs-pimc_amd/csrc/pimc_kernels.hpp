@@ -799,6 +799,18 @@ __global__ __launch_bounds__(BLOCK) void k_rotor3d(const Rep* __restrict__ reps,
     }
 }
 
+// ------------------------------------------------------------------------- restart
+// Rotation matrices of every non-linear rotor slice of replica r from its stored angles, with
+// the code the moves use when they accept (make_frame), so a resumed chain sees the same bits.
+__global__ void k_frames_from_angles(const Rep* __restrict__ reps, const RotBead* __restrict__ rotb,
+                                     Frame* __restrict__ frames, int r, int n) {
+    const Rep& rp = reps[r];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const RotBead& b = rotb[rp.rot_off + i];
+    frames[rp.rot_off + i] = make_frame(b.phi, b.cth, b.chi);
+}
+
 // ------------------------------------------------------------------------- rotational correlation
 // GetRCF (mc_estim.cc:1098-1139), one sample: for every time origin it0 and offset itc of the
 // first rotor of the rotating type, p = n(it0).n(it0+itc) with the unit axes of MCCosine;

@@ -13,6 +13,13 @@
 //                                         SaveDensities1D/2D mc_estim.cc:1327-1444,1666-1727, SaveRCF :1141-1184)
 //   <...>_sum.gra/_sum.g2d/_sum.rcf       accumulated distributions after every block (mc_main.cc:446-462)
 //   stdout                                the accept-ratio line every MCSKIP_RATIO steps (MCSaveAcceptRatio :879-943)
+//   yw001.stat, yw001.conf, yw001.rand    the checkpoint after every block, the previous one kept as
+//                                         *.old (mc_main.cc:471-478, StatusIO/ConfigIO mc_input.cc:517-609)
+// RESTART in qmc.input resumes from the checkpoint: the start block, every chain's coordinates,
+// angles and direction cosines, and its generator state (the pass counter that addresses every
+// draw), so the resumed run continues exactly as the uninterrupted one.  The reference's
+// yw001.conf holds one coordinate row and no angles; here it holds all three rows of
+// coordinates, cosines and angles per chain: an int64 byte count, then the three arrays.
 // The atom-atom g(r) and the atom density around a linear rotor are binned on the device
 // (300 radial bins to 15 A, 50 polar bins); the 3D densities around a non-linear rotor
 // (SaveDensities3D, SaveRhoThetaChi) are not.
@@ -61,7 +68,7 @@ struct Deck {
     std::vector<Species> types;
     int P = 0, Prot = 0;
     long passes = 0, blocks = 0, eqblocks = 0, startblock = 0;
-    bool rotation = false, read_coords = false;
+    bool rotation = false, read_coords = false, restart = false;
     std::string rot_type;
     double rot_step = 0.0;
     long skip_ratio = 100000, skip_total = 10000, skip_averg = 1;  // mc_setup.cc:91-93
@@ -99,7 +106,7 @@ Deck read_deck(const std::string& path) {
         if (!s) die("IOReadParams", "missing or bad field after " + k);
     };
     while (in >> key) {
-        if (key == "RESTART") die("IOReadParams", "RESTART is not supported by this driver");
+        if (key == "RESTART") d.restart = true;
         else if (key == "MASTERDIR") { in >> d.master; need(in, key); }
         else if (key == "OUTPUTDIR") { in >> d.outdir; need(in, key); }
         else if (key == "FILENAMEPREFIX") { in >> d.prefix; need(in, key); }
@@ -540,6 +547,101 @@ void print_ratios(const Deck& d, Context& c, Chain& ch, int r, long block, long 
     std::cout << o.str() << std::endl;
 }
 
+// ------------------------------------------------------------------------------ checkpoint
+const char* const F_STATUS = "yw001.stat";
+const char* const F_CONFIG = "yw001.conf";
+const char* const F_RANDOM = "yw001.rand";
+
+bool file_exists(const char* name) { return std::ifstream(name).good(); }
+
+void back_up(const char* name) {  // IOFileBackUp: the previous file as <name>.old
+    if (!file_exists(name)) return;
+    std::ifstream src(name, std::ios::binary);
+    std::ofstream dst(std::string(name) + ".old", std::ios::binary | std::ios::trunc);
+    dst << src.rdbuf();
+}
+
+struct Checkpoint {
+    long block = 0;
+    unsigned seed = 0;
+    std::vector<unsigned> pass;
+    std::vector<std::vector<double>> coords, cosines, angles;
+};
+
+void write_checkpoint(const Checkpoint& k, size_t n3) {
+    back_up(F_STATUS);
+    back_up(F_CONFIG);
+    back_up(F_RANDOM);
+    {
+        std::ofstream f(F_STATUS, std::ios::trunc);
+        f << "STARTBLOCK " << k.block << "\nREPLICAS " << k.pass.size() << "\n";
+        if (!f.good()) die("StatusIO", std::string("cannot write ") + F_STATUS);
+    }
+    {
+        std::ofstream f(F_CONFIG, std::ios::binary | std::ios::trunc);
+        const long long bytes = (long long)(n3 * sizeof(double));
+        for (size_t r = 0; r < k.pass.size(); ++r) {
+            f.write(reinterpret_cast<const char*>(&bytes), sizeof bytes);
+            f.write(reinterpret_cast<const char*>(k.coords[r].data()), bytes);
+            f.write(reinterpret_cast<const char*>(k.cosines[r].data()), bytes);
+            f.write(reinterpret_cast<const char*>(k.angles[r].data()), bytes);
+        }
+        if (!f.good()) die("ConfigIO", std::string("cannot write ") + F_CONFIG);
+    }
+    {
+        std::ofstream f(F_RANDOM, std::ios::trunc);
+        f << "PHILOX4X32-10 " << k.seed << "\n";
+        for (size_t r = 0; r < k.pass.size(); ++r) f << r << " " << k.pass[r] << "\n";
+        if (!f.good()) die("RandomIO", std::string("cannot write ") + F_RANDOM);
+    }
+}
+
+Checkpoint read_checkpoint(int nrep, size_t n3) {
+    Checkpoint k;
+    {
+        std::ifstream f(F_STATUS);
+        if (!f.good()) die("StatusIO", std::string("RESTART: cannot open ") + F_STATUS);
+        std::string key;
+        long reps = -1;
+        while (f >> key) {
+            if (key == "STARTBLOCK") f >> k.block;
+            else if (key == "REPLICAS") f >> reps;
+            std::string rest;
+            std::getline(f, rest);
+        }
+        if (reps != nrep) die("StatusIO", "RESTART: the checkpoint holds " + std::to_string(reps) + " chains, not " + std::to_string(nrep));
+    }
+    {
+        std::ifstream f(F_RANDOM);
+        std::string gen;
+        if (!(f >> gen >> k.seed) || gen != "PHILOX4X32-10") die("RandomIO", std::string("RESTART: bad ") + F_RANDOM);
+        k.pass.resize(nrep);
+        for (int r = 0; r < nrep; ++r) {
+            int idx;
+            if (!(f >> idx >> k.pass[r]) || idx != r) die("RandomIO", std::string("RESTART: bad ") + F_RANDOM);
+        }
+    }
+    {
+        std::ifstream f(F_CONFIG, std::ios::binary);
+        if (!f.good()) die("ConfigIO", std::string("RESTART: cannot open ") + F_CONFIG);
+        k.coords.resize(nrep);
+        k.cosines.resize(nrep);
+        k.angles.resize(nrep);
+        for (int r = 0; r < nrep; ++r) {
+            long long bytes = 0;
+            f.read(reinterpret_cast<char*>(&bytes), sizeof bytes);
+            if (!f.good() || bytes != (long long)(n3 * sizeof(double)))
+                die("ConfigIO", std::string("RESTART: ") + F_CONFIG + " does not match the deck's particles and slices");
+            for (auto* v : {&k.coords[r], &k.cosines[r], &k.angles[r]}) {
+                v->resize(n3);
+                f.read(reinterpret_cast<char*>(v->data()), bytes);
+            }
+            if (!f.good()) die("ConfigIO", std::string("RESTART: ") + F_CONFIG + " is truncated");
+        }
+    }
+    return k;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -584,6 +686,16 @@ int main(int argc, char** argv) {
         return 0;
     }
 
+    Checkpoint resume;
+    if (d.restart) {
+        resume = read_checkpoint(nrep, 3 * (size_t)n);
+        seed = resume.seed;
+        d.startblock = resume.block;
+    } else if (file_exists(F_STATUS) || file_exists(F_CONFIG) || file_exists(F_RANDOM)) {
+        // a new run does not overwrite a checkpoint (mc_main.cc:126-136)
+        die("QMC", "a checkpoint (yw001.*) exists here: add RESTART to qmc.input to resume, or remove it");
+    }
+
     Context c;
     if (pimc_create(nrep, flags, seed, &c.ctx) != 0)
         die("pimc_create", c.ctx ? pimc_last_error(c.ctx) : "no HIP device");
@@ -591,6 +703,11 @@ int main(int argc, char** argv) {
     for (int r = 0; r < nrep; ++r) c.check(pimc_set_replica(c.ctx, r, &desc), "pimc_set_replica");
     c.check(pimc_commit(c.ctx), "pimc_commit");
     for (int r = 0; r < nrep; ++r) c.check(pimc_set_state(c.ctx, r, coords.data(), angles.data()), "pimc_set_state");
+    if (d.restart)
+        for (int r = 0; r < nrep; ++r)
+            c.check(pimc_restore_state(c.ctx, r, resume.coords[r].data(), resume.angles[r].data(),
+                                       resume.cosines[r].data(), resume.pass[r]),
+                    "pimc_restore_state");
 
     std::vector<Chain> chains(nrep);
     for (int r = 0; r < nrep; ++r) {
@@ -718,6 +835,20 @@ int main(int argc, char** argv) {
             }
             save_xyz(ch, d, cg, sg);
         }
+        // checkpoint of every chain after the block (its exact device state)
+        Checkpoint k;
+        k.block = block;
+        k.seed = seed;
+        k.pass.resize(nrep);
+        k.coords.assign(nrep, std::vector<double>(3 * (size_t)n));
+        k.cosines = k.coords;
+        k.angles = k.coords;
+        for (int r = 0; r < nrep; ++r) {
+            c.check(pimc_get_state(c.ctx, r, k.coords[r].data(), k.angles[r].data(), k.cosines[r].data()),
+                    "pimc_get_state");
+            c.check(pimc_get_pass(c.ctx, r, &k.pass[r]), "pimc_get_pass");
+        }
+        write_checkpoint(k, 3 * (size_t)n);
     }
     double pe = 0, bm = 0;
     c.check(pimc_get_work(c.ctx, &pe, &bm), "pimc_get_work");

@@ -65,6 +65,8 @@ SIGNATURES = {
     "pimc_set_option": (_i, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_long]),
     "pimc_set_state": (_i, [ctypes.c_void_p, _i, _dp, _dp]),
     "pimc_get_state": (_i, [ctypes.c_void_p, _i, _dp, _dp, _dp]),
+    "pimc_restore_state": (_i, [ctypes.c_void_p, _i, _dp, _dp, _dp, _u]),
+    "pimc_get_pass": (_i, [ctypes.c_void_p, _i, ctypes.POINTER(ctypes.c_uint)]),
     "pimc_run": (_i, [ctypes.c_void_p, _i, _i]),
     "pimc_get_samples": (_i, [ctypes.c_void_p, _i, _dp, _i]),
     "pimc_clear_samples": (_i, [ctypes.c_void_p]),
@@ -213,6 +215,16 @@ class Context:
         cs = np.empty(3 * n)
         self._check(self.lib.pimc_get_state(self.h, r, _dptr(c), _dptr(a), _dptr(cs)))
         return c.reshape(3, n), a.reshape(3, n), cs.reshape(3, n)
+
+    def restore_state(self, r, coords, angles, cosines, pass_counter):
+        coords, angles, cosines = _f64(coords), _f64(angles), _f64(cosines)
+        self._check(self.lib.pimc_restore_state(self.h, int(r), _dptr(coords), _dptr(angles), _dptr(cosines),
+                                                int(pass_counter)))
+
+    def pass_counter(self, r):
+        p = ctypes.c_uint()
+        self._check(self.lib.pimc_get_pass(self.h, int(r), ctypes.byref(p)))
+        return p.value
 
     def run(self, n_passes, sample_every=0):
         return self._check(self.lib.pimc_run(self.h, int(n_passes), int(sample_every)))

@@ -1,5 +1,6 @@
 """The host driver moribs-pimc_amd/pimc on the CPU: qmc.input parsing, refusals of methods
-outside the sampling path, the initial lattice, and a loud failure without a device."""
+outside the sampling path, the initial lattice, checkpoint guards, and a loud failure without
+a device."""
 import os
 import subprocess
 
@@ -87,7 +88,7 @@ def test_xyz_init_is_read(tmp_path):
 
 
 @pytest.mark.parametrize("extra,msg", [("WORM He4 0.13 16", "worm"), ("MINIMAGE", "MINIMAGE"),
-                                       ("RESTART", "RESTART"), ("CRYSTAL FCC 1 1 1", "crystal"),
+                                       ("CRYSTAL FCC 1 1 1", "crystal"),
                                        ("REFLECTY 1", "REFLECTY"), ("ROTDENSI 1 -1 0 1 1 1 1", "ROTDENSI")])
 def test_methods_outside_the_path_are_refused(tmp_path, extra, msg):
     write(tmp_path, extra=extra)
@@ -124,3 +125,17 @@ def test_no_device_fails_loudly(tmp_path):
     decks.materialise("co2_100K", str(tmp_path), blocks=3)
     r = run(tmp_path)
     assert r.returncode == 1 and "pimc_create" in r.stderr
+
+
+def test_restart_needs_a_checkpoint(tmp_path):
+    write(tmp_path, extra="RESTART")
+    r = run(tmp_path)
+    assert r.returncode == 1 and "yw001.stat" in r.stderr, r.stderr
+
+
+def test_new_run_keeps_an_existing_checkpoint(tmp_path):
+    """As the reference refuses to start over its status files (mc_main.cc:126-136)."""
+    write(tmp_path)
+    (tmp_path / "yw001.stat").write_text("STARTBLOCK 3\n")
+    r = run(tmp_path)
+    assert r.returncode == 1 and "RESTART" in r.stderr, r.stderr

@@ -111,3 +111,26 @@ def test_driver_output_layouts(tmp_path):
     assert np.allclose(c0[:, 1], cl[:, 1], atol=1e-9)                             # P_1(p) = p
     assert np.allclose(c0[:, 1], c0[::-1, 1], atol=1e-9)                          # symmetric in tau
     assert np.allclose(c0[:, 0], np.arange(65) / (0.37 * 64), rtol=1e-5)
+
+
+def test_restart_continues_exactly(tmp_path):
+    """Four blocks in one run against two blocks, then RESTART for two more: the resumed run's
+    .eng lines and final configuration are identical to the uninterrupted run's."""
+    a, b = tmp_path / "a", tmp_path / "b"
+    for d, blocks in ((a, 4), (b, 2)):
+        decks.materialise("he4_co2", str(d), blocks=blocks)
+        deck = (d / "qmc.input").read_text().replace(f"NUMBEROFBLOCKS     {blocks}  10", f"NUMBEROFBLOCKS     {blocks}  1")
+        (d / "qmc.input").write_text(deck)
+        r = subprocess.run([DRIVER], cwd=d, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+    assert (b / "yw001.stat").read_text().startswith("STARTBLOCK 2")
+    deck = (b / "qmc.input").read_text() + "RESTART\n"
+    (b / "qmc.input").write_text(deck)
+    r = subprocess.run([DRIVER], cwd=b, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    eng_a = (a / "he4co2.eng").read_text().splitlines()
+    eng_b = (b / "he4co2.eng").read_text().splitlines()
+    assert [ln.split()[0] for ln in eng_a] == ["2", "3", "4"]
+    assert eng_b == eng_a[1:]
+    assert (b / "he4co2.xyz").read_text() == (a / "he4co2.xyz").read_text()
+    assert (b / "yw001.stat.old").exists()

@@ -173,3 +173,23 @@ def test_mixed_families_in_one_context(rot3d):
         assert np.abs(cg - co).max() < 1e-9, r
         rot = rotor_columns(wl["replicas"][r])
         assert np.abs(ag[:2, rot] - ao[:2, rot]).max() < 1e-9, r
+
+
+@pytest.mark.parametrize("kind", ["h2o_h2", "water2"])
+def test_restore_state_continues_exactly(rot3d, kind):
+    """pimc_get_state + pimc_get_pass into a fresh context, then more passes: bit-identical to
+    one context running all passes (rotation matrices rebuilt on the device)."""
+    wl = nl_workload(kind, rot3d)
+    one = gpu_ctx(wl)
+    one.run(4)
+    two = gpu_ctx(wl)
+    two.run(2)
+    c, a, s = two.get_state(0)
+    p = two.pass_counter(0)
+    assert p == 2
+    three = gpu_ctx(wl)
+    three.restore_state(0, c, a, s, p)
+    three.run(2)
+    c1, a1, s1 = one.get_state(0)
+    c3, a3, s3 = three.get_state(0)
+    assert np.array_equal(c1, c3) and np.array_equal(a1, a3) and np.array_equal(s1, s3)
