@@ -57,6 +57,34 @@ struct Family {
     long n = 0;
 };
 
+// Workgroup i of a launch runs on XCD i mod 8.  The parts of one chain (consecutive in `parts`,
+// W of them) all go to one XCD, so their exchanges stay in one L2; chains are spread over the
+// XCDs by `work` (heaviest first, to the least loaded XCD) and the grid is padded with r = -1.
+template <typename Part, typename Work>
+std::vector<Part> xcd_order(const std::vector<Part>& parts, Work work) {
+    constexpr int NXCD = 8;
+    std::vector<std::vector<Part>> bins(NXCD);
+    std::vector<double> load(NXCD, 0.0);
+    for (size_t i = 0; i < parts.size();) {
+        const int W = parts[i].W;
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[x] += work(parts[i].r);
+        for (int w = 0; w < W; ++w) bins[x].push_back(parts[i + w]);
+        i += W;
+    }
+    size_t len = 0;
+    for (auto& b : bins) len = std::max(len, b.size());
+    std::vector<Part> grid(len * NXCD);
+    for (size_t j = 0; j < len; ++j)
+        for (int x = 0; x < NXCD; ++x) {
+            Part q{};
+            q.r = -1;
+            if (j < bins[x].size()) q = bins[x][j];
+            grid[j * NXCD + x] = q;
+        }
+    return grid;
+}
+
 template <typename T>
 T* dalloc(size_t n) {
     T* p = nullptr;
@@ -142,6 +170,11 @@ struct pimc_ctx {
     unsigned* d_flags = nullptr;  // half-sweeps completed per part
     int nparts = 0, nflags = 0;
     int opt_rotor_part_slices = 0;  // most rotor slices per part, 0: from the LDS budget
+    RotPart* d_cparts0 = nullptr;    // centroid parts (slice ranges) of the linear family
+    RotPart* d_cparts1 = nullptr;    // and of the non-linear family
+    int ncp0 = 0, ncp1 = 0, ncflags = 0;
+    unsigned* d_cflags = nullptr;    // atoms done per centroid part
+    double* d_cslots = nullptr;      // partial Delta V per part, two buffers
     // surrogate network of the delayed-acceptance rotor move
     bool opt_da = false;
     MlpPacked* d_mlp = nullptr;     // the sampler's packed copy
@@ -236,7 +269,7 @@ void pimc_destroy(pimc_ctx* c) {
                     c->d_t1, c->d_v2f, c->d_v2d, c->d_g2d, c->d_g2f, c->d_t2, c->d_rk, c->d_re, c->d_tr,
                     c->d_work, c->d_all, c->d_rotreps, c->d_est, c->d_rows, c->d_rowbuf, c->d_parts, c->d_flags,
                     c->d_r3, c->d_v3d, c->d_v3f, c->d_t3, c->d_frames, c->d_work1, c->d_all0, c->d_all1, c->d_rot3, c->d_rows1, c->d_mlp,
-                    c->d_rcf, c->d_rcfreps};
+                    c->d_rcf, c->d_rcfreps, c->d_cparts0, c->d_cparts1, c->d_cflags, c->d_cslots};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -598,33 +631,73 @@ int pimc_commit(pimc_ctx* c) {
     }
     if (lds_r > 160 * 1024) return fail(c, "rotor chain exceeds LDS");
     c->lds_rotor = lds_r;
-    // Workgroup i of a launch runs on XCD i mod 8.  All parts of a chain go to one XCD, so its
-    // boundary exchange and its .rot table stay in one L2; chains are spread over the XCDs by
-    // work (heaviest first, to the least loaded), and each XCD takes its parts in that order.
+    // parts of a chain on one XCD: its boundary exchange and its .rot table stay in one L2
+    c->nflags = (int)parts.size();
+    parts = xcd_order(parts, [&](int r) {
+        const Rep& p = c->rep[r];
+        return (double)p.P * p.Prot * p.natoms;
+    });
+    // centroid parts: each replica's slices split so that parts carry similar shares of the
+    // work sum N^2 P (about 1024 parts in all; at most 64 parts and at least 8 slices a part)
     {
-        constexpr int NXCD = 8;
-        std::vector<std::vector<RotPart>> bins(NXCD);
-        std::vector<double> load(NXCD, 0.0);
-        for (size_t i = 0; i < parts.size();) {
-            const Rep& p = c->rep[parts[i].r];
-            const int W = parts[i].W;
-            int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            load[x] += (double)p.P * p.Prot * p.natoms;
-            for (int w = 0; w < W; ++w) bins[x].push_back(parts[i + w]);
-            i += W;
+        // every part of a replica must be resident while its parts wait for one another: keep
+        // the parts within what the device holds at once
+        int ncu = 0, occ0 = 0, occ1 = 0;
+        hipDeviceProp_t prop;
+        HIPCHECK(c, hipGetDeviceProperties(&prop, 0));
+        ncu = prop.multiProcessorCount;
+        if (c->fp64) {
+            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, k_centroid_parts<double, 0>, 256, 0));
+            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_centroid_parts<double, 1>, 256, 0));
+        } else {
+            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, k_centroid_parts<float, 0>, 256, 0));
+            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_centroid_parts<float, 1>, 256, 0));
         }
-        size_t len = 0;
-        for (auto& b : bins) len = std::max(len, b.size());
-        std::vector<RotPart> grid(len * NXCD);
-        for (size_t j = 0; j < len; ++j)
-            for (int x = 0; x < NXCD; ++x) {
-                RotPart q{};
-                q.r = -1;
-                if (j < bins[x].size()) q = bins[x][j];
-                grid[j * NXCD + x] = q;
+        const double capacity = (double)ncu * std::max(1, std::min(occ0, occ1));
+        double target = std::min(1024.0, capacity / 2.0);
+        std::vector<RotPart> cp[2];
+        double total = 0.0;
+        for (int r = 0; r < c->nrep; ++r) total += (double)c->rep[r].natoms * c->rep[r].natoms * c->rep[r].P;
+        std::vector<int> order(c->nrep);
+        for (int r = 0; r < c->nrep; ++r) order[r] = r;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            const Rep &a = c->rep[x], &b = c->rep[y];
+            return (double)a.natoms * a.natoms * a.P > (double)b.natoms * b.natoms * b.P;
+        });
+        int fb = 0;
+        for (int r : order) {
+            const Rep& p = c->rep[r];
+            const double w = (double)p.natoms * p.natoms * p.P;
+            int W = (int)std::ceil(target * w / std::max(total, 1.0));
+            W = std::max(1, std::min({W, 64, std::max(1, p.P / 8)}));
+            for (int q = 0; q < W; ++q) {
+                RotPart x{};
+                x.r = r;
+                x.k0 = (int)((long long)p.P * q / W);
+                x.k1 = (int)((long long)p.P * (q + 1) / W);
+                x.w = q;
+                x.W = W;
+                x.fbase = fb;
+                cp[p.fam].push_back(x);
             }
-        c->nflags = (int)parts.size();
-        parts.swap(grid);
+            fb += W;
+        }
+        c->ncflags = fb;
+        auto cost = [&](int r) { return (double)c->rep[r].natoms * c->rep[r].natoms * c->rep[r].P; };
+        for (int f = 0; f < 2; ++f)
+            if (!cp[f].empty()) cp[f] = xcd_order(cp[f], cost);
+        c->ncp0 = (int)cp[0].size();
+        c->ncp1 = (int)cp[1].size();
+        c->d_cparts0 = dalloc<RotPart>(cp[0].size());
+        c->d_cparts1 = dalloc<RotPart>(cp[1].size());
+        c->d_cflags = dalloc<unsigned>((size_t)std::max(fb, 1));
+        c->d_cslots = dalloc<double>(2 * (size_t)std::max(fb, 1));
+        if (!c->d_cparts0 || !c->d_cparts1 || !c->d_cflags || !c->d_cslots) return fail(c, "hipMalloc failed");
+        HIPCHECK(c, hipMemset(c->d_cflags, 0, sizeof(unsigned) * std::max(fb, 1)));
+        if (!cp[0].empty())
+            HIPCHECK(c, hipMemcpy(c->d_cparts0, cp[0].data(), sizeof(RotPart) * cp[0].size(), hipMemcpyHostToDevice));
+        if (!cp[1].empty())
+            HIPCHECK(c, hipMemcpy(c->d_cparts1, cp[1].data(), sizeof(RotPart) * cp[1].size(), hipMemcpyHostToDevice));
     }
     c->nparts = (int)parts.size();
     c->d_parts = dalloc<RotPart>(parts.size());
@@ -973,12 +1046,14 @@ template <typename R>
 void launch_centroid(pimc_ctx* c, int type) {
     Run run{c->seed, 0, c->ref_order ? 1 : 0, type, type + 1};
     Timed t(c, "centroid");
-    if (c->nall0)
-        hipLaunchKernelGGL((k_centroid<R, 0>), dim3(c->nall0), dim3(1024), 0, c->stream, c->d_rep, c->d_mut,
-                           c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_all0, run);
-    if (c->nall1)
-        hipLaunchKernelGGL((k_centroid<R, 1>), dim3(c->nall1), dim3(1024), 0, c->stream, c->d_rep, c->d_mut,
-                           c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_all1, run);
+    if (c->ncp0)
+        hipLaunchKernelGGL((k_centroid_parts<R, 0>), dim3(c->ncp0), dim3(256), 0, c->stream, c->d_rep, c->d_mut,
+                           c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_cparts0, c->d_cflags,
+                           c->d_cslots, c->ncflags, run);
+    if (c->ncp1)
+        hipLaunchKernelGGL((k_centroid_parts<R, 1>), dim3(c->ncp1), dim3(256), 0, c->stream, c->d_rep, c->d_mut,
+                           c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_cparts1, c->d_cflags,
+                           c->d_cslots, c->ncflags, run);
 }
 
 template <typename R>
@@ -1050,6 +1125,7 @@ int check_errors(pimc_ctx* c) {
     for (int r = 0; r < c->nrep; ++r) {
         if (m[r].err & 2) return fail(c, "rotor sweep: a neighbouring part did not arrive (replica " + std::to_string(r) + ")");
         if (m[r].err & 4) return fail(c, "large matrix test error (relative Euler angle off the rotden grid, replica " + std::to_string(r) + ")");
+        if (m[r].err & 8) return fail(c, "centroid move: a part of the replica did not arrive (replica " + std::to_string(r) + ")");
         if (m[r].err) return fail(c, "Rotational Moves: Negative rot density (replica " + std::to_string(r) + ")");
     }
     return 0;

@@ -5,7 +5,8 @@
 //                axis there) is staged in LDS once and every atom of the selected types is
 //                moved in turn, as MCBisectionMove does (mc_piqmc.cc:194-294).  Segments of
 //                one offset share only fixed end points, so they run concurrently.
-//   k_centroid   one workgroup per replica, atoms in turn, Delta V over all slices x partners
+//   k_centroid_parts  a replica's slices split over workgroups; atoms in turn, Delta V over the
+//                part's slices x partners, partials exchanged through flags in memory
 //                (MCMolecularMove mc_piqmc.cc:54-102).
 //   k_rotor      one workgroup per part of a linear rotor's chain (a range of slices): the part's
 //                orientations and its rotor-partner geometry live in LDS for all sweeps of a
@@ -275,75 +276,6 @@ __global__ __launch_bounds__(256) void k_bisect_nl(const Rep* __restrict__ reps,
     bisect_body<R, 1>(reps, mut, beads, rotb, frames, T, T3, work, run);
 }
 
-// ------------------------------------------------------------------------- centroid
-template <typename R, int FAM>
-__global__ __launch_bounds__(1024) void k_centroid(const Rep* __restrict__ reps, Mut* __restrict__ mut,
-                                                   double4* __restrict__ beads, const RotBead* __restrict__ rotb,
-                                                   const Frame* __restrict__ frames, Tables T, Tables3 T3,
-                                                   const int* __restrict__ replist, Run run) {
-    __shared__ double red[16];
-    const int r = replist[blockIdx.x];
-    const Rep& rp = reps[r];
-    const int ty = run.type_begin;
-    if (ty >= rp.ntypes) return;
-    const uint32_t pass = mut[r].pass;
-    const int N = rp.natoms, P = rp.P, np = N - 1;
-    const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
-    const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
-    unsigned long long acc_n = 0;
-    for (int ia = 0; ia < rp.tnumb[ty]; ++ia) {
-        const int a = rp.tbegin[ty] + ia;
-        uint4 w0 = draw(run.seed, rp.key, pass, K_CENTROID, 0, (uint32_t)a, 0, 0);
-        uint4 w1 = draw(run.seed, rp.key, pass, K_CENTROID, 0, (uint32_t)a, 0, 1);
-        const double dx = rp.mcstep[ty] * (u53(w0.x, w0.y) - 0.5);
-        const double dy = rp.mcstep[ty] * (u53(w0.z, w0.w) - 0.5);
-        const double dz = rp.mcstep[ty] * (u53(w1.x, w1.y) - 0.5);
-        double acc = 0.0;
-        const int items = P * np;
-        const bool rot_a = FAM == 1 ? (rp.nl_atom >= 0 && a >= rp.nl_atom) : (a == rp.lin_atom);
-        for (int i = threadIdx.x; i < items; i += blockDim.x) {
-            int it = i / np, jp = i - it * np;
-            int j = jp + (jp >= a);
-            const bool rot_j = FAM == 1 ? (rp.nl_atom >= 0 && j >= rp.nl_atom) : (j == rp.lin_atom);
-            double4 xa = bead(beads, rp, a, it), xj = bead(beads, rp, j, it);
-            if constexpr (FAM == 1) {
-                const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
-                const Frame& fa = frame_at(frames, rp, rot_a ? a : rp.nl_atom, it);
-                const Frame& fj = frame_at(frames, rp, rot_j ? j : rp.nl_atom, it);
-                double vn = pair_nl<R>(l1, l1.k, l3, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z,
-                                       fa, fj);
-                double vo = pair_nl<R>(l1, l1.k, l3, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, fa, fj);
-                acc += vn - vo;
-            } else {
-                double nx, ny, nz;
-                rotor_axis(rotb, rp, it, nx, ny, nz);
-                R vn = pair_v<R>(l1, l1.k, l2, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, nx, ny,
-                                 nz);
-                R vo = pair_v<R>(l1, l1.k, l2, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz);
-                acc += (double)vn - (double)vo;
-            }
-        }
-        const double deltav = block_sum(acc, red);
-        bool accepted = false;
-        if (deltav < 0.0) accepted = true;
-        else if (exp(-deltav * rp.tau) > u53(w1.z, w1.w)) accepted = true;
-        if (accepted) {
-            acc_n += 1;
-            for (int it = threadIdx.x; it < P; it += blockDim.x) {
-                long long idx = rp.bead_off + (long long)a * P + it;
-                double4 b = beads[idx];
-                beads[idx] = make_double4(b.x + dx, b.y + dy, b.z + dz, 0.0);
-            }
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        atomicAdd(&mut[r].total[ty][0], (unsigned long long)rp.tnumb[ty]);
-        atomicAdd(&mut[r].accep[ty][0], acc_n);
-        atomicAdd(&mut[r].pair_evals[1], 2ull * (unsigned long long)rp.tnumb[ty] * P * (N - 1));
-    }
-}
-
 // ------------------------------------------------------------------------- linear rotor
 // The rotor chain of a replica is split into parts of contiguous slices [k0, k1) (even
 // length), one workgroup each.  A part builds the rotor-partner geometry of its translational
@@ -392,6 +324,117 @@ __device__ __forceinline__ double load_agent(const double* p) {
 __device__ __forceinline__ void store_agent(double* p, double x) {
     __hip_atomic_store((unsigned long long*)p, (unsigned long long)__double_as_longlong(x), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ------------------------------------------------------------------------- centroid, split
+// MCMolecularMove (mc_piqmc.cc:54-102) with a replica's slices split over W workgroups
+// (parts [k0, k1) of 0..P-1, RotPart reused): atoms in turn, each part sums Delta V over its
+// slices and all partners, publishes the partial, raises its flag, waits for the other parts'
+// flags (bounded spin), and every part then adds the W partials in part order -- the same
+// sum, so the same decision -- and moves its own slices of the atom.  Partner positions at a
+// part's slices change only inside that part, so no bead data crosses parts.  Partials are
+// double-buffered by atom parity: a part can overwrite a buffer only after every part has
+// raised for the following atom, that is after every part has read it.
+template <typename R, int FAM>
+__global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ reps, Mut* __restrict__ mut,
+                                                        double4* __restrict__ beads, const RotBead* __restrict__ rotb,
+                                                        const Frame* __restrict__ frames, Tables T, Tables3 T3,
+                                                        const RotPart* __restrict__ parts, unsigned* __restrict__ flags,
+                                                        double* __restrict__ slots, int nslots, Run run) {
+    __shared__ double red[4];
+    __shared__ double part_sum[64];
+    __shared__ double s_dv;
+    __shared__ int s_err;
+    const RotPart pt = parts[blockIdx.x];
+    if (pt.r < 0) return;  // padding of the XCD-ordered grid
+    const int r = pt.r;
+    const Rep& rp = reps[r];
+    const int ty = run.type_begin;
+    if (ty >= rp.ntypes) return;
+    const uint32_t pass = mut[r].pass;
+    const int N = rp.natoms, P = rp.P, np = N - 1;
+    const int t0 = pt.k0, nt = pt.k1 - pt.k0, W = pt.W;
+    const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
+    const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
+    unsigned* myflag = flags + pt.fbase + pt.w;
+    const unsigned base = __hip_atomic_load(myflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) s_err = 0;
+    unsigned long long acc_n = 0;
+    for (int ia = 0; ia < rp.tnumb[ty]; ++ia) {
+        const int a = rp.tbegin[ty] + ia;
+        uint4 w0 = draw(run.seed, rp.key, pass, K_CENTROID, 0, (uint32_t)a, 0, 0);
+        uint4 w1 = draw(run.seed, rp.key, pass, K_CENTROID, 0, (uint32_t)a, 0, 1);
+        const double dx = rp.mcstep[ty] * (u53(w0.x, w0.y) - 0.5);
+        const double dy = rp.mcstep[ty] * (u53(w0.z, w0.w) - 0.5);
+        const double dz = rp.mcstep[ty] * (u53(w1.x, w1.y) - 0.5);
+        double acc = 0.0;
+        const int items = nt * np;
+        const bool rot_a = FAM == 1 ? (rp.nl_atom >= 0 && a >= rp.nl_atom) : (a == rp.lin_atom);
+        for (int i = threadIdx.x; i < items; i += blockDim.x) {
+            const int it = t0 + i / np, jp = i % np;
+            const int j = jp + (jp >= a);
+            const bool rot_j = FAM == 1 ? (rp.nl_atom >= 0 && j >= rp.nl_atom) : (j == rp.lin_atom);
+            const double4 xa = bead(beads, rp, a, it), xj = bead(beads, rp, j, it);
+            if constexpr (FAM == 1) {
+                const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
+                const Frame& fa = frame_at(frames, rp, rot_a ? a : rp.nl_atom, it);
+                const Frame& fj = frame_at(frames, rp, rot_j ? j : rp.nl_atom, it);
+                const double vn = pair_nl<R>(l1, l1.k, l3, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y,
+                                             xj.z, fa, fj);
+                const double vo = pair_nl<R>(l1, l1.k, l3, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, fa, fj);
+                acc += vn - vo;
+            } else {
+                double nx, ny, nz;
+                rotor_axis(rotb, rp, it, nx, ny, nz);
+                const R vn = pair_v<R>(l1, l1.k, l2, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z,
+                                       nx, ny, nz);
+                const R vo = pair_v<R>(l1, l1.k, l2, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz);
+                acc += (double)vn - (double)vo;
+            }
+        }
+        const double partial = block_sum(acc, red);
+        double* buf = slots + (size_t)(ia & 1) * nslots + pt.fbase;
+        if (W == 1) {
+            if (threadIdx.x == 0) s_dv = partial;
+        } else {
+            if (threadIdx.x == 0) {
+                store_agent(buf + pt.w, partial);
+                raise_count(myflag, base + (unsigned)ia + 1u);
+            }
+            if (threadIdx.x < W) {
+                if (!wait_count(flags + pt.fbase + threadIdx.x, base + (unsigned)ia + 1u)) s_err = 8;
+                part_sum[threadIdx.x] = load_agent(buf + threadIdx.x);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double s = 0.0;
+                for (int q = 0; q < W; ++q) s += part_sum[q];
+                s_dv = s;
+            }
+        }
+        __syncthreads();
+        const double deltav = s_dv;
+        bool accepted = false;
+        if (deltav < 0.0) accepted = true;
+        else if (exp(-deltav * rp.tau) > u53(w1.z, w1.w)) accepted = true;
+        if (accepted) {
+            acc_n += 1;
+            for (int it = t0 + (int)threadIdx.x; it < t0 + nt; it += blockDim.x) {
+                const long long idx = rp.bead_off + (long long)a * P + it;
+                const double4 b = beads[idx];
+                beads[idx] = make_double4(b.x + dx, b.y + dy, b.z + dz, 0.0);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (s_err) atomicOr(&mut[r].err, s_err);
+        if (pt.w == 0) {
+            atomicAdd(&mut[r].total[ty][0], (unsigned long long)rp.tnumb[ty]);
+            atomicAdd(&mut[r].accep[ty][0], acc_n);
+        }
+        atomicAdd(&mut[r].pair_evals[1], 2ull * (unsigned long long)rp.tnumb[ty] * nt * np);
+    }
 }
 
 // Sum over a slice's partners of V(r, cos theta) with the rotor axis n: cos = -(n . dr)/r
