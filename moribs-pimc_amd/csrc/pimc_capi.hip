@@ -85,6 +85,10 @@ std::vector<Part> xcd_order(const std::vector<Part>& parts, Work work) {
     return grid;
 }
 
+// bisection size classes of the linear family: lanes per workgroup by atom count
+inline int bisect_class(int natoms) { return natoms <= 24 ? 0 : (natoms <= 64 ? 1 : 2); }
+inline int bisect_lanes(int k) { return 64 << k; }
+
 template <typename T>
 T* dalloc(size_t n) {
     T* p = nullptr;
@@ -123,7 +127,7 @@ struct pimc_ctx {
     unsigned* d_hist = nullptr;
     double* d_rcf = nullptr;      // rotational correlation sums of the rotating replicas
     int* d_rcfreps = nullptr;
-    int nrcf = 0;
+    int nrcf = 0, rcf_protmax = 1;
     long long nrcfvals = 0;
     double* d_samples = nullptr;
     int sample_cap = 0;
@@ -163,6 +167,12 @@ struct pimc_ctx {
     size_t lds_bisect = 0, lds_rotor = 0;
     int Smax = 1, Pmax = 1, Nmax = 1, max_types = 1;
     int bisect_block = 64;
+    // linear-family bisection in three size classes (N <= 24, <= 64, larger) with 64, 128 and
+    // 256 lanes and their own LDS sizes, launched as parallel branches of the pass
+    int bw_n[3] = {0, 0, 0}, bw_off[3] = {0, 0, 0};
+    size_t bw_lds[3] = {0, 0, 0};
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
     int2* d_rows = nullptr;     // (replica, atom) rows of the estimator pair sum
     double2* d_rowbuf = nullptr;
     int nrows = 0;
@@ -170,6 +180,11 @@ struct pimc_ctx {
     unsigned* d_flags = nullptr;  // half-sweeps completed per part
     int nparts = 0, nflags = 0;
     int opt_rotor_part_slices = 0;  // most rotor slices per part, 0: from the LDS budget
+    RotPart* d_parts3 = nullptr;     // non-linear chains split by slice range
+    unsigned* d_flags3 = nullptr;
+    double* d_pub3 = nullptr;        // boundary frames published per part
+    int nparts3 = 0, pub3_stride = 0;
+    size_t lds_parts3 = 0;
     RotPart* d_cparts0 = nullptr;    // centroid parts (slice ranges) of the linear family
     RotPart* d_cparts1 = nullptr;    // and of the non-linear family
     int ncp0 = 0, ncp1 = 0, ncflags = 0;
@@ -269,9 +284,15 @@ void pimc_destroy(pimc_ctx* c) {
                     c->d_t1, c->d_v2f, c->d_v2d, c->d_g2d, c->d_g2f, c->d_t2, c->d_rk, c->d_re, c->d_tr,
                     c->d_work, c->d_all, c->d_rotreps, c->d_est, c->d_rows, c->d_rowbuf, c->d_parts, c->d_flags,
                     c->d_r3, c->d_v3d, c->d_v3f, c->d_t3, c->d_frames, c->d_work1, c->d_all0, c->d_all1, c->d_rot3, c->d_rows1, c->d_mlp,
-                    c->d_rcf, c->d_rcfreps, c->d_cparts0, c->d_cparts1, c->d_cflags, c->d_cslots};
+                    c->d_rcf, c->d_rcfreps, c->d_cparts0, c->d_cparts1, c->d_cflags, c->d_cslots,
+                    c->d_parts3, c->d_flags3, c->d_pub3};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    for (int k = 0; k < 2; ++k) {
+        if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+        if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
+    }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -575,6 +596,7 @@ int pimc_commit(pimc_ctx* c) {
             p.rcf_off = rcfoff;
             rcfoff += (long long)RCF_L * p.Prot;
             rcfreps.push_back(r);
+            c->rcf_protmax = std::max(c->rcf_protmax, p.Prot);
         }
         p.row_off = rowoff;
         for (int a = 0; a < p.natoms; ++a) (p.fam ? rows1 : rows).push_back(make_int2(r, a));
@@ -584,6 +606,10 @@ int pimc_commit(pimc_ctx* c) {
         size_t knots = p.pot1d_id >= 0 ? (size_t)c->t1[p.pot1d_id].n * (c->fp64 ? sizeof(double4) : sizeof(float4)) : 0;
         const size_t lds_w = (size_t)(((3 * p.natoms * W + 9 * W + 16 + 3) & ~3) + 12 * p.nrot * W) * sizeof(double) + knots;
         (p.fam ? lds_b1 : lds_b) = std::max(p.fam ? lds_b1 : lds_b, lds_w);
+        if (!p.fam) {
+            const int k = bisect_class(p.natoms);
+            c->bw_lds[k] = std::max(c->bw_lds[k], lds_w);
+        }
         int nseg = c->ref_order ? 1 : p.P / S;
         for (int k = 0; k < nseg; ++k) (p.fam ? work1 : work).push_back(make_int2(r, k));
         c->Nmax = std::max(c->Nmax, p.natoms);
@@ -725,6 +751,90 @@ int pimc_commit(pimc_ctx* c) {
     };
     std::stable_sort(work.begin(), work.end(), heavier);
     std::stable_sort(work1.begin(), work1.end(), heavier);
+    {  // size classes of the linear family, heaviest class first in the work array
+        std::vector<int2> byc[3];
+        for (const int2& w : work) byc[bisect_class(c->rep[w.x].natoms)].push_back(w);
+        work.clear();
+        for (int k = 2; k >= 0; --k) {
+            c->bw_off[k] = (int)work.size();
+            c->bw_n[k] = (int)byc[k].size();
+            work.insert(work.end(), byc[k].begin(), byc[k].end());
+        }
+        int busy = 0;
+        for (int k = 0; k < 3; ++k) busy += c->bw_n[k] > 0;
+        if (busy > 1)
+            for (int k = 0; k < 2; ++k) {
+                HIPCHECK(c, hipStreamCreate(&c->side[k]));  // default priority, like the main stream
+                HIPCHECK(c, hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming));
+            }
+        if (busy > 1) HIPCHECK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    }
+    // Non-linear chains split by slice range when whole chains would leave CUs idle: as many
+    // parts (powers of two, at least 16 slices each) as the device holds resident at once, since
+    // the parts of a chain wait for one another.  Not with delayed acceptance, whose kernel runs
+    // whole chains.
+    if (!c->opt_da && !rot3.empty()) {
+        hipDeviceProp_t prop;
+        HIPCHECK(c, hipGetDeviceProperties(&prop, 0));
+        int protmax = 2, nrmax = 1;
+        for (int r : rot3) {
+            protmax = std::max(protmax, c->rep[r].Prot);
+            nrmax = std::max(nrmax, c->rep[r].nrot);
+        }
+        auto lds_for = [&](int ns) {
+            const int maxmoves = (ns + 1) / 2;
+            int G = 1;
+            while (G < 64 && G * 2 * maxmoves <= 256) G *= 2;
+            return (size_t)((nrmax * (ns + 1) + 1) & ~1) * sizeof(double) + (size_t)2 * nrmax * sizeof(Frame) +
+                   (size_t)(256 / G) * (nrmax + 1) * sizeof(Frame);
+        };
+        int W = 1;
+        for (int w = 2; w <= 64; w *= 2) {
+            const int ns = ((protmax + w - 1) / w + 1) & ~1;
+            if (ns < 16 || lds_for(ns) > 160 * 1024) break;
+            int occ = 0;
+            if (c->fp64) HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_rotor3d_parts<double>, 256, lds_for(ns)));
+            else HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_rotor3d_parts<float>, 256, lds_for(ns)));
+            if ((long long)w * (long long)rot3.size() > (long long)occ * prop.multiProcessorCount) break;
+            W = w;
+        }
+        if (W > 1) {
+            std::vector<RotPart> p3;
+            std::vector<int> whole;
+            for (int r : rot3) {
+                const Rep& p = c->rep[r];
+                const int ns = ((p.Prot + W - 1) / W + 1) & ~1;
+                const int Wr = (p.Prot + ns - 1) / ns;
+                if (Wr < 2) {
+                    whole.push_back(r);
+                    continue;
+                }
+                const int fbase = (int)p3.size();
+                for (int w = 0; w < Wr; ++w) {
+                    RotPart q{};
+                    q.r = r;
+                    q.k0 = w * ns;
+                    q.k1 = std::min(p.Prot, (w + 1) * ns);
+                    q.w = w;
+                    q.W = Wr;
+                    q.fbase = fbase;
+                    p3.push_back(q);
+                    c->lds_parts3 = std::max(c->lds_parts3, lds_for(q.k1 - q.k0));
+                }
+            }
+            rot3.swap(whole);
+            const int nflags3 = (int)p3.size();
+            c->pub3_stride = 2 * nrmax * (int)(sizeof(Frame) / sizeof(double));
+            p3 = xcd_order(p3, [&](int r) { return (double)c->rep[r].Prot * c->rep[r].nrot; });
+            c->nparts3 = (int)p3.size();
+            c->d_parts3 = dalloc<RotPart>(p3.size());
+            c->d_flags3 = dalloc<unsigned>((size_t)nflags3);
+            c->d_pub3 = dalloc<double>((size_t)nflags3 * c->pub3_stride);
+            if (!c->d_parts3 || !c->d_flags3 || !c->d_pub3) return fail(c, "hipMalloc failed");
+            HIPCHECK(c, hipMemcpy(c->d_parts3, p3.data(), sizeof(RotPart) * p3.size(), hipMemcpyHostToDevice));
+            HIPCHECK(c, hipMemset(c->d_flags3, 0, sizeof(unsigned) * nflags3));
+        }
+    }
     c->nwork = (int)work.size();
     c->nwork1 = (int)work1.size();
     c->nall0 = (int)all0.size();
@@ -1034,9 +1144,28 @@ template <typename R>
 void launch_bisect(pimc_ctx* c, int u, int tb, int te) {
     Run run{c->seed, u, c->ref_order ? 1 : 0, tb, te};
     Timed t(c, "bisect");
-    if (c->nwork)
-        hipLaunchKernelGGL(k_bisect<R>, dim3(c->nwork), dim3(c->bisect_block), c->lds_bisect, c->stream, c->d_rep,
-                           c->d_mut, c->d_beads, c->d_rotb, c->tables(), c->d_work, run);
+    // size classes: the heaviest present on the main stream, the others forked onto side
+    // streams and joined again (parallel branches of the captured pass)
+    int busy = 0, top = -1;
+    for (int k = 2; k >= 0; --k)
+        if (c->bw_n[k]) {
+            ++busy;
+            if (top < 0) top = k;
+        }
+    if (busy > 1) (void)hipEventRecord(c->ev_fork, c->stream);
+    int nside = 0;
+    for (int k = 2; k >= 0; --k) {
+        if (!c->bw_n[k]) continue;
+        hipStream_t s = c->stream;
+        if (k != top) {
+            s = c->side[nside];
+            (void)hipStreamWaitEvent(s, c->ev_fork, 0);
+        }
+        hipLaunchKernelGGL(k_bisect<R>, dim3(c->bw_n[k]), dim3(bisect_lanes(k)), c->bw_lds[k], s, c->d_rep, c->d_mut,
+                           c->d_beads, c->d_rotb, c->tables(), c->d_work + c->bw_off[k], run);
+        if (k != top) (void)hipEventRecord(c->ev_join[nside++], s);
+    }
+    for (int i = 0; i < nside; ++i) (void)hipStreamWaitEvent(c->stream, c->ev_join[i], 0);
     if (c->nwork1)
         hipLaunchKernelGGL(k_bisect_nl<R>, dim3(c->nwork1), dim3(c->bisect_block), c->lds_bisect1, c->stream,
                            c->d_rep, c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_work1, run);
@@ -1058,12 +1187,16 @@ void launch_centroid(pimc_ctx* c, int type) {
 
 template <typename R>
 void launch_rotor(pimc_ctx* c, int u) {
-    if (c->nparts == 0 && c->nrot3 == 0) return;
+    if (c->nparts == 0 && c->nrot3 == 0 && c->nparts3 == 0) return;
     Run run{c->seed, u, c->ref_order ? 1 : 0, 0, 0};
     Timed t(c, "rotor");
     if (c->nparts)
         hipLaunchKernelGGL(k_rotor<R>, dim3(c->nparts), dim3(256), c->lds_rotor, c->stream, c->d_rep, c->d_mut,
                            c->d_beads, c->d_rotb, c->tables(), c->tables3(), c->d_parts, c->d_flags, run);
+    if (c->nparts3)
+        hipLaunchKernelGGL(k_rotor3d_parts<R>, dim3(c->nparts3), dim3(256), c->lds_parts3, c->stream, c->d_rep,
+                           c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_parts3,
+                           c->d_flags3, c->d_pub3, c->pub3_stride, run);
     if (c->nrot3 && c->opt_da)
         hipLaunchKernelGGL(k_rotor3d_da<R>, dim3(c->nrot3), dim3(256), c->lds_rot3_da, c->stream, c->d_rep, c->d_mut,
                            c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rot3, run, c->d_mlp);
@@ -1085,7 +1218,8 @@ void launch_estimate(pimc_ctx* c, double* out) {
         hipLaunchKernelGGL((k_est_rows<R, 1>), dim3(c->nrows1), dim3(256), 0, c->stream, c->d_rep, c->d_beads,
                            c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rows1, out ? nullptr : c->d_hist, c->d_rowbuf);
     if (!out && c->nrcf)
-        hipLaunchKernelGGL(k_rcf, dim3(c->nrcf), dim3(256), 0, c->stream, c->d_rep, c->d_rotb, c->d_rcfreps, c->d_rcf);
+        hipLaunchKernelGGL(k_rcf, dim3(c->nrcf, (c->rcf_protmax + 255) / 256), dim3(256), 0, c->stream, c->d_rep,
+                           c->d_rotb, c->d_rcfreps, c->d_rcf);
     hipLaunchKernelGGL(k_est_final, dim3(c->nrep), dim3(256), 0, c->stream, c->d_rep, c->d_mut, c->d_rotb,
                        c->d_frames, c->tables(), c->tables3(),
                        c->d_all, c->d_rowbuf, c->d_samples, c->sample_cap, out);

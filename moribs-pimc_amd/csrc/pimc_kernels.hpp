@@ -162,7 +162,8 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                 nw[3 * St + tid] = src[a * W + St];
             }
             // every draw of this move at once: Gaussians of midpoint m (level-major order)
-            // and the level acceptance uniforms
+            // and the level acceptance uniforms u, kept as -log(u): the level test
+            // exp(-dV) > u (mc_piqmc.cc:265-273) is then dV < -log(u), a comparison
             const int nm = St - 1;
             for (int i = tid; i < 3 * nm + L; i += nth) {
                 if (i < 3 * nm) {
@@ -176,7 +177,7 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                     int level = i - 3 * nm;
                     uint4 w4 = draw(run.seed, rp.key, pass, K_BISECT, 0, (uint32_t)a, (uint32_t)s0,
                                     ((uint32_t)level << 24) | (0xFFFFu << 8));
-                    gw[i] = u53(w4.x, w4.y);
+                    gw[i] = -log(u53(w4.x, w4.y));
                 }
             }
             __syncthreads();
@@ -206,7 +207,7 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                     const bool rot_j = FAM == 1 ? (rp.nl_atom >= 0 && j >= rp.nl_atom) : (j == rp.lin_atom);
                     double xj = wx[j * W + slot], yj = wy[j * W + slot], zj = wz[j * W + slot];
                     double nx = nh[3 * slot], ny = nh[3 * slot + 1], nz = nh[3 * slot + 2];
-#pragma unroll 1
+#pragma unroll
                     for (int no = 0; no < 2; ++no) {  // new position, then old
                         double xa = no ? wx[a * W + slot] : nw[3 * slot];
                         double ya = no ? wy[a * W + slot] : nw[3 * slot + 1];
@@ -228,9 +229,7 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                 const double prev = cum;
                 cum += acc;
                 double deltav = (cum - 2.0 * prev) * (rp.tau * (double)half);
-                accepted = false;
-                if (deltav < 0.0) accepted = true;
-                else if (exp(-deltav) > gw[3 * nm + level]) accepted = true;
+                accepted = deltav < 0.0 || deltav < gw[3 * nm + level];
                 if (!accepted) break;
             }
             tot[ty] += 1;
@@ -842,6 +841,211 @@ __global__ __launch_bounds__(BLOCK) void k_rotor3d(const Rep* __restrict__ reps,
     }
 }
 
+// MCRotations3D with a chain split over W >= 2 workgroups by slice range [k0, k1) (even
+// length), as k_rotor splits linear chains: a half-sweep is a parallel step over the part's
+// slices with the moves of k_rotor3d.  Before its even half a part needs the frames of slice
+// k0-1 (the previous part's last, odd) and before its odd half those of slice k1 (the next
+// part's first, even).  After its even half a part publishes its first slice's frames, after its
+// odd half its last slice's (device-coherent stores from wave 0, then the part's flag), and it
+// recomputes its two boundary links from the frames it receives.  A published slot is
+// rewritten only after the neighbour has raised past the half that reads it.  Bounded waits set
+// error bit 2.
+// LDS: Lk[q][kl], kl = 0..ns (link kl joins local slices kl and kl+1; local slice 0 is k0-1 and
+// ns+1 is k1), the neighbours' frames fnb[2][nrot] (k0-1, then k1), then the groups' frames.
+template <typename R>
+__global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ reps, Mut* __restrict__ mut,
+                                                       const double4* __restrict__ beads, RotBead* __restrict__ rotb,
+                                                       Frame* __restrict__ frames, Tables T, Tables3 T3,
+                                                       const RotPart* __restrict__ parts, unsigned* __restrict__ flags,
+                                                       double* __restrict__ pub, int pub_stride, Run run) {
+    extern __shared__ double smem[];
+    __shared__ unsigned long long red_acc[4];
+    __shared__ int s_err;
+    const RotPart pt = parts[blockIdx.x];
+    if (pt.r < 0) return;  // padding of the XCD-ordered grid
+    const int r = pt.r;
+    const Rep& rp = reps[r];
+    if (rp.rotkind != 2 || rp.rot3d_id < 0) return;
+    const int Prot = rp.Prot, RR = rp.RotRatio, P = rp.P, N = rp.natoms, nrot = rp.nrot;
+    int nsweep, j0, jstride;
+    if (run.ref_order) {
+        if (run.u >= P) return;
+        nsweep = 1; j0 = run.u; jstride = 0;
+    } else {
+        if (run.u >= rp.S) return;
+        nsweep = P / rp.S; j0 = run.u; jstride = rp.S;
+    }
+    const int k0 = pt.k0, ns = pt.k1 - pt.k0, W = pt.W;
+    const int L = ns + 1;  // links per rotor
+    const int np = N - 1;
+    const int items = RR * np;
+    const long long rb0 = rp.rot_off;
+    const double4* tab = T3.r3 + (long long)rp.rot3d_id * 3 * ROT3D_SIZE;
+    const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
+    const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
+    const int maxmoves = (ns + 1) / 2;
+    int G = 1;
+    while (G < 64 && G * 2 * maxmoves <= (int)blockDim.x) G *= 2;
+    const int groups = blockDim.x / G;
+    const int g = threadIdx.x / G, gl = threadIdx.x % G;
+    double* Lk = smem;
+    Frame* fnb = reinterpret_cast<Frame*>(smem + ((nrot * L + 1) & ~1));
+    Frame* gf = fnb + 2 * nrot + (long long)g * (nrot + 1);
+    unsigned* myflag = flags + pt.fbase + pt.w;
+    unsigned* prevflag = flags + pt.fbase + (pt.w + W - 1) % W;
+    unsigned* nextflag = flags + pt.fbase + (pt.w + 1) % W;
+    constexpr int FD = (int)(sizeof(Frame) / sizeof(double));  // doubles per frame
+    // published frames: per part, the first slice's nrot frames, then the last slice's
+    double* mypub = pub + (long long)(pt.fbase + pt.w) * pub_stride;
+    const double* prevlast = pub + (long long)(pt.fbase + (pt.w + W - 1) % W) * pub_stride + nrot * FD;
+    const double* nextfirst = pub + (long long)(pt.fbase + (pt.w + 1) % W) * pub_stride;
+    const unsigned base = __hip_atomic_load(myflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t pass = mut[r].pass;
+    const int rtype = rp.imtype;
+    const double twopi = 2.0 * PI_D;
+    auto own = [&](int q, int k) -> const Frame& { return frames[rb0 + (long long)q * Prot + k]; };
+    if (threadIdx.x == 0) s_err = 0;
+    unsigned long long nacc = 0;
+    int err = 0;
+    // at launch start no neighbour has moved its boundary slice yet (its first move on it waits
+    // for this part), so the boundary frames come straight from memory
+    for (int i = threadIdx.x; i < 2 * nrot; i += blockDim.x) {
+        const int side = i / nrot, q = i - side * nrot;
+        int k = side ? k0 + ns : k0 - 1;
+        if (k < 0) k += Prot;
+        if (k >= Prot) k -= Prot;
+        fnb[i] = own(q, k);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nrot * L; i += blockDim.x) {
+        const int q = i / L, kl = i - q * L;
+        const Frame& fa = kl == 0 ? fnb[q] : own(q, k0 + kl - 1);
+        const Frame& fb = kl == ns ? fnb[nrot + q] : own(q, k0 + kl);
+        double er, es;
+        if (rotden_frames(fa, fb, tab, Lk[i], er, es)) err = 1;
+    }
+    __syncthreads();
+    for (int sw = 0; sw < nsweep; ++sw) {
+        const uint32_t jsweep = (uint32_t)(j0 + sw * jstride);
+        for (int parity = 0; parity < 2; ++parity) {
+            const unsigned h = 2u * (unsigned)sw + (unsigned)parity;  // half-sweeps done before this one
+            if (!(sw == 0 && parity == 0)) {
+                // even half: the previous part's odd half of the last sweep; odd half: the next
+                // part's even half of this sweep
+                if (threadIdx.x == 0 && !wait_count(parity == 0 ? prevflag : nextflag, base + h)) s_err = 2;
+                __syncthreads();
+                const double* src = parity == 0 ? prevlast : nextfirst;
+                double* dst = reinterpret_cast<double*>(fnb + (parity == 0 ? 0 : nrot));
+                for (int i = threadIdx.x; i < nrot * FD; i += blockDim.x) dst[i] = load_agent(src + i);
+                __syncthreads();
+                for (int q = threadIdx.x; q < nrot; q += blockDim.x) {
+                    double er, es;
+                    if (parity == 0) {
+                        if (rotden_frames(fnb[q], own(q, k0), tab, Lk[q * L], er, es)) err = 1;
+                    } else if (rotden_frames(own(q, k0 + ns - 1), fnb[nrot + q], tab, Lk[q * L + ns], er, es)) {
+                        err = 1;
+                    }
+                }
+                __syncthreads();
+            }
+            const int nmoves = (ns - parity + 1) / 2;  // k0 is even: local slice kl has parity (kl - 1) & 1
+            for (int mv = g; mv < nmoves; mv += groups) {
+                const int kl = 2 * mv + parity + 1;
+                const int k = k0 + kl - 1;
+                for (int q = gl; q < nrot; q += G) gf[q] = own(q, k);
+                __builtin_amdgcn_wave_barrier();
+                for (int q = 0; q < nrot; ++q) {
+                    const int gatom = rp.nl_atom + q;
+                    const long long ik = rb0 + (long long)q * Prot + k;
+                    const RotBead cur = rotb[ik];
+                    const uint4 w0 = draw(run.seed, rp.key, pass, K_ROT3D, jsweep, (uint32_t)gatom, (uint32_t)k, 0);
+                    const uint4 w1 = draw(run.seed, rp.key, pass, K_ROT3D, jsweep, (uint32_t)gatom, (uint32_t)k, 1);
+                    const double rand1 = u53(w0.x, w0.y), rand2 = u53(w0.z, w0.w);
+                    const double rand3 = u53(w1.x, w1.y), rand4 = u53(w1.z, w1.w);
+                    double cost = cur.cth + rp.rtstep * (rand1 - 0.5);
+                    double phi = cur.phi + twopi * (rp.rtstep * (rand2 - 0.5));
+                    double chi = cur.chi + twopi * (rp.rtstep * (rand3 - 0.5));
+                    if (phi < 0.0) phi = twopi + phi;
+                    if (chi < 0.0) chi = twopi + chi;
+                    phi = fmod(phi, twopi);
+                    chi = fmod(chi, twopi);
+                    if (cost > 1.0) cost = 2.0 - cost;
+                    if (cost < -1.0) cost = -2.0 - cost;
+                    if (gl == 0) gf[nrot] = make_frame(phi, cost, chi);
+                    __builtin_amdgcn_wave_barrier();
+                    double d[4] = {Lk[q * L + kl - 1], Lk[q * L + kl], 0.0, 0.0};
+                    for (int c = 2 + gl; c < 4; c += G) {
+                        const Frame& fa = (c == 3) ? gf[nrot] : (kl == 1 ? fnb[q] : own(q, k - 1));
+                        const Frame& fb = (c == 2) ? gf[nrot] : (kl == ns ? fnb[nrot + q] : own(q, k + 1));
+                        double er, es;
+                        if (rotden_frames(fa, fb, tab, d[c], er, es)) err = 1;
+                    }
+                    if (G > 1)
+                        for (int c = 2; c < 4; ++c) d[c] = __shfl(d[c], (c - 2) % G, G);
+                    double eo = 0.0, en = 0.0;
+                    for (int i = gl; i < items; i += G) {
+                        const int it = k * RR + i / np, jp = i % np;
+                        const int j = jp + (jp >= gatom);
+                        const bool rj = j >= rp.nl_atom;
+                        const double4 xa = bead(beads, rp, gatom, it), xj = bead(beads, rp, j, it);
+                        const Frame& fj = gf[rj ? j - rp.nl_atom : q];
+                        eo += pair_nl<R>(l1, l1.k, l3, true, rj, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, gf[q], fj);
+                        en += pair_nl<R>(l1, l1.k, l3, true, rj, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, gf[nrot], fj);
+                    }
+                    for (int o = G >> 1; o > 0; o >>= 1) {
+                        eo += __shfl_xor(eo, o, G);
+                        en += __shfl_xor(en, o, G);
+                    }
+                    double dold = d[0] * d[1];
+                    if (fabs(dold) < RZERO) dold = 0.0;
+                    if (dold < 0.0) dold = fabs(dold);
+                    double dnew = d[2] * d[3];
+                    if (fabs(dnew) < RZERO) dnew = 0.0;
+                    if (dnew < 0.0) dnew = fabs(dnew);
+                    double rd = (dold > RZERO) ? dnew / dold : 1.0;
+                    rd *= exp(-rp.tau * (en - eo));
+                    const bool accepted = rd > 1.0 || rd > rand4;
+                    if (accepted && gl == 0) {
+                        ++nacc;
+                        const double sint = sqrt(1.0 - cost * cost);
+                        RotBead b = cur;
+                        b.cth = cost;
+                        b.phi = phi;
+                        b.chi = chi;
+                        b.nx = sint * cos(phi);
+                        b.ny = sint * sin(phi);
+                        b.nz = cost;
+                        rotb[ik] = b;
+                        frames[ik] = gf[nrot];
+                        gf[q] = gf[nrot];
+                        Lk[q * L + kl - 1] = d[2];
+                        Lk[q * L + kl] = d[3];
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            __syncthreads();
+            // publish the boundary slice this half moved, then raise the flag
+            if (threadIdx.x < 64) {
+                const int ks = parity == 0 ? k0 : k0 + ns - 1;
+                double* dst = mypub + (parity == 0 ? 0 : nrot * FD);
+                for (int i = threadIdx.x; i < nrot * FD; i += 64)
+                    store_agent(dst + i, reinterpret_cast<const double*>(&own(i / FD, ks))[i % FD]);
+                if (threadIdx.x == 0) raise_count(myflag, base + h + 1u);
+            }
+        }
+    }
+    if (err) atomicOr(&mut[r].err, 4);
+    unsigned long long sacc = block_sum(nacc, red_acc);
+    if (threadIdx.x == 0) {
+        if (s_err) atomicOr(&mut[r].err, s_err);
+        const unsigned long long moves = (unsigned long long)nsweep * ns * nrot;
+        atomicAdd(&mut[r].total[rtype][2], moves);
+        atomicAdd(&mut[r].accep[rtype][2], sacc);
+        atomicAdd(&mut[r].pair_evals[2], 2ull * moves * (unsigned long long)items);
+    }
+}
+
 // ------------------------------------------------------------------------- restart
 // Rotation matrices of every non-linear rotor slice of replica r from its stored angles, with
 // the code the moves use when they accept (make_frame), so a resumed chain sees the same bits.
@@ -859,14 +1063,14 @@ __global__ void k_frames_from_angles(const Rep* __restrict__ reps, const RotBead
 // first rotor of the rotating type, p = n(it0).n(it0+itc) with the unit axes of MCCosine;
 // rcf[0][itc] += p and rcf[l][itc] += P_l(p) for l = 1..9.  (The reference adds 1.0 for l >= 1:
 // its Legendre call is commented out; here P_l comes from the three-term recurrence.)
-// One workgroup per replica, a lane per offset, accumulated in FP64.
+// Grid (replica, offset block of 256): a lane per offset, accumulated in FP64.
 __global__ __launch_bounds__(256) void k_rcf(const Rep* __restrict__ reps, const RotBead* __restrict__ rotb,
                                              const int* __restrict__ list, double* __restrict__ rcf) {
     const Rep& rp = reps[list[blockIdx.x]];
     const int Prot = rp.Prot;
     const RotBead* b = rotb + rp.rot_off;
     double* out = rcf + rp.rcf_off;
-    for (int itc = threadIdx.x; itc < Prot; itc += blockDim.x) {
+    for (int itc = blockIdx.y * blockDim.x + threadIdx.x; itc < Prot; itc += gridDim.y * blockDim.x) {
         double acc[RCF_L];
         for (int l = 0; l < RCF_L; ++l) acc[l] = 0.0;
         int t1 = itc;
@@ -1157,17 +1361,22 @@ __global__ __launch_bounds__(256) void k_est_rows(const Rep* __restrict__ reps, 
             R cost = ((R)nx * dx + (R)ny * dy + (R)nz * dz) / rr;
             if (m0 == 1) cost = -cost;
             pot += (double)lpot2d<R>(l2, rr, cost);
-            if (g2h) {
-                int br = (int)floor((double)rr / drad);
+            if (g2h) {  // bins from FP64 geometry, as bin_2Ddensity gets it (mc_estim.cc:1233-1250)
+                const double ex = b0.x - b1.x, ey = b0.y - b1.y, ez = b0.z - b1.z;
+                const double rd = sqrt(ex * ex + ey * ey + ez * ez);
+                double cd = (nx * ex + ny * ey + nz * ez) / rd;
+                if (m0 == 1) cd = -cd;
+                int br = (int)floor(rd / drad);
                 if (br < BINSR && br >= 0) {
-                    int bt = (int)floor(acos((double)cost) / dth);
+                    int bt = (int)floor(acos(cd) / dth);
                     if (bt < BINST && bt >= 0) atomicAdd(&g2h[br * BINST + bt], 1u);
                 }
             }
         } else if (t0 == t1 && m0 == 0) {
             pot += (double)spot1d<R>(l1, l1.k, rr);
-            if (g1) {
-                int br = (int)floor((double)rr / drad);
+            if (g1) {  // bin_1Ddensity (mc_estim.cc:1275-1286) on the FP64 distance
+                const double ex = b0.x - b1.x, ey = b0.y - b1.y, ez = b0.z - b1.z;
+                int br = (int)floor(sqrt(ex * ex + ey * ey + ez * ez) / drad);
                 if (br < BINSR && br >= 0) atomicAdd(&g1[br], 1u);
             }
         }

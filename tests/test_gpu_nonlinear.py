@@ -193,3 +193,38 @@ def test_restore_state_continues_exactly(rot3d, kind):
     c1, a1, s1 = one.get_state(0)
     c3, a3, s3 = three.get_state(0)
     assert np.array_equal(c1, c3) and np.array_equal(a1, a3) and np.array_equal(s1, s3)
+
+
+@pytest.mark.parametrize("ref_order", [False, True])
+@pytest.mark.parametrize("kind", ["h2o_h2", "water2"])
+def test_nl_trajectory_split_chain_fp64(rot3d, kind, ref_order):
+    """P = Prot = 64: the rotor chain is split into parts of 16 slices that trade boundary frames
+    each half-sweep; FP64 trajectories still follow the oracle move for move."""
+    wl = nl_workload(kind, rot3d, n_h2=6, P=64, Prot=64, seed=5)
+    flags = ph.FLAG_FP64 | (ph.FLAG_REF_ORDER if ref_order else 0)
+    ctx = gpu_ctx(wl, flags)
+    ctx.run(2)
+    S = oracle_system(wl)
+    S.run(0, 2, order=O.ORDER_REFERENCE if ref_order else O.ORDER_SEGMENTS)
+    cg, ag, _ = ctx.get_state(0)
+    co, ao, _ = S.state()
+    tg, acg = ctx.counters(0)
+    to, aco = S.counters()
+    assert np.array_equal(tg, to) and np.array_equal(acg, aco), (acg, aco)
+    assert np.abs(cg - co).max() < 1e-9
+    rot = rotor_columns(wl["replicas"][0])
+    assert np.abs(ag[:, rot] - ao[:, rot]).max() < 1e-9
+
+
+def test_split_chain_restore_continues_exactly(rot3d):
+    wl = nl_workload("water2", rot3d, P=64, Prot=64, seed=5)
+    one = gpu_ctx(wl)
+    one.run(3)
+    two = gpu_ctx(wl)
+    two.run(1)
+    c, a, s = two.get_state(0)
+    three = gpu_ctx(wl)
+    three.restore_state(0, c, a, s, two.pass_counter(0))
+    three.run(2)
+    for x, y in zip(one.get_state(0), three.get_state(0)):
+        assert np.array_equal(x, y)
