@@ -184,9 +184,9 @@ def traffic_per_launch(workload_tag, kernel):
 
 
 KERNEL_NAMES = {"bisect": "k_bisect<float>", "centroid": "k_centroid_parts<float,0>", "rotor": "k_rotor<float>",
-                "estimate": "k_est_rows<float,0>"}
+                "estimate": "k_est_tiles<float,0>"}
 KERNEL_NAMES_NL = {"bisect": "k_bisect_nl<float>", "centroid": "k_centroid_parts<float,1>", "rotor": "k_rotor3d<float>",
-                   "estimate": "k_est_rows<float,1>"}
+                   "estimate": "k_est_tiles<float,1>"}
 WORKLOAD_TAG = {"c2": "c2_256replicas", "c1": "c1_64he", "c0": "c0_8he", "c3": "c3_asymtop_32h2",
                 "c4": "c4_water2_128replicas"}
 

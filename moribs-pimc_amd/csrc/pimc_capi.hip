@@ -173,9 +173,10 @@ struct pimc_ctx {
     size_t bw_lds[3] = {0, 0, 0};
     hipStream_t side[2] = {nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
-    int2* d_rows = nullptr;     // (replica, atom) rows of the estimator pair sum
-    double2* d_rowbuf = nullptr;
+    int2* d_rows = nullptr;     // (replica, tile) work of the estimator pair sum
+    double2* d_rowbuf = nullptr;  // a (potential, kinetic) partial per tile
     int nrows = 0;
+    size_t lds_est = 0;
     RotPart* d_parts = nullptr;  // rotor chains split into slice ranges, one workgroup each
     unsigned* d_flags = nullptr;  // half-sweeps completed per part
     int nparts = 0, nflags = 0;
@@ -598,10 +599,18 @@ int pimc_commit(pimc_ctx* c) {
             rcfreps.push_back(r);
             c->rcf_protmax = std::max(c->rcf_protmax, p.Prot);
         }
+        // estimator tiles: the largest power of two of slices whose window of all atoms
+        // (one slice more than the tile) stays within 1152 beads, or two slices
+        int ts = 1;
+        while (2 * ts <= p.P && (long long)p.natoms * (2 * ts + 1) <= std::max(1152, 2 * p.natoms)) ts *= 2;
+        p.est_ts = ts;
+        const int ntiles = (p.P + ts - 1) / ts;
         p.row_off = rowoff;
-        for (int a = 0; a < p.natoms; ++a) (p.fam ? rows1 : rows).push_back(make_int2(r, a));
+        for (int k = 0; k < ntiles; ++k) (p.fam ? rows1 : rows).push_back(make_int2(r, k));
         (p.fam ? all1 : all0).push_back(r);
-        rowoff += p.natoms;
+        rowoff += ntiles;
+        c->lds_est = std::max(c->lds_est, (size_t)(3 * p.natoms * (ts + 1) + 3 * ts) * sizeof(double) +
+                                              (size_t)p.natoms * sizeof(int));
         int W = S + 1;
         size_t knots = p.pot1d_id >= 0 ? (size_t)c->t1[p.pot1d_id].n * (c->fp64 ? sizeof(double4) : sizeof(float4)) : 0;
         const size_t lds_w = (size_t)(((3 * p.natoms * W + 9 * W + 16 + 3) & ~3) + 12 * p.nrot * W) * sizeof(double) + knots;
@@ -619,6 +628,7 @@ int pimc_commit(pimc_ctx* c) {
     }
     if (std::max(lds_b, lds_b1) > 160 * 1024)
         return fail(c, "segment window exceeds LDS (too many atoms for this segment size)");
+    if (c->lds_est > 150 * 1024) return fail(c, "estimator tile exceeds LDS (too many atoms)");
     c->lds_bisect = lds_b;
     c->lds_bisect1 = lds_b1;
     // rotor parts: slice ranges whose partner geometry (vec4 per translational slice and partner)
@@ -1212,11 +1222,13 @@ template <typename R>
 void launch_estimate(pimc_ctx* c, double* out) {
     Timed t(c, "estimate");
     if (c->nrows)
-        hipLaunchKernelGGL((k_est_rows<R, 0>), dim3(c->nrows), dim3(256), 0, c->stream, c->d_rep, c->d_beads,
-                           c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rows, out ? nullptr : c->d_hist, c->d_rowbuf);
+        hipLaunchKernelGGL((k_est_tiles<R, 0>), dim3(c->nrows), dim3(256), c->lds_est, c->stream, c->d_rep,
+                           c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rows,
+                           out ? nullptr : c->d_hist, c->d_rowbuf);
     if (c->nrows1)
-        hipLaunchKernelGGL((k_est_rows<R, 1>), dim3(c->nrows1), dim3(256), 0, c->stream, c->d_rep, c->d_beads,
-                           c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rows1, out ? nullptr : c->d_hist, c->d_rowbuf);
+        hipLaunchKernelGGL((k_est_tiles<R, 1>), dim3(c->nrows1), dim3(256), c->lds_est, c->stream, c->d_rep,
+                           c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rows1,
+                           out ? nullptr : c->d_hist, c->d_rowbuf);
     if (!out && c->nrcf)
         hipLaunchKernelGGL(k_rcf, dim3(c->nrcf, (c->rcf_protmax + 255) / 256), dim3(256), 0, c->stream, c->d_rep,
                            c->d_rotb, c->d_rcfreps, c->d_rcf);

@@ -38,9 +38,11 @@ struct Rep {
     int pot3d_id;  // 3D table of the atom / non-linear rotor pairs, -1 if none
     int rot3d_id;  // rho/E/E^2 tables of the non-linear rotor, -1 if none
     int key;       // replica index, second word of the generator key
+    int est_ts;    // slices per estimator tile (a power of two)
     double lambda[MAXT], mcstep[MAXT];
     double T, beta, tau, rtstep;
-    long long bead_off, rot_off, hist_off, sample_off, row_off;
+    long long bead_off, rot_off, hist_off, sample_off;
+    long long row_off;  // first estimator tile partial of the replica
     long long rcf_off;  // [RCF_L][Prot] accumulators of the rotational correlation functions, -1 if none
 };
 constexpr int RCF_L = 10;  // NUMB_RCF mc_confg.h:58

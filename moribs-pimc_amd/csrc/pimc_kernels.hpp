@@ -13,10 +13,10 @@
 //                launch; each half of the even/odd checkerboard (mc_piqmc.cc:490-512) is a
 //                parallel step over slices, a lane group per slice sums Delta V over partners
 //                (MCRotLinStep :781-936); parts trade one boundary slice per half-sweep.
-//   k_est_rows   one workgroup per (replica, atom a0): the a0 row of the pair sum of
-//                GetPotEnergy_Densities (mc_estim.cc:500-687, pairs a0 < a1, all slices, with
-//                the g(r) / g(r,theta) bins) and a0's term of GetKinEnergy (mc_estim.cc:876-937).
-//   k_est_final  one workgroup per replica: rows summed in order (deterministic), GetRotEnergy
+//   k_est_tiles  one workgroup per (replica, tile of slices): the tile's part of the pair sum of
+//                GetPotEnergy_Densities (mc_estim.cc:500-687, every pair, with the g(r) /
+//                g(r,theta) bins) and of GetKinEnergy (mc_estim.cc:876-937), beads read once.
+//   k_est_final  one workgroup per replica: tiles summed in order (deterministic), GetRotEnergy
 //                (mc_estim.cc:939-987) and the Cv terms of MCGetAverage (mc_main.cc:551-616).
 //   k_pot_slice  batched PotEnergy(atom, pos, it) (mc_piqmc.cc:1796-1965).
 //   k_rotor3d_da the non-linear rotor sweep with a surrogate first stage (delayed acceptance).
@@ -200,7 +200,35 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                 const int items = nmid * np;
                 double acc = 0.0;
                 const bool rot_a = FAM == 1 ? (rp.nl_atom >= 0 && a >= rp.nl_atom) : (a == rp.lin_atom);
-                for (int i = tid; i < items; i += nth) {
+                // an atom with the linear rotor last among its partners: the atom-atom terms and
+                // the rotor terms in separate loops, so no wave runs both tables' code
+                const bool split = FAM == 0 && !rot_a && rp.lin_atom == N - 1;
+                if (split) {  // items: nmid*(np-1) atom-atom terms first, then the nmid rotor terms
+                    const int np1 = np - 1, n1 = nmid * np1;
+                    for (int i = tid; i < items; i += nth) {
+                        const bool rj = i >= n1;
+                        const int q = rj ? i - n1 : i / np1, jp = rj ? 0 : i - q * np1;
+                        const int j = rj ? N - 1 : jp + (jp >= a);
+                        const int slot = half + q * lss;
+                        const double xj = wx[j * W + slot], yj = wy[j * W + slot], zj = wz[j * W + slot];
+                        double nx = 0.0, ny = 0.0, nz = 1.0;
+                        if (rj) {
+                            nx = nh[3 * slot];
+                            ny = nh[3 * slot + 1];
+                            nz = nh[3 * slot + 2];
+                        }
+#pragma unroll
+                        for (int no = 0; no < 2; ++no) {
+                            const double xa = no ? wx[a * W + slot] : nw[3 * slot];
+                            const double ya = no ? wy[a * W + slot] : nw[3 * slot + 1];
+                            const double za = no ? wz[a * W + slot] : nw[3 * slot + 2];
+                            const R v = rj ? pair_v<R>(l1, kn, l2, false, true, xa, ya, za, xj, yj, zj, nx, ny, nz)
+                                           : pair_v<R>(l1, kn, l2, false, false, xa, ya, za, xj, yj, zj, 0.0, 0.0, 1.0);
+                            acc += no ? -(double)v : (double)v;
+                        }
+                    }
+                }
+                for (int i = split ? items : tid; i < items; i += nth) {
                     int q = i / np, jp = i - q * np;
                     int j = jp + (jp >= a);
                     int slot = half + q * lss;
@@ -369,7 +397,31 @@ __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ 
         double acc = 0.0;
         const int items = nt * np;
         const bool rot_a = FAM == 1 ? (rp.nl_atom >= 0 && a >= rp.nl_atom) : (a == rp.lin_atom);
-        for (int i = threadIdx.x; i < items; i += blockDim.x) {
+        // an atom with the linear rotor last among its partners: atom-atom and rotor terms in
+        // separate loops (no wave runs both tables' code)
+        const bool split = FAM == 0 && !rot_a && rp.lin_atom == N - 1;
+        if (split) {  // items: nt*(np-1) atom-atom terms first, then the nt rotor terms
+            const int np1 = np - 1, n1 = nt * np1;
+            for (int i = threadIdx.x; i < items; i += blockDim.x) {
+                const bool rj = i >= n1;
+                const int it = t0 + (rj ? i - n1 : i / np1), jp = rj ? 0 : i % np1;
+                const int j = rj ? N - 1 : jp + (jp >= a);
+                const double4 xa = bead(beads, rp, a, it), xj = bead(beads, rp, j, it);
+                double nx = 0.0, ny = 0.0, nz = 1.0;
+                if (rj) rotor_axis(rotb, rp, it, nx, ny, nz);
+                R vn, vo;
+                if (rj) {
+                    vn = pair_v<R>(l1, l1.k, l2, false, true, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, nx, ny, nz);
+                    vo = pair_v<R>(l1, l1.k, l2, false, true, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz);
+                } else {
+                    vn = pair_v<R>(l1, l1.k, l2, false, false, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, 0.0,
+                                   0.0, 1.0);
+                    vo = pair_v<R>(l1, l1.k, l2, false, false, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, 0.0, 0.0, 1.0);
+                }
+                acc += (double)vn - (double)vo;
+            }
+        }
+        for (int i = split ? items : (int)threadIdx.x; i < items; i += blockDim.x) {
             const int it = t0 + i / np, jp = i % np;
             const int j = jp + (jp >= a);
             const bool rot_j = FAM == 1 ? (rp.nl_atom >= 0 && j >= rp.nl_atom) : (j == rp.lin_atom);
@@ -1309,81 +1361,118 @@ __global__ void k_pass_end(Mut* mut, int n) {
 }
 
 // ------------------------------------------------------------------------- estimators
-// rows[row_off + a0] = (pair potential of row a0 summed over slices, kinetic term of a0)
+// One workgroup per (replica, tile of est_ts slices).  All atoms' beads on the tile's slices and
+// one slice past it are staged in LDS once; lanes then take every pair at every slice of the
+// tile, the pairs enumerated as (a, a + d mod N) for d = 1 .. N/2 so that a tile has the same
+// work at any N.  g(r) is counted in LDS and added to the replica's bins once per tile.
+// part[row_off + tile] = (pair potential of the tile, its kinetic links / (4 beta lambda)).
+// The potential and the bins follow GetPotEnergy_Densities (mc_estim.cc:500-687), the kinetic
+// term GetKinEnergy (mc_estim.cc:876-937).
 template <typename R, int FAM>
-__global__ __launch_bounds__(256) void k_est_rows(const Rep* __restrict__ reps, const double4* __restrict__ beads,
-                                                  const RotBead* __restrict__ rotb, const Frame* __restrict__ frames,
-                                                  Tables T, Tables3 T3, const int2* __restrict__ rowlist,
-                                                  unsigned* __restrict__ hist, double2* __restrict__ rows) {
+__global__ __launch_bounds__(256) void k_est_tiles(const Rep* __restrict__ reps, const double4* __restrict__ beads,
+                                                   const RotBead* __restrict__ rotb, const Frame* __restrict__ frames,
+                                                   Tables T, Tables3 T3, const int2* __restrict__ tiles,
+                                                   unsigned* __restrict__ hist, double2* __restrict__ part) {
+    extern __shared__ double smem[];
     __shared__ double red[16];
-    const int2 rw = rowlist[blockIdx.x];
-    const Rep& rp = reps[rw.x];
-    const int a0 = rw.y;
-    const int N = rp.natoms, P = rp.P;
-    const int t0 = type_of(rp, a0);
-    const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
-    const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
-    // kinetic, GetKinEnergy mc_estim.cc:876-937
-    double kin = 0.0;
-    for (int it = threadIdx.x; it < P; it += blockDim.x) {
-        double4 b0 = bead(beads, rp, a0, it), b1 = bead(beads, rp, a0, (it + 1) % P);
-        double dx = b0.x - b1.x, dy = b0.y - b1.y, dz = b0.z - b1.z;
-        kin += (dx * dx + dy * dy + dz * dz);
+    __shared__ unsigned g1s[BINSR];
+    const int2 tk = tiles[blockIdx.x];
+    const Rep& rp = reps[tk.x];
+    const int N = rp.natoms, P = rp.P, TS = rp.est_ts, lts = __ffs(TS) - 1;
+    const int s0 = tk.y * TS, nts = min(TS, P - s0), W = nts + 1;
+    double* lx = smem;
+    double* ly = lx + N * W;
+    double* lz = ly + N * W;
+    double* ax = lz + N * W;  // rotor axis at the tile's slices
+    double* ay = ax + TS;
+    double* az = ay + TS;
+    int* kind = reinterpret_cast<int*>(az + TS);  // type | molecule kind << 8
+    for (int i = threadIdx.x; i < N * W; i += blockDim.x) {
+        const int a = i / W, w = i - a * W;
+        const int it = s0 + w < P ? s0 + w : s0 + w - P;
+        const double4 b = bead(beads, rp, a, it);
+        lx[i] = b.x;
+        ly[i] = b.y;
+        lz[i] = b.z;
     }
-    // potential and densities, GetPotEnergy_Densities mc_estim.cc:500-687
-    const double drad = MAX_RADIUS / (double)BINSR;
-    const double dth = 3.14159265358979323846 / (double)(BINST - 1);
+    for (int w = threadIdx.x; w < nts; w += blockDim.x) rotor_axis(rotb, rp, s0 + w, ax[w], ay[w], az[w]);
+    for (int a = threadIdx.x; a < N; a += blockDim.x) {
+        const int t = type_of(rp, a);
+        kind[a] = t | (rp.tmol[t] << 8);
+    }
     unsigned* g1 = hist ? hist + rp.hist_off : nullptr;
     unsigned* g2h = g1 ? g1 + BINSR : nullptr;
+    for (int i = threadIdx.x; i < BINSR; i += blockDim.x) g1s[i] = 0u;
+    __syncthreads();
+    // kinetic links (a, s) - (a, s + 1)
+    double kin = 0.0;
+    for (int i = threadIdx.x; i < N * nts; i += blockDim.x) {
+        const int a = i / nts, w = i - a * nts;
+        const int j = a * W + w;
+        const double dx = lx[j] - lx[j + 1], dy = ly[j] - ly[j + 1], dz = lz[j] - lz[j + 1];
+        kin += (dx * dx + dy * dy + dz * dz) / (4.0 * rp.beta * rp.lambda[kind[a] & 255]);
+    }
+    const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
+    const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
+    const double drad = MAX_RADIUS / (double)BINSR;
+    const double dth = 3.14159265358979323846 / (double)(BINST - 1);
+    const int npairs = N * (N - 1) / 2;
+    const float invN = 1.0f / (float)N;
     double pot = 0.0;
-    const int nb = N - 1 - a0;
-    const int m0 = rp.tmol[t0];
-    for (int i = threadIdx.x; i < nb * P; i += blockDim.x) {
-        int b = i / P, it = i - b * P;
-        int a1 = a0 + 1 + b;
-        int t1 = type_of(rp, a1);
-        int m1 = rp.tmol[t1];
-        double4 b0 = bead(beads, rp, a0, it), b1 = bead(beads, rp, a1, it);
-        R dx = (R)(b0.x - b1.x), dy = (R)(b0.y - b1.y), dz = (R)(b0.z - b1.z);
-        R rr = sqrt(dx * dx + dy * dy + dz * dz);
+    for (int i = threadIdx.x; i < (npairs << lts); i += blockDim.x) {
+        const int w = i & (TS - 1);
+        if (w >= nts) continue;
+        const int p = i >> lts;
+        // p = N (d - 1) + a; the float quotient is exact while N^2 < 2^23
+        const int d1 = N < 2896 ? (int)(((float)p + 0.5f) * invN) : p / N;
+        const int a0 = p - d1 * N;
+        int a1 = a0 + d1 + 1;
+        if (a1 >= N) a1 -= N;
+        const int k0 = kind[a0], k1 = kind[a1];
+        const int t0 = k0 & 255, t1 = k1 & 255, m0 = k0 >> 8, m1 = k1 >> 8;
+        const int j0 = a0 * W + w, j1 = a1 * W + w;
+        const double ex = lx[j0] - lx[j1], ey = ly[j0] - ly[j1], ez = lz[j0] - lz[j1];
+        const R dx = (R)ex, dy = (R)ey, dz = (R)ez;
+        const R rr = sqrt(dx * dx + dy * dy + dz * dz);
         if constexpr (FAM == 1) {
             if (m0 == 2 || m1 == 2) {  // mc_estim.cc:582-672: vcord or caleng, no density bins
                 const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
+                const int it = s0 + w;
                 const Frame& f0 = frame_at(frames, rp, m0 == 2 ? a0 : rp.nl_atom, it);
                 const Frame& f1 = frame_at(frames, rp, m1 == 2 ? a1 : rp.nl_atom, it);
-                pot += pair_nl<R>(l1, l1.k, l3, m0 == 2, m1 == 2, b0.x, b0.y, b0.z, b1.x, b1.y, b1.z, f0, f1);
+                pot += pair_nl<R>(l1, l1.k, l3, m0 == 2, m1 == 2, lx[j0], ly[j0], lz[j0], lx[j1], ly[j1], lz[j1], f0, f1);
                 continue;
             }
         }
-        if (m0 == 1 || m1 == 1) {
-            double nx, ny, nz;
-            rotor_axis(rotb, rp, it, nx, ny, nz);
+        if (m0 == 1 || m1 == 1) {  // axis to the partner: from the rotor to the atom
+            const double nx = ax[w], ny = ay[w], nz = az[w];
             R cost = ((R)nx * dx + (R)ny * dy + (R)nz * dz) / rr;
             if (m0 == 1) cost = -cost;
             pot += (double)lpot2d<R>(l2, rr, cost);
             if (g2h) {  // bins from FP64 geometry, as bin_2Ddensity gets it (mc_estim.cc:1233-1250)
-                const double ex = b0.x - b1.x, ey = b0.y - b1.y, ez = b0.z - b1.z;
                 const double rd = sqrt(ex * ex + ey * ey + ez * ez);
                 double cd = (nx * ex + ny * ey + nz * ez) / rd;
                 if (m0 == 1) cd = -cd;
-                int br = (int)floor(rd / drad);
+                const int br = (int)floor(rd / drad);
                 if (br < BINSR && br >= 0) {
-                    int bt = (int)floor(acos(cd) / dth);
+                    const int bt = (int)floor(acos(cd) / dth);
                     if (bt < BINST && bt >= 0) atomicAdd(&g2h[br * BINST + bt], 1u);
                 }
             }
         } else if (t0 == t1 && m0 == 0) {
             pot += (double)spot1d<R>(l1, l1.k, rr);
             if (g1) {  // bin_1Ddensity (mc_estim.cc:1275-1286) on the FP64 distance
-                const double ex = b0.x - b1.x, ey = b0.y - b1.y, ez = b0.z - b1.z;
-                int br = (int)floor(sqrt(ex * ex + ey * ey + ez * ez) / drad);
-                if (br < BINSR && br >= 0) atomicAdd(&g1[br], 1u);
+                const int br = (int)floor(sqrt(ex * ex + ey * ey + ez * ez) / drad);
+                if (br < BINSR && br >= 0) atomicAdd(&g1s[br], 1u);
             }
         }
     }
-    double sp = block_sum(pot, red);
-    double sk = block_sum(kin, red) / (4.0 * rp.beta * rp.lambda[t0]);
-    if (threadIdx.x == 0) rows[rp.row_off + a0] = make_double2(sp, sk);
+    const double sp = block_sum(pot, red);
+    const double sk = block_sum(kin, red);
+    if (g1)
+        for (int i = threadIdx.x; i < BINSR; i += blockDim.x)
+            if (g1s[i]) atomicAdd(&g1[i], g1s[i]);
+    if (threadIdx.x == 0) part[rp.row_off + tk.y] = make_double2(sp, sk);
 }
 
 // out (if not null) receives 8 doubles per replica; otherwise the sample ring.
@@ -1434,7 +1523,8 @@ __global__ __launch_bounds__(256) void k_est_final(const Rep* __restrict__ reps,
     }
     if (threadIdx.x == 0) {
         double r2avr = 0.0, spot = 0.0;
-        for (int a = 0; a < N; ++a) {
+        const int ntiles = (P + rp.est_ts - 1) / rp.est_ts;
+        for (int a = 0; a < ntiles; ++a) {
             double2 v = rows[rp.row_off + a];
             spot += v.x;
             r2avr += v.y;
