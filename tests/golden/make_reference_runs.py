@@ -1,5 +1,7 @@
 """Writes reference_runs.json: the block averages of the reference program on the decks of
-decks.py, for the statistical parity tests of the host driver.
+decks.py, for the statistical parity tests of the host driver: every .eng block line, and
+the per-bin mean and standard error over blocks of the block distributions (.gra He-He g(r),
+.gri / .grt density around the rotor, column 1 of .rcf).
 
 The program is oracle/_ref/pimc, compiled from the upstream sources by oracle/ref.mk (run
 `make -C oracle ref` first; the reference tree is only present in the build container).  It
@@ -20,7 +22,44 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "moribs-pimc_amd"))
 sys.path.insert(0, HERE)
 
+import numpy as np  # noqa: E402
+
 import decks  # noqa: E402
+
+DISTRIBUTIONS = {"co2_100K": (".rcf",), "he4_co2": (".gra", ".gri", ".grt", ".rcf")}
+
+
+GROUP = 10  # consecutive blocks averaged before the spread is taken (slow modes)
+
+
+def symmetric_part(v):
+    """Angular density averaged over theta and pi - theta: bin i with bin 48 - i (bin 49 holds
+    only theta = pi)."""
+    return 0.5 * (v[..., :49] + v[..., 48::-1])
+
+
+def block_stats(d, prefix, ext, blocks):
+    """Per-bin mean over the block files <prefix><block><ext>, and its standard error from the
+    spread of averages over groups of GROUP consecutive blocks.  For .grt, the symmetric part
+    of every block is taken first."""
+    vals = []
+    for b in blocks:
+        path = os.path.join(d, f"{prefix}{b:03d}{ext}")
+        if ext == ".rcf":
+            with open(path) as f:
+                first = f.read().split("\n\n")[0]
+            a = np.loadtxt(first.splitlines(), ndmin=2)
+        else:
+            a = np.loadtxt(path, ndmin=2)
+        vals.append(a[:, 1])
+        x = a[:, 0]
+    v = np.array(vals)
+    if ext == ".grt":
+        v = symmetric_part(v)
+        x = x[:49]
+    g = v[: len(v) // GROUP * GROUP].reshape(-1, GROUP, v.shape[1]).mean(axis=1)
+    return dict(x=x.tolist(), mean=v.mean(axis=0).tolist(), se=(g.std(axis=0, ddof=1) / np.sqrt(len(g))).tolist(),
+                blocks=len(v), group=GROUP)
 
 
 def run(name):
@@ -34,8 +73,11 @@ def run(name):
         wall = time.time() - t0
         prefix = "CO2_monomer" if name == "co2_100K" else "he4co2"
         eng = decks.read_eng(os.path.join(d, prefix + ".eng"))
+        measured = [int(b) for b in eng[:, 0]]
+        dist = {ext: block_stats(d, prefix, ext, measured) for ext in DISTRIBUTIONS[name]}
     return dict(deck=decks.DECKS[name].format(blocks=decks.BLOCKS[name]), columns=["block", *decks.ENG_COLUMNS],
-                eng=eng.tolist(), wall_s=round(wall, 1), program="oracle/_ref/pimc, OMP_NUM_THREADS=1")
+                eng=eng.tolist(), distributions=dist, wall_s=round(wall, 1),
+                program="oracle/_ref/pimc, OMP_NUM_THREADS=1")
 
 
 def main():

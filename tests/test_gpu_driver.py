@@ -37,20 +37,29 @@ def mean_se(chains):
     return x.mean(), se
 
 
+@pytest.fixture(scope="module")
+def driver_runs(tmp_path_factory):
+    """The driver on each deck with CHAINS independent chains: {name: (directory, [eng per chain])}."""
+    out = {}
+    for name in ("co2_100K", "he4_co2"):
+        d = str(tmp_path_factory.mktemp(name))
+        decks.materialise(name, d, blocks=GPU_BLOCKS[name])
+        r = subprocess.run([DRIVER, "--replicas", str(CHAINS)], cwd=d, capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stderr[-2000:]
+        engs = []
+        for c in range(CHAINS):
+            engs.append(decks.read_eng(os.path.join(d, PREFIX[name] + (f"_r{c}" if c else "") + ".eng")))
+        out[name] = (d, engs)
+    return out
+
+
 @pytest.mark.parametrize("name", ["co2_100K", "he4_co2"])
-def test_driver_matches_reference_statistics(name, tmp_path):
-    d = str(tmp_path)
-    decks.materialise(name, d, blocks=GPU_BLOCKS[name])
-    r = subprocess.run([DRIVER, "--replicas", str(CHAINS)], cwd=d, capture_output=True, text=True, timeout=600)
-    assert r.returncode == 0, r.stderr[-2000:]
+def test_driver_matches_reference_statistics(name, driver_runs):
+    d, engs = driver_runs[name]
     neq = 5 if name == "co2_100K" else 10
-    engs = []
-    for c in range(CHAINS):
-        stem = os.path.join(d, PREFIX[name] + (f"_r{c}" if c else ""))
-        e = decks.read_eng(stem + ".eng")
+    for e in engs:
         assert e.shape == (GPU_BLOCKS[name] - neq, 10)
         assert np.array_equal(e[:, 0], np.arange(neq + 1, GPU_BLOCKS[name] + 1))
-        engs.append(e)
     ref = np.array(REF[name]["eng"])
     cols = REF[name]["columns"]
     report = []
@@ -64,6 +73,66 @@ def test_driver_matches_reference_statistics(name, tmp_path):
             continue
         assert abs(mg - mr) <= 2.0 * np.hypot(sg, sr), report
     print(name, report)
+
+
+def block_columns(d, prefix, ext, blocks):
+    """Column 1 of the block files <prefix><block><ext>, per chain: [chains, blocks, bins]."""
+    out = []
+    for c in range(CHAINS):
+        stem = prefix + (f"_r{c}" if c else "")
+        rows = []
+        for b in blocks:
+            with open(os.path.join(d, f"{stem}{b:03d}{ext}")) as f:
+                text = f.read()
+            if ext == ".rcf":
+                text = text.split("\n\n")[0]
+            rows.append(np.loadtxt(text.splitlines(), ndmin=2)[:, 1])
+        out.append(rows)
+    return np.array(out)
+
+
+def grouped_mean_se(v, group):
+    """Mean over [chains, blocks, bins] and the standard error from averages over groups of
+    `group` consecutive blocks of each chain (slow modes make single blocks correlated)."""
+    c, b, n = v.shape
+    g = v[:, : b // group * group].reshape(c, -1, group, n).mean(axis=2).reshape(-1, n)
+    return v.reshape(-1, n).mean(axis=0), g.std(axis=0, ddof=1) / np.sqrt(len(g))
+
+
+@pytest.mark.parametrize("name,ext", [("he4_co2", ".gra"), ("he4_co2", ".gri"), ("he4_co2", ".grt"),
+                                      ("he4_co2", ".rcf"), ("co2_100K", ".rcf")])
+def test_distributions_match_reference(name, ext, driver_runs):
+    """He-He g(r), the radial and angular He density around the rotor, and the rotor's
+    orientational correlation <n(tau).n(0)>: per-bin means of the driver's block files against
+    the reference program's, in units of the combined standard error (both from groups of 10
+    consecutive blocks).  Over the bins that hold density the sum of z^2 stays below the 0.999
+    quantile of chi^2 with that many degrees of freedom, few bins pass 2 and none passes 4.5.
+
+    The angular density is compared through its symmetric part, bins theta and pi - theta
+    averaged in every block (bin i with bin 48 - i; bin 49 holds only theta = pi).  The rotor-He potential has
+    only even Legendre terms, so the exact density is symmetric; its antisymmetric part is a
+    slow mode (He crossing from one end of the rotor to the other) that neither run converges,
+    and the two runs show it with opposite signs."""
+    d, engs = driver_runs[name]
+    neq = 5 if name == "co2_100K" else 10
+    ref = REF[name]["distributions"][ext]
+    v = block_columns(d, PREFIX[name], ext, range(neq + 1, GPU_BLOCKS[name] + 1))
+    if ext == ".grt":  # the symmetric part of every block
+        v = 0.5 * (v[..., :49] + v[..., 48::-1])
+    mg, sg = grouped_mean_se(v, ref["group"])
+    mr, sr = np.array(ref["mean"]), np.array(ref["se"])
+    assert mg.shape == mr.shape
+    se = np.hypot(sg, sr)
+    keep = (np.maximum(np.abs(mg), np.abs(mr)) > 1e-3 * np.abs(mr).max()) & (se > 0)
+    z = (mg - mr)[keep] / se[keep]
+    print(name, ext, "bins", keep.sum(), "mean z^2", round(float(np.mean(z * z)), 3), "max |z|",
+          round(float(np.abs(z).max()), 2))
+    from scipy.stats import chi2
+    k = int(keep.sum())
+    assert k >= 3  # a P = 4 rotor has 3 free offsets: n(0).n(0) = 1 on both sides
+    assert np.sum(z * z) < chi2.ppf(0.999, k)
+    assert np.mean(np.abs(z) > 2) < 0.12 or k < 10
+    assert np.abs(z).max() < 4.5
 
 
 def test_driver_output_layouts(tmp_path):
