@@ -300,6 +300,17 @@ void pimc_destroy(pimc_ctx* c) {
 
 const char* pimc_last_error(const pimc_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
+// a grid counts as uniform when every point lies within 1e-6 (relative to the span) of the
+// straight line through its end points
+static bool uniform_grid(const double* g, int n, double& g0, double& step) {
+    g0 = g[0];
+    step = (g[n - 1] - g[0]) / (double)(n - 1);
+    double span = std::fabs(g[n - 1] - g[0]);
+    for (int i = 0; i < n; ++i)
+        if (std::fabs(g[i] - (g0 + i * step)) > 1e-6 * span) return false;
+    return step > 0;
+}
+
 int pimc_add_pot1d(pimc_ctx* c, int n, const double* r, const double* v) {
     if (c->committed) return fail(c, "tables must be added before pimc_commit");
     if (n < 2) return fail(c, "init_pot1D: Not enough data points for extrapolation");
@@ -313,6 +324,9 @@ int pimc_add_pot1d(pimc_ctx* c, int n, const double* r, const double* v) {
     t.x0 = x[0];
     t.xmax = x[n - 1];
     t.inv_h = (double)(n - 1) / (x[n - 1] - x[0]);
+    double g0, step;
+    t.uniform = uniform_grid(x.data(), n, g0, step) ? 1 : 0;
+    t.pad = 0;
     // U0 exp(-alpha r) below the grid, -C6/r^6 above (mc_poten.cc:420-437)
     t.alpha = std::log(y[0] / y[1]) / (x[1] - x[0]);
     t.unode = y[0] * std::exp(t.alpha * x[0]);
@@ -323,17 +337,6 @@ int pimc_add_pot1d(pimc_ctx* c, int n, const double* r, const double* v) {
         c->k1.push_back(make_double4(x[i], y[i], y2[i], i + 1 < n ? 1.0 / (x[i + 1] - x[i]) : 0.0));
     c->t1.push_back(t);
     return (int)c->t1.size() - 1;
-}
-
-// a grid counts as uniform when every point lies within 1e-6 (relative to the span) of the
-// straight line through its end points
-static bool uniform_grid(const double* g, int n, double& g0, double& step) {
-    g0 = g[0];
-    step = (g[n - 1] - g[0]) / (double)(n - 1);
-    double span = std::fabs(g[n - 1] - g[0]);
-    for (int i = 0; i < n; ++i)
-        if (std::fabs(g[i] - (g0 + i * step)) > 1e-6 * span) return false;
-    return step > 0;
 }
 
 int pimc_add_pot2d(pimc_ctx* c, int nr, int nc, double dr, double dc, const double* rg, const double* cg,

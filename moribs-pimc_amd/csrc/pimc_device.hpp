@@ -65,6 +65,7 @@ struct RotBead {
 
 struct Tab1D {
     int n, off;
+    int uniform, pad;  // uniform: every knot within 1e-6 of the span of the straight line
     double x0, xmax, inv_h, alpha, unode, c6;
 };
 struct Tab2D {
@@ -169,7 +170,7 @@ using vec4_t = typename std::conditional<std::is_same<R, float>::value, float4, 
 template <typename R>
 struct Lut1D {
     const vec4_t<R>* k;  // knots (x, y, y'', 1/(x[i+1]-x[i]))
-    int n;
+    int n, uniform;
     R x0, xmax, inv_h, alpha, unode, c6;
 };
 
@@ -195,12 +196,14 @@ __device__ __forceinline__ Lut1D<R> lut1d(const Tables& T, int tid) {
     if (tid < 0) {
         L.k = nullptr;
         L.n = 0;
+        L.uniform = 0;
         L.x0 = L.xmax = L.inv_h = L.alpha = L.unode = L.c6 = (R)0;
         return L;
     }
     const Tab1D t = T.t1[tid];
     L.k = knots_ptr(T, R()) + t.off;
     L.n = t.n;
+    L.uniform = t.uniform;
     L.x0 = (R)t.x0;
     L.xmax = (R)t.xmax;
     L.inv_h = (R)t.inv_h;
@@ -246,20 +249,39 @@ constexpr bool is_fp32() {
     return std::is_same<R, float>::value;
 }
 
+// FP32 square root and reciprocal of the move kernels: the hardware instructions (1 ulp)
+// instead of the correctly rounded sequences; FP64 keeps IEEE arithmetic.
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double fast_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float fast_div(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+__device__ __forceinline__ double fast_div(double x, double y) { return x / y; }
+
 // SPot1D: spline inside the grid, U0 exp(-alpha r) below it, -C6/r^6 above it.
 // `k` is the knot array (LDS or global copy of L.k).  FP64 keeps the reference's
-// arithmetic; FP32 multiplies by the stored inverse cell width instead of dividing.
+// arithmetic; FP32 multiplies by the stored inverse cell width instead of dividing, and on
+// uniform knots takes the cell from the grid arithmetic alone (at a cell boundary the two
+// cells' cubics agree to rounding).
 template <typename R>
 __device__ __forceinline__ R spot1d(const Lut1D<R>& L, const vec4_t<R>* __restrict__ k, R r) {
     if (r >= L.xmax) {
         R r2 = r * r;
-        return -L.c6 / (r2 * r2 * r2);
+        return -fast_div(L.c6, r2 * r2 * r2);
     }
-    if (r <= L.x0) return L.unode * exp(-L.alpha * r);
+    if constexpr (is_fp32<R>()) {
+        if (r <= L.x0) return L.unode * __expf(-L.alpha * r);
+    } else {
+        if (r <= L.x0) return L.unode * exp(-L.alpha * r);
+    }
     R f = floor((r - L.x0) * L.inv_h);
     f = fmin(fmax(f, (R)0), (R)(L.n - 2));
     int i = (int)f;
     vec4_t<R> a = k[i], b = k[i + 1];
+    if constexpr (is_fp32<R>()) {
+        if (L.uniform) {
+            R A = (b.x - r) * a.w, B = (r - a.x) * a.w, h = b.x - a.x;
+            return A * a.y + B * b.y + ((A * A * A - A) * a.z + (B * B * B - B) * b.z) * (h * h) * (R)(1.0 / 6.0);
+        }
+    }
     while (i > 0 && a.x > r) {
         --i;
         b = a;
@@ -398,11 +420,9 @@ __device__ __forceinline__ R pair_v(const Lut1D<R>& l1, const vec4_t<R>* __restr
                                     bool rot_a, bool rot_j, double xa, double ya, double za, double xj, double yj,
                                     double zj, double nx, double ny, double nz) {
     R dx = (R)(xa - xj), dy = (R)(ya - yj), dz = (R)(za - zj);
-    R r = sqrt(dx * dx + dy * dy + dz * dz);
+    R r = fast_sqrt(dx * dx + dy * dy + dz * dz);
     if (rot_a || rot_j) {
-        R cost;
-        if constexpr (is_fp32<R>()) cost = __fdividef((R)nx * dx + (R)ny * dy + (R)nz * dz, r);
-        else cost = ((R)nx * dx + (R)ny * dy + (R)nz * dz) / r;
+        R cost = fast_div((R)nx * dx + (R)ny * dy + (R)nz * dz, r);
         if (rot_a) cost = -cost;
         return lpot2d<R>(l2, r, cost);
     }
