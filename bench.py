@@ -285,6 +285,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--da", type=int, default=0, metavar="PASSES",
                     help="compare surrogate delayed acceptance with plain moves on the asymmetric-top workload")
     args = ap.parse_args()
@@ -303,6 +304,7 @@ def main():
     nonlinear = args.workload in ("c3", "c4")
     wl = pc.nl_workload(args.workload) if nonlinear else pc.workload(args.workload)
     ctx = pc.load_into_context(ph.Context(len(wl["replicas"]), flags=ph.FLAG_PROFILE), wl)
+    shapes = None if args.no_tune else ctx.autotune(passes=2)  # untimed, like the warmup
     bm_pass = float(sum(pc.bead_moves_per_pass(rp) for rp in wl["replicas"]))
     ctx.run(args.warmup, sample_every=1)
     ctx.synchronize()
@@ -358,7 +360,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": DESC[args.workload], "replicas": len(wl["replicas"]),
                    "bead_moves_per_step": bm_pass, "pair_terms": "fp32, fp64 accumulation",
-                   "estimators": "one MCGetAverage sample per replica per pass"},
+                   "estimators": "one MCGetAverage sample per replica per pass",
+                   "launch_shapes": shapes if shapes else "defaults"},
         "kernels_ms_per_step": {f: round(kt[f][0] / args.steps, 4) for f in fams},
         "roofline": roof,
     }

@@ -87,7 +87,6 @@ std::vector<Part> xcd_order(const std::vector<Part>& parts, Work work) {
 
 // bisection size classes of the linear family: lanes per workgroup by atom count
 inline int bisect_class(int natoms) { return natoms <= 24 ? 0 : (natoms <= 64 ? 1 : 2); }
-inline int bisect_lanes(int k) { return 64 << k; }
 
 template <typename T>
 T* dalloc(size_t n) {
@@ -170,6 +169,7 @@ struct pimc_ctx {
     // linear-family bisection in three size classes (N <= 24, <= 64, larger) with 64, 128 and
     // 256 lanes and their own LDS sizes, launched as parallel branches of the pass
     int bw_n[3] = {0, 0, 0}, bw_off[3] = {0, 0, 0};
+    int bw_lanes[3] = {64, 128, 256};  // lanes per segment of each class
     size_t bw_lds[3] = {0, 0, 0};
     hipStream_t side[2] = {nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
@@ -181,6 +181,9 @@ struct pimc_ctx {
     unsigned* d_flags = nullptr;  // half-sweeps completed per part
     int nparts = 0, nflags = 0;
     int opt_rotor_part_slices = 0;  // most rotor slices per part, 0: from the LDS budget
+    int opt_rotor_lds_kb = 52;      // LDS budget of a rotor part (52 KB: three parts per CU)
+    int rotor_lds_kb = 52;          // budget the current parts were planned with
+    std::vector<int> rot_chains;    // replicas with a linear rotor, heaviest first
     RotPart* d_parts3 = nullptr;     // non-linear chains split by slice range
     unsigned* d_flags3 = nullptr;
     double* d_pub3 = nullptr;        // boundary frames published per part
@@ -476,12 +479,81 @@ int pimc_set_option(pimc_ctx* c, const char* name, long value) {
         c->opt_rotor_part_slices = (int)value;
         return 0;
     }
+    static const char* lanes_opt[3] = {"bisect_lanes_small", "bisect_lanes_medium", "bisect_lanes_large"};
+    for (int k = 0; k < 3; ++k)
+        if (std::strcmp(name, lanes_opt[k]) == 0) {
+            if (value != 64 && value != 128 && value != 256)
+                return fail(c, std::string(name) + " is 64, 128 or 256");
+            c->bw_lanes[k] = (int)value;
+            return 0;
+        }
+    if (std::strcmp(name, "rotor_lds_kb") == 0) {
+        if (value < 8 || value > 150) return fail(c, "rotor_lds_kb is 8 to 150");
+        c->opt_rotor_lds_kb = (int)value;
+        return 0;
+    }
     if (std::strcmp(name, "delayed_acceptance") == 0) {
         if (value != 0 && value != 1) return fail(c, "delayed_acceptance is 0 or 1");
         c->opt_da = value == 1;
         return 0;
     }
     return fail(c, std::string("unknown option ") + name);
+}
+
+// Linear-rotor parts: slice ranges whose partner geometry (vec4 per translational slice and
+// partner) stays within an LDS budget of lds_kb; the heaviest chains first, the parts of a
+// chain adjacent and on one XCD (its boundary exchange and its .rot table stay in one L2).
+// Called at commit and again by the autotuner, which compares budgets.
+static int plan_rotor_parts(pimc_ctx* c, int lds_kb) {
+    const size_t geo_bytes = c->fp64 ? sizeof(double4) : sizeof(float4);
+    const long long budget = (long long)lds_kb * 1024;
+    std::vector<RotPart> parts;
+    size_t lds_r = 0;
+    for (int r : c->rot_chains) {
+        const Rep& p = c->rep[r];
+        const long long per_slice = (long long)p.RotRatio * (p.natoms - 1) * (long long)geo_bytes;
+        int ns = (int)std::max<long long>(2, (budget - 16 * (long long)sizeof(double)) /
+                                                 (7 * (long long)sizeof(double) + per_slice)) & ~1;
+        if (c->opt_rotor_part_slices > 0) ns = std::max(2, std::min(ns, c->opt_rotor_part_slices & ~1));
+        if (ns >= p.Prot) ns = p.Prot;
+        int W = (p.Prot + ns - 1) / ns;
+        ns = ((p.Prot + W - 1) / W + 1) & ~1;  // even, balanced
+        W = (p.Prot + ns - 1) / ns;
+        const int fbase = (int)parts.size();
+        for (int w = 0; w < W; ++w) {
+            RotPart q{};
+            q.r = r;
+            q.k0 = w * ns;
+            q.k1 = std::min(p.Prot, (w + 1) * ns);
+            q.w = w;
+            q.W = W;
+            q.fbase = fbase;
+            parts.push_back(q);
+            const int n = q.k1 - q.k0;
+            lds_r = std::max(lds_r, (size_t)((7 * (n + 2) + 3) & ~3) * sizeof(double) + (size_t)(n * per_slice));
+        }
+    }
+    if (lds_r > 160 * 1024) return fail(c, "rotor chain exceeds LDS");
+    const int nflags = (int)parts.size();
+    parts = xcd_order(parts, [&](int r) {
+        const Rep& p = c->rep[r];
+        return (double)p.P * p.Prot * p.natoms;
+    });
+    if (c->d_parts) (void)hipFree(c->d_parts);
+    if (c->d_flags) (void)hipFree(c->d_flags);
+    c->d_parts = nullptr;
+    c->d_flags = nullptr;
+    c->lds_rotor = lds_r;
+    c->nflags = nflags;
+    c->nparts = (int)parts.size();
+    c->rotor_lds_kb = lds_kb;
+    if (parts.empty()) return 0;
+    c->d_parts = dalloc<RotPart>(parts.size());
+    c->d_flags = dalloc<unsigned>(nflags);
+    if (!c->d_parts || !c->d_flags) return fail(c, "hipMalloc failed");
+    HIPCHECK(c, hipMemcpy(c->d_parts, parts.data(), sizeof(RotPart) * parts.size(), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemset(c->d_flags, 0, sizeof(unsigned) * nflags));
+    return 0;
 }
 
 int pimc_commit(pimc_ctx* c) {
@@ -634,48 +706,12 @@ int pimc_commit(pimc_ctx* c) {
     if (c->lds_est > 150 * 1024) return fail(c, "estimator tile exceeds LDS (too many atoms)");
     c->lds_bisect = lds_b;
     c->lds_bisect1 = lds_b1;
-    // rotor parts: slice ranges whose partner geometry (vec4 per translational slice and partner)
-    // stays within a fixed LDS budget; the heaviest chains first, the parts of a chain adjacent
-    const size_t geo_bytes = c->fp64 ? sizeof(double4) : sizeof(float4);
-    const long long budget = 52 * 1024;  // three parts per CU
     std::stable_sort(rotreps.begin(), rotreps.end(), [&](int x, int y) {
         const Rep &a = c->rep[x], &b = c->rep[y];
         return (long long)a.P * a.Prot * a.natoms > (long long)b.P * b.Prot * b.natoms;
     });
-    std::vector<RotPart> parts;
-    size_t lds_r = 0;
-    for (int r : rotreps) {
-        const Rep& p = c->rep[r];
-        const long long per_slice = (long long)p.RotRatio * (p.natoms - 1) * (long long)geo_bytes;
-        int ns = (int)std::max<long long>(2, (budget - 16 * (long long)sizeof(double)) /
-                                                 (7 * (long long)sizeof(double) + per_slice)) & ~1;
-        if (c->opt_rotor_part_slices > 0) ns = std::max(2, std::min(ns, c->opt_rotor_part_slices & ~1));
-        if (ns >= p.Prot) ns = p.Prot;
-        int W = (p.Prot + ns - 1) / ns;
-        ns = ((p.Prot + W - 1) / W + 1) & ~1;  // even, balanced
-        W = (p.Prot + ns - 1) / ns;
-        const int fbase = (int)parts.size();
-        for (int w = 0; w < W; ++w) {
-            RotPart q{};
-            q.r = r;
-            q.k0 = w * ns;
-            q.k1 = std::min(p.Prot, (w + 1) * ns);
-            q.w = w;
-            q.W = W;
-            q.fbase = fbase;
-            parts.push_back(q);
-            const int n = q.k1 - q.k0;
-            lds_r = std::max(lds_r, (size_t)((7 * (n + 2) + 3) & ~3) * sizeof(double) + (size_t)(n * per_slice));
-        }
-    }
-    if (lds_r > 160 * 1024) return fail(c, "rotor chain exceeds LDS");
-    c->lds_rotor = lds_r;
-    // parts of a chain on one XCD: its boundary exchange and its .rot table stay in one L2
-    c->nflags = (int)parts.size();
-    parts = xcd_order(parts, [&](int r) {
-        const Rep& p = c->rep[r];
-        return (double)p.P * p.Prot * p.natoms;
-    });
+    c->rot_chains = rotreps;
+    if (plan_rotor_parts(c, c->opt_rotor_lds_kb)) return -1;
     // centroid parts: each replica's slices split so that parts carry similar shares of the
     // work sum N^2 P (about 1024 parts in all; at most 64 parts and at least 8 slices a part)
     {
@@ -737,14 +773,6 @@ int pimc_commit(pimc_ctx* c) {
             HIPCHECK(c, hipMemcpy(c->d_cparts0, cp[0].data(), sizeof(RotPart) * cp[0].size(), hipMemcpyHostToDevice));
         if (!cp[1].empty())
             HIPCHECK(c, hipMemcpy(c->d_cparts1, cp[1].data(), sizeof(RotPart) * cp[1].size(), hipMemcpyHostToDevice));
-    }
-    c->nparts = (int)parts.size();
-    c->d_parts = dalloc<RotPart>(parts.size());
-    c->d_flags = dalloc<unsigned>(c->nflags);
-    if (!c->d_parts || !c->d_flags) return fail(c, "hipMalloc failed");
-    if (!parts.empty()) {
-        HIPCHECK(c, hipMemcpy(c->d_parts, parts.data(), sizeof(RotPart) * parts.size(), hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemset(c->d_flags, 0, sizeof(unsigned) * c->nflags));
     }
     c->nbeads = boff;
     c->nrotb = roff;
@@ -1174,7 +1202,7 @@ void launch_bisect(pimc_ctx* c, int u, int tb, int te) {
             s = c->side[nside];
             (void)hipStreamWaitEvent(s, c->ev_fork, 0);
         }
-        hipLaunchKernelGGL(k_bisect<R>, dim3(c->bw_n[k]), dim3(bisect_lanes(k)), c->bw_lds[k], s, c->d_rep, c->d_mut,
+        hipLaunchKernelGGL(k_bisect<R>, dim3(c->bw_n[k]), dim3(c->bw_lanes[k]), c->bw_lds[k], s, c->d_rep, c->d_mut,
                            c->d_beads, c->d_rotb, c->tables(), c->d_work + c->bw_off[k], run);
         if (k != top) (void)hipEventRecord(c->ev_join[nside++], s);
     }
@@ -1238,6 +1266,22 @@ void launch_estimate(pimc_ctx* c, double* out) {
     hipLaunchKernelGGL(k_est_final, dim3(c->nrep), dim3(256), 0, c->stream, c->d_rep, c->d_mut, c->d_rotb,
                        c->d_frames, c->tables(), c->tables3(),
                        c->d_all, c->d_rowbuf, c->d_samples, c->sample_cap, out);
+}
+
+// Autotuner launches: one bisection size class alone, and the linear-rotor sweep alone, over
+// the start slices of one pass as the context issues them.
+template <typename R>
+void launch_bisect_class(pimc_ctx* c, int k, int u) {
+    Run run{c->seed, u, c->ref_order ? 1 : 0, 0, c->max_types};
+    hipLaunchKernelGGL(k_bisect<R>, dim3(c->bw_n[k]), dim3(c->bw_lanes[k]), c->bw_lds[k], c->stream, c->d_rep,
+                       c->d_mut, c->d_beads, c->d_rotb, c->tables(), c->d_work + c->bw_off[k], run);
+}
+
+template <typename R>
+void launch_rotor_linear(pimc_ctx* c, int u) {
+    Run run{c->seed, u, c->ref_order ? 1 : 0, 0, 0};
+    hipLaunchKernelGGL(k_rotor<R>, dim3(c->nparts), dim3(256), c->lds_rotor, c->stream, c->d_rep, c->d_mut,
+                       c->d_beads, c->d_rotb, c->tables(), c->tables3(), c->d_parts, c->d_flags, run);
 }
 
 // one pass in the order described at the top of this file
@@ -1311,6 +1355,106 @@ int pimc_run(pimc_ctx* c, int n_passes, int sample_every) {
     }
     HIPCHECK(c, hipGetLastError());
     return check_errors(c);
+}
+
+int pimc_autotune(pimc_ctx* c, int passes, long* chosen) {
+    if (!c->committed) return fail(c, "pimc_commit first");
+    if (passes < 1 || passes > 64) return fail(c, "autotune: passes is 1 to 64");
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    // every candidate starts from the same state; the chains get it back at the end
+    double4* sb = dalloc<double4>((size_t)std::max<long long>(c->nbeads, 1));
+    RotBead* sr = dalloc<RotBead>((size_t)std::max<long long>(c->nrotb, 1));
+    Mut* sm = dalloc<Mut>(c->nrep);
+    if (!sb || !sr || !sm) {
+        for (void* q : {(void*)sb, (void*)sr, (void*)sm})
+            if (q) (void)hipFree(q);
+        return fail(c, "autotune: hipMalloc failed");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    auto copy_all = [&](bool save) -> int {
+        auto cp = [&](void* dst, const void* src, size_t bytes) {
+            return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream) == hipSuccess;
+        };
+        const size_t nb = sizeof(double4) * (size_t)c->nbeads, nr = sizeof(RotBead) * (size_t)c->nrotb;
+        const size_t nm = sizeof(Mut) * (size_t)c->nrep;
+        bool ok = save ? (cp(sb, c->d_beads, nb) && cp(sr, c->d_rotb, nr) && cp(sm, c->d_mut, nm))
+                       : (cp(c->d_beads, sb, nb) && cp(c->d_rotb, sr, nr) && cp(c->d_mut, sm, nm));
+        return ok ? 0 : -1;
+    };
+    // time `passes` passes' worth of one kind of launch; < 0 if a launch failed or reported an error
+    auto timed = [&](auto&& issue) -> double {
+        if (copy_all(false)) return -1.0;
+        const int nu = c->ref_order ? c->Pmax : c->Smax;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) return -1.0;
+        for (int p = 0; p < passes; ++p)
+            for (int u = 0; u < nu; ++u) issue(u);
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1.0;
+        if (hipGetLastError() != hipSuccess) return -1.0;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
+        std::vector<Mut> m(c->nrep);
+        if (hipMemcpy(m.data(), c->d_mut, sizeof(Mut) * c->nrep, hipMemcpyDeviceToHost) != hipSuccess) return -1.0;
+        for (const Mut& x : m)
+            if (x.err) return -1.0;
+        return ms;
+    };
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess || copy_all(true)) {
+        rc = fail(c, "autotune: event or copy failed");
+    } else {
+        static const int lanes[3] = {64, 128, 256};
+        for (int k = 0; k < 3 && rc == 0; ++k) {
+            if (!c->bw_n[k]) continue;
+            int best = c->bw_lanes[k];
+            double tbest = -1.0;
+            for (int L : lanes) {
+                c->bw_lanes[k] = L;
+                const double t = timed([&](int u) {
+                    if (c->fp64) launch_bisect_class<double>(c, k, u);
+                    else launch_bisect_class<float>(c, k, u);
+                });
+                if (t >= 0.0 && (tbest < 0.0 || t < tbest)) {
+                    tbest = t;
+                    best = L;
+                }
+            }
+            c->bw_lanes[k] = best;
+        }
+        if (rc == 0 && !c->rot_chains.empty()) {
+            static const int budgets[4] = {40, 52, 64, 78};
+            int best = c->rotor_lds_kb;
+            double tbest = -1.0;
+            for (int kb : budgets) {
+                if (plan_rotor_parts(c, kb)) {
+                    rc = -1;
+                    break;
+                }
+                const double t = timed([&](int u) {
+                    if (c->fp64) launch_rotor_linear<double>(c, u);
+                    else launch_rotor_linear<float>(c, u);
+                });
+                if (t >= 0.0 && (tbest < 0.0 || t < tbest)) {
+                    tbest = t;
+                    best = kb;
+                }
+            }
+            if (rc == 0 && c->rotor_lds_kb != best && plan_rotor_parts(c, best)) rc = -1;
+        }
+        if (copy_all(false) || hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = rc ? rc : fail(c, "autotune: restoring the state failed");
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    for (void* q : {(void*)sb, (void*)sr, (void*)sm}) (void)hipFree(q);
+    if (c->graph_exec) {  // launch shapes changed: the next pass is captured again
+        (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+    }
+    if (rc == 0 && chosen) {
+        for (int k = 0; k < 3; ++k) chosen[k] = c->bw_lanes[k];
+        chosen[3] = c->rotor_lds_kb;
+    }
+    return rc;
 }
 
 int pimc_synchronize(pimc_ctx* c) {

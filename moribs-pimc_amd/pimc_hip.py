@@ -68,6 +68,7 @@ SIGNATURES = {
     "pimc_restore_state": (_i, [ctypes.c_void_p, _i, _dp, _dp, _dp, _u]),
     "pimc_get_pass": (_i, [ctypes.c_void_p, _i, ctypes.POINTER(ctypes.c_uint)]),
     "pimc_run": (_i, [ctypes.c_void_p, _i, _i]),
+    "pimc_autotune": (_i, [ctypes.c_void_p, _i, ctypes.POINTER(ctypes.c_long)]),
     "pimc_get_samples": (_i, [ctypes.c_void_p, _i, _dp, _i]),
     "pimc_clear_samples": (_i, [ctypes.c_void_p]),
     "pimc_get_counters": (_i, [ctypes.c_void_p, _i, _dp, _dp]),
@@ -228,6 +229,13 @@ class Context:
 
     def run(self, n_passes, sample_every=0):
         return self._check(self.lib.pimc_run(self.h, int(n_passes), int(sample_every)))
+
+    def autotune(self, passes=2):
+        """Launch shapes picked by timing candidates on this workload (state restored after)."""
+        out = (ctypes.c_long * 4)()
+        self._check(self.lib.pimc_autotune(self.h, int(passes), out))
+        return dict(bisect_lanes_small=out[0], bisect_lanes_medium=out[1], bisect_lanes_large=out[2],
+                    rotor_lds_kb=out[3])
 
     def samples(self, r, max_samples=1 << 16):
         out = np.empty(8 * max_samples)
