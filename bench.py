@@ -183,9 +183,11 @@ def traffic_per_launch(workload_tag, kernel):
     return None
 
 
-KERNEL_NAMES = {"bisect": "k_bisect<float>", "centroid": "k_centroid_parts<float,0>", "rotor": "k_rotor<float>",
+# the linear family's bisection is one fork-join launch of its size classes; the traffic file
+# holds that launch's bytes as k_bisect<float>[launch]
+KERNEL_NAMES = {"bisect": "k_bisect<float>[launch]", "centroid": "k_centroid_parts<float,0>", "rotor": "k_rotor<float>",
                 "estimate": "k_est_tiles<float,0>"}
-KERNEL_NAMES_NL = {"bisect": "k_bisect_nl<float>", "centroid": "k_centroid_parts<float,1>", "rotor": "k_rotor3d<float>",
+KERNEL_NAMES_NL = {"bisect": "k_bisect_nl<float>", "centroid": "k_centroid_parts<float,1>", "rotor": "k_rotor3d_parts<float>",
                    "estimate": "k_est_tiles<float,1>"}
 WORKLOAD_TAG = {"c2": "c2_256replicas", "c1": "c1_64he", "c0": "c0_8he", "c3": "c3_asymtop_32h2",
                 "c4": "c4_water2_128replicas"}
@@ -286,6 +288,8 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--shapes", default=None, metavar="JSON",
+                    help="launch options to use instead of autotuning, e.g. the launch_shapes of a bench line")
     ap.add_argument("--da", type=int, default=0, metavar="PASSES",
                     help="compare surrogate delayed acceptance with plain moves on the asymmetric-top workload")
     args = ap.parse_args()
@@ -303,8 +307,13 @@ def main():
         return
     nonlinear = args.workload in ("c3", "c4")
     wl = pc.nl_workload(args.workload) if nonlinear else pc.workload(args.workload)
-    ctx = pc.load_into_context(ph.Context(len(wl["replicas"]), flags=ph.FLAG_PROFILE), wl)
-    shapes = None if args.no_tune else ctx.autotune(passes=2)  # untimed, like the warmup
+    ctx = ph.Context(len(wl["replicas"]), flags=ph.FLAG_PROFILE)
+    shapes = json.loads(args.shapes) if args.shapes else None
+    for k, v in (shapes or {}).items():
+        ctx.set_option(k, int(v))
+    pc.load_into_context(ctx, wl)
+    if shapes is None and not args.no_tune:
+        shapes = ctx.autotune(passes=2)  # untimed, like the warmup
     bm_pass = float(sum(pc.bead_moves_per_pass(rp) for rp in wl["replicas"]))
     ctx.run(args.warmup, sample_every=1)
     ctx.synchronize()
