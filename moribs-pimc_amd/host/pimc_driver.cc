@@ -29,7 +29,9 @@
 // the device moves every slice at once, so sampling, sum lines and ratio lines are taken at
 // pass granularity: every max(1, MCSKIP_AVERG / NUMBEROFSLICES) passes, and so on.
 //
-// Usage:  pimc [qmc.input] [--seed N] [--fp64] [--ref-order] [--replicas R] [--init-only]
+// Usage:  pimc [qmc.input] [--seed N] [--fp64] [--ref-order] [--replicas R] [--autotune] [--init-only]
+// --autotune picks launch shapes by timing them on the deck before the first block (results
+// are the same as without it)
 // --init-only writes the initial configuration to <OUTPUTDIR><FILENAMEPREFIX>.xyz and stops
 // before any device work.  With R > 1 the same deck runs as R independent chains in one context; chain r > 0 writes
 // its files under the prefix <OUTPUTDIR><FILENAMEPREFIX>_r<r>.
@@ -649,7 +651,7 @@ int main(int argc, char** argv) {
     unsigned seed = 985456376u;  // mc_main.cc:151
     unsigned flags = 0;
     int nrep = 1;
-    bool init_only = false;
+    bool init_only = false, autotune = false;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--seed" && i + 1 < argc) seed = (unsigned)std::strtoul(argv[++i], nullptr, 10);
@@ -657,8 +659,10 @@ int main(int argc, char** argv) {
         else if (a == "--ref-order") flags |= PIMC_FLAG_REF_ORDER;
         else if (a == "--replicas" && i + 1 < argc) nrep = std::atoi(argv[++i]);
         else if (a == "--init-only") init_only = true;
+        else if (a == "--autotune") autotune = true;
         else if (a == "-h" || a == "--help") {
-            std::cout << "usage: pimc [qmc.input] [--seed N] [--fp64] [--ref-order] [--replicas R] [--init-only]\n";
+            std::cout << "usage: pimc [qmc.input] [--seed N] [--fp64] [--ref-order] [--replicas R] [--autotune] "
+                         "[--init-only]\n";
             return 0;
         } else if (!a.empty() && a[0] != '-') input = a;
         else die("main", "unknown option " + a);
@@ -708,6 +712,12 @@ int main(int argc, char** argv) {
             c.check(pimc_restore_state(c.ctx, r, resume.coords[r].data(), resume.angles[r].data(),
                                        resume.cosines[r].data(), resume.pass[r]),
                     "pimc_restore_state");
+    if (autotune) {
+        long shapes[4];
+        c.check(pimc_autotune(c.ctx, 2, shapes), "pimc_autotune");
+        std::cerr << "launch shapes: bisection lanes " << shapes[0] << "/" << shapes[1] << "/" << shapes[2]
+                  << ", rotor part LDS " << shapes[3] << " KB\n";
+    }
 
     std::vector<Chain> chains(nrep);
     for (int r = 0; r < nrep; ++r) {
