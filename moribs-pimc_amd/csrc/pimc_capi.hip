@@ -134,6 +134,7 @@ struct pimc_ctx {
     double4* d_k1d = nullptr;
     Tab1D* d_t1 = nullptr;
     float* d_v2f = nullptr;
+    float4* d_v2c = nullptr;  // corner-packed cells of the uniform 2D tables
     double* d_v2d = nullptr;
     double* d_g2d = nullptr;
     float* d_g2f = nullptr;
@@ -177,6 +178,7 @@ struct pimc_ctx {
     double2* d_rowbuf = nullptr;  // a (potential, kinetic) partial per tile
     int nrows = 0;
     size_t lds_est = 0;
+    size_t lds_knots = 0;       // the largest 1D knot array, staged in LDS by the centroid move
     RotPart* d_parts = nullptr;  // rotor chains split into slice ranges, one workgroup each
     unsigned* d_flags = nullptr;  // half-sweeps completed per part
     int nparts = 0, nflags = 0;
@@ -217,6 +219,7 @@ struct pimc_ctx {
         T.k1d = d_k1d;
         T.t1 = d_t1;
         T.v2f = d_v2f;
+        T.v2c = d_v2c;
         T.v2d = d_v2d;
         T.g2d = d_g2d;
         T.g2f = d_g2f;
@@ -285,7 +288,7 @@ void pimc_destroy(pimc_ctx* c) {
     }
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     void* ptrs[] = {c->d_rep, c->d_mut, c->d_beads, c->d_rotb, c->d_hist, c->d_samples, c->d_k1f, c->d_k1d,
-                    c->d_t1, c->d_v2f, c->d_v2d, c->d_g2d, c->d_g2f, c->d_t2, c->d_rk, c->d_re, c->d_tr,
+                    c->d_t1, c->d_v2f, c->d_v2c, c->d_v2d, c->d_g2d, c->d_g2f, c->d_t2, c->d_rk, c->d_re, c->d_tr,
                     c->d_work, c->d_all, c->d_rotreps, c->d_est, c->d_rows, c->d_rowbuf, c->d_parts, c->d_flags,
                     c->d_r3, c->d_v3d, c->d_v3f, c->d_t3, c->d_frames, c->d_work1, c->d_all0, c->d_all1, c->d_rot3, c->d_rows1, c->d_mlp,
                     c->d_rcf, c->d_rcfreps, c->d_cparts0, c->d_cparts1, c->d_cflags, c->d_cslots,
@@ -350,6 +353,7 @@ int pimc_add_pot2d(pimc_ctx* c, int nr, int nc, double dr, double dc, const doub
     t.nr = nr;
     t.nc = nc;
     t.off = (long long)c->v2.size();
+    t.coff = -1;  // set at pimc_commit for uniform tables
     t.goff = (int)c->g2.size();
     t.dr = dr;
     t.dc = dc;
@@ -706,6 +710,8 @@ int pimc_commit(pimc_ctx* c) {
     if (c->lds_est > 150 * 1024) return fail(c, "estimator tile exceeds LDS (too many atoms)");
     c->lds_bisect = lds_b;
     c->lds_bisect1 = lds_b1;
+    for (const Tab1D& t : c->t1)
+        c->lds_knots = std::max(c->lds_knots, (size_t)t.n * (c->fp64 ? sizeof(double4) : sizeof(float4)));
     std::stable_sort(rotreps.begin(), rotreps.end(), [&](int x, int y) {
         const Rep &a = c->rep[x], &b = c->rep[y];
         return (long long)a.P * a.Prot * a.natoms > (long long)b.P * b.Prot * b.natoms;
@@ -722,11 +728,11 @@ int pimc_commit(pimc_ctx* c) {
         HIPCHECK(c, hipGetDeviceProperties(&prop, 0));
         ncu = prop.multiProcessorCount;
         if (c->fp64) {
-            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, k_centroid_parts<double, 0>, 256, 0));
-            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_centroid_parts<double, 1>, 256, 0));
+            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, k_centroid_parts<double, 0>, 256, c->lds_knots));
+            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_centroid_parts<double, 1>, 256, c->lds_knots));
         } else {
-            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, k_centroid_parts<float, 0>, 256, 0));
-            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_centroid_parts<float, 1>, 256, 0));
+            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, k_centroid_parts<float, 0>, 256, c->lds_knots));
+            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_centroid_parts<float, 1>, 256, c->lds_knots));
         }
         const double capacity = (double)ncu * std::max(1, std::min(occ0, occ1));
         double target = std::min(1024.0, capacity / 2.0);
@@ -975,10 +981,23 @@ int pimc_commit(pimc_ctx* c) {
     for (size_t i = 0; i < c->k1.size(); ++i)
         k1f[i] = make_float4((float)c->k1[i].x, (float)c->k1[i].y, (float)c->k1[i].z, (float)c->k1[i].w);
     std::vector<float> v2f(c->v2.begin(), c->v2.end()), g2f(c->g2.begin(), c->g2.end());
+    // uniform 2D tables also as one float4 of corner values per cell (one load per FP32 look-up)
+    std::vector<float4> v2c;
+    for (Tab2D& t : c->t2) {
+        t.coff = -1;
+        if (!t.uniform) continue;
+        t.coff = (long long)v2c.size();
+        for (int ir = 0; ir + 1 < t.nr; ++ir)
+            for (int ic = 0; ic + 1 < t.nc; ++ic) {
+                const float* v = v2f.data() + t.off + (long long)ir * t.nc + ic;
+                v2c.push_back(make_float4(v[0], v[1], v[t.nc], v[t.nc + 1]));
+            }
+    }
     c->d_k1f = dalloc<float4>(k1f.size());
     c->d_k1d = dalloc<double4>(c->k1.size());
     c->d_t1 = dalloc<Tab1D>(c->t1.size());
     c->d_v2f = dalloc<float>(v2f.size());
+    c->d_v2c = dalloc<float4>(std::max<size_t>(v2c.size(), 1));
     c->d_v2d = dalloc<double>(c->v2.size());
     c->d_g2d = dalloc<double>(c->g2.size());
     c->d_g2f = dalloc<float>(g2f.size());
@@ -992,6 +1011,7 @@ int pimc_commit(pimc_ctx* c) {
     };
     if (up(c->d_k1f, k1f.data(), k1f.size() * sizeof(float4)) || up(c->d_k1d, c->k1.data(), c->k1.size() * sizeof(double4)) ||
         up(c->d_t1, c->t1.data(), c->t1.size() * sizeof(Tab1D)) || up(c->d_v2f, v2f.data(), v2f.size() * sizeof(float)) ||
+        up(c->d_v2c, v2c.data(), v2c.size() * sizeof(float4)) ||
         up(c->d_v2d, c->v2.data(), c->v2.size() * sizeof(double)) ||
         up(c->d_g2d, c->g2.data(), c->g2.size() * sizeof(double)) ||
         up(c->d_g2f, g2f.data(), g2f.size() * sizeof(float)) || up(c->d_t2, c->t2.data(), c->t2.size() * sizeof(Tab2D)) ||
@@ -1217,11 +1237,11 @@ void launch_centroid(pimc_ctx* c, int type) {
     Run run{c->seed, 0, c->ref_order ? 1 : 0, type, type + 1};
     Timed t(c, "centroid");
     if (c->ncp0)
-        hipLaunchKernelGGL((k_centroid_parts<R, 0>), dim3(c->ncp0), dim3(256), 0, c->stream, c->d_rep, c->d_mut,
+        hipLaunchKernelGGL((k_centroid_parts<R, 0>), dim3(c->ncp0), dim3(256), c->lds_knots, c->stream, c->d_rep, c->d_mut,
                            c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_cparts0, c->d_cflags,
                            c->d_cslots, c->ncflags, run);
     if (c->ncp1)
-        hipLaunchKernelGGL((k_centroid_parts<R, 1>), dim3(c->ncp1), dim3(256), 0, c->stream, c->d_rep, c->d_mut,
+        hipLaunchKernelGGL((k_centroid_parts<R, 1>), dim3(c->ncp1), dim3(256), c->lds_knots, c->stream, c->d_rep, c->d_mut,
                            c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_cparts1, c->d_cflags,
                            c->d_cslots, c->ncflags, run);
 }

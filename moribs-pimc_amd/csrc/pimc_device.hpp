@@ -70,7 +70,8 @@ struct Tab1D {
 };
 struct Tab2D {
     int nr, nc;
-    long long off;  // into values
+    long long off;   // into values
+    long long coff;  // into the corner-packed FP32 cells of a uniform table, -1 if none
     int goff;       // into grids: r grid at goff, cos grid at goff + nr
     int uniform;    // both grids uniform to 1e-6 relative: FP32 lookups use grid arithmetic
     double dr, dc;  // steps of the file header (cell index)
@@ -93,6 +94,7 @@ struct Tables {
     const double4* k1d;
     const Tab1D* t1;
     const float* v2f;
+    const float4* v2c;  // per cell of a uniform table: V[ir][ic], V[ir][ic+1], V[ir+1][ic], V[ir+1][ic+1]
     const double* v2d;
     const double* g2d;
     const float* g2f;
@@ -178,6 +180,7 @@ template <typename R>
 struct Lut2D {
     const R* v;  // V[r][cos]
     const R* g;  // r grid, then cos grid
+    const float4* vc;  // corner-packed cells (FP32 uniform tables)
     int nr, nc, uniform;
     R rmin, cmin, dr, dc, inv_dr, inv_dc;  // grid origins and file-header steps
     R gdr, gdc, inv_gdr, inv_gdc;          // actual grid spacings (uniform grids)
@@ -219,6 +222,7 @@ __device__ __forceinline__ Lut2D<R> lut2d(const Tables& T, int tid) {
     if (tid < 0) {
         L.v = nullptr;
         L.g = nullptr;
+        L.vc = nullptr;
         L.nr = L.nc = 2;
         L.uniform = 0;
         L.rmin = L.cmin = L.dr = L.dc = L.inv_dr = L.inv_dc = (R)1;
@@ -228,6 +232,7 @@ __device__ __forceinline__ Lut2D<R> lut2d(const Tables& T, int tid) {
     const Tab2D t = T.t2[tid];
     L.v = values_ptr(T, R()) + t.off;
     L.g = grid_ptr(T, R()) + t.goff;
+    L.vc = t.coff >= 0 ? T.v2c + t.coff : nullptr;
     L.nr = t.nr;
     L.nc = t.nc;
     L.uniform = t.uniform;
@@ -302,22 +307,34 @@ __device__ __forceinline__ R spot1d(const Lut1D<R>& L, const vec4_t<R>* __restri
     }
 }
 
+// LPot2D on a uniform FP32 grid in two steps, so that the loads of many look-ups can be in
+// flight together: the cell and the weights, then the lerps of its four corner values, which
+// the corner-packed copy holds in one 16-byte load.
+__device__ __forceinline__ void lpot2d_u_cell(const Lut2D<float>& L, float r, float c, int& cell, float& wr,
+                                              float& wc) {
+    float fr = floor((r - L.rmin) * L.inv_dr), fc = floor((c - L.cmin) * L.inv_dc);
+    fr = fmin(fmax(fr, 0.f), (float)(L.nr - 2));
+    fc = fmin(fmax(fc, 0.f), (float)(L.nc - 2));
+    cell = (int)fr * (L.nc - 1) + (int)fc;
+    wr = (r - (L.rmin + fr * L.gdr)) * L.inv_gdr;
+    wc = (c - (L.cmin + fc * L.gdc)) * L.inv_gdc;
+}
+
+__device__ __forceinline__ float lpot2d_u_lerp(float y1, float y4, float y2, float y3, float wr, float wc) {
+    const float lo = y1 + wc * (y4 - y1), hi = y2 + wc * (y3 - y2);
+    return lo + wr * (hi - lo);
+}
+
 // LPot2D: cells clamped to the grid, weights extrapolate linearly off-grid.
 template <typename R>
 __device__ __forceinline__ R lpot2d(const Lut2D<R>& L, R r, R c) {
     if constexpr (is_fp32<R>()) {
         if (L.uniform) {  // grid positions by arithmetic, reciprocal multiplies, lerp form
-            R fr = floor((r - L.rmin) * L.inv_dr), fc = floor((c - L.cmin) * L.inv_dc);
-            fr = fmin(fmax(fr, (R)0), (R)(L.nr - 2));
-            fc = fmin(fmax(fc, (R)0), (R)(L.nc - 2));
-            int ir = (int)fr, ic = (int)fc;
-            const R* v = L.v + ir * L.nc + ic;
-            R y1 = v[0], y4 = v[1];
-            R y2 = v[L.nc], y3 = v[L.nc + 1];
-            R wr = (r - (L.rmin + fr * L.gdr)) * L.inv_gdr;
-            R wc = (c - (L.cmin + fc * L.gdc)) * L.inv_gdc;
-            R lo = y1 + wc * (y4 - y1), hi = y2 + wc * (y3 - y2);
-            return lo + wr * (hi - lo);
+            int cell;
+            float wr, wc;
+            lpot2d_u_cell(L, r, c, cell, wr, wc);
+            const float4 q = L.vc[cell];
+            return lpot2d_u_lerp(q.x, q.y, q.z, q.w, wr, wc);
         }
     }
     R fr = floor((r - L.rmin) / L.dr), fc = floor((c - L.cmin) / L.dc);
@@ -349,6 +366,50 @@ __device__ __forceinline__ double rot_rho(const Tables& T, const TabRot& t, doub
     f = fmin(fmax(f, 0.0), (double)(t.n - 2));
     int i = (int)f;
     double4 a = k[i], b = k[i + 1];
+    while (i > 0 && a.x > g) {
+        --i;
+        b = a;
+        a = k[i];
+    }
+    while (i < t.n - 2 && b.x <= g) {
+        ++i;
+        a = b;
+        b = k[i + 1];
+    }
+    double h = b.x - a.x;
+    double A = (b.x - g) / h, B = (g - a.x) / h;
+    return A * a.y + B * b.y + ((A * A * A - A) * a.z + (B * B * B - B) * b.z) * (h * h) / 6.;
+}
+
+// rot_rho in two steps, so that its knot loads can be in flight with other loads: the knots of
+// the cell the grid arithmetic guesses, then the value (with the cell corrected if needed).
+struct RhoCell {
+    double4 a, b;
+    int i;
+};
+
+__device__ __forceinline__ RhoCell rot_rho_load(const Tables& T, const TabRot& t, double g) {
+    const double4* k = T.rk + t.off;
+    double f = floor((g - t.x0) * t.inv_h);
+    f = fmin(fmax(f, 0.0), (double)(t.n - 2));
+    RhoCell c;
+    c.i = (int)f;
+    c.a = k[c.i];
+    c.b = k[c.i + 1];
+    return c;
+}
+
+__device__ __forceinline__ double rot_rho_eval(const Tables& T, const TabRot& t, double g, RhoCell c) {
+    const double4* k = T.rk + t.off;
+    if (g > t.xmax) return k[t.n - 1].y;
+    if (g < t.x0) {
+        double4 k0 = k[0], k1 = k[1];
+        double sa = (k1.y - k0.y) / (k1.x - k0.x);
+        double sb = (k0.y * k1.x - k1.y * k0.x) / (k1.x - k0.x);
+        return sa * g + sb;
+    }
+    int i = c.i;
+    double4 a = c.a, b = c.b;
     while (i > 0 && a.x > g) {
         --i;
         b = a;

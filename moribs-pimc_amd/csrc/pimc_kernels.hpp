@@ -368,6 +368,7 @@ __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ 
                                                         const Frame* __restrict__ frames, Tables T, Tables3 T3,
                                                         const RotPart* __restrict__ parts, unsigned* __restrict__ flags,
                                                         double* __restrict__ slots, int nslots, Run run) {
+    extern __shared__ double smem[];  // the 1D spline knots
     __shared__ double red[4];
     __shared__ double part_sum[64];
     __shared__ double s_dv;
@@ -383,6 +384,9 @@ __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ 
     const int t0 = pt.k0, nt = pt.k1 - pt.k0, W = pt.W;
     const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
     const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
+    vec4_t<R>* kn = reinterpret_cast<vec4_t<R>*>(smem);
+    for (int i = threadIdx.x; i < l1.n; i += blockDim.x) kn[i] = l1.k[i];
+    __syncthreads();
     unsigned* myflag = flags + pt.fbase + pt.w;
     const unsigned base = __hip_atomic_load(myflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (threadIdx.x == 0) s_err = 0;
@@ -411,12 +415,12 @@ __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ 
                 if (rj) rotor_axis(rotb, rp, it, nx, ny, nz);
                 R vn, vo;
                 if (rj) {
-                    vn = pair_v<R>(l1, l1.k, l2, false, true, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, nx, ny, nz);
-                    vo = pair_v<R>(l1, l1.k, l2, false, true, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz);
+                    vn = pair_v<R>(l1, kn, l2, false, true, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, nx, ny, nz);
+                    vo = pair_v<R>(l1, kn, l2, false, true, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz);
                 } else {
-                    vn = pair_v<R>(l1, l1.k, l2, false, false, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, 0.0,
+                    vn = pair_v<R>(l1, kn, l2, false, false, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, 0.0,
                                    0.0, 1.0);
-                    vo = pair_v<R>(l1, l1.k, l2, false, false, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, 0.0, 0.0, 1.0);
+                    vo = pair_v<R>(l1, kn, l2, false, false, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, 0.0, 0.0, 1.0);
                 }
                 acc += (double)vn - (double)vo;
             }
@@ -430,16 +434,16 @@ __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ 
                 const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
                 const Frame& fa = frame_at(frames, rp, rot_a ? a : rp.nl_atom, it);
                 const Frame& fj = frame_at(frames, rp, rot_j ? j : rp.nl_atom, it);
-                const double vn = pair_nl<R>(l1, l1.k, l3, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y,
+                const double vn = pair_nl<R>(l1, kn, l3, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y,
                                              xj.z, fa, fj);
-                const double vo = pair_nl<R>(l1, l1.k, l3, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, fa, fj);
+                const double vo = pair_nl<R>(l1, kn, l3, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, fa, fj);
                 acc += vn - vo;
             } else {
                 double nx, ny, nz;
                 rotor_axis(rotb, rp, it, nx, ny, nz);
-                const R vn = pair_v<R>(l1, l1.k, l2, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z,
+                const R vn = pair_v<R>(l1, kn, l2, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z,
                                        nx, ny, nz);
-                const R vo = pair_v<R>(l1, l1.k, l2, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz);
+                const R vo = pair_v<R>(l1, kn, l2, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz);
                 acc += (double)vn - (double)vo;
             }
         }
@@ -494,9 +498,31 @@ __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ 
 template <typename R>
 __device__ __forceinline__ double slice_energy(const Lut2D<R>& l2, const vec4_t<R>* pk, int items, int gl, int G,
                                                double nx, double ny, double nz) {
-    constexpr int B = 8;
+    constexpr int B = 8;  // partner look-ups in flight per lane
     double acc = 0.0;
     const R fx = (R)nx, fy = (R)ny, fz = (R)nz;
+    if constexpr (is_fp32<R>()) {
+        if (l2.uniform) {  // all cells of a batch first, then all loads, then the lerps
+            for (int base = gl; base < items; base += B * G) {
+                int cell[B];
+                float wr[B], wc[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const int i = base + b * G;
+                    const float4 d = i < items ? pk[i] : make_float4(0.f, 0.f, 1.f, 1.f);
+                    lpot2d_u_cell(l2, d.w, -(fx * d.x + fy * d.y + fz * d.z), cell[b], wr[b], wc[b]);
+                }
+                float4 q[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) q[b] = l2.vc[cell[b]];
+#pragma unroll
+                for (int b = 0; b < B; ++b)
+                    if (base + b * G < items) acc += (double)lpot2d_u_lerp(q[b].x, q[b].y, q[b].z, q[b].w, wr[b], wc[b]);
+            }
+            for (int o = G >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
+            return acc;
+        }
+    }
     for (int base = gl; base < items; base += B * G) {
         vec4_t<R> d[B];
 #pragma unroll
@@ -656,20 +682,26 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
                 if (cost < -1.0) cost = -2.0 - cost;
                 double sint = sqrt(1.0 - cost * cost);
                 double nnx = sint * cos(ph), nny = sint * sin(ph), nnz = cost;
-                // new link factors rho(n_k-1 . n') and rho(n' . n_k+1), on two lanes when G >= 2
-                double rho0 = 0.0, rho1 = 0.0;
+                // new link factors rho(n_k-1 . n') and rho(n' . n_k+1), on two lanes when G >= 2;
+                // their knots are loaded before the partner gathers, so the loads overlap
+                double rho0 = 0.0, rho1 = 0.0, pa = 0.0, pb = 0.0;
+                RhoCell ca{}, cb{};
                 if (gl < 2) {
                     const int kn = gl == 0 ? kl - 1 : kl + 1;
-                    double p = gl == 0 ? nv[3 * kn] * nnx + nv[3 * kn + 1] * nny + nv[3 * kn + 2] * nnz
-                                       : nnx * nv[3 * kn] + nny * nv[3 * kn + 1] + nnz * nv[3 * kn + 2];
-                    if (gl == 0) rho0 = rot_rho(T, trt, p);
-                    else rho1 = rot_rho(T, trt, p);
+                    pa = gl == 0 ? nv[3 * kn] * nnx + nv[3 * kn + 1] * nny + nv[3 * kn + 2] * nnz
+                                 : nnx * nv[3 * kn] + nny * nv[3 * kn + 1] + nnz * nv[3 * kn + 2];
+                    ca = rot_rho_load(T, trt, pa);
                 }
                 if (G == 1) {
-                    double p = nnx * nv[3 * kl + 3] + nny * nv[3 * kl + 4] + nnz * nv[3 * kl + 5];
-                    rho1 = rot_rho(T, trt, p);
+                    pb = nnx * nv[3 * kl + 3] + nny * nv[3 * kl + 4] + nnz * nv[3 * kl + 5];
+                    cb = rot_rho_load(T, trt, pb);
                 }
                 double enew = slice_energy<R>(l2, geo + (long long)(kl - 1) * items, items, gl, G, nnx, nny, nnz);
+                if (gl < 2) {
+                    if (gl == 0) rho0 = rot_rho_eval(T, trt, pa, ca);
+                    else rho1 = rot_rho_eval(T, trt, pa, ca);
+                }
+                if (G == 1) rho1 = rot_rho_eval(T, trt, pb, cb);
                 if (G > 1) rho1 = __shfl_xor(rho1, 1, G);
                 if (gl == 0) {
                     nev += (unsigned long long)items;
