@@ -265,9 +265,24 @@ __device__ __forceinline__ double fast_div(double x, double y) { return x / y; }
 // `k` is the knot array (LDS or global copy of L.k).  FP64 keeps the reference's
 // arithmetic; FP32 multiplies by the stored inverse cell width instead of dividing, and on
 // uniform knots takes the cell from the grid arithmetic alone (at a cell boundary the two
-// cells' cubics agree to rounding).
+// cells' cubics agree to rounding) and selects between the spline and the tails without
+// branches, so that independent look-ups interleave.
 template <typename R>
 __device__ __forceinline__ R spot1d(const Lut1D<R>& L, const vec4_t<R>* __restrict__ k, R r) {
+    if constexpr (is_fp32<R>()) {
+        if (L.uniform) {  // without branches: both tails are computed and the range selects
+            float f = floor((r - L.x0) * L.inv_h);
+            f = fmin(fmax(f, 0.f), (float)(L.n - 2));
+            const int i = (int)f;
+            const float4 a = k[i], b = k[i + 1];
+            const float A = (b.x - r) * a.w, B = (r - a.x) * a.w, h = b.x - a.x;
+            const float v = A * a.y + B * b.y + ((A * A * A - A) * a.z + (B * B * B - B) * b.z) * (h * h) * (1.0f / 6.0f);
+            const float r2 = r * r;
+            const float above = -fast_div(L.c6, r2 * r2 * r2);
+            const float below = L.unode * __expf(-L.alpha * r);
+            return r >= L.xmax ? above : (r <= L.x0 ? below : v);
+        }
+    }
     if (r >= L.xmax) {
         R r2 = r * r;
         return -fast_div(L.c6, r2 * r2 * r2);
@@ -281,12 +296,6 @@ __device__ __forceinline__ R spot1d(const Lut1D<R>& L, const vec4_t<R>* __restri
     f = fmin(fmax(f, (R)0), (R)(L.n - 2));
     int i = (int)f;
     vec4_t<R> a = k[i], b = k[i + 1];
-    if constexpr (is_fp32<R>()) {
-        if (L.uniform) {
-            R A = (b.x - r) * a.w, B = (r - a.x) * a.w, h = b.x - a.x;
-            return A * a.y + B * b.y + ((A * A * A - A) * a.z + (B * B * B - B) * b.z) * (h * h) * (R)(1.0 / 6.0);
-        }
-    }
     while (i > 0 && a.x > r) {
         --i;
         b = a;
