@@ -691,8 +691,17 @@ int pimc_commit(pimc_ctx* c) {
         c->lds_est = std::max(c->lds_est, (size_t)(3 * p.natoms * (ts + 1) + 3 * ts) * sizeof(double) +
                                               (size_t)p.natoms * sizeof(int));
         int W = S + 1;
-        size_t knots = p.pot1d_id >= 0 ? (size_t)c->t1[p.pot1d_id].n * (c->fp64 ? sizeof(double4) : sizeof(float4)) : 0;
-        const size_t lds_w = (size_t)(((3 * p.natoms * W + 9 * W + 16 + 3) & ~3) + 12 * p.nrot * W) * sizeof(double) + knots;
+        // (y, y'') pairs of uniform FP32 knots in the linear family, vec4 knots otherwise
+        size_t knots = 0;
+        if (p.pot1d_id >= 0) {
+            const Tab1D& t = c->t1[p.pot1d_id];
+            knots = (size_t)t.n * (c->fp64 ? sizeof(double4) : (p.fam == 0 && t.uniform ? sizeof(float2) : sizeof(float4)));
+        }
+        // bisection LDS: axes, new positions and draws (FP64), the window in the pair
+        // precision, frames, knots (the layout of bisect_body)
+        const size_t lds_w = (size_t)((9 * W + 16 + 3) & ~3) * sizeof(double) +
+                             (size_t)p.natoms * W * (c->fp64 ? sizeof(double4) : sizeof(float4)) +
+                             (size_t)12 * p.nrot * W * sizeof(double) + knots;
         (p.fam ? lds_b1 : lds_b) = std::max(p.fam ? lds_b1 : lds_b, lds_w);
         if (!p.fam) {
             const int k = bisect_class(p.natoms);

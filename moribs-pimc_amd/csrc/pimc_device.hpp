@@ -316,6 +316,23 @@ __device__ __forceinline__ R spot1d(const Lut1D<R>& L, const vec4_t<R>* __restri
     }
 }
 
+// SPot1D on uniform FP32 knots from (y, y'') pairs alone (x and the cell width follow from the
+// grid), for kernels that stage the knots in LDS at half the size; the same branchless form
+__device__ __forceinline__ float spot1d_u2(const Lut1D<float>& L, const float2* __restrict__ k2, float r) {
+    const float t = (r - L.x0) * L.inv_h;
+    float f = floor(t);
+    f = fmin(fmax(f, 0.f), (float)(L.n - 2));
+    const int i = (int)f;
+    const float2 a = k2[i], b = k2[i + 1];
+    const float h = fast_div(1.0f, L.inv_h);
+    const float B = t - f, A = 1.0f - B;
+    const float v = A * a.x + B * b.x + ((A * A * A - A) * a.y + (B * B * B - B) * b.y) * (h * h) * (1.0f / 6.0f);
+    const float r2 = r * r;
+    const float above = -fast_div(L.c6, r2 * r2 * r2);
+    const float below = L.unode * __expf(-L.alpha * r);
+    return r >= L.xmax ? above : (r <= L.x0 ? below : v);
+}
+
 // LPot2D on a uniform FP32 grid in two steps, so that the loads of many look-ups can be in
 // flight together: the cell and the weights, then the lerps of its four corner values, which
 // the corner-packed copy holds in one 16-byte load.
@@ -485,18 +502,35 @@ __device__ __forceinline__ double rot_spline(const Tables& T, int tid, double g,
 // mc_piqmc.cc:1847-1958, for one atom type and one linear-molecule type per system,
 // mc_input.cc:386-387).  rot_a / rot_j: the mover / partner is the linear molecule; (nx,ny,nz)
 // is its axis at that slice.
-template <typename R>
-__device__ __forceinline__ R pair_v(const Lut1D<R>& l1, const vec4_t<R>* __restrict__ k1, const Lut2D<R>& l2,
-                                    bool rot_a, bool rot_j, double xa, double ya, double za, double xj, double yj,
-                                    double zj, double nx, double ny, double nz) {
-    R dx = (R)(xa - xj), dy = (R)(ya - yj), dz = (R)(za - zj);
-    R r = fast_sqrt(dx * dx + dy * dy + dz * dz);
+// ... on the separation (dx, dy, dz) = mover - partner.  HALF: k1 points to (y, y'') pairs of
+// uniform FP32 knots (spot1d_u2), otherwise to vec4 knots.
+template <typename R, bool HALF = false>
+__device__ __forceinline__ R pair_d(const Lut1D<R>& l1, const void* __restrict__ k1, const Lut2D<R>& l2,
+                                    bool rot_a, bool rot_j, R dx, R dy, R dz, double nx, double ny, double nz) {
+    const R r = fast_sqrt(dx * dx + dy * dy + dz * dz);
     if (rot_a || rot_j) {
         R cost = fast_div((R)nx * dx + (R)ny * dy + (R)nz * dz, r);
         if (rot_a) cost = -cost;
         return lpot2d<R>(l2, r, cost);
     }
-    return spot1d<R>(l1, k1, r);
+    if constexpr (HALF && is_fp32<R>()) return spot1d_u2(l1, static_cast<const float2*>(k1), r);
+    else return spot1d<R>(l1, static_cast<const vec4_t<R>*>(k1), r);
+}
+
+// positions in FP64: the separation is formed in FP64 and rounded once
+template <typename R>
+__device__ __forceinline__ R pair_v(const Lut1D<R>& l1, const vec4_t<R>* __restrict__ k1, const Lut2D<R>& l2,
+                                    bool rot_a, bool rot_j, double xa, double ya, double za, double xj, double yj,
+                                    double zj, double nx, double ny, double nz) {
+    return pair_d<R>(l1, k1, l2, rot_a, rot_j, (R)(xa - xj), (R)(ya - yj), (R)(za - zj), nx, ny, nz);
+}
+
+// positions already in the pair precision (the bisection window)
+template <typename R, bool HALF = false>
+__device__ __forceinline__ R pair_vr(const Lut1D<R>& l1, const void* __restrict__ k1, const Lut2D<R>& l2,
+                                     bool rot_a, bool rot_j, R xa, R ya, R za, R xj, R yj, R zj, double nx, double ny,
+                                     double nz) {
+    return pair_d<R, HALF>(l1, k1, l2, rot_a, rot_j, xa - xj, ya - yj, za - zj, nx, ny, nz);
 }
 
 template <typename T>

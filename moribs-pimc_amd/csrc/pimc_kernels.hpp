@@ -88,7 +88,9 @@ __device__ __forceinline__ const Frame& frame_at(const Frame* __restrict__ frame
 }
 
 // ------------------------------------------------------------------------- bisection
-// LDS: wx,wy,wz [natoms*(S+1)], nh [3*(S+1)], nw [3*(S+1)]; blockDim a multiple of 64.
+// The window of all atoms' positions on the segment's S+1 slots is held in the pair precision
+// (float4 in FP32 runs, double4 in FP64 runs); new positions are built in FP64 from FP64 end
+// points, and accepted ones go to HBM in FP64 at once.  blockDim a multiple of 64.
 // FAM 1 (non-linear rotors) adds the rotors' frames at the window's slots: fr [nrot*(S+1)].
 template <typename R, int FAM>
 __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* __restrict__ mut,
@@ -110,31 +112,41 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
         s0 = run.u + wk.y * S;
     }
     const int W = S + 1;
-    double* wx = smem;
-    double* wy = wx + N * W;
-    double* wz = wy + N * W;
-    double* nh = wz + N * W;
+    using P4 = vec4_t<R>;
+    // LDS: rotor axes, the moving atom's new positions and its draws (FP64), then the window of
+    // all atoms' positions in the pair precision (x, y, z, -), rotor frames (FAM 1) and the 1D
+    // spline knots: (y, y'') pairs of uniform FP32 knots (linear family), vec4 knots otherwise
+    double* nh = smem;
     double* nw = nh + 3 * W;
     double* gw = nw + 3 * W;  // draws of the moving atom: 3 per midpoint, then one per level
-    // rotor frames (FAM 1), then the 1D spline knots (16-byte aligned)
-    Frame* fr = reinterpret_cast<Frame*>(smem + ((3 * N * W + 9 * W + 16 + 3) & ~3));
+    P4* win = reinterpret_cast<P4*>(smem + ((9 * W + 16 + 3) & ~3));
+    Frame* fr = reinterpret_cast<Frame*>(win + N * W);
     const int nfr = FAM == 1 ? rp.nrot * W : 0;
-    vec4_t<R>* kn = reinterpret_cast<vec4_t<R>*>(smem + ((3 * N * W + 9 * W + 16 + 3) & ~3) + 12 * nfr);
+    P4* kn = reinterpret_cast<P4*>(fr + nfr);
     const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
     const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
     const Lut3D<R> l3 = lut3d<R>(T3, FAM == 1 ? rp.pot3d_id : -1);
-    for (int i = tid; i < l1.n; i += nth) kn[i] = l1.k[i];
+    const bool half_knots = FAM == 0 && is_fp32<R>() && l1.uniform;
+    if (half_knots) {
+        float2* k2 = reinterpret_cast<float2*>(kn);
+        for (int i = tid; i < l1.n; i += nth) k2[i] = make_float2((float)l1.k[i].y, (float)l1.k[i].z);
+    } else {
+        for (int i = tid; i < l1.n; i += nth) kn[i] = l1.k[i];
+    }
     if constexpr (FAM == 1)
         for (int i = tid; i < nfr; i += nth) {
             const int q = i / W, w = i - q * W;
             fr[i] = frame_at(frames, rp, rp.nl_atom + q, (s0 + w) % P);
         }
     for (int i = tid; i < N * W; i += nth) {
-        int a = i / W, w = i - a * W;
-        double4 b = bead(beads, rp, a, (s0 + w) % P);
-        wx[i] = b.x;
-        wy[i] = b.y;
-        wz[i] = b.z;
+        const int a = i / W, w = i - a * W;
+        const double4 b = bead(beads, rp, a, (s0 + w) % P);
+        P4 q;
+        q.x = (R)b.x;
+        q.y = (R)b.y;
+        q.z = (R)b.z;
+        q.w = (R)0;
+        win[i] = q;
     }
     for (int w = tid; w < W; w += nth) {
         double x, y, z;
@@ -156,10 +168,13 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
         const double bnorm = 1.0 / (rp.lambda[ty] * rp.tau);
         for (int ia = 0; ia < rp.tnumb[ty]; ++ia) {
             const int a = rp.tbegin[ty] + ia;
-            if (tid < 3) {  // end points of the moving atom's segment
-                double* src = tid == 0 ? wx : (tid == 1 ? wy : wz);
-                nw[tid] = src[a * W];
-                nw[3 * St + tid] = src[a * W + St];
+            // end points of the moving atom's segment (slots 0 and St) in FP64 from HBM: no move
+            // of this launch changes them; the loads overlap the draws
+            double e0 = 0.0, e1 = 0.0;
+            if (tid < 3) {
+                const double4 b0 = bead(beads, rp, a, s0 % P), b1 = bead(beads, rp, a, (s0 + St) % P);
+                e0 = tid == 0 ? b0.x : (tid == 1 ? b0.y : b0.z);
+                e1 = tid == 0 ? b1.x : (tid == 1 ? b1.y : b1.z);
             }
             // every draw of this move at once: Gaussians of midpoint m (level-major order)
             // and the level acceptance uniforms u, kept as -log(u): the level test
@@ -180,6 +195,10 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                     gw[i] = -log(u53(w4.x, w4.y));
                 }
             }
+            if (tid < 3) {
+                nw[tid] = e0;
+                nw[3 * St + tid] = e1;
+            }
             __syncthreads();
             double cum = 0.0;
             bool accepted = false;
@@ -198,59 +217,56 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                 // Delta V at the new midpoints: items (midpoint, partner)
                 const int np = N - 1;
                 const int items = nmid * np;
-                double acc = 0.0;
                 const bool rot_a = FAM == 1 ? (rp.nl_atom >= 0 && a >= rp.nl_atom) : (a == rp.lin_atom);
                 // an atom with the linear rotor last among its partners: the atom-atom terms and
-                // the rotor terms in separate loops, so no wave runs both tables' code
+                // the rotor terms in one index space, rotor terms last, so a wave runs one
+                // table's code except at the seam
                 const bool split = FAM == 0 && !rot_a && rp.lin_atom == N - 1;
-                if (split) {  // items: nmid*(np-1) atom-atom terms first, then the nmid rotor terms
-                    const int np1 = np - 1, n1 = nmid * np1;
+                auto item_sum = [&](auto half_c) -> double {
+                    constexpr bool H = decltype(half_c)::value;
+                    double acc = 0.0;
                     for (int i = tid; i < items; i += nth) {
-                        const bool rj = i >= n1;
-                        const int q = rj ? i - n1 : i / np1, jp = rj ? 0 : i - q * np1;
-                        const int j = rj ? N - 1 : jp + (jp >= a);
-                        const int slot = half + q * lss;
-                        const double xj = wx[j * W + slot], yj = wy[j * W + slot], zj = wz[j * W + slot];
-                        double nx = 0.0, ny = 0.0, nz = 1.0;
-                        if (rj) {
-                            nx = nh[3 * slot];
-                            ny = nh[3 * slot + 1];
-                            nz = nh[3 * slot + 2];
-                        }
-#pragma unroll
-                        for (int no = 0; no < 2; ++no) {
-                            const double xa = no ? wx[a * W + slot] : nw[3 * slot];
-                            const double ya = no ? wy[a * W + slot] : nw[3 * slot + 1];
-                            const double za = no ? wz[a * W + slot] : nw[3 * slot + 2];
-                            const R v = rj ? pair_v<R>(l1, kn, l2, false, true, xa, ya, za, xj, yj, zj, nx, ny, nz)
-                                           : pair_v<R>(l1, kn, l2, false, false, xa, ya, za, xj, yj, zj, 0.0, 0.0, 1.0);
-                            acc += no ? -(double)v : (double)v;
-                        }
-                    }
-                }
-                for (int i = split ? items : tid; i < items; i += nth) {
-                    int q = i / np, jp = i - q * np;
-                    int j = jp + (jp >= a);
-                    int slot = half + q * lss;
-                    const bool rot_j = FAM == 1 ? (rp.nl_atom >= 0 && j >= rp.nl_atom) : (j == rp.lin_atom);
-                    double xj = wx[j * W + slot], yj = wy[j * W + slot], zj = wz[j * W + slot];
-                    double nx = nh[3 * slot], ny = nh[3 * slot + 1], nz = nh[3 * slot + 2];
-#pragma unroll
-                    for (int no = 0; no < 2; ++no) {  // new position, then old
-                        double xa = no ? wx[a * W + slot] : nw[3 * slot];
-                        double ya = no ? wy[a * W + slot] : nw[3 * slot + 1];
-                        double za = no ? wz[a * W + slot] : nw[3 * slot + 2];
-                        if constexpr (FAM == 1) {
-                            double v = pair_nl<R>(l1, kn, l3, rot_a, rot_j, xa, ya, za, xj, yj, zj,
-                                                  fr[(rot_a ? a - rp.nl_atom : 0) * W + slot],
-                                                  fr[(rot_j ? j - rp.nl_atom : 0) * W + slot]);
-                            acc += no ? -v : v;
+                        int q, j;
+                        bool rj;
+                        if (split) {
+                            const int np1 = np - 1, n1 = nmid * np1;
+                            rj = i >= n1;
+                            q = rj ? i - n1 : i / np1;
+                            const int jp = rj ? 0 : i - q * np1;
+                            j = rj ? N - 1 : jp + (jp >= a);
                         } else {
-                            R v = pair_v<R>(l1, kn, l2, rot_a, rot_j, xa, ya, za, xj, yj, zj, nx, ny, nz);
-                            acc += no ? -(double)v : (double)v;
+                            q = i / np;
+                            const int jp = i - q * np;
+                            j = jp + (jp >= a);
+                            rj = FAM == 1 ? (rp.nl_atom >= 0 && j >= rp.nl_atom) : (j == rp.lin_atom);
+                        }
+                        const int slot = half + q * lss;
+                        const P4 pj = win[j * W + slot], pa = win[a * W + slot];
+                        const R xn = (R)nw[3 * slot], yn = (R)nw[3 * slot + 1], zn = (R)nw[3 * slot + 2];
+                        if constexpr (FAM == 1) {
+                            const Frame& fa = fr[(rot_a ? a - rp.nl_atom : 0) * W + slot];
+                            const Frame& fj = fr[(rj ? j - rp.nl_atom : 0) * W + slot];
+                            acc += pair_nl<R>(l1, kn, l3, rot_a, rj, xn, yn, zn, pj.x, pj.y, pj.z, fa, fj) -
+                                   pair_nl<R>(l1, kn, l3, rot_a, rj, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, fa, fj);
+                        } else if (split) {
+                            if (rj) {
+                                const double nx = nh[3 * slot], ny = nh[3 * slot + 1], nz = nh[3 * slot + 2];
+                                acc += (double)pair_vr<R, H>(l1, kn, l2, false, true, xn, yn, zn, pj.x, pj.y, pj.z, nx, ny, nz) -
+                                       (double)pair_vr<R, H>(l1, kn, l2, false, true, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, nx, ny, nz);
+                            } else {
+                                acc += (double)pair_vr<R, H>(l1, kn, l2, false, false, xn, yn, zn, pj.x, pj.y, pj.z, 0.0, 0.0, 1.0) -
+                                       (double)pair_vr<R, H>(l1, kn, l2, false, false, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, 0.0, 0.0, 1.0);
+                            }
+                        } else {
+                            const double nx = nh[3 * slot], ny = nh[3 * slot + 1], nz = nh[3 * slot + 2];
+                            acc += (double)pair_vr<R, H>(l1, kn, l2, rot_a, rj, xn, yn, zn, pj.x, pj.y, pj.z, nx, ny, nz) -
+                                   (double)pair_vr<R, H>(l1, kn, l2, rot_a, rj, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, nx, ny, nz);
                         }
                     }
-                }
+                    return acc;
+                };
+                double acc = half_knots ? item_sum(std::integral_constant<bool, true>{})
+                                        : item_sum(std::integral_constant<bool, false>{});
                 acc = block_sum(acc, red);
                 evals += 2ull * (unsigned long long)items;
                 // sum over all points up to this level; level acceptance of mc_piqmc.cc:263-273
@@ -261,22 +277,22 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                 if (!accepted) break;
             }
             tot[ty] += 1;
-            if (accepted) {
+            if (accepted) {  // the new midpoints into the window and, in FP64, to HBM
                 acc_n[ty] += 1;
-                for (int i = tid; i < 3 * (St - 1); i += nth) {
-                    int slot = 1 + i / 3, id = i % 3;
-                    double* dst = id == 0 ? wx : (id == 1 ? wy : wz);
-                    dst[a * W + slot] = nw[3 * slot + id];
+                for (int i = tid; i < St - 1; i += nth) {
+                    const int slot = 1 + i;
+                    const double x = nw[3 * slot], y = nw[3 * slot + 1], z = nw[3 * slot + 2];
+                    P4 q;
+                    q.x = (R)x;
+                    q.y = (R)y;
+                    q.z = (R)z;
+                    q.w = (R)0;
+                    win[a * W + slot] = q;
+                    beads[rp.bead_off + (long long)a * P + (s0 + slot) % P] = make_double4(x, y, z, 0.0);
                 }
             }
             __syncthreads();
         }
-    }
-    // interior slots back to HBM (end points were never modified)
-    for (int i = tid; i < N * (S - 1); i += nth) {
-        int a = i / (S - 1), w = 1 + i - a * (S - 1);
-        int idx = a * W + w;
-        beads[rp.bead_off + (long long)a * P + (s0 + w) % P] = make_double4(wx[idx], wy[idx], wz[idx], 0.0);
     }
     if (tid == 0) {
         for (int ty = run.type_begin; ty < tend; ++ty) {
