@@ -88,6 +88,11 @@ __device__ __forceinline__ const Frame& frame_at(const Frame* __restrict__ frame
 }
 
 // ------------------------------------------------------------------------- bisection
+constexpr int BIS_RING = 128;  // doubles of draws staged for a batch of atoms
+
+// doubles of the draw ring for a window of W slots: a batch of atoms, or one atom's draws
+__host__ __device__ constexpr int bisect_ring(int W) { return BIS_RING > 3 * W + 16 ? BIS_RING : 3 * W + 16; }
+
 // The window of all atoms' positions on the segment's S+1 slots is held in the pair precision
 // (float4 in FP32 runs, double4 in FP64 runs); new positions are built in FP64 from FP64 end
 // points, and accepted ones go to HBM in FP64 at once.  blockDim a multiple of 64.
@@ -118,8 +123,8 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
     // spline knots: (y, y'') pairs of uniform FP32 knots (linear family), vec4 knots otherwise
     double* nh = smem;
     double* nw = nh + 3 * W;
-    double* gw = nw + 3 * W;  // draws of the moving atom: 3 per midpoint, then one per level
-    P4* win = reinterpret_cast<P4*>(smem + ((9 * W + 16 + 3) & ~3));
+    double* gw = nw + 3 * W;  // draws of a batch of atoms: per atom 3 per midpoint, then one per level
+    P4* win = reinterpret_cast<P4*>(smem + ((6 * W + bisect_ring(W) + 3) & ~3));
     Frame* fr = reinterpret_cast<Frame*>(win + N * W);
     const int nfr = FAM == 1 ? rp.nrot * W : 0;
     P4* kn = reinterpret_cast<P4*>(fr + nfr);
@@ -166,40 +171,55 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
         const int L = rp.tlev[ty];
         const int St = 1 << L;
         const double bnorm = 1.0 / (rp.lambda[ty] * rp.tau);
+        const int nm = St - 1;
+        const int per = 3 * nm + L;  // draws of one atom's move
+        const int K = per <= BIS_RING ? (nth < BIS_RING ? nth : BIS_RING) / per : 1;  // atoms per draw batch
+        // end points of an atom's segment (slots 0 and St) in FP64 from HBM (no move of this launch
+        // changes them), loaded one atom ahead on the three coordinate lanes
+        double e0 = 0.0, e1 = 0.0;
+        if (tid < 3 && rp.tnumb[ty] > 0) {
+            const int a = rp.tbegin[ty];
+            const double4 b0 = bead(beads, rp, a, s0 % P), b1 = bead(beads, rp, a, (s0 + St) % P);
+            e0 = tid == 0 ? b0.x : (tid == 1 ? b0.y : b0.z);
+            e1 = tid == 0 ? b1.x : (tid == 1 ? b1.y : b1.z);
+        }
         for (int ia = 0; ia < rp.tnumb[ty]; ++ia) {
             const int a = rp.tbegin[ty] + ia;
-            // end points of the moving atom's segment (slots 0 and St) in FP64 from HBM: no move
-            // of this launch changes them; the loads overlap the draws
-            double e0 = 0.0, e1 = 0.0;
-            if (tid < 3) {
-                const double4 b0 = bead(beads, rp, a, s0 % P), b1 = bead(beads, rp, a, (s0 + St) % P);
-                e0 = tid == 0 ? b0.x : (tid == 1 ? b0.y : b0.z);
-                e1 = tid == 0 ? b1.x : (tid == 1 ? b1.y : b1.z);
-            }
-            // every draw of this move at once: Gaussians of midpoint m (level-major order)
-            // and the level acceptance uniforms u, kept as -log(u): the level test
-            // exp(-dV) > u (mc_piqmc.cc:265-273) is then dV < -log(u), a comparison
-            const int nm = St - 1;
-            for (int i = tid; i < 3 * nm + L; i += nth) {
-                if (i < 3 * nm) {
-                    int m = i / 3, id = i - 3 * m;
-                    int level = 31 - __clz(m + 1);
-                    int q = m + 1 - (1 << level);
-                    uint4 w4 = draw(run.seed, rp.key, pass, K_BISECT, 0, (uint32_t)a, (uint32_t)s0,
-                                    ((uint32_t)level << 24) | ((uint32_t)q << 8) | (uint32_t)id);
-                    gw[i] = gauss_from(w4, bnorm / (double)(1 << (L - level)));
-                } else {
-                    int level = i - 3 * nm;
-                    uint4 w4 = draw(run.seed, rp.key, pass, K_BISECT, 0, (uint32_t)a, (uint32_t)s0,
-                                    ((uint32_t)level << 24) | (0xFFFFu << 8));
-                    gw[i] = -log(u53(w4.x, w4.y));
+            const int kk = ia % K;
+            if (kk == 0) {
+                // every draw of the next K moves at once: Gaussians of midpoint m (level-major
+                // order) and the level acceptance uniforms u, kept as -log(u): the level test
+                // exp(-dV) > u (mc_piqmc.cc:265-273) is then dV < -log(u), a comparison
+                const int nk = rp.tnumb[ty] - ia < K ? rp.tnumb[ty] - ia : K;
+                for (int i = tid; i < nk * per; i += nth) {
+                    const int k = i / per, d = i - k * per;
+                    const uint32_t ab = (uint32_t)(a + k);
+                    if (d < 3 * nm) {
+                        int m = d / 3, id = d - 3 * m;
+                        int level = 31 - __clz(m + 1);
+                        int q = m + 1 - (1 << level);
+                        uint4 w4 = draw(run.seed, rp.key, pass, K_BISECT, 0, ab, (uint32_t)s0,
+                                        ((uint32_t)level << 24) | ((uint32_t)q << 8) | (uint32_t)id);
+                        gw[i] = gauss_from(w4, bnorm / (double)(1 << (L - level)));
+                    } else {
+                        int level = d - 3 * nm;
+                        uint4 w4 = draw(run.seed, rp.key, pass, K_BISECT, 0, ab, (uint32_t)s0,
+                                        ((uint32_t)level << 24) | (0xFFFFu << 8));
+                        gw[i] = -log(u53(w4.x, w4.y));
+                    }
                 }
             }
+            const double* g = gw + kk * per;  // this move's draws
             if (tid < 3) {
                 nw[tid] = e0;
                 nw[3 * St + tid] = e1;
+                if (ia + 1 < rp.tnumb[ty]) {  // the next atom's end points, used one move later
+                    const double4 b0 = bead(beads, rp, a + 1, s0 % P), b1 = bead(beads, rp, a + 1, (s0 + St) % P);
+                    e0 = tid == 0 ? b0.x : (tid == 1 ? b0.y : b0.z);
+                    e1 = tid == 0 ? b1.x : (tid == 1 ? b1.y : b1.z);
+                }
             }
-            __syncthreads();
+            if (kk == 0) __syncthreads();  // the batch's draws are visible to every lane
             double cum = 0.0;
             bool accepted = false;
             for (int level = 0; level < L; ++level) {
@@ -211,7 +231,7 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                     int q = i / 3, id = i - 3 * q;
                     int slot = half + q * lss;
                     nw[3 * slot + id] = 0.5 * (nw[3 * (slot - half) + id] + nw[3 * (slot + half) + id]) +
-                                        gw[3 * (nmid - 1 + q) + id];
+                                        g[3 * (nmid - 1 + q) + id];
                 }
                 __syncthreads();
                 // Delta V at the new midpoints: items (midpoint, partner)
@@ -273,7 +293,7 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                 const double prev = cum;
                 cum += acc;
                 double deltav = (cum - 2.0 * prev) * (rp.tau * (double)half);
-                accepted = deltav < 0.0 || deltav < gw[3 * nm + level];
+                accepted = deltav < 0.0 || deltav < g[3 * nm + level];
                 if (!accepted) break;
             }
             tot[ty] += 1;
