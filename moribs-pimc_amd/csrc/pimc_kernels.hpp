@@ -859,8 +859,7 @@ __global__ __launch_bounds__(BLOCK) void k_rotor3d(const Rep* __restrict__ reps,
     for (int i = threadIdx.x; i < nrot * Prot; i += blockDim.x) {
         const int q = i / Prot, k = i - q * Prot;
         const long long b = rb0 + (long long)q * Prot;
-        double er, es;
-        if (rotden_frames(frames[b + k], frames[b + (k + 1 == Prot ? 0 : k + 1)], tab, Lk[i], er, es)) err = 1;
+        if (rotden_rho(frames[b + k], frames[b + (k + 1 == Prot ? 0 : k + 1)], tab, Lk[i])) err = 1;
     }
     __syncthreads();
     for (int sw = 0; sw < nsweep; ++sw) {
@@ -899,8 +898,7 @@ __global__ __launch_bounds__(BLOCK) void k_rotor3d(const Rep* __restrict__ reps,
                     for (int c = 2 + gl; c < 4; c += G) {
                         const Frame& fa = (c == 3) ? gf[nrot] : frames[rb0 + (long long)q * Prot + k0];
                         const Frame& fb = (c == 2) ? gf[nrot] : frames[rb0 + (long long)q * Prot + k2];
-                        double er, es;
-                        if (rotden_frames(fa, fb, tab, d[c], er, es)) err = 1;
+                        if (rotden_rho(fa, fb, tab, d[c])) err = 1;
                     }
                     if (G > 1)
                         for (int c = 2; c < 4; ++c) d[c] = __shfl(d[c], (c - 2) % G, G);
@@ -1041,8 +1039,7 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
         const int q = i / L, kl = i - q * L;
         const Frame& fa = kl == 0 ? fnb[q] : own(q, k0 + kl - 1);
         const Frame& fb = kl == ns ? fnb[nrot + q] : own(q, k0 + kl);
-        double er, es;
-        if (rotden_frames(fa, fb, tab, Lk[i], er, es)) err = 1;
+        if (rotden_rho(fa, fb, tab, Lk[i])) err = 1;
     }
     __syncthreads();
     for (int sw = 0; sw < nsweep; ++sw) {
@@ -1059,10 +1056,9 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
                 for (int i = threadIdx.x; i < nrot * FD; i += blockDim.x) dst[i] = load_agent(src + i);
                 __syncthreads();
                 for (int q = threadIdx.x; q < nrot; q += blockDim.x) {
-                    double er, es;
                     if (parity == 0) {
-                        if (rotden_frames(fnb[q], own(q, k0), tab, Lk[q * L], er, es)) err = 1;
-                    } else if (rotden_frames(own(q, k0 + ns - 1), fnb[nrot + q], tab, Lk[q * L + ns], er, es)) {
+                        if (rotden_rho(fnb[q], own(q, k0), tab, Lk[q * L])) err = 1;
+                    } else if (rotden_rho(own(q, k0 + ns - 1), fnb[nrot + q], tab, Lk[q * L + ns])) {
                         err = 1;
                     }
                 }
@@ -1097,8 +1093,7 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
                     for (int c = 2 + gl; c < 4; c += G) {
                         const Frame& fa = (c == 3) ? gf[nrot] : (kl == 1 ? fnb[q] : own(q, k - 1));
                         const Frame& fb = (c == 2) ? gf[nrot] : (kl == ns ? fnb[nrot + q] : own(q, k + 1));
-                        double er, es;
-                        if (rotden_frames(fa, fb, tab, d[c], er, es)) err = 1;
+                        if (rotden_rho(fa, fb, tab, d[c])) err = 1;
                     }
                     if (G > 1)
                         for (int c = 2; c < 4; ++c) d[c] = __shfl(d[c], (c - 2) % G, G);
@@ -1269,8 +1264,7 @@ __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps
     int err = 0;
     mlp_stage(Wg, W);
     for (int i = threadIdx.x; i < Prot; i += blockDim.x) {
-        double er, es;
-        if (rotden_frames(frames[rb0 + i], frames[rb0 + (i + 1 == Prot ? 0 : i + 1)], tab, Lk[i], er, es)) err = 1;
+        if (rotden_rho(frames[rb0 + i], frames[rb0 + (i + 1 == Prot ? 0 : i + 1)], tab, Lk[i])) err = 1;
     }
     __syncthreads();
     for (int sw = 0; sw < nsweep; ++sw) {
@@ -1306,8 +1300,7 @@ __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps
                 // new density factors (t0,t1') on lane 0 and (t1',t2) on lane 1
                 double dl = 0.0;
                 if (l < 2) {
-                    double er, es;
-                    if (rotden_frames(l == 0 ? frames[rb0 + k0] : fn, l == 0 ? fn : frames[rb0 + k2], tab, dl, er, es))
+                    if (rotden_rho(l == 0 ? frames[rb0 + k0] : fn, l == 0 ? fn : frames[rb0 + k2], tab, dl))
                         err = 1;
                 }
                 const double d2 = __shfl(dl, 0), d3 = __shfl(dl, 1);

@@ -76,6 +76,23 @@ __device__ inline void deleul_frames(const Frame& f1, const Frame& f2, double& p
 // rotden (rotden.f:1-31) with rotpro (rotpro_sub.f:1-64): corner value plus one forward
 // difference per axis on the 1-degree grid, index (itheta*361 + iphi)*361 + ichi; E and E^2
 // converted from cm^-1 to Kelvin.  Returns 1 for an out-of-range angle (the reference's istop).
+// rho alone, for the moves: one 32-byte gather instead of three
+__device__ inline int rotden_rho(const Frame& f1, const Frame& f2, const double4* __restrict__ tab, double& rho) {
+    double p2, t2, c2;
+    deleul_frames(f1, f2, p2, t2, c2);
+    const double phi = p2 * 180.0 / PI_D, theta = t2 * 180.0 / PI_D, chi = c2 * 180.0 / PI_D;
+    int stop = 0;
+    int ichi = (int)chi, iphi = (int)phi, ith = (int)theta;
+    if (ichi > 360 || ichi < 0) { ichi = 0; stop = 1; }
+    if (iphi > 360 || iphi < 0) { iphi = 0; stop = 1; }
+    if (ith > 180 || ith < 0) { ith = 0; stop = 1; }
+    const long long i0 = ((long long)ith * 361 + iphi) * 361 + ichi;
+    const double xc = chi - (double)ichi, xp = phi - (double)iphi, xt = theta - (double)ith;
+    const double4 r = tab[i0];
+    rho = r.x + r.y * xc + r.z * xp + r.w * xt;
+    return stop;
+}
+
 __device__ inline int rotden_frames(const Frame& f1, const Frame& f2, const double4* __restrict__ tab, double& rho,
                                     double& erot, double& esq) {
     const double wno2k = 0.6950356;
