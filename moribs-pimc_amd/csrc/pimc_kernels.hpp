@@ -590,7 +590,7 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut* __restrict__ mut,
                                                const double4* __restrict__ beads, RotBead* __restrict__ rotb,
                                                Tables T, Tables3 T3, const RotPart* __restrict__ parts,
-                                               unsigned* __restrict__ flags, Run run) {
+                                               unsigned* __restrict__ flags, double* __restrict__ links, Run run) {
     extern __shared__ double smem[];
     __shared__ unsigned long long red_acc[4];
     __shared__ int s_err;
@@ -690,9 +690,14 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
                     nv[3 * kl] = nv[3 * src];
                     nv[3 * kl + 1] = nv[3 * src + 1];
                     nv[3 * kl + 2] = nv[3 * src + 2];
+                    const int lk = parity == 0 ? 0 : ns;
+                    double p = nv[3 * lk] * nv[3 * lk + 3] + nv[3 * lk + 1] * nv[3 * lk + 4] + nv[3 * lk + 2] * nv[3 * lk + 5];
+                    Lk[lk] = rot_rho(T, trt, p);
                 } else {
                     // the even half waits for the previous part's odd half of the last sweep,
-                    // the odd half for the next part's even half of this sweep
+                    // the odd half for the next part's even half of this sweep; the neighbour
+                    // publishes the density of the shared link with its orientation (the same
+                    // product of the same operands this part would form)
                     if (!wait_count(parity == 0 ? prevflag : nextflag, base + h)) s_err = 2;
                     int k = parity == 0 ? k0 - 1 : k0 + ns;
                     if (k < 0) k += Prot;
@@ -701,10 +706,9 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
                     nv[3 * kl] = load_agent(&b->nx);
                     nv[3 * kl + 1] = load_agent(&b->ny);
                     nv[3 * kl + 2] = load_agent(&b->nz);
+                    const int nb = pt.fbase + (parity == 0 ? (pt.w + W - 1) % W : (pt.w + 1) % W);
+                    Lk[parity == 0 ? 0 : ns] = load_agent(links + 2 * nb + (parity == 0 ? 1 : 0));
                 }
-                const int lk = parity == 0 ? 0 : ns;
-                double p = nv[3 * lk] * nv[3 * lk + 3] + nv[3 * lk + 1] * nv[3 * lk + 4] + nv[3 * lk + 2] * nv[3 * lk + 5];
-                Lk[lk] = rot_rho(T, trt, p);
             }
             __syncthreads();
             const int nmoves = (ns - parity + 1) / 2;  // k0 is even, so local slice kl has parity (kl - 1) & 1
@@ -780,6 +784,9 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
                 store_agent(&b->nx, nv[3 * kl]);
                 store_agent(&b->ny, nv[3 * kl + 1]);
                 store_agent(&b->nz, nv[3 * kl + 2]);
+                // the density of the link to the neighbour's adjacent slice: Lk[0] after the even
+                // half (read by the previous part), Lk[ns] after the odd half (by the next)
+                store_agent(links + 2 * (pt.fbase + pt.w) + parity, Lk[parity == 0 ? 0 : ns]);
                 raise_count(myflag, base + h + 1u);
             }
         }
