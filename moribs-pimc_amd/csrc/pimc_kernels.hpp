@@ -245,20 +245,32 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                 auto item_sum = [&](auto half_c) -> double {
                     constexpr bool H = decltype(half_c)::value;
                     double acc = 0.0;
+                    // item i = (midpoint qa, partner index jp) with den partners per midpoint in
+                    // the atom range; a lane's items advance by nth, so (qa, jp) is carried
+                    // along with one division per level
+                    const int den = split ? np - 1 : np;
+                    const int n1 = split ? nmid * den : items;  // items of the atom range
+                    const int dd = den > 0 ? den : 1;
+                    const int dq = nth / dd, dj = nth - dq * dd;
+                    int qa = tid / dd, jp = tid - qa * dd;
                     for (int i = tid; i < items; i += nth) {
                         int q, j;
                         bool rj;
-                        if (split) {
-                            const int np1 = np - 1, n1 = nmid * np1;
-                            rj = i >= n1;
-                            q = rj ? i - n1 : i / np1;
-                            const int jp = rj ? 0 : i - q * np1;
-                            j = rj ? N - 1 : jp + (jp >= a);
+                        if (i >= n1) {  // the rotor terms after the atom range (split)
+                            rj = true;
+                            q = i - n1;
+                            j = N - 1;
                         } else {
-                            q = i / np;
-                            const int jp = i - q * np;
+                            q = qa;
                             j = jp + (jp >= a);
-                            rj = FAM == 1 ? (rp.nl_atom >= 0 && j >= rp.nl_atom) : (j == rp.lin_atom);
+                            rj = split ? false
+                                       : (FAM == 1 ? (rp.nl_atom >= 0 && j >= rp.nl_atom) : (j == rp.lin_atom));
+                        }
+                        jp += dj;
+                        qa += dq;
+                        if (jp >= dd) {
+                            jp -= dd;
+                            ++qa;
                         }
                         const int slot = half + q * lss;
                         const P4 pj = win[j * W + slot], pa = win[a * W + slot];
