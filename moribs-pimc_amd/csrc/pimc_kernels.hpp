@@ -454,10 +454,21 @@ __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ 
         const bool split = FAM == 0 && !rot_a && rp.lin_atom == N - 1;
         if (split) {  // items: nt*(np-1) atom-atom terms first, then the nt rotor terms
             const int np1 = np - 1, n1 = nt * np1;
+            // item i = (slice t0 + q, partner index jp); a lane's items advance by blockDim,
+            // so (q, jp) is carried along with one division per atom
+            const int dd = np1 > 0 ? np1 : 1, nth = (int)blockDim.x;
+            const int dq = nth / dd, dj = nth - dq * dd;
+            int qa = (int)threadIdx.x / dd, jpa = (int)threadIdx.x - qa * dd;
             for (int i = threadIdx.x; i < items; i += blockDim.x) {
                 const bool rj = i >= n1;
-                const int it = t0 + (rj ? i - n1 : i / np1), jp = rj ? 0 : i % np1;
+                const int it = t0 + (rj ? i - n1 : qa), jp = rj ? 0 : jpa;
                 const int j = rj ? N - 1 : jp + (jp >= a);
+                jpa += dj;
+                qa += dq;
+                if (jpa >= dd) {
+                    jpa -= dd;
+                    ++qa;
+                }
                 const double4 xa = bead(beads, rp, a, it), xj = bead(beads, rp, j, it);
                 double nx = 0.0, ny = 0.0, nz = 1.0;
                 if (rj) rotor_axis(rotb, rp, it, nx, ny, nz);
