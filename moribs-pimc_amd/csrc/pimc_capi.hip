@@ -884,7 +884,7 @@ int pimc_commit(pimc_ctx* c) {
             }
             rot3.swap(whole);
             const int nflags3 = (int)p3.size();
-            c->pub3_stride = 2 * nrmax * (int)(sizeof(Frame) / sizeof(double));
+            c->pub3_stride = 2 * nrmax * (int)(sizeof(Frame) / sizeof(double)) + 2 * nrmax;  // frames, then link densities
             p3 = xcd_order(p3, [&](int r) { return (double)c->rep[r].Prot * c->rep[r].nrot; });
             c->nparts3 = (int)p3.size();
             c->d_parts3 = dalloc<RotPart>(p3.size());

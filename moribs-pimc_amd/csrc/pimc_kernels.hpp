@@ -738,7 +738,9 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
             for (int mv = g; mv < nmoves; mv += groups) {
                 const int kl = 2 * mv + parity + 1;
                 const int k = k0 + kl - 1;
-                uint4 w0 = draw(run.seed, rp.key, pass, K_ROTLIN, jsweep, (uint32_t)ar, (uint32_t)k, 0);
+                // both draws of the move up front (independent, so they interleave)
+                const uint4 w0 = draw(run.seed, rp.key, pass, K_ROTLIN, jsweep, (uint32_t)ar, (uint32_t)k, 0);
+                const uint4 w1 = draw(run.seed, rp.key, pass, K_ROTLIN, jsweep, (uint32_t)ar, (uint32_t)k, 1);
                 double cost = cth[kl] + rp.rtstep * (u53(w0.x, w0.y) - 0.5);
                 double ph = phi[kl] + rp.rtstep * (u53(w0.z, w0.w) - 0.5);
                 if (cost > 1.0) cost = 2.0 - cost;
@@ -777,11 +779,7 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
                     } else {
                         double rd = (dold > RZERO) ? dnew / dold : 1.0;
                         rd *= exp(-rp.tau * (enew - E[kl]));
-                        bool accepted = rd > 1.0;
-                        if (!accepted) {
-                            uint4 w1 = draw(run.seed, rp.key, pass, K_ROTLIN, jsweep, (uint32_t)ar, (uint32_t)k, 1);
-                            accepted = rd > u53(w1.x, w1.y);
-                        }
+                        const bool accepted = rd > 1.0 || rd > u53(w1.x, w1.y);
                         if (accepted) {
                             ++nacc;
                             cth[kl] = cost;
@@ -994,8 +992,8 @@ __global__ __launch_bounds__(BLOCK) void k_rotor3d(const Rep* __restrict__ reps,
 // slices with the moves of k_rotor3d.  Before its even half a part needs the frames of slice
 // k0-1 (the previous part's last, odd) and before its odd half those of slice k1 (the next
 // part's first, even).  After its even half a part publishes its first slice's frames, after its
-// odd half its last slice's (device-coherent stores from wave 0, then the part's flag), and it
-// recomputes its two boundary links from the frames it receives.  A published slot is
+// odd half its last slice's, each with the density of its link to the neighbour (device-coherent
+// stores from wave 0, then the part's flag); the receiver takes both.  A published slot is
 // rewritten only after the neighbour has raised past the half that reads it.  Bounded waits set
 // error bit 2.
 // LDS: Lk[q][kl], kl = 0..ns (link kl joins local slices kl and kl+1; local slice 0 is k0-1 and
@@ -1081,17 +1079,13 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
                 // part's even half of this sweep
                 if (threadIdx.x == 0 && !wait_count(parity == 0 ? prevflag : nextflag, base + h)) s_err = 2;
                 __syncthreads();
+                // the neighbour's boundary frames, and the densities of the shared links that it
+                // evaluated from the same frames in the same order
                 const double* src = parity == 0 ? prevlast : nextfirst;
                 double* dst = reinterpret_cast<double*>(fnb + (parity == 0 ? 0 : nrot));
                 for (int i = threadIdx.x; i < nrot * FD; i += blockDim.x) dst[i] = load_agent(src + i);
-                __syncthreads();
-                for (int q = threadIdx.x; q < nrot; q += blockDim.x) {
-                    if (parity == 0) {
-                        if (rotden_rho(fnb[q], own(q, k0), tab, Lk[q * L])) err = 1;
-                    } else if (rotden_rho(own(q, k0 + ns - 1), fnb[nrot + q], tab, Lk[q * L + ns])) {
-                        err = 1;
-                    }
-                }
+                const double* lsrc = parity == 0 ? prevlast + nrot * FD + nrot : nextfirst + 2 * nrot * FD;
+                for (int q = threadIdx.x; q < nrot; q += blockDim.x) Lk[q * L + (parity == 0 ? 0 : ns)] = load_agent(lsrc + q);
                 __syncthreads();
             }
             const int nmoves = (ns - parity + 1) / 2;  // k0 is even: local slice kl has parity (kl - 1) & 1
@@ -1176,6 +1170,9 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
                 double* dst = mypub + (parity == 0 ? 0 : nrot * FD);
                 for (int i = threadIdx.x; i < nrot * FD; i += 64)
                     store_agent(dst + i, reinterpret_cast<const double*>(&own(i / FD, ks))[i % FD]);
+                // the boundary link densities: Lk[q][0] after the even half, Lk[q][ns] after the odd
+                for (int q = threadIdx.x; q < nrot; q += 64)
+                    store_agent(mypub + 2 * nrot * FD + (parity == 0 ? 0 : nrot) + q, Lk[q * L + (parity == 0 ? 0 : ns)]);
                 if (threadIdx.x == 0) raise_count(myflag, base + h + 1u);
             }
         }
