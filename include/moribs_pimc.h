@@ -97,7 +97,9 @@ int  pimc_commit(pimc_ctx* ctx);   /* allocates device state after all replicas 
      "rotor_part_slices"  most rotor slices per workgroup of the rotor sweep (even, >= 2);
                           0 (default) sizes parts from the LDS budget.
      "rotor_lds_kb"       LDS budget of one part of the linear-rotor sweep, 8 to 150 KB
-                          (default 52: three parts per CU).  Results do not
+                          (default 52: three parts per CU).
+     "centroid_parts"     about how many workgroups the centroid move's replicas are split into
+                          in all (0, the default: min(1024, half of what the device holds)).  Results do not
                           depend on it: every draw is addressed by (sweep, slice).
      "bisect_lanes_small", "bisect_lanes_medium", "bisect_lanes_large"
                           lanes (64, 128 or 256) per bisection segment for replicas of at most
@@ -128,11 +130,12 @@ int  pimc_get_pass(pimc_ctx* ctx, int replica, unsigned* pass);
 int  pimc_run(pimc_ctx* ctx, int n_passes, int sample_every);
 /* Autotuner of launch shapes (no counterpart in the reference, whose OpenMP loops have no
    tunable shape).  After pimc_commit and the initial state: times each bisection size class
-   at 64, 128 and 256 lanes per segment and the linear-rotor sweep at part LDS budgets of 40,
-   52, 64 and 78 KB, each candidate over `passes` passes' launches from the same saved state,
-   and keeps the fastest.  The chains' coordinates, orientations, counters and pass numbers are
-   restored, so results are the same as without tuning.  chosen (may be NULL) receives the
-   lanes of the small, medium and large classes and the rotor budget in KB. */
+   at 64, 128 and 256 lanes per segment, the linear-rotor sweep at part LDS budgets of 40,
+   52, 64 and 78 KB and the centroid move split into about 256, 512 or 1024 parts, each
+   candidate over `passes` passes' launches from the same saved state, and keeps the fastest.  The chains' coordinates, orientations, counters and pass numbers are
+   restored, so results are the same as without tuning.  chosen (may be NULL, else 5 entries)
+   receives the lanes of the small, medium and large classes, the rotor budget in KB and the
+   centroid part target. */
 int  pimc_autotune(pimc_ctx* ctx, int passes, long* chosen);
 /* samples of one replica, 8 doubles each: K, V, E_rot, E_rot^2 term (ErotSQ), Cv, Cv_trans,
    Cv_rot, Erot_termSQ (mc_main.cc:551-616).  Returns the number copied. */

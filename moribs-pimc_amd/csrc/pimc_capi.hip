@@ -185,6 +185,8 @@ struct pimc_ctx {
     int nparts = 0, nflags = 0;
     int opt_rotor_part_slices = 0;  // most rotor slices per part, 0: from the LDS budget
     int opt_rotor_lds_kb = 52;      // LDS budget of a rotor part (52 KB: three parts per CU)
+    int opt_centroid_parts = 0;     // target number of centroid parts, 0: from the capacity
+    int centroid_target = 0;        // target the current centroid parts were planned with
     int rotor_lds_kb = 52;          // budget the current parts were planned with
     std::vector<int> rot_chains;    // replicas with a linear rotor, heaviest first
     RotPart* d_parts3 = nullptr;     // non-linear chains split by slice range
@@ -492,6 +494,11 @@ int pimc_set_option(pimc_ctx* c, const char* name, long value) {
             c->bw_lanes[k] = (int)value;
             return 0;
         }
+    if (std::strcmp(name, "centroid_parts") == 0) {
+        if (value < 0 || value > 4096) return fail(c, "centroid_parts is 0 to 4096");
+        c->opt_centroid_parts = (int)value;
+        return 0;
+    }
     if (std::strcmp(name, "rotor_lds_kb") == 0) {
         if (value < 8 || value > 150) return fail(c, "rotor_lds_kb is 8 to 150");
         c->opt_rotor_lds_kb = (int)value;
@@ -561,6 +568,74 @@ static int plan_rotor_parts(pimc_ctx* c, int lds_kb) {
     if (!c->d_parts || !c->d_flags || !c->d_links) return fail(c, "hipMalloc failed");
     HIPCHECK(c, hipMemcpy(c->d_parts, parts.data(), sizeof(RotPart) * parts.size(), hipMemcpyHostToDevice));
     HIPCHECK(c, hipMemset(c->d_flags, 0, sizeof(unsigned) * nflags));
+    return 0;
+}
+
+// Centroid parts: each replica's slices split so that parts carry similar shares of the work
+// sum N^2 P, about `target` parts in all (0: min(1024, half the resident capacity)), at most 64
+// parts and at least 8 slices a part.  Every part of a replica must be resident while its parts
+// wait for one another, so the target stays within what the device holds at once.  Called at
+// commit and again by the autotuner.
+static int plan_centroid_parts(pimc_ctx* c, int target_parts) {
+    int occ0 = 0, occ1 = 0;
+    hipDeviceProp_t prop;
+    HIPCHECK(c, hipGetDeviceProperties(&prop, 0));
+    const int ncu = prop.multiProcessorCount;
+    if (c->fp64) {
+        HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, k_centroid_parts<double, 0>, 256, c->lds_knots));
+        HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_centroid_parts<double, 1>, 256, c->lds_knots));
+    } else {
+        HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, k_centroid_parts<float, 0>, 256, c->lds_knots));
+        HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_centroid_parts<float, 1>, 256, c->lds_knots));
+    }
+    const double capacity = (double)ncu * std::max(1, std::min(occ0, occ1));
+    const double target = target_parts > 0 ? std::min((double)target_parts, capacity / 2.0) : std::min(1024.0, capacity / 2.0);
+    std::vector<RotPart> cp[2];
+    double total = 0.0;
+    for (int r = 0; r < c->nrep; ++r) total += (double)c->rep[r].natoms * c->rep[r].natoms * c->rep[r].P;
+    std::vector<int> order(c->nrep);
+    for (int r = 0; r < c->nrep; ++r) order[r] = r;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const Rep &a = c->rep[x], &b = c->rep[y];
+        return (double)a.natoms * a.natoms * a.P > (double)b.natoms * b.natoms * b.P;
+    });
+    int fb = 0;
+    for (int r : order) {
+        const Rep& p = c->rep[r];
+        const double w = (double)p.natoms * p.natoms * p.P;
+        int W = (int)std::ceil(target * w / std::max(total, 1.0));
+        W = std::max(1, std::min({W, 64, std::max(1, p.P / 8)}));
+        for (int q = 0; q < W; ++q) {
+            RotPart x{};
+            x.r = r;
+            x.k0 = (int)((long long)p.P * q / W);
+            x.k1 = (int)((long long)p.P * (q + 1) / W);
+            x.w = q;
+            x.W = W;
+            x.fbase = fb;
+            cp[p.fam].push_back(x);
+        }
+        fb += W;
+    }
+    auto cost = [&](int r) { return (double)c->rep[r].natoms * c->rep[r].natoms * c->rep[r].P; };
+    for (int f = 0; f < 2; ++f)
+        if (!cp[f].empty()) cp[f] = xcd_order(cp[f], cost);
+    for (void* q : {(void*)c->d_cparts0, (void*)c->d_cparts1, (void*)c->d_cflags, (void*)c->d_cslots})
+        if (q) (void)hipFree(q);
+    c->ncflags = fb;
+    c->centroid_target = (int)target;
+    c->ncp0 = (int)cp[0].size();
+    c->ncp1 = (int)cp[1].size();
+    c->d_cparts0 = dalloc<RotPart>(cp[0].size());
+    c->d_cparts1 = dalloc<RotPart>(cp[1].size());
+    c->d_cflags = dalloc<unsigned>((size_t)std::max(fb, 1));
+    c->d_cslots = dalloc<double>(2 * (size_t)std::max(fb, 1));
+    if (!c->d_cparts0 || !c->d_cparts1 || !c->d_cflags || !c->d_cslots) return fail(c, "hipMalloc failed");
+    HIPCHECK(c, hipMemset(c->d_cflags, 0, sizeof(unsigned) * std::max(fb, 1)));
+    if (!cp[0].empty())
+        HIPCHECK(c, hipMemcpy(c->d_cparts0, cp[0].data(), sizeof(RotPart) * cp[0].size(), hipMemcpyHostToDevice));
+    if (!cp[1].empty())
+        HIPCHECK(c, hipMemcpy(c->d_cparts1, cp[1].data(), sizeof(RotPart) * cp[1].size(), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -731,68 +806,7 @@ int pimc_commit(pimc_ctx* c) {
     });
     c->rot_chains = rotreps;
     if (plan_rotor_parts(c, c->opt_rotor_lds_kb)) return -1;
-    // centroid parts: each replica's slices split so that parts carry similar shares of the
-    // work sum N^2 P (about 1024 parts in all; at most 64 parts and at least 8 slices a part)
-    {
-        // every part of a replica must be resident while its parts wait for one another: keep
-        // the parts within what the device holds at once
-        int ncu = 0, occ0 = 0, occ1 = 0;
-        hipDeviceProp_t prop;
-        HIPCHECK(c, hipGetDeviceProperties(&prop, 0));
-        ncu = prop.multiProcessorCount;
-        if (c->fp64) {
-            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, k_centroid_parts<double, 0>, 256, c->lds_knots));
-            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_centroid_parts<double, 1>, 256, c->lds_knots));
-        } else {
-            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, k_centroid_parts<float, 0>, 256, c->lds_knots));
-            HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_centroid_parts<float, 1>, 256, c->lds_knots));
-        }
-        const double capacity = (double)ncu * std::max(1, std::min(occ0, occ1));
-        double target = std::min(1024.0, capacity / 2.0);
-        std::vector<RotPart> cp[2];
-        double total = 0.0;
-        for (int r = 0; r < c->nrep; ++r) total += (double)c->rep[r].natoms * c->rep[r].natoms * c->rep[r].P;
-        std::vector<int> order(c->nrep);
-        for (int r = 0; r < c->nrep; ++r) order[r] = r;
-        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-            const Rep &a = c->rep[x], &b = c->rep[y];
-            return (double)a.natoms * a.natoms * a.P > (double)b.natoms * b.natoms * b.P;
-        });
-        int fb = 0;
-        for (int r : order) {
-            const Rep& p = c->rep[r];
-            const double w = (double)p.natoms * p.natoms * p.P;
-            int W = (int)std::ceil(target * w / std::max(total, 1.0));
-            W = std::max(1, std::min({W, 64, std::max(1, p.P / 8)}));
-            for (int q = 0; q < W; ++q) {
-                RotPart x{};
-                x.r = r;
-                x.k0 = (int)((long long)p.P * q / W);
-                x.k1 = (int)((long long)p.P * (q + 1) / W);
-                x.w = q;
-                x.W = W;
-                x.fbase = fb;
-                cp[p.fam].push_back(x);
-            }
-            fb += W;
-        }
-        c->ncflags = fb;
-        auto cost = [&](int r) { return (double)c->rep[r].natoms * c->rep[r].natoms * c->rep[r].P; };
-        for (int f = 0; f < 2; ++f)
-            if (!cp[f].empty()) cp[f] = xcd_order(cp[f], cost);
-        c->ncp0 = (int)cp[0].size();
-        c->ncp1 = (int)cp[1].size();
-        c->d_cparts0 = dalloc<RotPart>(cp[0].size());
-        c->d_cparts1 = dalloc<RotPart>(cp[1].size());
-        c->d_cflags = dalloc<unsigned>((size_t)std::max(fb, 1));
-        c->d_cslots = dalloc<double>(2 * (size_t)std::max(fb, 1));
-        if (!c->d_cparts0 || !c->d_cparts1 || !c->d_cflags || !c->d_cslots) return fail(c, "hipMalloc failed");
-        HIPCHECK(c, hipMemset(c->d_cflags, 0, sizeof(unsigned) * std::max(fb, 1)));
-        if (!cp[0].empty())
-            HIPCHECK(c, hipMemcpy(c->d_cparts0, cp[0].data(), sizeof(RotPart) * cp[0].size(), hipMemcpyHostToDevice));
-        if (!cp[1].empty())
-            HIPCHECK(c, hipMemcpy(c->d_cparts1, cp[1].data(), sizeof(RotPart) * cp[1].size(), hipMemcpyHostToDevice));
-    }
+    if (plan_centroid_parts(c, c->opt_centroid_parts)) return -1;
     c->nbeads = boff;
     c->nrotb = roff;
     c->nhist = hoff;
@@ -1311,6 +1325,21 @@ void launch_bisect_class(pimc_ctx* c, int k, int u) {
 }
 
 template <typename R>
+void launch_centroid_all(pimc_ctx* c) {
+    for (int ty = 0; ty < c->max_types; ++ty) {
+        Run run{c->seed, 0, c->ref_order ? 1 : 0, ty, ty + 1};
+        if (c->ncp0)
+            hipLaunchKernelGGL((k_centroid_parts<R, 0>), dim3(c->ncp0), dim3(256), c->lds_knots, c->stream, c->d_rep,
+                               c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_cparts0,
+                               c->d_cflags, c->d_cslots, c->ncflags, run);
+        if (c->ncp1)
+            hipLaunchKernelGGL((k_centroid_parts<R, 1>), dim3(c->ncp1), dim3(256), c->lds_knots, c->stream, c->d_rep,
+                               c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_cparts1,
+                               c->d_cflags, c->d_cslots, c->ncflags, run);
+    }
+}
+
+template <typename R>
 void launch_rotor_linear(pimc_ctx* c, int u) {
     Run run{c->seed, u, c->ref_order ? 1 : 0, 0, 0};
     hipLaunchKernelGGL(k_rotor<R>, dim3(c->nparts), dim3(256), c->lds_rotor, c->stream, c->d_rep, c->d_mut,
@@ -1473,6 +1502,27 @@ int pimc_autotune(pimc_ctx* c, int passes, long* chosen) {
             }
             if (rc == 0 && c->rotor_lds_kb != best && plan_rotor_parts(c, best)) rc = -1;
         }
+        if (rc == 0 && c->ncp0 + c->ncp1 > 0) {  // centroid parts in all: fewer parts keep more in L2
+            static const int targets[3] = {256, 512, 1024};
+            int best = c->centroid_target;
+            double tbest = -1.0;
+            for (int tg : targets) {
+                if (plan_centroid_parts(c, tg)) {
+                    rc = -1;
+                    break;
+                }
+                const double t = timed([&](int u) {
+                    if (u != 0) return;
+                    if (c->fp64) launch_centroid_all<double>(c);
+                    else launch_centroid_all<float>(c);
+                });
+                if (t >= 0.0 && (tbest < 0.0 || t < tbest)) {
+                    tbest = t;
+                    best = c->centroid_target;
+                }
+            }
+            if (rc == 0 && c->centroid_target != best && plan_centroid_parts(c, best)) rc = -1;
+        }
         if (copy_all(false) || hipStreamSynchronize(c->stream) != hipSuccess)
             rc = rc ? rc : fail(c, "autotune: restoring the state failed");
     }
@@ -1486,6 +1536,7 @@ int pimc_autotune(pimc_ctx* c, int passes, long* chosen) {
     if (rc == 0 && chosen) {
         for (int k = 0; k < 3; ++k) chosen[k] = c->bw_lanes[k];
         chosen[3] = c->rotor_lds_kb;
+        chosen[4] = c->centroid_target;
     }
     return rc;
 }

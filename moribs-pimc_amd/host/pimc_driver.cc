@@ -713,10 +713,10 @@ int main(int argc, char** argv) {
                                        resume.cosines[r].data(), resume.pass[r]),
                     "pimc_restore_state");
     if (autotune) {
-        long shapes[4];
+        long shapes[5];
         c.check(pimc_autotune(c.ctx, 2, shapes), "pimc_autotune");
         std::cerr << "launch shapes: bisection lanes " << shapes[0] << "/" << shapes[1] << "/" << shapes[2]
-                  << ", rotor part LDS " << shapes[3] << " KB\n";
+                  << ", rotor part LDS " << shapes[3] << " KB, centroid parts " << shapes[4] << "\n";
     }
 
     std::vector<Chain> chains(nrep);

@@ -232,10 +232,10 @@ class Context:
 
     def autotune(self, passes=2):
         """Launch shapes picked by timing candidates on this workload (state restored after)."""
-        out = (ctypes.c_long * 4)()
+        out = (ctypes.c_long * 5)()
         self._check(self.lib.pimc_autotune(self.h, int(passes), out))
         return dict(bisect_lanes_small=out[0], bisect_lanes_medium=out[1], bisect_lanes_large=out[2],
-                    rotor_lds_kb=out[3])
+                    rotor_lds_kb=out[3], centroid_parts=out[4])
 
     def samples(self, r, max_samples=1 << 16):
         out = np.empty(8 * max_samples)

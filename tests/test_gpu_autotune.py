@@ -24,6 +24,7 @@ def test_autotune_keeps_the_chains(flags):
     for k in ("bisect_lanes_small", "bisect_lanes_medium", "bisect_lanes_large"):
         assert chosen[k] in (64, 128, 256)
     assert chosen["rotor_lds_kb"] in (40, 52, 64, 78)
+    assert 0 < chosen["centroid_parts"] <= 1024
     for r in range(len(wl["replicas"])):
         assert tuned.pass_counter(r) == plain.pass_counter(r) == 0
         assert np.array_equal(tuned.counters(r)[0], plain.counters(r)[0])
