@@ -146,6 +146,16 @@ int  pimc_get_counters(pimc_ctx* ctx, int replica, double* total, double* accep)
 /* accumulated g(r) (300 bins) and g(r,theta) (300x50) counts of one replica
    (bin_1Ddensity/bin_2Ddensity mc_estim.cc:1233-1286) */
 int  pimc_get_histograms(pimc_ctx* ctx, int replica, double* gr1d, double* gr2d);
+/* accumulated bins around a non-linear rotor of one replica since pimc_clear_samples:
+   gr3d[(ir*50 + it)*100 + ic] counts the atoms at (r, theta, chi) of the rotor's body frame
+   (300 radial bins to 15 A, theta by pi/49, chi by 2 pi/99; bin_3Ddensity mc_estim.cc:1252-1273
+   on vcord's radret/theret/chiret, GetPotEnergy_Densities mc_estim.cc:582-641); rel[0..49],
+   rel[50..149], rel[150..249] count the relative Euler angles theta, phi, chi of neighbouring
+   rotor slices (GetRotE3D mc_estim.cc:1046-1058).  Either pointer may be NULL.  Returns a
+   bit mask of what the replica has: 1 the 3D bins (non-linear rotor and atoms), 2 the
+   relative angle bins (rotating non-linear rotor with a density table); arrays it lacks are
+   left untouched. */
+int  pimc_get_histograms3d(pimc_ctx* ctx, int replica, double* gr3d, double* rel);
 /* rotational correlation sums of one replica since pimc_clear_samples, out[10][Prot]:
    row 0 sum of n(t0).n(t0+t), rows l = 1..9 sum of P_l of it, over time origins and samples
    (GetRCF mc_estim.cc:1098-1139; the reference adds 1.0 in rows 1..9).  Returns Prot, or 0

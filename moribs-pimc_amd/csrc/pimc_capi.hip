@@ -749,7 +749,22 @@ int pimc_commit(pimc_ctx* c) {
         p.rot_off = roff;
         roff += ((p.rotkind || p.fam == 1) ? (long long)p.tnumb[p.imtype] * p.Prot : 0);
         p.hist_off = hoff;
-        hoff += BINSR + BINSR * BINST;
+        {  // atom-atom g(r), the atoms around a linear rotor, then the bins around a non-linear rotor
+            long long h = BINSR + BINSR * BINST;
+            bool atoms = false;
+            for (int t = 0; t < p.ntypes; ++t) atoms |= p.tmol[t] == 0 && p.tnumb[t] > 0;
+            p.h3_off = -1;
+            p.hrel_off = -1;
+            if (p.fam == 1 && p.nl_atom >= 0 && atoms) {
+                p.h3_off = (int)h;
+                h += (long long)BINSR * BINST * BINSC;
+            }
+            if (p.rotkind == 2 && p.rot3d_id >= 0) {
+                p.hrel_off = (int)h;
+                h += BINST + 2 * BINSC;
+            }
+            hoff += h;
+        }
         p.rcf_off = -1;
         if (p.rotkind) {  // GetRCF runs with ROTATION (mc_main.cc:571-587)
             p.rcf_off = rcfoff;
@@ -1312,7 +1327,7 @@ void launch_estimate(pimc_ctx* c, double* out) {
                            c->d_rotb, c->d_rcfreps, c->d_rcf);
     hipLaunchKernelGGL(k_est_final, dim3(c->nrep), dim3(256), 0, c->stream, c->d_rep, c->d_mut, c->d_rotb,
                        c->d_frames, c->tables(), c->tables3(),
-                       c->d_all, c->d_rowbuf, c->d_samples, c->sample_cap, out);
+                       c->d_all, c->d_rowbuf, c->d_samples, c->sample_cap, out, out ? nullptr : c->d_hist);
 }
 
 // Autotuner launches: one bisection size class alone, and the linear-rotor sweep alone, over
@@ -1591,6 +1606,31 @@ int pimc_get_histograms(pimc_ctx* c, int r, double* gr1d, double* gr2d) {
     if (gr2d)
         for (int i = 0; i < BINSR * BINST; ++i) gr2d[i] = h[BINSR + i];
     return 0;
+}
+
+int pimc_get_histograms3d(pimc_ctx* c, int r, double* gr3d, double* rel) {
+    if (!c->committed || r < 0 || r >= c->nrep) return fail(c, "bad replica");
+    const Rep& p = c->rep[r];
+    int has = 0;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (p.h3_off >= 0) {
+        has |= 1;
+        if (gr3d) {
+            std::vector<unsigned> h((size_t)BINSR * BINST * BINSC);
+            HIPCHECK(c, hipMemcpy(h.data(), c->d_hist + p.hist_off + p.h3_off, sizeof(unsigned) * h.size(),
+                                  hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < h.size(); ++i) gr3d[i] = h[i];
+        }
+    }
+    if (p.hrel_off >= 0) {
+        has |= 2;
+        if (rel) {
+            unsigned h[BINST + 2 * BINSC];
+            HIPCHECK(c, hipMemcpy(h, c->d_hist + p.hist_off + p.hrel_off, sizeof(h), hipMemcpyDeviceToHost));
+            for (int i = 0; i < BINST + 2 * BINSC; ++i) rel[i] = h[i];
+        }
+    }
+    return has;
 }
 
 int pimc_get_rcf(pimc_ctx* c, int r, double* out, int max_values) {

@@ -20,6 +20,7 @@ constexpr int MAXT = 4;
 constexpr double RZERO = 1.0e-10;  // zero of the rotational density (mc_confg.h:60)
 constexpr int BINSR = 300;         // radial bins (mc_estim.cc:25)
 constexpr int BINST = 50;          // polar bins (mc_estim.cc:26)
+constexpr int BINSC = 100;         // chi bins around a non-linear rotor (mc_estim.cc:27)
 constexpr double MAX_RADIUS = 15.0;
 
 enum : uint32_t { K_CENTROID = 1u, K_BISECT = 2u, K_ROTLIN = 3u, K_ROT3D = 4u };
@@ -39,6 +40,8 @@ struct Rep {
     int rot3d_id;  // rho/E/E^2 tables of the non-linear rotor, -1 if none
     int key;       // replica index, second word of the generator key
     int est_ts;    // slices per estimator tile (a power of two)
+    int h3_off;    // (r, theta, chi) bins around a non-linear rotor, from hist_off; -1 if none
+    int hrel_off;  // relative Euler angle bins (theta, phi, chi) of rotor links, from hist_off; -1 if none
     double lambda[MAXT], mcstep[MAXT];
     double T, beta, tau, rtstep;
     long long bead_off, rot_off, hist_off, sample_off;

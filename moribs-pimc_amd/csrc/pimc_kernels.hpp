@@ -1490,6 +1490,7 @@ __global__ __launch_bounds__(256) void k_est_tiles(const Rep* __restrict__ reps,
     }
     unsigned* g1 = hist ? hist + rp.hist_off : nullptr;
     unsigned* g2h = g1 ? g1 + BINSR : nullptr;
+    unsigned* g3 = (g1 && rp.h3_off >= 0) ? g1 + rp.h3_off : nullptr;
     for (int i = threadIdx.x; i < BINSR; i += blockDim.x) g1s[i] = 0u;
     __syncthreads();
     // kinetic links (a, s) - (a, s + 1)
@@ -1504,6 +1505,7 @@ __global__ __launch_bounds__(256) void k_est_tiles(const Rep* __restrict__ reps,
     const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
     const double drad = MAX_RADIUS / (double)BINSR;
     const double dth = 3.14159265358979323846 / (double)(BINST - 1);
+    const double dch = 2.0 * 3.14159265358979323846 / (double)(BINSC - 1);
     const int npairs = N * (N - 1) / 2;
     const float invN = 1.0f / (float)N;
     double pot = 0.0;
@@ -1523,12 +1525,25 @@ __global__ __launch_bounds__(256) void k_est_tiles(const Rep* __restrict__ reps,
         const R dx = (R)ex, dy = (R)ey, dz = (R)ez;
         const R rr = sqrt(dx * dx + dy * dy + dz * dz);
         if constexpr (FAM == 1) {
-            if (m0 == 2 || m1 == 2) {  // mc_estim.cc:582-672: vcord or caleng, no density bins
+            if (m0 == 2 || m1 == 2) {  // mc_estim.cc:582-672: vcord or caleng
                 const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
                 const int it = s0 + w;
                 const Frame& f0 = frame_at(frames, rp, m0 == 2 ? a0 : rp.nl_atom, it);
                 const Frame& f1 = frame_at(frames, rp, m1 == 2 ? a1 : rp.nl_atom, it);
                 pot += pair_nl<R>(l1, l1.k, l3, m0 == 2, m1 == 2, lx[j0], ly[j0], lz[j0], lx[j1], ly[j1], lz[j1], f0, f1);
+                if (g3 && m0 != m1) {  // bin_3Ddensity of vcord's FP64 geometry (mc_estim.cc:637,1252-1273)
+                    double rad, the, chi;
+                    if (m0 == 2) vcord_geometry(f0, lx[j0], ly[j0], lz[j0], lx[j1], ly[j1], lz[j1], rad, the, chi);
+                    else vcord_geometry(f1, lx[j1], ly[j1], lz[j1], lx[j0], ly[j0], lz[j0], rad, the, chi);
+                    const int br = (int)floor(rad / drad);
+                    if (br < BINSR && br >= 0) {
+                        const int bt = (int)floor(the / dth);
+                        if (bt < BINST && bt >= 0) {
+                            const int bc = (int)floor(chi / dch);
+                            if (bc < BINSC && bc >= 0) atomicAdd(&g3[(br * BINST + bt) * BINSC + bc], 1u);
+                        }
+                    }
+                }
                 continue;
             }
         }
@@ -1568,7 +1583,8 @@ __global__ __launch_bounds__(256) void k_est_final(const Rep* __restrict__ reps,
                                                    const RotBead* __restrict__ rotb, const Frame* __restrict__ frames,
                                                    Tables T, Tables3 T3,
                                                    const int* __restrict__ replist, const double2* __restrict__ rows,
-                                                   double* __restrict__ samples, int sample_cap, double* __restrict__ out) {
+                                                   double* __restrict__ samples, int sample_cap, double* __restrict__ out,
+                                                   unsigned* __restrict__ hist) {
     __shared__ double red[16];
     const int r = replist[blockIdx.x];
     const Rep& rp = reps[r];
@@ -1593,12 +1609,23 @@ __global__ __launch_bounds__(256) void k_est_final(const Rep* __restrict__ reps,
     }
     if (rp.rotkind == 2 && rp.rot3d_id >= 0) {  // GetRotE3D mc_estim.cc:989-1096, per rotor
         const double4* tab = T3.r3 + (long long)rp.rot3d_id * 3 * ROT3D_SIZE;
+        // relative Euler angle bins (mc_estim.cc:1046-1058): theta by pi/49, phi and chi by 2 pi/99
+        unsigned* hr = (hist && rp.hrel_off >= 0) ? hist + rp.hist_off + rp.hrel_off : nullptr;
+        const double dth = PI_D / (double)(BINST - 1), dch = 2.0 * PI_D / (double)(BINSC - 1);
         for (int q = 0; q < rp.nrot; ++q) {
             const long long fb = rp.rot_off + (long long)q * rp.Prot;
             double a1 = 0, a2 = 0, a3 = 0;
             for (int k = threadIdx.x; k < rp.Prot; k += blockDim.x) {
-                double rho, er, es;
-                rotden_frames(frames[fb + k], frames[fb + (k + 1) % rp.Prot], tab, rho, er, es);
+                double rho, er, es, rel[3];
+                rotden_frames(frames[fb + k], frames[fb + (k + 1) % rp.Prot], tab, rho, er, es, hr ? rel : nullptr);
+                if (hr) {
+                    const int bt = (int)floor(rel[1] / dth);
+                    if (bt < BINST && bt >= 0) atomicAdd(&hr[bt], 1u);
+                    const int bp = (int)floor(rel[0] / dch);
+                    if (bp < BINSC && bp >= 0) atomicAdd(&hr[BINST + bp], 1u);
+                    const int bc = (int)floor(rel[2] / dch);
+                    if (bc < BINSC && bc >= 0) atomicAdd(&hr[BINST + BINSC + bc], 1u);
+                }
                 a1 += er;
                 a2 += es;
                 a3 += er * er;

@@ -93,11 +93,17 @@ __device__ inline int rotden_rho(const Frame& f1, const Frame& f2, const double4
     return stop;
 }
 
+// rel (if not null) receives the relative Euler angles (phi, theta, chi) in radians
 __device__ inline int rotden_frames(const Frame& f1, const Frame& f2, const double4* __restrict__ tab, double& rho,
-                                    double& erot, double& esq) {
+                                    double& erot, double& esq, double* rel = nullptr) {
     const double wno2k = 0.6950356;
     double p2, t2, c2;
     deleul_frames(f1, f2, p2, t2, c2);
+    if (rel) {
+        rel[0] = p2;
+        rel[1] = t2;
+        rel[2] = c2;
+    }
     const double phi = p2 * 180.0 / PI_D, theta = t2 * 180.0 / PI_D, chi = c2 * 180.0 / PI_D;
     int stop = 0;
     int ichi = (int)chi, iphi = (int)phi, ith = (int)theta;
@@ -211,6 +217,35 @@ __device__ __forceinline__ R vcord_v(const Lut3D<R>& L, const Frame& f, double c
     }
     const R rad = rn / (R)0.529177249;
     return vcalc<R>(L, rad, the * (R)180 / pi, chiw * (R)180 / pi);
+}
+
+// vcord's geometry for the density bins (vcord.f:40-75), FP64: radret = |rpt - rcom| in
+// Angstrom, theret = angle to the body z axis, chiret = azimuth about it in [0, 2 pi)
+__device__ inline void vcord_geometry(const Frame& f, double cx, double cy, double cz, double px, double py, double pz,
+                                      double& rad, double& the, double& chi) {
+    const double rx = px - cx, ry = py - cy, rz = pz - cz;
+    double dz = 0.0;
+    dz = dz + f.m[6] * rx;
+    dz = dz + f.m[7] * ry;
+    dz = dz + f.m[8] * rz;
+    double rr = 0.0;
+    rr = rr + rx * rx;
+    rr = rr + ry * ry;
+    rr = rr + rz * rz;
+    rad = sqrt(rr);
+    the = acos(within1(dz / (f.zn * rad)));
+    double rdx = 0.0, rdy = 0.0;
+    rdx = rdx + rx * f.m[0];
+    rdx = rdx + ry * f.m[1];
+    rdx = rdx + rz * f.m[2];
+    rdy = rdy + rx * f.m[3];
+    rdy = rdy + ry * f.m[4];
+    rdy = rdy + rz * f.m[5];
+    const double chiw = fabs(rdx) < 1.0e-08 ? PI_D / 2.0 : atan(fabs(rdy / rdx));
+    if (rdx >= 0.0 && rdy >= 0.0) chi = chiw;
+    else if (rdx < 0.0 && rdy >= 0.0) chi = PI_D - chiw;
+    else if (rdx < 0.0 && rdy < 0.0) chi = PI_D + chiw;
+    else chi = 2 * PI_D - chiw;
 }
 
 // caleng (caleng_tip4p_gg.f:2-186): rigid TIP4P dimer in Kelvin, always FP64 (the Coulomb

@@ -11,7 +11,11 @@
 //   <OUTPUTDIR><FILENAMEPREFIX>.xyz       the configuration at the end of every block (IOxyz mc_input.cc:690-746)
 //   <...><block>.gra/.gri/.grt/.g2d/.rcf  block distributions (MCSaveBlockAverages mc_main.cc:703-735,
 //                                         SaveDensities1D/2D mc_estim.cc:1327-1444,1666-1727, SaveRCF :1141-1184)
+//   <...><block>.grc/.gtc                 around a non-linear rotor, .gri/.grt/.grc from the 3D bins
+//                                         (SaveRho1D mc_estim.cc:1445-1590, SaveRhoThetaChi :1930-1995)
 //   <...>_sum.gra/_sum.g2d/_sum.rcf       accumulated distributions after every block (mc_main.cc:446-462)
+//   <...>_sum.g3d/_sum.gri/.grt/.grc      around a non-linear rotor (SaveDensities3D mc_estim.cc:1728-1820,
+//   <...>_sum.eulphi/.eulchi/.eulthe      SaveRho1D with the relative Euler angles :1591-1662)
 //   stdout                                the accept-ratio line every MCSKIP_RATIO steps (MCSaveAcceptRatio :879-943)
 //   yw001.stat, yw001.conf, yw001.rand    the checkpoint after every block, the previous one kept as
 //                                         *.old (mc_main.cc:471-478, StatusIO/ConfigIO mc_input.cc:517-609)
@@ -20,9 +24,9 @@
 // draw), so the resumed run continues exactly as the uninterrupted one.  The reference's
 // yw001.conf holds one coordinate row and no angles; here it holds all three rows of
 // coordinates, cosines and angles per chain: an int64 byte count, then the three arrays.
-// The atom-atom g(r) and the atom density around a linear rotor are binned on the device
-// (300 radial bins to 15 A, 50 polar bins); the 3D densities around a non-linear rotor
-// (SaveDensities3D, SaveRhoThetaChi) are not.
+// The atom-atom g(r), the atom density around a linear rotor (300 radial bins to 15 A, 50
+// polar bins), the atom density around a non-linear rotor in its body frame (300 x 50 x 100
+// chi bins) and the relative Euler angles of neighbouring rotor slices are binned on the device.
 //
 // Time steps and passes.  The reference counts "steps" (one time slice of one pass) and
 // samples when the step count is a multiple of MCSKIP_AVERG (mc_main.cc:340-484).  A pass on
@@ -387,11 +391,16 @@ struct Chain {
     std::vector<double> tot0, acc0;  // counters at the start of the block
     // bin counts and rotational correlation sums: this block, and all measured blocks
     std::vector<double> gr1, gr2, rcf, gr1_sum, gr2_sum, rcf_sum;
+    // around a non-linear rotor: (r, theta, chi) bins of this block and of all measured blocks,
+    // and the relative Euler angle bins of all measured blocks (never reset per block, as
+    // _relthe_sum and its like are not)
+    std::vector<double> gr3, gr3_sum, rel_sum;
 };
 
-constexpr int NBR = 300, NBT = 50, NRCF = 10;  // MC_BINSR, MC_BINST (mc_estim.cc:25-26), NUMB_RCF
-constexpr double DR = 15.0 / NBR;             // MAX_RADIUS / MC_BINSR, MIN_RADIUS = 0
-const double DTH = M_PI / (NBT - 1);
+constexpr int NBR = 300, NBT = 50, NBC = 100, NRCF = 10;  // MC_BINSR/T/C (mc_estim.cc:25-27), NUMB_RCF
+constexpr double DR = 15.0 / NBR;                        // MAX_RADIUS / MC_BINSR, MIN_RADIUS = 0
+const double DTH = M_PI / (NBT - 1), DCH = 2.0 * M_PI / (NBC - 1);
+constexpr size_t NB3 = (size_t)NBR * NBT * NBC;
 
 std::ofstream open_out(const std::string& name, const char* who) {
     std::ofstream f(name);
@@ -474,6 +483,150 @@ void save_block_densities(const Chain& c, const std::string& fname, double acoun
     save_g2d(c.gr2, fname + ".g2d", acount, d, n_atoms);
 }
 
+
+// Distributions around a non-linear rotor.  The reference keeps NUMB_DENS3D = atom types +
+// molecule types columns, one per type in input order (densities_init mc_estim.cc:244-246);
+// only the atom type's column has counts.  dens3d_columns gives, per column, the bins (or null)
+// and the type's count.
+struct Col3 {
+    const std::vector<double>* bins;
+    double numb;
+};
+std::vector<Col3> dens3d_columns(const Deck& d, const std::vector<double>& g3) {
+    std::vector<Col3> cols;
+    for (auto& s : d.types) cols.push_back({s.molecule == 0 ? &g3 : nullptr, (double)s.numb});
+    return cols;
+}
+
+// SaveRho1D (mc_estim.cc:1445-1590): rho(r), rho(theta), rho(chi) of the 3D bins, and with
+// rel (the accumulated files) the relative Euler angle distributions (:1591-1662).
+void save_rho1d(const std::vector<double>& g3, const std::string& fname, double acount, const Deck& d,
+                const std::vector<double>* rel) {
+    const char* sp = "   ";
+    const auto cols = dens3d_columns(d, g3);
+    {
+        auto f = open_out(fname + ".gri", "SaveRho1D");
+        const double norm1 = DR * d.P * acount;
+        for (int ir = 0; ir < NBR; ++ir) {
+            f << std::setw(14) << (ir + 0.5) * DR << sp;
+            for (auto& c : cols) {
+                double dens = 0.0;
+                if (c.bins)
+                    for (size_t k = (size_t)ir * NBT * NBC; k < (size_t)(ir + 1) * NBT * NBC; ++k) dens += (*c.bins)[k];
+                f << std::setw(14) << dens / norm1 << sp;
+            }
+            f << "\n";
+        }
+    }
+    {
+        auto f = open_out(fname + ".grt", "SaveRho1D");
+        const double norm2 = d.P * acount * DTH * (180.0 / M_PI);
+        for (int it = 0; it < NBT; ++it) {
+            f << std::setw(14) << (it + 0.5) * DTH * 180.0 / M_PI << sp;
+            for (auto& c : cols) {
+                double dens = 0.0;
+                if (c.bins)
+                    for (int ir = 0; ir < NBR; ++ir)
+                        for (int ic = 0; ic < NBC; ++ic) dens += (*c.bins)[((size_t)ir * NBT + it) * NBC + ic];
+                f << std::setw(14) << dens / (norm2 * c.numb) << sp;
+            }
+            f << "\n";
+        }
+    }
+    {
+        auto f = open_out(fname + ".grc", "SaveRho1D");
+        const double norm4 = d.P * acount * DCH * (180.0 / M_PI);
+        for (int ic = 0; ic < NBC; ++ic) {
+            f << std::setw(14) << (ic + 0.5) * DCH * 180.0 / M_PI << sp;
+            for (auto& c : cols) {
+                double dens = 0.0;
+                if (c.bins)
+                    for (int ir = 0; ir < NBR; ++ir)
+                        for (int it = 0; it < NBT; ++it) dens += (*c.bins)[((size_t)ir * NBT + it) * NBC + ic];
+                f << std::setw(14) << dens / (norm4 * c.numb) << sp;
+            }
+            f << "\n";
+        }
+    }
+    if (!rel) return;
+    const double nrot = d.rotation ? d.Prot : d.P;  // NumbRotTimes
+    const double norm5 = nrot * acount * DCH * (180.0 / M_PI), norm6 = nrot * acount * DTH * (180.0 / M_PI);
+    const std::vector<double>& q = *rel;
+    {
+        auto f = open_out(fname + ".eulphi", "SaveRho1D");
+        for (int ip = 0; ip < NBC; ++ip)
+            f << std::setw(14) << (ip + 0.5) * DCH * 180.0 / M_PI << sp << std::setw(14) << q[NBT + ip] / norm5 << "\n";
+    }
+    {
+        auto f = open_out(fname + ".eulchi", "SaveRho1D");
+        for (int ic = 0; ic < NBC; ++ic)
+            f << std::setw(14) << (ic + 0.5) * DCH * 180.0 / M_PI << sp << std::setw(14) << q[NBT + NBC + ic] / norm5
+              << "\n";
+    }
+    {
+        auto f = open_out(fname + ".eulthe", "SaveRho1D");
+        for (int it = 0; it < NBT; ++it)
+            f << std::setw(14) << (it + 0.5) * DTH * 180.0 / M_PI << sp << std::setw(14) << q[it] / norm6 << "\n";
+    }
+}
+
+// SaveRhoThetaChi (mc_estim.cc:1930-1995): rho(theta, chi) over all r, chi inner, a blank line
+// after each theta
+void save_gtc(const std::vector<double>& g3, const std::string& name, double acount, const Deck& d) {
+    const char* sp = "   ";
+    const auto cols = dens3d_columns(d, g3);
+    auto f = open_out(name, "SaveRhoThetaChi");
+    const double norm6 = DTH * DCH * d.P * acount * (180.0 * 180.0 / (M_PI * M_PI));
+    for (int it = 0; it < NBT; ++it) {
+        for (int ic = 0; ic < NBC; ++ic) {
+            f << std::setw(14) << (it + 0.5) * DTH * 180.0 / M_PI << sp << std::setw(14)
+              << (ic + 0.5) * DCH * 180.0 / M_PI << sp;
+            for (auto& c : cols) {
+                double dens = 0.0;
+                if (c.bins)
+                    for (int ir = 0; ir < NBR; ++ir) dens += (*c.bins)[((size_t)ir * NBT + it) * NBC + ic];
+                f << std::setw(14) << dens / (norm6 * c.numb) << sp;
+            }
+            f << "\n";
+        }
+        f << "\n";
+    }
+}
+
+// SaveDensities3D (mc_estim.cc:1728-1820): r, theta, chi and rho / (r^2 sin theta) per column,
+// one line per bin, chi fastest.  Formatted into a buffer: the file has 1.5 million lines.
+void save_g3d(const std::vector<double>& g3, const std::string& name, double acount, const Deck& d) {
+    const auto cols = dens3d_columns(d, g3);
+    auto f = open_out(name, "SaveDensities3D");
+    const double norm5 = DR * DTH * DCH * d.P * acount;
+    std::string buf;
+    buf.reserve(1 << 22);
+    char num[128];  // four fields of 14 characters and 3 blanks
+    for (int ir = 0; ir < NBR; ++ir) {
+        const double r = (ir + 0.5) * DR;
+        for (int it = 0; it < NBT; ++it) {
+            const double th = (it + 0.5) * DTH;
+            for (int ic = 0; ic < NBC; ++ic) {
+                const double ch = (ic + 0.5) * DCH;
+                const size_t k = ((size_t)ir * NBT + it) * NBC + ic;
+                for (auto& c : cols) {
+                    const double v = c.bins ? (*c.bins)[k] / (norm5 * r * r * std::sin(th)) : 0.0;
+                    const int n = std::snprintf(num, sizeof num, "%14.6e   %14.6e   %14.6e   %14.6e   ", r,
+                                                th * 180.0 / M_PI, ch * 180.0 / M_PI, v);
+                    if (n < 0 || n >= (int)sizeof num) die("SaveDensities3D", "number format overflow");
+                    buf.append(num, (size_t)n);
+                }
+                buf.push_back('\n');
+            }
+            if (buf.size() > (1u << 22) - 4096) {
+                f.write(buf.data(), (std::streamsize)buf.size());
+                buf.clear();
+            }
+        }
+    }
+    f.write(buf.data(), (std::streamsize)buf.size());
+    if (!f.good()) die("SaveDensities3D", "cannot write " + name);
+}
 
 void set_sci(std::ostream& o) { o << std::setprecision(6) << std::scientific; }
 
@@ -740,10 +893,11 @@ int main(int argc, char** argv) {
 
     const int nt = (int)d.types.size();
     int n_plain = 0;
-    bool linear = false;
+    bool linear = false, nonlinear = false;
     for (auto& s : d.types) {
         if (s.molecule == 0) n_plain += s.numb;
         if (s.molecule == 1) linear = true;
+        if (s.molecule == 2) nonlinear = true;
     }
     std::vector<double> samples;
     for (long block = d.startblock + 1; block <= d.startblock + d.blocks; ++block) {
@@ -760,6 +914,10 @@ int main(int argc, char** argv) {
             ch.gr1.assign(NBR, 0.0);
             ch.gr2.assign(NBR * NBT, 0.0);
             ch.rcf.assign(d.rotation ? (size_t)NRCF * d.Prot : 0, 0.0);
+            if (nonlinear) {
+                ch.gr3.assign(NB3, 0.0);
+                if (ch.rel_sum.empty()) ch.rel_sum.assign(NBT + 2 * NBC, 0.0);
+            }
         }
         for (long done = 0; done < d.passes;) {
             const long m = std::min(chunk, d.passes - done);
@@ -792,6 +950,15 @@ int main(int argc, char** argv) {
                     c.check(pimc_get_histograms(c.ctx, r, g1.data(), g2.data()), "pimc_get_histograms");
                     for (int i = 0; i < NBR; ++i) ch.gr1[i] += g1[i];
                     for (int i = 0; i < NBR * NBT; ++i) ch.gr2[i] += g2[i];
+                    if (nonlinear) {
+                        std::vector<double> g3(NB3), rl(NBT + 2 * NBC);
+                        const int has = pimc_get_histograms3d(c.ctx, r, g3.data(), rl.data());
+                        c.check(has, "pimc_get_histograms3d");
+                        if (has & 1)
+                            for (size_t i = 0; i < NB3; ++i) ch.gr3[i] += g3[i];
+                        if (has & 2)
+                            for (int i = 0; i < NBT + 2 * NBC; ++i) ch.rel_sum[i] += rl[i];
+                    }
                     if (d.rotation) {
                         std::vector<double> rc((size_t)NRCF * d.Prot);
                         c.check(pimc_get_rcf(c.ctx, r, rc.data(), (int)rc.size()), "pimc_get_rcf");
@@ -811,6 +978,10 @@ int main(int argc, char** argv) {
                 bn << std::setw(3) << std::setfill('0') << block;
                 const std::string fname = ch.stem + bn.str();
                 save_block_densities(ch, fname, ch.block.count, d, n_plain, linear);
+                if (nonlinear) {  // MCSaveBlockAverages mc_main.cc:722-735
+                    save_rho1d(ch.gr3, fname, ch.block.count, d, nullptr);
+                    save_gtc(ch.gr3, fname + ".gtc", ch.block.count, d);
+                }
                 if (d.rotation) save_rcf(ch.rcf, fname + ".rcf", ch.block.count, d);
                 save_energy(ch, block);
                 if (ch.gr1_sum.empty()) {
@@ -832,6 +1003,12 @@ int main(int argc, char** argv) {
                     }
                 }
                 if (linear) save_g2d(ch.gr2_sum, ch.stem + "_sum.g2d", ch.total.count, d, n_plain);
+                if (nonlinear) {  // mc_main.cc:456-461
+                    if (ch.gr3_sum.empty()) ch.gr3_sum.assign(NB3, 0.0);
+                    for (size_t i = 0; i < NB3; ++i) ch.gr3_sum[i] += ch.gr3[i];
+                    save_g3d(ch.gr3_sum, ch.stem + "_sum.g3d", ch.total.count, d);
+                    save_rho1d(ch.gr3_sum, ch.stem + "_sum", ch.total.count, d, &ch.rel_sum);
+                }
                 if (d.rotation) save_rcf(ch.rcf_sum, ch.stem + "_sum.rcf", ch.total.count, d);
             }
             c.check(pimc_get_state(c.ctx, r, cg.data(), ag.data(), sg.data()), "pimc_get_state");

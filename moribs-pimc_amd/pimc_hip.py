@@ -73,6 +73,7 @@ SIGNATURES = {
     "pimc_clear_samples": (_i, [ctypes.c_void_p]),
     "pimc_get_counters": (_i, [ctypes.c_void_p, _i, _dp, _dp]),
     "pimc_get_histograms": (_i, [ctypes.c_void_p, _i, _dp, _dp]),
+    "pimc_get_histograms3d": (_i, [ctypes.c_void_p, _i, _dp, _dp]),
     "pimc_get_work": (_i, [ctypes.c_void_p, _dp, _dp]),
     "pimc_get_rcf": (_i, [ctypes.c_void_p, _i, _dp, _i]),
     "pimc_pot_energy_slice": (_i, [ctypes.c_void_p, _i, _ip, _ip, _ip, _dp, _dp]),
@@ -257,6 +258,16 @@ class Context:
         g2 = np.zeros(300 * 50)
         self._check(self.lib.pimc_get_histograms(self.h, r, _dptr(g1), _dptr(g2)))
         return g1, g2.reshape(300, 50)
+
+    def histograms3d(self, r):
+        """Bins around a non-linear rotor since the last clear_samples: the (r, theta, chi)
+        counts as [300][50][100] (None without them) and the relative Euler angle counts as
+        (theta[50], phi[100], chi[100]) (None without them)."""
+        g3 = np.zeros(300 * 50 * 100)
+        rel = np.zeros(250)
+        has = self._check(self.lib.pimc_get_histograms3d(self.h, int(r), _dptr(g3), _dptr(rel)))
+        return (g3.reshape(300, 50, 100) if has & 1 else None,
+                (rel[:50], rel[50:150], rel[150:]) if has & 2 else None)
 
     def rcf(self, r):
         """[10][Prot] rotational correlation sums since the last clear_samples (None: no rotor)."""

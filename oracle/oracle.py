@@ -79,6 +79,7 @@ def lib():
             ("or_pot_energy_densities", _d, [_vp, _dp, _dp]),
             ("or_rot_energy", _d, [_vp, _dp, _dp]),
             ("or_rot_e3d", _d, [_vp, _dp, _dp]),
+            ("or_nl_densities", _i, [_vp, _dp, _dp]),
             ("or_sample", None, [_vp, _dp]),
             ("or_set_rng", None, [_vp, _u, _u]),
             ("or_molecular_move", _i, [_vp, _i, _u]),
@@ -227,6 +228,7 @@ def ref():
             ("h_pot_energy_densities", _d, [_dp, _dp]),
             ("h_rot_energy", _d, [_dp, _dp]),
             ("h_rot_e3d", _d, [_dp, _dp]),
+            ("h_nl_densities", _i, [_dp, _dp]),
             ("h_set_tape", None, [_i, _dp, _l]),
             ("h_tape_pos", _l, [_i]),
             ("h_tape_short", _i, []),

@@ -18,6 +18,8 @@
  *   or_caleng                            golden vectors + hand-computed TIP4P dimer
  *   or_kin_energy, or_pot_energy_densities, or_rot_energy, or_rot_e3d
  *                                        golden vectors (estimators on fixed paths)
+ *   or_nl_densities                      the compiled reference's own bins on the same
+ *                                        paths + hand-placed partners and frames
  *   or_linden_rho                        reference-held .rot fixtures + closed form
  *   moves / schedule                     statistical parity against the compiled
  *                                        reference program and closed forms
@@ -42,6 +44,7 @@
 #define OR_RZERO 1.0e-10
 #define OR_MC_BINSR 300
 #define OR_MC_BINST 50
+#define OR_MC_BINSC 100
 #define OR_MAX_RADIUS 15.0
 
 typedef struct {
@@ -1438,6 +1441,74 @@ double or_rot_e3d(or_sys* s, double* erotsq, double* erot_termsq) {
     if (erotsq) *erotsq = esq_all;
     if (erot_termsq) *erot_termsq = term_all;
     return e3d;
+}
+
+/* The bins around a non-linear rotor of one estimator call.  gr3d[(ir*50 + it)*100 + ic]:
+   bin_3Ddensity (mc_estim.cc:1252-1273) of vcord's (radret, theret, chiret) for every
+   atom / rotor pair and slice, as GetPotEnergy_Densities makes them (mc_estim.cc:582-641);
+   the atom partners are of the one atom type.  rel[0..49], rel[50..149], rel[150..249]:
+   the relative Euler angles theta, phi, chi of neighbouring rotor slices as GetRotE3D bins
+   them (mc_estim.cc:1046-1058, RotDenType 0 so each adds 1), with its rotor offsets.
+   Either array may be NULL; bins are added to what the arrays hold.  Returns -1 for more
+   than two rotors, as or_rot_e3d does. */
+int or_nl_densities(or_sys* s, double* gr3d, double* rel) {
+    const double dradius = OR_MAX_RADIUS / (double)OR_MC_BINSR;
+    const double dtheta = M_PI / (double)(OR_MC_BINST - 1);
+    const double dchi = 2.0 * M_PI / (double)(OR_MC_BINSC - 1);
+    if (s->imtype < 0 || s->type[s->imtype].molecule != 2) return 0;
+    if (s->type[s->imtype].numb > 2) { fail(s, "GetRotE3D restatement supports at most two rotors"); return -1; }
+    if (gr3d)
+        for (int atom0 = 0; atom0 < s->natoms - 1; atom0++)
+            for (int atom1 = atom0 + 1; atom1 < s->natoms; atom1++) {
+                int type0 = s->mctype[atom0], type1 = s->mctype[atom1];
+                int mol0 = s->type[type0].molecule, mol1 = s->type[type1].molecule;
+                if (!((mol0 == 2 || mol1 == 2) && mol0 != mol1)) continue;
+                int offset0 = s->P * atom0, offset1 = s->P * atom1;
+                for (int it = 0; it < s->P; it++) {
+                    int t0 = offset0 + it, t1 = offset1 + it, tm;
+                    double rcom[3], rpt[3], eul[3], v, rr, th, ch, hx[3], hy[3], hz[3];
+                    if (mol0 == 2) {
+                        tm = offset0 + it / s->RotRatio;
+                        for (int id = 0; id < 3; id++) { rcom[id] = C(s, id, t0); rpt[id] = C(s, id, t1); }
+                    } else {
+                        tm = offset1 + it / s->RotRatio;
+                        for (int id = 0; id < 3; id++) { rpt[id] = C(s, id, t0); rcom[id] = C(s, id, t1); }
+                    }
+                    eul[PHI] = AN(s, PHI, tm); eul[CTH] = acos(AN(s, CTH, tm)); eul[CHI] = AN(s, CHI, tm);
+                    or_vcord(eul, rcom, rpt, s->vtable, s->nr3, s->nth3, s->nch3, s->rvmax, s->rvmin,
+                             s->rvstep, &v, &rr, &th, &ch, hx, hy, hz, 0);
+                    int br = (int)floor(rr / dradius);
+                    if (br < OR_MC_BINSR && br >= 0) {
+                        int bt = (int)floor(th / dtheta);
+                        if (bt < OR_MC_BINST && bt >= 0) {
+                            int bc = (int)floor(ch / dchi);
+                            if (bc < OR_MC_BINSC && bc >= 0) gr3d[(br * OR_MC_BINST + bt) * OR_MC_BINSC + bc] += 1.0;
+                        }
+                    }
+                }
+            }
+    if (rel) {
+        int type = s->imtype;
+        int offset = s->type[type].offset;
+        for (int atom = 0; atom < s->type[type].numb; atom++) {
+            offset += (s->P * atom);
+            for (int it0 = 0; it0 < s->Prot; it0++) {
+                int t0 = offset + it0, t1 = offset + (it0 + 1) % s->Prot;
+                double e1[3], e2[3], er[3], rho, erot, esq;
+                int istop = 0;
+                e1[0] = AN(s, PHI, t0); e1[1] = acos(AN(s, CTH, t0)); e1[2] = AN(s, CHI, t0);
+                e2[0] = AN(s, PHI, t1); e2[1] = acos(AN(s, CTH, t1)); e2[2] = AN(s, CHI, t1);
+                or_rotden(e1, e2, er, &rho, &erot, &esq, s->rhoprp, s->erotpr, s->erotsq, &istop);
+                int bt = (int)floor(er[1] / dtheta);
+                if (bt < OR_MC_BINST && bt >= 0) rel[bt] += 1.0;
+                int bp = (int)floor(er[0] / dchi);
+                if (bp < OR_MC_BINSC && bp >= 0) rel[OR_MC_BINST + bp] += 1.0;
+                int bc = (int)floor(er[2] / dchi);
+                if (bc < OR_MC_BINSC && bc >= 0) rel[OR_MC_BINST + OR_MC_BINSC + bc] += 1.0;
+            }
+        }
+    }
+    return 0;
 }
 
 /* MCGetAverage mc_main.cc:551-616 (energy and Cv terms of one sample) */

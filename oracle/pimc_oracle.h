@@ -99,6 +99,10 @@ double or_kin_energy(or_sys* s);
 double or_pot_energy_densities(or_sys* s, double* gr1d, double* gr2d);  /* bins may be NULL */
 double or_rot_energy(or_sys* s, double* erotsq, double* erot_termsq);
 double or_rot_e3d(or_sys* s, double* erotsq, double* erot_termsq);
+/* bins around a non-linear rotor: gr3d[300*50*100] (r, theta, chi), rel[50+100+100]
+   (relative Euler theta, phi, chi of neighbouring rotor slices); either may be NULL;
+   -1 for more than two rotors */
+int    or_nl_densities(or_sys* s, double* gr3d, double* rel);
 /* one MCGetAverage sample (mc_main.cc:551-616): out[0..7] =
    kin, pot, rot, rotsq, Cv, Cv_trans, Cv_rot, rot_termsq */
 void   or_sample(or_sys* s, double out[8]);

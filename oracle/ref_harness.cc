@@ -31,6 +31,11 @@ void densities_malloc(void);
 void densities_reset(int);
 extern double** _gr1D;
 extern double*** _gr2D;
+extern double** _gr3D;
+extern double* _relthe_sum;
+extern double* _relphi_sum;
+extern double* _relchi_sum;
+extern int NUMB_DENS3D;
 
 extern "C" void rotden_(double*, double*, double*, double*, double*, double*, double*, double*, double*, int*);
 extern "C" void vcord_(double*, double*, double*, double*, int*, int*, int*, double*, double*, double*, double*,
@@ -219,6 +224,24 @@ double h_pot_energy_densities(double* gr1d, double* gr2d) {
         for (int ir = 0; ir < 300; ++ir)
             for (int it = 0; it < 50; ++it) gr2d[ir * 50 + it] = _gr2D[0][ir][it];
     return v;
+}
+// The bins of one estimator call around a non-linear rotor: GetPotEnergy_Densities fills
+// _gr3D (bin_3Ddensity mc_estim.cc:1252-1273), GetRotE3D the relative Euler angle bins
+// (mc_estim.cc:1046-1058).  gr3d takes NUMB_DENS3D x 300*50*100 values, rel the 50 theta,
+// 100 phi and 100 chi bins; returns NUMB_DENS3D.
+int h_nl_densities(double* gr3d, double* rel) {
+    if (!(IMPURITY && MCAtom[IMTYPE].molecule == 2)) return 0;
+    densities_reset(MC_BLOCK);
+    densities_reset(MC_TOTAL);
+    GetPotEnergy_Densities();
+    GetRotE3D();
+    const long nb = 300L * 50 * 100;
+    for (int id = 0; id < NUMB_DENS3D; ++id)
+        for (long i = 0; i < nb; ++i) gr3d[id * nb + i] = _gr3D[id][i];
+    for (int i = 0; i < 50; ++i) rel[i] = _relthe_sum[i];
+    for (int i = 0; i < 100; ++i) rel[50 + i] = _relphi_sum[i];
+    for (int i = 0; i < 100; ++i) rel[150 + i] = _relchi_sum[i];
+    return NUMB_DENS3D;
 }
 double h_rot_energy(double* esq, double* term) {
     double v = GetRotEnergy();

@@ -72,6 +72,40 @@ def materialise(name, d, blocks=None):
         pt.write_rot(os.path.join(d, pt.rot_filename("CO2", 0.37, 64)), c, rho, er, es)
 
 
+# A non-linear rotor deck for the driver's own tests (the reference program is not run on it):
+# 4 para-H2 around one H2O without rotation moves, the H2-H2 table of pimc_tables.h2h2_table
+# and the stand-in 3D table of tables3d.pot3d (40 x 181 x 181).
+NL_DECK = """MASTERDIR        ./
+OUTPUTDIR        ./
+FILENAMEPREFIX   h2oh2
+DIMENSION     3
+DENSITY       0.02
+TEMPERATURE   0.5
+ATOM       H2   4    BOLTZMANN     0.30     2         pH2      PRIMITIVE
+NONLINEAR  H2O  1    BOLTZMANN     0.05     2         H2Opot   PRIMITIVE
+NUMBEROFSLICES     16
+NUMBEROFPASSES     100
+NUMBEROFBLOCKS     {blocks}  1
+MCSKIP_RATIO     1600
+MCSKIP_TOTAL     1600
+MCSKIP_AVERG     16
+"""
+
+
+def materialise_nl(d, blocks=3):
+    """Writes NL_DECK and its tables into directory d."""
+    try:
+        from golden import tables3d
+    except ImportError:  # run from tests/golden itself
+        import tables3d
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "qmc.input"), "w") as f:
+        f.write(NL_DECK.format(blocks=blocks))
+    r, v = pt.h2h2_table()
+    pt.write_pot1d(os.path.join(d, "pH2.pot"), r, v)
+    tables3d.write_pot3d(os.path.join(d, "H2Opot.pot"), *tables3d.pot3d(nch=181))
+
+
 ENG_COLUMNS = ("kin", "pot", "kin+pot", "rot", "rotsq", "total", "Cv", "Cv_trans", "Cv_rot")
 
 

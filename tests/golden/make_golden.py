@@ -21,7 +21,8 @@ Sections
   nonlinear  rotden/deleul/rotpro, vcord/vcalc (CHgrd 91, 181, 361) and caleng on seeded
              inputs, with the stand-in tables of tables3d.py
   h2o_h2     4 para-H2 + 1 non-linear rotor, P = Prot = 8: PotEnergy, PotRotE3D, GetRotE3D,
-             GetPotEnergy_Densities and two reference-order passes on tapes
+             GetPotEnergy_Densities (with its 3D bins and the relative Euler angle bins) and
+             two reference-order passes on tapes
   water2     2 non-linear rotors (TIP4P via caleng), P = Prot = 8: the same quantities
 """
 import json
@@ -335,6 +336,16 @@ def _system3d(arrays, tag, ntypes, names, numb, molecule, pots, P, Prot, T, seed
     est, gr1, _ = estimators(R, nonlinear=True)
     out["estimators"] = est
     arrays[f"{tag}_gr1d"] = gr1
+    # bins around the non-linear rotor (_gr3D and the relative Euler angle bins), kept sparse
+    n3 = 300 * 50 * 100
+    g3, rel = np.zeros(2 * n3), np.zeros(250)
+    nd = R.h_nl_densities(O.P(g3), O.P(rel))
+    g3 = g3[:nd * n3].reshape(nd, n3)
+    nz = np.nonzero(g3)
+    arrays[f"{tag}_gr3d_idx"] = np.stack(nz).astype(np.int64)
+    arrays[f"{tag}_gr3d_val"] = g3[nz]
+    arrays[f"{tag}_rel"] = rel
+    out["ndens3d"] = nd
     tapes = make_tapes(rng, 40000)
     for k, t in tapes.items():
         R.h_set_tape(k, O.P(t), len(t))

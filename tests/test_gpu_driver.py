@@ -182,6 +182,51 @@ def test_driver_output_layouts(tmp_path):
     assert np.allclose(c0[:, 0], np.arange(65) / (0.37 * 64), rtol=1e-5)
 
 
+def test_driver_nonlinear_densities(tmp_path):
+    """Around a non-linear rotor (4 para-H2 + H2O, no rotation moves): the block files .gri,
+    .grt, .grc (SaveRho1D), .gtc (SaveRhoThetaChi) and the accumulated _sum.g3d
+    (SaveDensities3D), _sum.gri/.grt/.grc and _sum.eulphi/.eulchi/.eulthe.  Each normalisation
+    is checked by integration: every file gives the same number of H2 within 15 A of the rotor
+    per slice (at most 4), the H2O column is empty, and the accumulated files give the mean of
+    the two measured blocks (equal sample counts)."""
+    d = str(tmp_path)
+    decks.materialise_nl(d, blocks=3)
+    r = subprocess.run([DRIVER], cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    stem = os.path.join(d, "h2oh2")
+    dr, dth, dch, deg = 0.05, np.pi / 49, 2 * np.pi / 99, 180.0 / np.pi
+
+    def counts(prefix, gtc=True):
+        gri, grt, grc = (np.loadtxt(prefix + e) for e in (".gri", ".grt", ".grc"))
+        assert gri.shape == (300, 3) and grt.shape == (50, 3) and grc.shape == (100, 3)
+        assert not gri[:, 2].any() and not grt[:, 2].any() and not grc[:, 2].any()
+        assert np.allclose(grc[:, 0], (np.arange(100) + 0.5) * dch * deg)
+        n = gri[:, 1].sum() * dr
+        assert abs(grt[:, 1].sum() * dth * deg * 4 - n) < 1e-5 * n
+        assert abs(grc[:, 1].sum() * dch * deg * 4 - n) < 1e-5 * n
+        if gtc:
+            t = np.loadtxt(prefix + ".gtc")
+            assert t.shape == (5000, 4) and not t[:, 3].any()
+            assert abs(t[:, 2].sum() * dth * dch * deg * deg * 4 - n) < 1e-5 * n
+        return n
+
+    per_block = [counts(stem + b) for b in ("002", "003")]
+    assert all(0.5 < n <= 4.0 + 1e-9 for n in per_block), per_block
+    n_sum = counts(stem + "_sum", gtc=False)
+    assert abs(n_sum - np.mean(per_block)) < 1e-5 * n_sum
+    g3d = np.fromfile(stem + "_sum.g3d", sep=" ").reshape(-1, 8)
+    assert g3d.shape[0] == 300 * 50 * 100 and not g3d[:, 7].any()
+    ir, it, ic = np.unravel_index(np.arange(g3d.shape[0]), (300, 50, 100))
+    assert np.allclose(g3d[:, 0], (ir + 0.5) * dr, rtol=1e-6) and np.allclose(g3d[:, 2], (ic + 0.5) * dch * deg, rtol=1e-6)
+    th = (it + 0.5) * dth
+    assert np.allclose(g3d[:, 1], th * deg, rtol=1e-6)
+    n3 = np.sum(g3d[:, 3] * ((ir + 0.5) * dr) ** 2 * np.sin(th)) * dr * dth * dch
+    assert abs(n3 - n_sum) < 1e-5 * n_sum
+    for ext, nb in ((".eulphi", 100), (".eulchi", 100), (".eulthe", 50)):
+        e = np.loadtxt(stem + "_sum" + ext)
+        assert e.shape == (nb, 2) and not e[:, 1].any()  # no rotation moves, no rotor links binned
+
+
 def test_restart_continues_exactly(tmp_path):
     """Four blocks in one run against two blocks, then RESTART for two more: the resumed run's
     .eng lines and final configuration are identical to the uninterrupted run's."""

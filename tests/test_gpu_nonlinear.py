@@ -228,3 +228,52 @@ def test_split_chain_restore_continues_exactly(rot3d):
     three.run(2)
     for x, y in zip(one.get_state(0), three.get_state(0)):
         assert np.array_equal(x, y)
+
+
+N3D = 300 * 50 * 100
+
+
+@pytest.mark.parametrize("kind", ["h2o_h2", "water2"])
+def test_nl_density_bins_fp64(rot3d, kind):
+    """GetPotEnergy_Densities' (r, theta, chi) bins around the rotor and GetRotE3D's relative
+    Euler angle bins, accumulated over three sampled passes, equal the oracle's on the same
+    chain (FP64 runs follow it move for move)."""
+    wl = nl_workload(kind, rot3d)
+    ctx = gpu_ctx(wl, ph.FLAG_FP64)
+    ctx.clear_samples()
+    ctx.run(3, sample_every=1)
+    g3, rel = ctx.histograms3d(0)
+    S = oracle_system(wl)
+    w3, wr = np.zeros(N3D), np.zeros(250)
+    for p in range(3):
+        S.run(p, 1)
+        assert S.L.or_nl_densities(S.h, O.P(w3), O.P(wr)) == 0
+    nrot = 1 if kind == "h2o_h2" else 2
+    if kind == "h2o_h2":
+        assert g3 is not None and np.array_equal(g3.reshape(-1), w3)
+        assert w3.sum() == 3 * 4 * 8  # every H2 within 15 A at every slice
+    else:
+        assert g3 is None and not w3.any()
+    assert rel is not None and np.array_equal(np.concatenate(rel), wr)
+    assert wr[:50].sum() == 3 * nrot * 8
+
+
+def test_nl_density_bins_fp32_configuration(rot3d):
+    """An FP32 run bins from the FP64 coordinates and frames: one sampled pass, then the
+    oracle's bins of the configuration the device holds."""
+    wl = nl_workload("h2o_h2", rot3d)
+    ctx = gpu_ctx(wl)
+    ctx.run(2)
+    ctx.clear_samples()
+    ctx.run(1, sample_every=1)
+    g3, rel = ctx.histograms3d(0)
+    cg, ag, _ = ctx.get_state(0)
+    rp = wl["replicas"][0]
+    S = oracle_system(wl)
+    angles = np.array(rp["angles"], copy=True)
+    rot = rotor_columns(rp)
+    angles[:, rot] = ag[:, rot]
+    S.set_state(cg, angles)
+    w3, wr = np.zeros(N3D), np.zeros(250)
+    assert S.L.or_nl_densities(S.h, O.P(w3), O.P(wr)) == 0
+    assert np.array_equal(g3.reshape(-1), w3) and np.array_equal(np.concatenate(rel), wr)

@@ -369,3 +369,65 @@ def test_nonlinear_reference_order_passes(G, A, rot3d, tag):
     assert t.reshape(-1).tolist() == g["pass_total"] and acc.reshape(-1).tolist() == g["pass_accep"]
     est, _, _ = estimators(S, nonlinear=True)
     check_estimators(est, g["pass_estimators"])
+
+
+N3D = 300 * 50 * 100
+
+
+def nl_bins(S):
+    g3, rel = np.zeros(N3D), np.zeros(250)
+    assert S.L.or_nl_densities(S.h, O.P(g3), O.P(rel)) == 0
+    return g3, rel
+
+
+@pytest.mark.parametrize("tag", ["h2o_h2", "water2"])
+def test_nonlinear_density_bins(G, A, rot3d, tag):
+    """The (r, theta, chi) bins around the rotor and the relative Euler angle bins equal the
+    compiled reference's _gr3D and _relthe/_relphi/_relchi_sum on the same path."""
+    S = system3d(tag, rot3d)
+    S.set_state(A[f"{tag}_coords"], A[f"{tag}_angles"])
+    g3, rel = nl_bins(S)
+    ref = np.zeros((G[tag]["ndens3d"], N3D))
+    idx = A[f"{tag}_gr3d_idx"]
+    ref[idx[0], idx[1]] = A[f"{tag}_gr3d_val"]
+    assert np.array_equal(g3, ref[0])  # density type 0: the atom type (h2o_h2) or nothing (water2)
+    assert np.all(ref[1:] == 0.0)
+    assert np.array_equal(rel, A[f"{tag}_rel"])
+    nrot = 1 if tag == "h2o_h2" else 2
+    assert rel[:50].sum() == rel[50:150].sum() == rel[150:].sum() == nrot * 8
+
+
+def frame_matrix(phi, theta, chi):
+    """matpre (rotden.f:136-163): body axes are the columns."""
+    cp, sp, ct, st, ck, sk = np.cos(phi), np.sin(phi), np.cos(theta), np.sin(theta), np.cos(chi), np.sin(chi)
+    return np.array([[cp * ct * ck - sp * sk, -cp * ct * sk - sp * ck, cp * st],
+                     [sp * ct * ck + cp * sk, -sp * ct * sk + cp * ck, sp * st],
+                     [-st * ck, st * sk, ct]])
+
+
+def test_nonlinear_density_bins_hand_placed(rot3d):
+    """Atoms put at chosen body-frame (r, theta, chi) around a rotor of chosen orientation land
+    in bin (floor(r/0.05), floor(theta/(pi/49)), floor(chi/(2 pi/99))), one count per slice.
+    Identical rotor slices give the identity as relative rotation: theta and phi bin 0, and
+    chi = 2 pi - acos(1) in the last bin (deleul takes chi = 2 pi - acos when sin chi <= 0)."""
+    S = system3d("h2o_h2", rot3d)
+    P = 8
+    eul = (0.7, np.arccos(0.3), 2.1)
+    Rm = frame_matrix(*eul)
+    centre = np.array([0.4, -0.2, 0.3])
+    body = [(3.33, 1.0, 4.0), (5.12, 2.5, 0.3), (2.71, 0.4, 2.0), (7.77, 1.9, 5.5)]  # (r, theta, chi)
+    coords = np.zeros((3, 5 * P))
+    for a, (r, th, ch) in enumerate(body):
+        v = r * np.array([np.sin(th) * np.cos(ch), np.sin(th) * np.sin(ch), np.cos(th)])
+        coords[:, a * P:(a + 1) * P] = (centre + Rm @ v)[:, None]
+    coords[:, 4 * P:] = centre[:, None]
+    angles = np.zeros((3, 5 * P))
+    angles[0, 4 * P:], angles[1, 4 * P:], angles[2, 4 * P:] = eul[0], np.cos(eul[1]), eul[2]
+    S.set_state(coords, angles)
+    g3, rel = nl_bins(S)
+    want = np.zeros(N3D)
+    for r, th, ch in body:
+        br, bt, bc = int(r / 0.05), int(th / (np.pi / 49)), int(ch / (2 * np.pi / 99))
+        want[(br * 50 + bt) * 100 + bc] += P
+    assert np.array_equal(g3, want)
+    assert rel[0] == P and rel[50] == P and rel[249] == P and rel.sum() == 3 * P
