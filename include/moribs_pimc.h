@@ -88,6 +88,20 @@ int  pimc_add_pot3d(pimc_ctx* ctx, int nr, int nth, int nch, double rmin, double
 /* init_rot3D mc_poten.cc:440-506: the .rho, .eng and .esq tables, 181*361*361 values each in
    rotpro's order (rotpro_sub.f:7-26); copied to the device at once (566 MB) */
 int  pimc_add_rot3d(pimc_ctx* ctx, const double* rho, const double* erot, const double* esq);
+/* Free-rotor density matrices computed on the GPU, in the reference generators' units and
+   order; ctx needs no replicas or commit (any created context).
+   linden (linear_prop/linden.f:88-122): the four .rot columns cos(gamma), rho, E and E^2
+   numerators on npt points of [-1, 1], for rotational constant bconst (cm^-1), T (K) and
+   nslice rotor slices; iodevn -1 all l, 0 even, 1 odd; lmax <= 0 takes linden's cut-off (first
+   l with weight below 1e-16).  Returns lmax used.
+   asymrho (nmv_prop/asymrho.f:34-731): rho, E (cm^-1) and E^2 (cm^-2) on the 181 x 361 x 361
+   whole-degree (theta, phi, chi) grid that pimc_add_rot3d reads, for rotational constants
+   A, B, C (cm^-1), J up to maxj, iodevn -1/0/1 (both, even, odd k).  Fails like asymrho when
+   exp(-tau E_max) > 1e-8 ("raise maxj"). */
+int  pimc_linden(pimc_ctx* ctx, double T, int nslice, double bconst, int npt, int iodevn, int lmax,
+                 double* cost, double* rho, double* erot, double* esq);
+int  pimc_asymrho(pimc_ctx* ctx, double T, int nslice, double A, double B, double C, int maxj, int iodevn,
+                  double* rho, double* eng, double* esq);
 
 /* ---- replicas and state ---- */
 int  pimc_set_replica(pimc_ctx* ctx, int replica, const pimc_replica_desc* desc);

@@ -7,6 +7,7 @@
 // proposal per segment start and the rotor P sweeps per pass, as in mc_main.cc:348-381.
 // One pass is captured once as a HIP graph and replayed.
 #include "pimc_kernels.hpp"
+#include "pimc_gen.hpp"
 #include "../../include/moribs_pimc.h"
 #include <hip/hip_runtime.h>
 #include <cmath>
@@ -420,6 +421,109 @@ static void rot3d_points(const double* t, std::vector<double4>& out) {
                 out[i0] = make_double4(v, ich != 360 ? t[i0 + 1] - v : 0.0, iph != 360 ? t[i0 + 361] - v : 0.0,
                                        ith != 180 ? t[i0 + 361 * 361] - v : 0.0);
             }
+}
+
+namespace {
+struct DevMem {  // a device buffer freed on every return path
+    void* p = nullptr;
+    ~DevMem() {
+        if (p) (void)hipFree(p);
+    }
+    double* d() const { return static_cast<double*>(p); }
+};
+}  // namespace
+
+int pimc_linden(pimc_ctx* c, double T, int nslice, double bconst, int npt, int iodevn, int lmax, double* cost,
+                double* rho, double* erot, double* esq) {
+    if (!c) return -1;
+    if (!(T > 0.0) || nslice < 1 || npt < 2 || iodevn < -1 || iodevn > 1 || !cost || !rho || !erot || !esq)
+        return fail(c, "pimc_linden: T > 0, nslice >= 1, npt >= 2, iodevn in -1..1 and four output arrays");
+    const double tau = 1.4387752224 / (T * nslice);  // linden.f:5,24, in cm
+    if (lmax <= 0) {  // the first l whose weight falls below 1e-16 (linden.f:27-36)
+        lmax = 500;
+        for (int l = 0; l <= 500; ++l)
+            if (std::exp(-tau * bconst * l * (l + 1.0)) < 1.0e-16) {
+                lmax = l;
+                break;
+            }
+    }
+    DevMem d;
+    HIPCHECK(c, hipMalloc(&d.p, sizeof(double) * 4 * (size_t)npt));
+    hipLaunchKernelGGL(k_linden, dim3((npt + 255) / 256), dim3(256), 0, c->stream, npt, lmax, iodevn, tau, bconst,
+                       nslice * 0.69503476, d.d());
+    HIPCHECK(c, hipGetLastError());
+    std::vector<double> h(4 * (size_t)npt);
+    HIPCHECK(c, hipMemcpyAsync(h.data(), d.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    std::copy(h.begin(), h.begin() + npt, cost);
+    std::copy(h.begin() + npt, h.begin() + 2 * npt, rho);
+    std::copy(h.begin() + 2 * npt, h.begin() + 3 * npt, erot);
+    std::copy(h.begin() + 3 * npt, h.end(), esq);
+    return lmax;
+}
+
+int pimc_asymrho(pimc_ctx* c, double T, int nslice, double A, double B, double C, int maxj, int iodevn, double* rho,
+                 double* eng, double* esq) {
+    if (!c) return -1;
+    if (!(T > 0.0) || nslice < 1 || maxj < 0 || maxj > 160 || iodevn < -1 || iodevn > 1 || !rho || !eng || !esq)
+        return fail(c, "pimc_asymrho: T > 0, nslice >= 1, 0 <= maxj <= 160, iodevn in -1..1 and three output arrays");
+    const double beta = 1.0 / (0.6950356 * T), tau = beta / nslice;  // asymrho.f:42, in cm
+    std::vector<double> g;
+    const double emax = asym_blocks(maxj, A, B, C, tau, g);
+    if (std::exp(-tau * emax) > 1.0e-8) return fail(c, "pimc_asymrho: too large contribution from emax: raise maxj");
+    const int n = 2 * maxj + 1, NG = ASYM_NG, NT = 181;
+    const size_t nn = (size_t)n * n, np = (size_t)NG * NG;
+    const int npar = iodevn == -1 ? 2 : 1;
+    const int par0 = iodevn == -1 ? 0 : iodevn;
+    // e^{i k chi} split into cosine and sine tables [n][361], and [cos(m phi) | -sin(m phi)] [361][2n]
+    std::vector<double> xc(n * NG), xs(n * NG), fm(NG * 2 * (size_t)n);
+    for (int kk = 0; kk < n; ++kk)
+        for (int a = 0; a < NG; ++a) {
+            const double ang = (double)a * (M_PI / 180.0), x = ang * (double)(kk - maxj);
+            xc[(size_t)kk * NG + a] = std::cos(x);
+            xs[(size_t)kk * NG + a] = std::sin(x);
+            fm[(size_t)a * 2 * n + kk] = std::cos(x);
+            fm[(size_t)a * 2 * n + n + kk] = -std::sin(x);
+        }
+    const std::vector<int> src = asym_fill_map();
+    const size_t nbatch = (size_t)NT * npar * 3;
+    DevMem dg, dxc, dxs, dfm, dsrc, dm, dt, dx, dout;
+    HIPCHECK(c, hipMalloc(&dg.p, sizeof(double) * g.size()));
+    HIPCHECK(c, hipMalloc(&dxc.p, sizeof(double) * xc.size()));
+    HIPCHECK(c, hipMalloc(&dxs.p, sizeof(double) * xs.size()));
+    HIPCHECK(c, hipMalloc(&dfm.p, sizeof(double) * fm.size()));
+    HIPCHECK(c, hipMalloc(&dsrc.p, sizeof(int) * src.size()));
+    HIPCHECK(c, hipMalloc(&dm.p, sizeof(double) * nbatch * nn));
+    HIPCHECK(c, hipMalloc(&dt.p, sizeof(double) * nbatch * 2 * n * NG));
+    HIPCHECK(c, hipMalloc(&dx.p, sizeof(double) * nbatch * np));
+    HIPCHECK(c, hipMalloc(&dout.p, sizeof(double) * 3 * NT * np));
+    HIPCHECK(c, hipMemcpyAsync(dg.p, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(dxc.p, xc.data(), sizeof(double) * xc.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(dxs.p, xs.data(), sizeof(double) * xs.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(dfm.p, fm.data(), sizeof(double) * fm.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(dsrc.p, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice, c->stream));
+    // M_q(theta) of each parity, batch (theta, parity, q)
+    for (int pi = 0; pi < npar; ++pi)
+        hipLaunchKernelGGL(k_asym_m, dim3((unsigned)((nn + 255) / 256), NT), dim3(256), 0, c->stream, maxj, par0 + pi,
+                           npar, pi, dg.d() + (size_t)(par0 + pi) * (maxj + 1) * 3 * nn, dm.d());
+    HIPCHECK(c, hipGetLastError());
+    // T = M [cos(k chi); sin(k chi)] stacked as [2n][361], then X = [cos(m phi) | -sin(m phi)] T
+    const dim3 g1((NG + 63) / 64, (n + 63) / 64, (unsigned)nbatch), g2((NG + 63) / 64, (NG + 63) / 64, (unsigned)nbatch);
+    hipLaunchKernelGGL(k_dgemm, g1, dim3(256), 0, c->stream, n, NG, n, dm.d(), n, (long long)nn, dxc.d(), NG, 0LL,
+                       dt.d(), NG, (long long)2 * n * NG);
+    hipLaunchKernelGGL(k_dgemm, g1, dim3(256), 0, c->stream, n, NG, n, dm.d(), n, (long long)nn, dxs.d(), NG, 0LL,
+                       dt.d() + (size_t)n * NG, NG, (long long)2 * n * NG);
+    hipLaunchKernelGGL(k_dgemm, g2, dim3(256), 0, c->stream, NG, NG, 2 * n, dfm.d(), 2 * n, 0LL, dt.d(), NG,
+                       (long long)2 * n * NG, dx.d(), NG, (long long)np);
+    HIPCHECK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_asym_fill, dim3((unsigned)((np + 255) / 256), NT), dim3(256), 0, c->stream, npar,
+                       static_cast<const int*>(dsrc.p), dx.d(), dout.d(), dout.d() + NT * np, dout.d() + 2 * NT * np);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipMemcpyAsync(rho, dout.d(), sizeof(double) * NT * np, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(eng, dout.d() + NT * np, sizeof(double) * NT * np, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(esq, dout.d() + 2 * NT * np, sizeof(double) * NT * np, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 int pimc_add_rot3d(pimc_ctx* c, const double* rho, const double* erot, const double* esq) {

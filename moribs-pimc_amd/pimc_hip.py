@@ -60,6 +60,9 @@ SIGNATURES = {
     "pimc_add_rotdens": (_i, [ctypes.c_void_p, _i, _dp, _dp, _dp, _dp]),
     "pimc_add_pot3d": (_i, [ctypes.c_void_p, _i, _i, _i, _d, _d, _dp]),
     "pimc_add_rot3d": (_i, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "pimc_linden": (_i, [ctypes.c_void_p, ctypes.c_double, _i, ctypes.c_double, _i, _i, _i, _dp, _dp, _dp, _dp]),
+    "pimc_asymrho": (_i, [ctypes.c_void_p, ctypes.c_double, _i, ctypes.c_double, ctypes.c_double, ctypes.c_double, _i, _i,
+                          _dp, _dp, _dp]),
     "pimc_set_replica": (_i, [ctypes.c_void_p, _i, ctypes.POINTER(ReplicaDesc)]),
     "pimc_commit": (_i, [ctypes.c_void_p]),
     "pimc_set_option": (_i, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_long]),
@@ -175,6 +178,21 @@ class Context:
         if any(x.size != 181 * 361 * 361 for x in a):
             raise PimcError("non-linear density tables hold 181*361*361 values each")
         return self._check(self.lib.pimc_add_rot3d(self.h, *[_dptr(x) for x in a]))
+
+    # density-matrix generators on the device
+    def linden(self, T, nslice, bconst, npt=2001, iodevn=-1, lmax=0):
+        """The four .rot columns (cos, rho, E, E^2 numerators) of linden.f:88-122 and the l cut-off used."""
+        out = [np.zeros(npt) for _ in range(4)]
+        used = self._check(self.lib.pimc_linden(self.h, float(T), int(nslice), float(bconst), int(npt), int(iodevn),
+                                                int(lmax), *[_dptr(x) for x in out]))
+        return (*out, used)
+
+    def asymrho(self, T, nslice, A, B, C, maxj, iodevn=-1):
+        """rho, E, E^2 of asymrho.f:34-731 on the (181, 361, 361) whole-degree grid."""
+        out = [np.zeros(181 * 361 * 361) for _ in range(3)]
+        self._check(self.lib.pimc_asymrho(self.h, float(T), int(nslice), float(A), float(B), float(C), int(maxj),
+                                          int(iodevn), *[_dptr(x) for x in out]))
+        return tuple(x.reshape(181, 361, 361) for x in out)
 
     # replicas
     def set_replica(self, r, ntypes, numb, molecule, levels, mcstep, mass, P, Prot, T, rtstep, pot1d=None, pot2d=None,
