@@ -182,7 +182,7 @@ struct pimc_ctx {
     size_t lds_knots = 0;       // the largest 1D knot array, staged in LDS by the centroid move
     RotPart* d_parts = nullptr;  // rotor chains split into slice ranges, one workgroup each
     unsigned* d_flags = nullptr;  // half-sweeps completed per part
-    double* d_links = nullptr;    // per part, the densities of its two boundary links as last published
+    unsigned long long* d_xrec = nullptr;  // per part, its two boundary records (8 words each) as last published
     int nparts = 0, nflags = 0;
     int opt_rotor_part_slices = 0;  // most rotor slices per part, 0: from the LDS budget
     int opt_rotor_lds_kb = 52;      // LDS budget of a rotor part (52 KB: three parts per CU)
@@ -293,7 +293,7 @@ void pimc_destroy(pimc_ctx* c) {
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     void* ptrs[] = {c->d_rep, c->d_mut, c->d_beads, c->d_rotb, c->d_hist, c->d_samples, c->d_k1f, c->d_k1d,
                     c->d_t1, c->d_v2f, c->d_v2c, c->d_v2d, c->d_g2d, c->d_g2f, c->d_t2, c->d_rk, c->d_re, c->d_tr,
-                    c->d_work, c->d_all, c->d_rotreps, c->d_est, c->d_rows, c->d_rowbuf, c->d_parts, c->d_flags, c->d_links,
+                    c->d_work, c->d_all, c->d_rotreps, c->d_est, c->d_rows, c->d_rowbuf, c->d_parts, c->d_flags, c->d_xrec,
                     c->d_r3, c->d_v3d, c->d_v3f, c->d_t3, c->d_frames, c->d_work1, c->d_all0, c->d_all1, c->d_rot3, c->d_rows1, c->d_mlp,
                     c->d_rcf, c->d_rcfreps, c->d_cparts0, c->d_cparts1, c->d_cflags, c->d_cslots,
                     c->d_parts3, c->d_flags3, c->d_pub3};
@@ -657,10 +657,10 @@ static int plan_rotor_parts(pimc_ctx* c, int lds_kb) {
     });
     if (c->d_parts) (void)hipFree(c->d_parts);
     if (c->d_flags) (void)hipFree(c->d_flags);
-    if (c->d_links) (void)hipFree(c->d_links);
+    if (c->d_xrec) (void)hipFree(c->d_xrec);
     c->d_parts = nullptr;
     c->d_flags = nullptr;
-    c->d_links = nullptr;
+    c->d_xrec = nullptr;
     c->lds_rotor = lds_r;
     c->nflags = nflags;
     c->nparts = (int)parts.size();
@@ -668,10 +668,11 @@ static int plan_rotor_parts(pimc_ctx* c, int lds_kb) {
     if (parts.empty()) return 0;
     c->d_parts = dalloc<RotPart>(parts.size());
     c->d_flags = dalloc<unsigned>(nflags);
-    c->d_links = dalloc<double>(2 * (size_t)nflags);
-    if (!c->d_parts || !c->d_flags || !c->d_links) return fail(c, "hipMalloc failed");
+    c->d_xrec = dalloc<unsigned long long>(16 * (size_t)nflags);
+    if (!c->d_parts || !c->d_flags || !c->d_xrec) return fail(c, "hipMalloc failed");
     HIPCHECK(c, hipMemcpy(c->d_parts, parts.data(), sizeof(RotPart) * parts.size(), hipMemcpyHostToDevice));
     HIPCHECK(c, hipMemset(c->d_flags, 0, sizeof(unsigned) * nflags));
+    HIPCHECK(c, hipMemset(c->d_xrec, 0, sizeof(unsigned long long) * 16 * nflags));  // no record matches a wanted number
     return 0;
 }
 
@@ -1399,7 +1400,7 @@ void launch_rotor(pimc_ctx* c, int u) {
     Timed t(c, "rotor");
     if (c->nparts)
         hipLaunchKernelGGL(k_rotor<R>, dim3(c->nparts), dim3(256), c->lds_rotor, c->stream, c->d_rep, c->d_mut,
-                           c->d_beads, c->d_rotb, c->tables(), c->tables3(), c->d_parts, c->d_flags, c->d_links, run);
+                           c->d_beads, c->d_rotb, c->tables(), c->tables3(), c->d_parts, c->d_flags, c->d_xrec, run);
     if (c->nparts3)
         hipLaunchKernelGGL(k_rotor3d_parts<R>, dim3(c->nparts3), dim3(256), c->lds_parts3, c->stream, c->d_rep,
                            c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_parts3,
@@ -1462,7 +1463,7 @@ template <typename R>
 void launch_rotor_linear(pimc_ctx* c, int u) {
     Run run{c->seed, u, c->ref_order ? 1 : 0, 0, 0};
     hipLaunchKernelGGL(k_rotor<R>, dim3(c->nparts), dim3(256), c->lds_rotor, c->stream, c->d_rep, c->d_mut,
-                       c->d_beads, c->d_rotb, c->tables(), c->tables3(), c->d_parts, c->d_flags, c->d_links, run);
+                       c->d_beads, c->d_rotb, c->tables(), c->tables3(), c->d_parts, c->d_flags, c->d_xrec, run);
 }
 
 // one pass in the order described at the top of this file

@@ -613,7 +613,7 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut* __restrict__ mut,
                                                const double4* __restrict__ beads, RotBead* __restrict__ rotb,
                                                Tables T, Tables3 T3, const RotPart* __restrict__ parts,
-                                               unsigned* __restrict__ flags, double* __restrict__ links, Run run) {
+                                               unsigned* __restrict__ flags, unsigned long long* __restrict__ xrec, Run run) {
     extern __shared__ double smem[];
     __shared__ unsigned long long red_acc[4];
     __shared__ int s_err;
@@ -642,9 +642,7 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
     vec4_t<R>* geo = reinterpret_cast<vec4_t<R>*>(smem + ((7 * (ns + 2) + 3) & ~3));
     const int ar = rp.rot_atom;
     const long long rbase = rp.rot_off + (long long)(ar - rp.tbegin[rp.imtype]) * Prot;
-    unsigned* myflag = flags + pt.fbase + pt.w;
-    unsigned* prevflag = flags + pt.fbase + (pt.w + W - 1) % W;
-    unsigned* nextflag = flags + pt.fbase + (pt.w + 1) % W;
+    unsigned* myflag = flags + pt.fbase + pt.w;  // half-sweeps done by the launches before this one
     if (threadIdx.x == 0) s_err = 0;
     // own slices and both neighbours' boundary slices
     for (int kl = threadIdx.x; kl < ns + 2; kl += blockDim.x) {
@@ -706,31 +704,52 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
         for (int parity = 0; parity < 2; ++parity) {
             const unsigned h = 2u * (unsigned)sw + (unsigned)parity;  // half-sweeps done before this one
             // the neighbour's boundary slice this half reads: k0-1 (even half), k1 (odd half)
-            if (threadIdx.x == 0 && !(sw == 0 && parity == 0)) {
+            if (!(sw == 0 && parity == 0)) {
                 const int kl = parity == 0 ? 0 : ns + 1;
                 if (W == 1) {
-                    const int src = parity == 0 ? ns : 1;
-                    nv[3 * kl] = nv[3 * src];
-                    nv[3 * kl + 1] = nv[3 * src + 1];
-                    nv[3 * kl + 2] = nv[3 * src + 2];
-                    const int lk = parity == 0 ? 0 : ns;
-                    double p = nv[3 * lk] * nv[3 * lk + 3] + nv[3 * lk + 1] * nv[3 * lk + 4] + nv[3 * lk + 2] * nv[3 * lk + 5];
-                    Lk[lk] = rot_rho(T, trt, p);
-                } else {
-                    // the even half waits for the previous part's odd half of the last sweep,
-                    // the odd half for the next part's even half of this sweep; the neighbour
-                    // publishes the density of the shared link with its orientation (the same
-                    // product of the same operands this part would form)
-                    if (!wait_count(parity == 0 ? prevflag : nextflag, base + h)) s_err = 2;
-                    int k = parity == 0 ? k0 - 1 : k0 + ns;
-                    if (k < 0) k += Prot;
-                    if (k >= Prot) k -= Prot;
-                    RotBead* b = rotb + rbase + k;
-                    nv[3 * kl] = load_agent(&b->nx);
-                    nv[3 * kl + 1] = load_agent(&b->ny);
-                    nv[3 * kl + 2] = load_agent(&b->nz);
-                    const int nb = pt.fbase + (parity == 0 ? (pt.w + W - 1) % W : (pt.w + 1) % W);
-                    Lk[parity == 0 ? 0 : ns] = load_agent(links + 2 * nb + (parity == 0 ? 1 : 0));
+                    if (threadIdx.x == 0) {
+                        const int src = parity == 0 ? ns : 1;
+                        nv[3 * kl] = nv[3 * src];
+                        nv[3 * kl + 1] = nv[3 * src + 1];
+                        nv[3 * kl + 2] = nv[3 * src + 2];
+                        const int lk = parity == 0 ? 0 : ns;
+                        double p = nv[3 * lk] * nv[3 * lk + 3] + nv[3 * lk + 1] * nv[3 * lk + 4] + nv[3 * lk + 2] * nv[3 * lk + 5];
+                        Lk[lk] = rot_rho(T, trt, p);
+                    }
+                } else if (threadIdx.x < 8) {
+                    // The even half reads what the previous part published after its odd half of
+                    // the last sweep, the odd half what the next part published after its even
+                    // half of this sweep: the orientation and the density of the shared link (the
+                    // same product of the same operands this part would form), as eight words
+                    // that each carry the publishing half's number (base + h) above 32 bits of a
+                    // value.  Polling the words themselves returns the data with its check.  The
+                    // neighbour cannot publish that record again before this part's next half.
+                    const int nb = parity == 0 ? (pt.w + W - 1) % W : (pt.w + 1) % W;
+                    const unsigned long long* rec = xrec + ((size_t)(pt.fbase + nb) * 2 + (parity == 0 ? 1 : 0)) * 8;
+                    const unsigned want = base + h;
+                    unsigned long long wv = 0;
+                    unsigned spins = 0;
+                    for (;;) {
+                        wv = __hip_atomic_load(rec + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if ((__ballot((unsigned)(wv >> 32) == want) & 0xffull) == 0xffull) break;
+                        if (++spins > ROT_SPIN_LIMIT) {
+                            s_err = 2;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                    const unsigned lo = (unsigned)wv;
+                    double v[4];
+                    for (int i = 0; i < 4; ++i) {
+                        const unsigned a = __shfl(lo, 2 * i, 8), b = __shfl(lo, 2 * i + 1, 8);
+                        v[i] = __longlong_as_double((long long)(((unsigned long long)b << 32) | a));
+                    }
+                    if (threadIdx.x == 0) {
+                        nv[3 * kl] = v[0];
+                        nv[3 * kl + 1] = v[1];
+                        nv[3 * kl + 2] = v[2];
+                        Lk[parity == 0 ? 0 : ns] = v[3];
+                    }
                 }
             }
             __syncthreads();
@@ -797,21 +816,24 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
             __syncthreads();
             // publish the boundary slice this half moved (k0 after the even half, k1-1 after
             // the odd half), then the count of half-sweeps done
-            if (W > 1 && threadIdx.x == 0) {
+            if (W > 1 && threadIdx.x < 8) {
+                // orientation and the density of the link to the neighbour's adjacent slice
+                // (Lk[0] after the even half, for the previous part; Lk[ns] after the odd half,
+                // for the next), two words per double; no wait for the stores, the reader
+                // checks every word's number
                 const int kl = parity == 0 ? 1 : ns;
-                RotBead* b = rotb + rbase + (k0 + kl - 1);
-                store_agent(&b->phi, phi[kl]);
-                store_agent(&b->cth, cth[kl]);
-                store_agent(&b->nx, nv[3 * kl]);
-                store_agent(&b->ny, nv[3 * kl + 1]);
-                store_agent(&b->nz, nv[3 * kl + 2]);
-                // the density of the link to the neighbour's adjacent slice: Lk[0] after the even
-                // half (read by the previous part), Lk[ns] after the odd half (by the next)
-                store_agent(links + 2 * (pt.fbase + pt.w) + parity, Lk[parity == 0 ? 0 : ns]);
-                raise_count(myflag, base + h + 1u);
+                const int i = threadIdx.x >> 1;
+                const double v = i < 3 ? nv[3 * kl + i] : Lk[parity == 0 ? 0 : ns];
+                const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+                const unsigned half = (threadIdx.x & 1) ? (unsigned)(bits >> 32) : (unsigned)bits;
+                __hip_atomic_store(xrec + ((size_t)(pt.fbase + pt.w) * 2 + parity) * 8 + threadIdx.x,
+                                   ((unsigned long long)(base + h + 1u) << 32) | half, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     }
+    if (W > 1 && threadIdx.x == 0)  // every part of the chain ran the same halves
+        __hip_atomic_store(myflag, base + 2u * (unsigned)nsweep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int kl = 1 + threadIdx.x; kl <= ns; kl += blockDim.x) {
         RotBead b = rotb[rbase + k0 + kl - 1];
         b.phi = phi[kl];
