@@ -192,7 +192,7 @@ struct pimc_ctx {
     std::vector<int> rot_chains;    // replicas with a linear rotor, heaviest first
     RotPart* d_parts3 = nullptr;     // non-linear chains split by slice range
     unsigned* d_flags3 = nullptr;
-    double* d_pub3 = nullptr;        // boundary frames published per part
+    unsigned long long* d_pub3 = nullptr;  // boundary records published per part
     int nparts3 = 0, pub3_stride = 0;
     size_t lds_parts3 = 0;
     RotPart* d_cparts0 = nullptr;    // centroid parts (slice ranges) of the linear family
@@ -1018,15 +1018,17 @@ int pimc_commit(pimc_ctx* c) {
             }
             rot3.swap(whole);
             const int nflags3 = (int)p3.size();
-            c->pub3_stride = 2 * nrmax * (int)(sizeof(Frame) / sizeof(double)) + 2 * nrmax;  // frames, then link densities
+            // two records per part, each 2 words per double of nrmax frames, then nrmax link densities
+            c->pub3_stride = 4 * nrmax * ((int)(sizeof(Frame) / sizeof(double)) + 1);
             p3 = xcd_order(p3, [&](int r) { return (double)c->rep[r].Prot * c->rep[r].nrot; });
             c->nparts3 = (int)p3.size();
             c->d_parts3 = dalloc<RotPart>(p3.size());
             c->d_flags3 = dalloc<unsigned>((size_t)nflags3);
-            c->d_pub3 = dalloc<double>((size_t)nflags3 * c->pub3_stride);
+            c->d_pub3 = dalloc<unsigned long long>((size_t)nflags3 * c->pub3_stride);
             if (!c->d_parts3 || !c->d_flags3 || !c->d_pub3) return fail(c, "hipMalloc failed");
             HIPCHECK(c, hipMemcpy(c->d_parts3, p3.data(), sizeof(RotPart) * p3.size(), hipMemcpyHostToDevice));
             HIPCHECK(c, hipMemset(c->d_flags3, 0, sizeof(unsigned) * nflags3));
+            HIPCHECK(c, hipMemset(c->d_pub3, 0, sizeof(unsigned long long) * nflags3 * c->pub3_stride));
         }
     }
     c->nwork = (int)work.size();

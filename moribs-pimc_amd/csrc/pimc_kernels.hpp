@@ -1014,10 +1014,13 @@ __global__ __launch_bounds__(BLOCK) void k_rotor3d(const Rep* __restrict__ reps,
 // slices with the moves of k_rotor3d.  Before its even half a part needs the frames of slice
 // k0-1 (the previous part's last, odd) and before its odd half those of slice k1 (the next
 // part's first, even).  After its even half a part publishes its first slice's frames, after its
-// odd half its last slice's, each with the density of its link to the neighbour (device-coherent
-// stores from wave 0, then the part's flag); the receiver takes both.  A published slot is
-// rewritten only after the neighbour has raised past the half that reads it.  Bounded waits set
-// error bit 2.
+// odd half its last slice's, each with the density of its link to the neighbour, as a record of
+// 64-bit words that each carry the publishing half's number above 32 bits of a double; the
+// receiver's lanes poll the words themselves, so the data arrives with its check, and write the
+// halves into LDS.  A record is rewritten only after the neighbour has read it (its next half
+// needs this part's).  Bounded waits set error bit 2.
+// Record (part, parity) at pub + part * pub_stride + parity * pub_stride / 2: 2 words per double
+// of the nrot frames, then of the nrot link densities.
 // LDS: Lk[q][kl], kl = 0..ns (link kl joins local slices kl and kl+1; local slice 0 is k0-1 and
 // ns+1 is k1), the neighbours' frames fnb[2][nrot] (k0-1, then k1), then the groups' frames.
 template <typename R>
@@ -1025,7 +1028,7 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
                                                        const double4* __restrict__ beads, RotBead* __restrict__ rotb,
                                                        Frame* __restrict__ frames, Tables T, Tables3 T3,
                                                        const RotPart* __restrict__ parts, unsigned* __restrict__ flags,
-                                                       double* __restrict__ pub, int pub_stride, Run run) {
+                                                       unsigned long long* __restrict__ pub, int pub_stride, Run run) {
     extern __shared__ double smem[];
     __shared__ unsigned long long red_acc[4];
     __shared__ int s_err;
@@ -1059,14 +1062,13 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
     double* Lk = smem;
     Frame* fnb = reinterpret_cast<Frame*>(smem + ((nrot * L + 1) & ~1));
     Frame* gf = fnb + 2 * nrot + (long long)g * (nrot + 1);
-    unsigned* myflag = flags + pt.fbase + pt.w;
-    unsigned* prevflag = flags + pt.fbase + (pt.w + W - 1) % W;
-    unsigned* nextflag = flags + pt.fbase + (pt.w + 1) % W;
+    unsigned* myflag = flags + pt.fbase + pt.w;  // half-sweeps done by the launches before this one
     constexpr int FD = (int)(sizeof(Frame) / sizeof(double));  // doubles per frame
-    // published frames: per part, the first slice's nrot frames, then the last slice's
-    double* mypub = pub + (long long)(pt.fbase + pt.w) * pub_stride;
-    const double* prevlast = pub + (long long)(pt.fbase + (pt.w + W - 1) % W) * pub_stride + nrot * FD;
-    const double* nextfirst = pub + (long long)(pt.fbase + (pt.w + 1) % W) * pub_stride;
+    const int nwords = 2 * nrot * (FD + 1);                      // words of one record
+    const int half_rec = pub_stride / 2;
+    unsigned long long* mypub = pub + (long long)(pt.fbase + pt.w) * pub_stride;   // + parity * half_rec
+    const unsigned long long* prevlast = pub + (long long)(pt.fbase + (pt.w + W - 1) % W) * pub_stride + half_rec;
+    const unsigned long long* nextfirst = pub + (long long)(pt.fbase + (pt.w + 1) % W) * pub_stride;
     const unsigned base = __hip_atomic_load(myflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t pass = mut[r].pass;
     const int rtype = rp.imtype;
@@ -1097,17 +1099,29 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
         for (int parity = 0; parity < 2; ++parity) {
             const unsigned h = 2u * (unsigned)sw + (unsigned)parity;  // half-sweeps done before this one
             if (!(sw == 0 && parity == 0)) {
-                // even half: the previous part's odd half of the last sweep; odd half: the next
-                // part's even half of this sweep
-                if (threadIdx.x == 0 && !wait_count(parity == 0 ? prevflag : nextflag, base + h)) s_err = 2;
-                __syncthreads();
-                // the neighbour's boundary frames, and the densities of the shared links that it
-                // evaluated from the same frames in the same order
-                const double* src = parity == 0 ? prevlast : nextfirst;
-                double* dst = reinterpret_cast<double*>(fnb + (parity == 0 ? 0 : nrot));
-                for (int i = threadIdx.x; i < nrot * FD; i += blockDim.x) dst[i] = load_agent(src + i);
-                const double* lsrc = parity == 0 ? prevlast + nrot * FD + nrot : nextfirst + 2 * nrot * FD;
-                for (int q = threadIdx.x; q < nrot; q += blockDim.x) Lk[q * L + (parity == 0 ? 0 : ns)] = load_agent(lsrc + q);
+                // even half: the previous part's record of its odd half of the last sweep; odd
+                // half: the next part's record of its even half of this sweep.  The neighbour's
+                // boundary frames, and the densities of the shared links that it evaluated from
+                // the same frames in the same order; each lane polls its own words.
+                const unsigned long long* src = parity == 0 ? prevlast : nextfirst;
+                unsigned* dstf = reinterpret_cast<unsigned*>(fnb + (parity == 0 ? 0 : nrot));
+                const unsigned want = base + h;
+                for (int w = threadIdx.x; w < nwords; w += blockDim.x) {
+                    unsigned long long wv;
+                    unsigned spins = 0;
+                    for (;;) {
+                        wv = __hip_atomic_load(src + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if ((unsigned)(wv >> 32) == want) break;
+                        if (++spins > ROT_SPIN_LIMIT) {
+                            s_err = 2;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                    const int i = w >> 1;  // the double: frames first, then link densities
+                    if (i < nrot * FD) dstf[w] = (unsigned)wv;
+                    else reinterpret_cast<unsigned*>(&Lk[(i - nrot * FD) * L + (parity == 0 ? 0 : ns)])[w & 1] = (unsigned)wv;
+                }
                 __syncthreads();
             }
             const int nmoves = (ns - parity + 1) / 2;  // k0 is even: local slice kl has parity (kl - 1) & 1
@@ -1186,19 +1200,26 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
                 }
             }
             __syncthreads();
-            // publish the boundary slice this half moved, then raise the flag
-            if (threadIdx.x < 64) {
+            // publish the boundary slice this half moved: its frames and the boundary link
+            // densities (Lk[q][0] after the even half, Lk[q][ns] after the odd), two words per
+            // double numbered base + h + 1; no wait for the stores, the reader checks every word
+            {
                 const int ks = parity == 0 ? k0 : k0 + ns - 1;
-                double* dst = mypub + (parity == 0 ? 0 : nrot * FD);
-                for (int i = threadIdx.x; i < nrot * FD; i += 64)
-                    store_agent(dst + i, reinterpret_cast<const double*>(&own(i / FD, ks))[i % FD]);
-                // the boundary link densities: Lk[q][0] after the even half, Lk[q][ns] after the odd
-                for (int q = threadIdx.x; q < nrot; q += 64)
-                    store_agent(mypub + 2 * nrot * FD + (parity == 0 ? 0 : nrot) + q, Lk[q * L + (parity == 0 ? 0 : ns)]);
-                if (threadIdx.x == 0) raise_count(myflag, base + h + 1u);
+                unsigned long long* dst = mypub + (parity == 0 ? 0 : half_rec);
+                const unsigned long long tag = (unsigned long long)(base + h + 1u) << 32;
+                for (int w = threadIdx.x; w < nwords; w += blockDim.x) {
+                    const int i = w >> 1;
+                    const double v = i < nrot * FD ? reinterpret_cast<const double*>(&own(i / FD, ks))[i % FD]
+                                                   : Lk[(i - nrot * FD) * L + (parity == 0 ? 0 : ns)];
+                    const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+                    __hip_atomic_store(dst + w, tag | ((w & 1) ? (bits >> 32) : (bits & 0xffffffffull)), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
         }
     }
+    if (threadIdx.x == 0)  // every part of the chain ran the same halves
+        __hip_atomic_store(myflag, base + 2u * (unsigned)nsweep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (err) atomicOr(&mut[r].err, 4);
     unsigned long long sacc = block_sum(nacc, red_acc);
     if (threadIdx.x == 0) {
