@@ -199,7 +199,7 @@ struct pimc_ctx {
     RotPart* d_cparts1 = nullptr;    // and of the non-linear family
     int ncp0 = 0, ncp1 = 0, ncflags = 0;
     unsigned* d_cflags = nullptr;    // atoms done per centroid part
-    double* d_cslots = nullptr;      // partial Delta V per part, two buffers
+    unsigned long long* d_cslots = nullptr;  // partial Delta V per part as two numbered words, two buffers
     // surrogate network of the delayed-acceptance rotor move
     bool opt_da = false;
     MlpPacked* d_mlp = nullptr;     // the sampler's packed copy
@@ -734,9 +734,10 @@ static int plan_centroid_parts(pimc_ctx* c, int target_parts) {
     c->d_cparts0 = dalloc<RotPart>(cp[0].size());
     c->d_cparts1 = dalloc<RotPart>(cp[1].size());
     c->d_cflags = dalloc<unsigned>((size_t)std::max(fb, 1));
-    c->d_cslots = dalloc<double>(2 * (size_t)std::max(fb, 1));
+    c->d_cslots = dalloc<unsigned long long>(4 * (size_t)std::max(fb, 1));
     if (!c->d_cparts0 || !c->d_cparts1 || !c->d_cflags || !c->d_cslots) return fail(c, "hipMalloc failed");
     HIPCHECK(c, hipMemset(c->d_cflags, 0, sizeof(unsigned) * std::max(fb, 1)));
+    HIPCHECK(c, hipMemset(c->d_cslots, 0, sizeof(unsigned long long) * 4 * std::max(fb, 1)));  // no word matches a wanted number
     if (!cp[0].empty())
         HIPCHECK(c, hipMemcpy(c->d_cparts0, cp[0].data(), sizeof(RotPart) * cp[0].size(), hipMemcpyHostToDevice));
     if (!cp[1].empty())

@@ -404,18 +404,19 @@ __device__ __forceinline__ void store_agent(double* p, double x) {
 // ------------------------------------------------------------------------- centroid, split
 // MCMolecularMove (mc_piqmc.cc:54-102) with a replica's slices split over W workgroups
 // (parts [k0, k1) of 0..P-1, RotPart reused): atoms in turn, each part sums Delta V over its
-// slices and all partners, publishes the partial, raises its flag, waits for the other parts'
-// flags (bounded spin), and every part then adds the W partials in part order -- the same
-// sum, so the same decision -- and moves its own slices of the atom.  Partner positions at a
-// part's slices change only inside that part, so no bead data crosses parts.  Partials are
-// double-buffered by atom parity: a part can overwrite a buffer only after every part has
-// raised for the following atom, that is after every part has read it.
+// slices and all partners and publishes the partial as two 64-bit words that carry the atom's
+// number (base + ia + 1) above 32 bits of the double; lanes 0..2W-1 poll every part's words
+// (bounded spin), and every part then adds the W partials in part order -- the same sum, so the
+// same decision -- and moves its own slices of the atom.  Partner positions at a part's slices
+// change only inside that part, so no bead data crosses parts.  Partials are double-buffered by
+// atom parity: a part can overwrite a buffer only after every part has published the following
+// atom, that is after every part has read it.
 template <typename R, int FAM>
 __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ reps, Mut* __restrict__ mut,
                                                         double4* __restrict__ beads, const RotBead* __restrict__ rotb,
                                                         const Frame* __restrict__ frames, Tables T, Tables3 T3,
                                                         const RotPart* __restrict__ parts, unsigned* __restrict__ flags,
-                                                        double* __restrict__ slots, int nslots, Run run) {
+                                                        unsigned long long* __restrict__ slots, int nslots, Run run) {
     extern __shared__ double smem[];  // the 1D spline knots
     __shared__ double red[4];
     __shared__ double part_sum[64];
@@ -507,17 +508,30 @@ __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ 
             }
         }
         const double partial = block_sum(acc, red);
-        double* buf = slots + (size_t)(ia & 1) * nslots + pt.fbase;
+        unsigned long long* buf = slots + ((size_t)(ia & 1) * nslots + pt.fbase) * 2;
         if (W == 1) {
             if (threadIdx.x == 0) s_dv = partial;
         } else {
-            if (threadIdx.x == 0) {
-                store_agent(buf + pt.w, partial);
-                raise_count(myflag, base + (unsigned)ia + 1u);
+            const unsigned tag = base + (unsigned)ia + 1u;
+            if (threadIdx.x < 2) {
+                const unsigned long long bits = (unsigned long long)__double_as_longlong(partial);
+                __hip_atomic_store(buf + 2 * pt.w + threadIdx.x,
+                                   ((unsigned long long)tag << 32) | (threadIdx.x ? bits >> 32 : bits & 0xffffffffull),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            if (threadIdx.x < W) {
-                if (!wait_count(flags + pt.fbase + threadIdx.x, base + (unsigned)ia + 1u)) s_err = 8;
-                part_sum[threadIdx.x] = load_agent(buf + threadIdx.x);
+            if (threadIdx.x < 2 * W) {  // word 2q + 1 is the high half of part q's partial
+                unsigned long long wv;
+                unsigned spins = 0;
+                for (;;) {
+                    wv = __hip_atomic_load(buf + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((unsigned)(wv >> 32) == tag) break;
+                    if (++spins > ROT_SPIN_LIMIT) {
+                        s_err = 8;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                reinterpret_cast<unsigned*>(part_sum)[threadIdx.x] = (unsigned)wv;
             }
             __syncthreads();
             if (threadIdx.x == 0) {
@@ -542,6 +556,8 @@ __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ 
         __syncthreads();
     }
     if (threadIdx.x == 0) {
+        if (W > 1)  // every part of the replica ran the same atoms
+            __hip_atomic_store(myflag, base + (unsigned)rp.tnumb[ty], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (s_err) atomicOr(&mut[r].err, s_err);
         if (pt.w == 0) {
             atomicAdd(&mut[r].total[ty][0], (unsigned long long)rp.tnumb[ty]);
