@@ -329,11 +329,20 @@ __device__ __forceinline__ float spot1d_u2(const Lut1D<float>& L, const float2* 
     const float2 a = k2[i], b = k2[i + 1];
     const float h = fast_div(1.0f, L.inv_h);
     const float B = t - f, A = 1.0f - B;
-    const float v = A * a.x + B * b.x + ((A * A * A - A) * a.y + (B * B * B - B) * b.y) * (h * h) * (1.0f / 6.0f);
-    const float r2 = r * r;
-    const float above = -fast_div(L.c6, r2 * r2 * r2);
-    const float below = L.unode * __expf(-L.alpha * r);
-    return r >= L.xmax ? above : (r <= L.x0 ? below : v);
+    float v = A * a.x + B * b.x + ((A * A * A - A) * a.y + (B * B * B - B) * b.y) * (h * h) * (1.0f / 6.0f);
+    // the tails in branches that a wave skips unless one of its lanes needs them (a wave-uniform
+    // test: a masked region this short would still be issued); the repulsive wall's exponential
+    // is a quarter-rate instruction and rarely needed
+    if (__any(r <= L.x0)) {
+        float rw = r;
+        asm volatile("" : "+v"(rw));  // keeps the exponential inside the branch (no instruction)
+        if (rw <= L.x0) v = L.unode * __expf(-L.alpha * rw);
+    }
+    if (r >= L.xmax) {
+        const float r2 = r * r;
+        v = -fast_div(L.c6, r2 * r2 * r2);
+    }
+    return v;
 }
 
 // LPot2D on a uniform FP32 grid in two steps, so that the loads of many look-ups can be in
