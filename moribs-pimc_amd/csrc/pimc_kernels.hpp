@@ -119,13 +119,15 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
     const int W = S + 1;
     using P4 = vec4_t<R>;
     // LDS: rotor axes, the moving atom's new positions and its draws (FP64), then the window of
-    // all atoms' positions in the pair precision (x, y, z, -), rotor frames (FAM 1) and the 1D
-    // spline knots: (y, y'') pairs of uniform FP32 knots (linear family), vec4 knots otherwise
+    // all atoms' positions in the pair precision (x, y, z, -), the new midpoints again in the pair
+    // precision (one read per item), rotor frames (FAM 1) and the 1D spline knots: (y, y'') pairs
+    // of uniform FP32 knots (linear family), vec4 knots otherwise
     double* nh = smem;
     double* nw = nh + 3 * W;
     double* gw = nw + 3 * W;  // draws of a batch of atoms: per atom 3 per midpoint, then one per level
     P4* win = reinterpret_cast<P4*>(smem + ((6 * W + bisect_ring(W) + 3) & ~3));
-    Frame* fr = reinterpret_cast<Frame*>(win + N * W);
+    P4* nwf = win + N * W;
+    Frame* fr = reinterpret_cast<Frame*>(nwf + W);
     const int nfr = FAM == 1 ? rp.nrot * W : 0;
     P4* kn = reinterpret_cast<P4*>(fr + nfr);
     const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
@@ -230,8 +232,10 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                 for (int i = tid; i < 3 * nmid; i += nth) {
                     int q = i / 3, id = i - 3 * q;
                     int slot = half + q * lss;
-                    nw[3 * slot + id] = 0.5 * (nw[3 * (slot - half) + id] + nw[3 * (slot + half) + id]) +
-                                        g[3 * (nmid - 1 + q) + id];
+                    const double v = 0.5 * (nw[3 * (slot - half) + id] + nw[3 * (slot + half) + id]) +
+                                     g[3 * (nmid - 1 + q) + id];
+                    nw[3 * slot + id] = v;
+                    reinterpret_cast<R*>(nwf + slot)[id] = (R)v;
                 }
                 __syncthreads();
                 // Delta V at the new midpoints: items (midpoint, partner)
@@ -274,7 +278,8 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                         }
                         const int slot = half + q * lss;
                         const P4 pj = win[j * W + slot], pa = win[a * W + slot];
-                        const R xn = (R)nw[3 * slot], yn = (R)nw[3 * slot + 1], zn = (R)nw[3 * slot + 2];
+                        const P4 pn = nwf[slot];
+                        const R xn = pn.x, yn = pn.y, zn = pn.z;
                         if constexpr (FAM == 1) {
                             const Frame& fa = fr[(rot_a ? a - rp.nl_atom : 0) * W + slot];
                             const Frame& fj = fr[(rj ? j - rp.nl_atom : 0) * W + slot];
