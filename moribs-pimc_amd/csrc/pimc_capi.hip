@@ -897,10 +897,10 @@ int pimc_commit(pimc_ctx* c) {
             const Tab1D& t = c->t1[p.pot1d_id];
             knots = (size_t)t.n * (c->fp64 ? sizeof(double4) : (p.fam == 0 && t.uniform ? sizeof(float2) : sizeof(float4)));
         }
-        // bisection LDS: axes, new positions and the draw ring (FP64), the window and the new
-        // midpoints in the pair precision, frames, knots (the layout of bisect_body)
-        const size_t lds_w = (size_t)((6 * W + bisect_ring(W) + 3) & ~3) * sizeof(double) +
-                             (size_t)(p.natoms + 1) * W * (c->fp64 ? sizeof(double4) : sizeof(float4)) +
+        // bisection LDS: new positions and the draw ring (FP64), the window, the new midpoints
+        // and the rotor axes in the pair precision, frames, knots (the layout of bisect_body)
+        const size_t lds_w = (size_t)((3 * W + bisect_ring(W) + 3) & ~3) * sizeof(double) +
+                             (size_t)(p.natoms + 2) * W * (c->fp64 ? sizeof(double4) : sizeof(float4)) +
                              (size_t)12 * p.nrot * W * sizeof(double) + knots;
         (p.fam ? lds_b1 : lds_b) = std::max(p.fam ? lds_b1 : lds_b, lds_w);
         if (!p.fam) {

@@ -118,16 +118,16 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
     }
     const int W = S + 1;
     using P4 = vec4_t<R>;
-    // LDS: rotor axes, the moving atom's new positions and its draws (FP64), then the window of
-    // all atoms' positions in the pair precision (x, y, z, -), the new midpoints again in the pair
-    // precision (one read per item), rotor frames (FAM 1) and the 1D spline knots: (y, y'') pairs
-    // of uniform FP32 knots (linear family), vec4 knots otherwise
-    double* nh = smem;
-    double* nw = nh + 3 * W;
+    // LDS: the moving atom's new positions and its draws (FP64), then in the pair precision the
+    // window of all atoms' positions (x, y, z, -), the new midpoints again (one read per item)
+    // and the rotor axes at the window's slices, then rotor frames (FAM 1) and the 1D spline
+    // knots: (y, y'') pairs of uniform FP32 knots (linear family), vec4 knots otherwise
+    double* nw = smem;
     double* gw = nw + 3 * W;  // draws of a batch of atoms: per atom 3 per midpoint, then one per level
-    P4* win = reinterpret_cast<P4*>(smem + ((6 * W + bisect_ring(W) + 3) & ~3));
+    P4* win = reinterpret_cast<P4*>(smem + ((3 * W + bisect_ring(W) + 3) & ~3));
     P4* nwf = win + N * W;
-    Frame* fr = reinterpret_cast<Frame*>(nwf + W);
+    P4* nhf = nwf + W;
+    Frame* fr = reinterpret_cast<Frame*>(nhf + W);
     const int nfr = FAM == 1 ? rp.nrot * W : 0;
     P4* kn = reinterpret_cast<P4*>(fr + nfr);
     const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
@@ -158,9 +158,12 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
     for (int w = tid; w < W; w += nth) {
         double x, y, z;
         rotor_axis(rotb, rp, (s0 + w) % P, x, y, z);
-        nh[3 * w] = x;
-        nh[3 * w + 1] = y;
-        nh[3 * w + 2] = z;
+        P4 q;
+        q.x = (R)x;  // the pair functions take the axis in the pair precision
+        q.y = (R)y;
+        q.z = (R)z;
+        q.w = (R)0;
+        nhf[w] = q;
     }
     __syncthreads();
 
@@ -287,7 +290,8 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                                    pair_nl<R>(l1, kn, l3, rot_a, rj, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, fa, fj);
                         } else if (split) {
                             if (rj) {
-                                const double nx = nh[3 * slot], ny = nh[3 * slot + 1], nz = nh[3 * slot + 2];
+                                const P4 ax = nhf[slot];
+                                const double nx = ax.x, ny = ax.y, nz = ax.z;
                                 acc += (double)pair_vr<R, H>(l1, kn, l2, false, true, xn, yn, zn, pj.x, pj.y, pj.z, nx, ny, nz) -
                                        (double)pair_vr<R, H>(l1, kn, l2, false, true, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, nx, ny, nz);
                             } else {
@@ -295,7 +299,8 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                                        (double)pair_vr<R, H>(l1, kn, l2, false, false, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, 0.0, 0.0, 1.0);
                             }
                         } else {
-                            const double nx = nh[3 * slot], ny = nh[3 * slot + 1], nz = nh[3 * slot + 2];
+                            const P4 ax = nhf[slot];
+                            const double nx = ax.x, ny = ax.y, nz = ax.z;
                             acc += (double)pair_vr<R, H>(l1, kn, l2, rot_a, rj, xn, yn, zn, pj.x, pj.y, pj.z, nx, ny, nz) -
                                    (double)pair_vr<R, H>(l1, kn, l2, rot_a, rj, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, nx, ny, nz);
                         }
