@@ -270,20 +270,38 @@ __device__ __forceinline__ double fast_div(double x, double y) { return x / y; }
 // uniform knots takes the cell from the grid arithmetic alone (at a cell boundary the two
 // cells' cubics agree to rounding) and selects between the spline and the tails without
 // branches, so that independent look-ups interleave.
+// The tails of the uniform FP32 spline (mc_poten.cc:629-634): U0 e^{-alpha r} at or below the
+// first knot, -C6/r^6 at or beyond the last, in branches that a wave skips unless one of its
+// lanes needs them.  The tests are wave-uniform because a masked region this short would still
+// be issued, and the empty asm keeps the compiler from computing the tails ahead of the test.
+__device__ __forceinline__ float spot1d_tails(const Lut1D<float>& L, float r, float v) {
+    if (__any(r <= L.x0)) {
+        float rw = r;
+        asm volatile("" : "+v"(rw));
+        if (rw <= L.x0) v = L.unode * __expf(-L.alpha * rw);
+    }
+    if (__any(r >= L.xmax)) {
+        float rw = r;
+        asm volatile("" : "+v"(rw));
+        if (rw >= L.xmax) {
+            const float r2 = rw * rw;
+            v = -fast_div(L.c6, r2 * r2 * r2);
+        }
+    }
+    return v;
+}
+
 template <typename R>
 __device__ __forceinline__ R spot1d(const Lut1D<R>& L, const vec4_t<R>* __restrict__ k, R r) {
     if constexpr (is_fp32<R>()) {
-        if (L.uniform) {  // without branches: both tails are computed and the range selects
+        if (L.uniform) {  // the table without branches, the tails behind wave-uniform tests
             float f = floor((r - L.x0) * L.inv_h);
             f = fmin(fmax(f, 0.f), (float)(L.n - 2));
             const int i = (int)f;
             const float4 a = k[i], b = k[i + 1];
             const float A = (b.x - r) * a.w, B = (r - a.x) * a.w, h = b.x - a.x;
             const float v = A * a.y + B * b.y + ((A * A * A - A) * a.z + (B * B * B - B) * b.z) * (h * h) * (1.0f / 6.0f);
-            const float r2 = r * r;
-            const float above = -fast_div(L.c6, r2 * r2 * r2);
-            const float below = L.unode * __expf(-L.alpha * r);
-            return r >= L.xmax ? above : (r <= L.x0 ? below : v);
+            return spot1d_tails(L, r, v);
         }
     }
     if (r >= L.xmax) {
@@ -329,20 +347,8 @@ __device__ __forceinline__ float spot1d_u2(const Lut1D<float>& L, const float2* 
     const float2 a = k2[i], b = k2[i + 1];
     const float h = fast_div(1.0f, L.inv_h);
     const float B = t - f, A = 1.0f - B;
-    float v = A * a.x + B * b.x + ((A * A * A - A) * a.y + (B * B * B - B) * b.y) * (h * h) * (1.0f / 6.0f);
-    // the tails in branches that a wave skips unless one of its lanes needs them (a wave-uniform
-    // test: a masked region this short would still be issued); the repulsive wall's exponential
-    // is a quarter-rate instruction and rarely needed
-    if (__any(r <= L.x0)) {
-        float rw = r;
-        asm volatile("" : "+v"(rw));  // keeps the exponential inside the branch (no instruction)
-        if (rw <= L.x0) v = L.unode * __expf(-L.alpha * rw);
-    }
-    if (r >= L.xmax) {
-        const float r2 = r * r;
-        v = -fast_div(L.c6, r2 * r2 * r2);
-    }
-    return v;
+    const float v = A * a.x + B * b.x + ((A * A * A - A) * a.y + (B * B * B - B) * b.y) * (h * h) * (1.0f / 6.0f);
+    return spot1d_tails(L, r, v);
 }
 
 // LPot2D on a uniform FP32 grid in two steps, so that the loads of many look-ups can be in
