@@ -6,18 +6,22 @@
 //                moved in turn, as MCBisectionMove does (mc_piqmc.cc:194-294).  Segments of
 //                one offset share only fixed end points, so they run concurrently.
 //   k_centroid_parts  a replica's slices split over workgroups; atoms in turn, Delta V over the
-//                part's slices x partners, partials exchanged through flags in memory
+//                part's slices x partners, partials exchanged as numbered words in memory
 //                (MCMolecularMove mc_piqmc.cc:54-102).
 //   k_rotor      one workgroup per part of a linear rotor's chain (a range of slices): the part's
 //                orientations and its rotor-partner geometry live in LDS for all sweeps of a
 //                launch; each half of the even/odd checkerboard (mc_piqmc.cc:490-512) is a
 //                parallel step over slices, a lane group per slice sums Delta V over partners
-//                (MCRotLinStep :781-936); parts trade one boundary slice per half-sweep.
+//                (MCRotLinStep :781-936); parts trade one boundary slice per half-sweep as a
+//                record of numbered words (each 64-bit word: the publishing half's number over
+//                32 bits of a double), so a poll of the record returns the data with its check.
 //   k_est_tiles  one workgroup per (replica, tile of slices): the tile's part of the pair sum of
-//                GetPotEnergy_Densities (mc_estim.cc:500-687, every pair, with the g(r) /
-//                g(r,theta) bins) and of GetKinEnergy (mc_estim.cc:876-937), beads read once.
+//                GetPotEnergy_Densities (mc_estim.cc:500-687, every pair, with the g(r),
+//                g(r,theta) and (r,theta,chi) bins) and of GetKinEnergy (mc_estim.cc:876-937),
+//                beads read once.
 //   k_est_final  one workgroup per replica: tiles summed in order (deterministic), GetRotEnergy
-//                (mc_estim.cc:939-987) and the Cv terms of MCGetAverage (mc_main.cc:551-616).
+//                (mc_estim.cc:939-987) or GetRotE3D with its relative Euler angle bins
+//                (:989-1096), and the Cv terms of MCGetAverage (mc_main.cc:551-616).
 //   k_pot_slice  batched PotEnergy(atom, pos, it) (mc_piqmc.cc:1796-1965).
 //   k_rotor3d_da the non-linear rotor sweep with a surrogate first stage (delayed acceptance).
 #pragma once
@@ -378,38 +382,9 @@ struct RotPart {
     int r, k0, k1, w, W, fbase, pad0, pad1;
 };
 
+// Polls of a numbered word before a wait gives up and sets the part's error bit (each poll
+// sleeps between loads, so this bounds a wait at seconds, far beyond any normal exchange)
 constexpr unsigned ROT_SPIN_LIMIT = 1u << 22;
-
-// spin until *f >= target (device-coherent loads; the loads that follow are issued after the
-// flag value has returned, and read device-coherent memory themselves)
-__device__ __forceinline__ bool wait_count(unsigned* f, unsigned target) {
-    unsigned spins = 0;
-    while ((int)(__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
-        if (++spins > ROT_SPIN_LIMIT) return false;
-        __builtin_amdgcn_s_sleep(2);
-    }
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    return true;
-}
-
-// store the flag after every earlier store of this lane has completed
-__device__ __forceinline__ void raise_count(unsigned* f, unsigned v) {
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ double load_agent(const double* p) {
-    unsigned long long v =
-        __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)v);
-}
-
-__device__ __forceinline__ void store_agent(double* p, double x) {
-    __hip_atomic_store((unsigned long long*)p, (unsigned long long)__double_as_longlong(x), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // ------------------------------------------------------------------------- centroid, split
 // MCMolecularMove (mc_piqmc.cc:54-102) with a replica's slices split over W workgroups
