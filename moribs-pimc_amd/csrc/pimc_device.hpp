@@ -551,11 +551,46 @@ __device__ __forceinline__ R pair_vr(const Lut1D<R>& l1, const void* __restrict_
     return pair_d<R, HALF>(l1, k1, l2, rot_a, rot_j, xa - xj, ya - yj, za - zj, nx, ny, nz);
 }
 
+// One DPP lane move within rows of 16 lanes (ctrl: quad_perm 0x00-0xff, row_mirror 0x140,
+// row_half_mirror 0x141) of a 32- or 64-bit value
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_f(T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+    T r;
+    if constexpr (sizeof(T) == 4) {
+        int b;
+        __builtin_memcpy(&b, &v, 4);
+        b = __builtin_amdgcn_update_dpp(0, b, CTRL, 0xf, 0xf, true);
+        __builtin_memcpy(&r, &b, 4);
+    } else {
+        long long b;
+        __builtin_memcpy(&b, &v, 8);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xf, 0xf, true);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, true);
+        const long long c = ((long long)hi << 32) | (long long)(unsigned)lo;
+        __builtin_memcpy(&r, &c, 8);
+    }
+    return r;
+}
+
+// Sum over aligned groups of G lanes (G a power of two up to 64, all lanes of a group active);
+// every lane of a group gets the same sum.  Within a row of 16 the butterfly runs on DPP moves
+// (xor 1, xor 2 by quad permutes; then the half-row and row mirrors, which pair the quads and the
+// half-rows once each holds one value), across rows on lane shuffles.
+template <typename T>
+__device__ __forceinline__ T group_sum(T v, int G) {
+    if (G >= 2) v += dpp_f<0xb1>(v);   // quad_perm [1,0,3,2]
+    if (G >= 4) v += dpp_f<0x4e>(v);   // quad_perm [2,3,0,1]
+    if (G >= 8) v += dpp_f<0x141>(v);  // row_half_mirror
+    if (G >= 16) v += dpp_f<0x140>(v); // row_mirror
+    if (G >= 32) v += __shfl_xor(v, 16, 64);
+    if (G >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    return group_sum(v, 64);
 }
 
 }  // namespace pimc

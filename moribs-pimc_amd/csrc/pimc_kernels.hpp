@@ -579,7 +579,7 @@ __device__ __forceinline__ double slice_energy(const Lut2D<R>& l2, const vec4_t<
                 for (int b = 0; b < B; ++b)
                     if (base + b * G < items) acc += (double)lpot2d_u_lerp(q[b].x, q[b].y, q[b].z, q[b].w, wr[b], wc[b]);
             }
-            for (int o = G >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
+            acc = group_sum(acc, G);
             return acc;
         }
     }
@@ -600,7 +600,7 @@ __device__ __forceinline__ double slice_energy(const Lut2D<R>& l2, const vec4_t<
             if (base + b * G < items) acc += (double)v;
         }
     }
-    for (int o = G >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
+    acc = group_sum(acc, G);
     return acc;
 }
 
@@ -964,7 +964,7 @@ __global__ __launch_bounds__(BLOCK) void k_rotor3d(const Rep* __restrict__ reps,
                         eo += pair_nl<R>(l1, l1.k, l3, true, rj, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, gf[q], fj);
                         en += pair_nl<R>(l1, l1.k, l3, true, rj, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, gf[nrot], fj);
                     }
-                    for (int o = G >> 1; o > 0; o >>= 1) {
+                    for (int o = G >> 1; o > 0; o >>= 1) {  // the two sums' shuffles interleave
                         eo += __shfl_xor(eo, o, G);
                         en += __shfl_xor(en, o, G);
                     }
@@ -1168,7 +1168,7 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
                         eo += pair_nl<R>(l1, l1.k, l3, true, rj, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, gf[q], fj);
                         en += pair_nl<R>(l1, l1.k, l3, true, rj, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, gf[nrot], fj);
                     }
-                    for (int o = G >> 1; o > 0; o >>= 1) {
+                    for (int o = G >> 1; o > 0; o >>= 1) {  // the two sums' shuffles interleave
                         eo += __shfl_xor(eo, o, G);
                         en += __shfl_xor(en, o, G);
                     }
