@@ -573,18 +573,39 @@ __device__ __forceinline__ T dpp_f(T v) {
     return r;
 }
 
+// Lane l's copy of a 32- or 64-bit value, wave-uniform (v_readlane)
+template <typename T>
+__device__ __forceinline__ T read_lane(T v, int l) {
+    T r;
+    if constexpr (sizeof(T) == 4) {
+        int b;
+        __builtin_memcpy(&b, &v, 4);
+        b = __builtin_amdgcn_readlane(b, l);
+        __builtin_memcpy(&r, &b, 4);
+    } else {
+        long long b;
+        __builtin_memcpy(&b, &v, 8);
+        const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+        const long long c = ((long long)hi << 32) | (long long)(unsigned)lo;
+        __builtin_memcpy(&r, &c, 8);
+    }
+    return r;
+}
+
 // Sum over aligned groups of G lanes (G a power of two up to 64, all lanes of a group active);
 // every lane of a group gets the same sum.  Within a row of 16 the butterfly runs on DPP moves
 // (xor 1, xor 2 by quad permutes; then the half-row and row mirrors, which pair the quads and the
-// half-rows once each holds one value), across rows on lane shuffles.
+// half-rows once each holds one value), across rows on lane shuffles, or for the whole wave as
+// the four row sums read into scalar registers and added as the shuffles would add them.
 template <typename T>
 __device__ __forceinline__ T group_sum(T v, int G) {
     if (G >= 2) v += dpp_f<0xb1>(v);   // quad_perm [1,0,3,2]
     if (G >= 4) v += dpp_f<0x4e>(v);   // quad_perm [2,3,0,1]
     if (G >= 8) v += dpp_f<0x141>(v);  // row_half_mirror
     if (G >= 16) v += dpp_f<0x140>(v); // row_mirror
+    if (G == 64) return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
     if (G >= 32) v += __shfl_xor(v, 16, 64);
-    if (G >= 64) v += __shfl_xor(v, 32, 64);
     return v;
 }
 
