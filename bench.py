@@ -27,9 +27,13 @@ import pimc_tables as pt  # noqa: E402
 
 METRIC = "bead-moves/sec & pair-evals/sec on 1 MI355X; surrogate-DA vs plain at equal σ_E"
 FP32_PEAK_TFLOPS = 157.3
-# algorithmic FP32 FLOP per pair evaluation: displacement, distance and interpolation
+# LDS: 128 bytes per clock per CU, 256 CUs at 2.4 GHz
+LDS_PEAK_GBS = 128.0 * 256 * 2.4
+# algorithmic FP32 FLOP and LDS operand bytes per pair evaluation
 FLOP_1D = 28.0   # clamped cubic spline on the 1D pair table
 FLOP_2D = 36.0   # bilinear (r, cos theta) on the rotor table
+BYTES_1D = 28.0  # partner xyz (12 B) and the cell's y, y'' pairs (16 B)
+BYTES_2D = 40.0  # partner xyz (12 B), rotor axis (12 B), four cell corners (16 B)
 DESC = {
     "c2": "256 replicas, N~U{1..100} He + 1 linear rotor (B=0.39 cm^-1), P in {64,128,256,512}, T~U(0.3,2) K",
     "c1": "64 He + 1 linear rotor, P=Prot=512, T=0.37 K",
@@ -43,15 +47,26 @@ VCALC_BYTES = 4 * 4     # vcalc: corner and three forward neighbours of V (FP32)
 HBM_PEAK_GBS = 8000.0
 
 
-def flop_mix(wl):
-    """Mean FLOP per pair evaluation of the bisection kernel's pair mix, per replica
-    weighted by its pair count: He movers see N-1 He (1D) + 1 rotor (2D), the rotor N He (2D)."""
+def pair_mix(wl, per_1d, per_2d):
+    """Mean of per_1d / per_2d over the bisection kernel's pair mix, per replica weighted by its
+    pair count: He movers see N-1 He (1D) + 1 rotor (2D), the rotor N He (2D)."""
     n1 = n2 = 0.0
     for rp in wl["replicas"]:
         n = rp["numb"][0]
         n1 += n * (n - 1) * rp["P"]
         n2 += 2 * n * rp["P"]
-    return (n1 * FLOP_1D + n2 * FLOP_2D) / (n1 + n2)
+    return (n1 * per_1d + n2 * per_2d) / (n1 + n2)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 # ----------------------------------------------------------------------------- CPU baseline
@@ -129,7 +144,7 @@ def cpu_baseline(wl, budget_s=12.0):
         per_pass = (t2 - t1) / (n2 - n1)
         bm_sum += pc.bead_moves_per_pass(rp)
         t_sum += max(per_pass, 1e-9)
-    return dict(value=cores * bm_sum / t_sum, unit="bead-moves/s", cores=cores, kind="reference",
+    return dict(value=cores * bm_sum / t_sum, unit="bead-moves/s", cores=cores, kind="reference", cpu=cpu_model(),
                 sample=f"first {len(reps)} replicas of the workload, upstream pimc (-Ofast, 1 thread each), "
                        f"{cores} concurrent processes; per-pass time from two pass counts")
 
@@ -165,7 +180,7 @@ def cpu_baseline_port(wl, budget_s=15.0):
         S.run(n, 1, order=O.ORDER_SEGMENTS)
         n += 1
     dt = time.perf_counter() - t0
-    return dict(value=n * pc.bead_moves_per_pass(rp) / dt, unit="bead-moves/s", cores=1, kind="port",
+    return dict(value=n * pc.bead_moves_per_pass(rp) / dt, unit="bead-moves/s", cores=1, kind="port", cpu=cpu_model(),
                 sample=f"{n} pass(es) of the first replica on the oracle (FP64, one host core)")
 
 
@@ -194,6 +209,13 @@ WORKLOAD_TAG = {"c2": "c2_256replicas", "c1": "c1_64he", "c0": "c0_8he", "c3": "
 
 
 # ----------------------------------------------------------------------------- GPU
+def device_asymrho():
+    """The density-matrix generator on the device (pimc_asymrho) for the non-linear workloads'
+    tables; the host restatement takes tens of seconds at maxj 44."""
+    gen = ph.Context(1)
+    return lambda T, P, A, B, C, maxj: gen.asymrho(T, P, A, B, C, maxj)
+
+
 def blocked_se(x):
     """Standard error of the mean of a correlated series: the plateau (largest value) of the
     blocking estimate over block lengths 1, 2, 4, ... with at least 16 blocks."""
@@ -216,7 +238,7 @@ def da_comparison(passes, rows_wanted=1_000_000, equil=100):
     def note(msg):
         print(f"[da] {msg}", file=sys.stderr, flush=True)
 
-    wl = pc.nl_workload("c3")
+    wl = pc.nl_workload("c3", asymrho=device_asymrho())
     bm_pass = float(pc.bead_moves_per_pass(wl["replicas"][0]))
     ctx = pc.load_into_context(ph.Context(1), wl)
     ctx.run(equil)
@@ -306,8 +328,8 @@ def main():
                           "config": {"workload": r["workload"]}, "surrogate_da": r}))
         return
     nonlinear = args.workload in ("c3", "c4")
-    wl = pc.nl_workload(args.workload) if nonlinear else pc.workload(args.workload)
-    ctx = ph.Context(len(wl["replicas"]), flags=ph.FLAG_PROFILE)
+    wl = pc.nl_workload(args.workload, asymrho=device_asymrho()) if nonlinear else pc.workload(args.workload)
+    ctx = ph.Context(len(wl["replicas"]))
     shapes = json.loads(args.shapes) if args.shapes else None
     for k, v in (shapes or {}).items():
         ctx.set_option(k, int(v))
@@ -315,44 +337,66 @@ def main():
     if shapes is None and not args.no_tune:
         shapes = ctx.autotune(passes=2)  # untimed, like the warmup
     bm_pass = float(sum(pc.bead_moves_per_pass(rp) for rp in wl["replicas"]))
-    ctx.run(args.warmup, sample_every=1)
+    ctx.run(args.warmup, sample_every=1)  # the first pass captures the pass graph
     ctx.synchronize()
-    fams = ["bisect", "centroid", "rotor", "estimate"]
-    efams = ["bisect", "centroid", "rotor", "estimate"]
-    k0 = {f: ctx.kernel_time(f) for f in fams}
-    e0 = {f: ctx.family_evals(f) for f in efams}
+    # timed region: the production path, each pass one replay of the captured graph
     pe0, _ = ctx.work()
     t0 = time.perf_counter()
     ctx.run(args.steps, sample_every=1)
     ctx.synchronize()
     t1 = time.perf_counter()
     pe1, _ = ctx.work()
-    k1 = {f: ctx.kernel_time(f) for f in fams}
-    e1 = {f: ctx.family_evals(f) for f in efams}
     dt = t1 - t0
+    # profiled region: the same passes again, launch by launch between HIP events on the launch
+    # stream, for the per-kernel split and the roofline of the dominant kernel
+    fams = ["bisect", "centroid", "rotor", "estimate"]
+    ctx.set_option("profile", 1)
+    k0 = {f: ctx.kernel_time(f) for f in fams}
+    e0 = {f: ctx.family_evals(f) for f in fams}
+    t2 = time.perf_counter()
+    ctx.run(args.steps, sample_every=1)
+    ctx.synchronize()
+    t3 = time.perf_counter()
+    k1 = {f: ctx.kernel_time(f) for f in fams}
+    e1 = {f: ctx.family_evals(f) for f in fams}
+    ctx.set_option("profile", 0)
     kt = {f: (k1[f][0] - k0[f][0], k1[f][1] - k0[f][1]) for f in fams}
-    ev = {f: e1[f] - e0[f] for f in efams}
-    dom = max(efams, key=lambda f: kt[f][0])
+    ev = {f: e1[f] - e0[f] for f in fams}
+    dom = max(fams, key=lambda f: kt[f][0])
     ms, nl = kt[dom]
+    launch_s = ms / max(nl, 1) * 1e-3
     if nonlinear and dom == "rotor":
         # density look-ups dominate: 4 rotden calls per move, four FP64 values of rho each, plus
         # vcalc's four FP32 values per rotor-partner evaluation
         moves = args.steps * sum(rp["P"] * rp["Prot"] * rp["numb"][-1] for rp in wl["replicas"])
         bytes_per_launch = (4.0 * moves * ROTDEN_BYTES + ev[dom] * VCALC_BYTES) / max(nl, 1)
-        achieved = bytes_per_launch / (ms / max(nl, 1) * 1e-3) / 1e9
+        achieved = bytes_per_launch / launch_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "note": "algorithmic gather bytes of rotden and vcalc"}
     else:
-        flop_per_eval = {"bisect": flop_mix(wl), "centroid": flop_mix(wl), "rotor": FLOP_2D,
-                         "estimate": flop_mix(wl)}[dom] if not nonlinear else FLOP_2D
-        flop_per_launch = ev[dom] * flop_per_eval / max(nl, 1)
-        achieved = flop_per_launch / (ms / max(nl, 1) * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-                "note": "FP32 vector peak; the path has no matrix contraction, FLOPs are algorithmic "
-                        f"({FLOP_1D:g}/1D and {FLOP_2D:g}/2D pair evaluation)"}
+        if nonlinear:
+            flop_per_eval, bytes_per_eval = FLOP_2D, BYTES_2D
+        elif dom == "rotor":
+            flop_per_eval, bytes_per_eval = FLOP_2D, BYTES_2D
+        else:
+            flop_per_eval, bytes_per_eval = pair_mix(wl, FLOP_1D, FLOP_2D), pair_mix(wl, BYTES_1D, BYTES_2D)
+        evals_per_launch = ev[dom] / max(nl, 1)
+        tflops = evals_per_launch * flop_per_eval / launch_s / 1e12
+        gbs = evals_per_launch * bytes_per_eval / launch_s / 1e9
+        legs = {"valu": {"achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(tflops / FP32_PEAK_TFLOPS, 5),
+                         "per_eval": round(flop_per_eval, 2)},
+                "lds": {"achieved": round(gbs, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / LDS_PEAK_GBS, 5), "per_eval": round(bytes_per_eval, 2)}}
+        top = max(legs, key=lambda k: legs[k]["frac"])
+        roof = {"bound": top, "achieved": legs[top]["achieved"], "peak": legs[top]["peak"], "unit": legs[top]["unit"],
+                "frac": legs[top]["frac"], "legs": legs,
+                "note": "FP32 vector and LDS legs (no matrix contraction on this path); algorithmic FLOPs and "
+                        f"operand bytes per pair evaluation ({FLOP_1D:g} FLOP / {BYTES_1D:g} B 1D spline, "
+                        f"{FLOP_2D:g} FLOP / {BYTES_2D:g} B 2D bilinear); bound = the leg nearer its peak"}
     roof["traffic"] = traffic_per_launch(WORKLOAD_TAG[args.workload], (KERNEL_NAMES_NL if nonlinear else KERNEL_NAMES)[dom])
     roof["kernel"] = dom
+    roof["launch_ms"] = round(launch_s * 1e3, 4)
     out = {
         "metric": METRIC,
         "value": round(args.steps * bm_pass / dt, 1),
@@ -370,8 +414,10 @@ def main():
         "config": {"workload": DESC[args.workload], "replicas": len(wl["replicas"]),
                    "bead_moves_per_step": bm_pass, "pair_terms": "fp32, fp64 accumulation",
                    "estimators": "one MCGetAverage sample per replica per pass",
+                   "timed_path": "captured pass graph, one replay per pass",
                    "launch_shapes": shapes if shapes else "defaults"},
         "kernels_ms_per_step": {f: round(kt[f][0] / args.steps, 4) for f in fams},
+        "profiled_ms_per_step": round(1e3 * (t3 - t2) / args.steps, 4),
         "roofline": roof,
     }
     if not args.no_cpu:

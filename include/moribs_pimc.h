@@ -105,7 +105,34 @@ int  pimc_asymrho(pimc_ctx* ctx, double T, int nslice, double A, double B, doubl
 
 /* ---- replicas and state ---- */
 int  pimc_set_replica(pimc_ctx* ctx, int replica, const pimc_replica_desc* desc);
+/* The checks pimc_set_replica applies, without a context or a device: the reference's input
+   checks on the sampled path (MCInit mc_setup.cc:384-398, IOReadParams mc_input.cc:210-330, with
+   their messages; the reference exits where this returns -1), the limits of this library, and an
+   even rotor slice count for the checkerboard sweeps (mc_piqmc.cc:490-512, :728-760).  Returns 0
+   if accepted, else -1 with the reason in msg (len bytes, may be NULL). */
+int  pimc_check_replica(const pimc_replica_desc* desc, char* msg, int len);
 int  pimc_commit(pimc_ctx* ctx);   /* allocates device state after all replicas are set */
+/* How pimc_commit splits linear-rotor chains over workgroups, without a device (no counterpart
+   in the reference, whose sweep is one OpenMP loop per chain, mc_piqmc.cc:490-512).  Chain i has
+   prot[i] rotor slices (even), rot_ratio[i] translational slices per rotor slice and natoms[i]
+   atoms.  Parts hold an even number of slices, sized so their LDS stays within lds_kb (at most
+   part_slices_max slices when > 0); a chain's parts wait on one another on one XCD, so a chain
+   gets at most cus_per_xcd * min(blocks_per_cu, lds_per_cu / lds) parts, fewer and larger parts
+   when the LDS budget alone would give more.  Writes parts_per_chain[i], slices_per_part[i] and
+   the launch's LDS bytes; returns -1 with a message when no plan fits (a part above 160 KB, or
+   parts that cannot be resident together). */
+int  pimc_plan_rotor_parts(int nchains, const int* prot, const int* rot_ratio, const int* natoms, int fp64,
+                           int lds_kb, int part_slices_max, int blocks_per_cu, int lds_per_cu, int cus_per_xcd,
+                           int* parts_per_chain, int* slices_per_part, long* lds_bytes, char* msg, int len);
+/* How pimc_commit splits non-linear rotor chains (MCRotations3D's checkerboard, mc_piqmc.cc:728-
+   760) over workgroups, without a device; no counterpart in the reference.  One count W for all
+   nchains chains of up to protmax slices with nrot rotors: the largest power of two up to 64 whose
+   parts hold at least 16 slices and fit LDS, with all parts resident at once (ncu * occupancy)
+   and the W parts of one chain, which wait on one another on one XCD, within cus_per_xcd *
+   occupancy; occupancy = min(blocks_per_cu, lds_per_cu / part LDS).  Returns W (1: whole
+   chains), -1 for arguments out of range. */
+int  pimc_plan_rotor3d_split(int protmax, int nchains, int nrot, int blocks_per_cu, int lds_per_cu, int ncu,
+                             int cus_per_xcd);
 /* Launch-shape options, set before pimc_commit (no counterpart in the reference, whose
    OpenMP loops have no tunable shape):
      "rotor_part_slices"  most rotor slices per workgroup of the rotor sweep (even, >= 2);
@@ -122,7 +149,10 @@ int  pimc_commit(pimc_ctx* ctx);   /* allocates device state after all replicas 
      "delayed_acceptance" 1: the non-linear rotor move (MCRot3Dstep mc_piqmc.cc:938-1199) runs
                           as a two-stage test with the surrogate network as the first stage
                           (see the surrogate section below); needs one non-linear rotor among
-                          atoms with a 3D potential, and a surrogate loaded before pimc_run. */
+                          atoms with a 3D potential, and a surrogate loaded before pimc_run.
+   One option may be set at any time, also after pimc_commit:
+     "profile"            1: time every launch with HIP events (pimc_kernel_time), issuing passes
+                          launch by launch; 0: replay the captured pass graph (the default). */
 int  pimc_set_option(pimc_ctx* ctx, const char* name, long value);
 /* coords/angles [3][natoms*P] like MCCoords/MCAngles; cosines follow from angles
    (mc_main.cc:186-197).  angles may be NULL (phi=0, cos(theta)=1, chi=0, mc_setup.cc:476-489).
@@ -187,8 +217,10 @@ int  pimc_pot_energy_slice(pimc_ctx* ctx, int nq, const int* replica, const int*
 int  pimc_estimate(pimc_ctx* ctx, double* out);
 
 /* ---- measurement ---- */
-/* accumulated device time (ms) and launch count of a kernel family when PIMC_FLAG_PROFILE is
-   set: "bisect", "centroid", "rotor", "estimate" */
+/* accumulated device time (ms) and launch count of a kernel family while profiling is on
+   (PIMC_FLAG_PROFILE at creation, or the option "profile" set to 1 at any time; passes are then
+   issued launch by launch between HIP events instead of replaying the captured pass graph):
+   "bisect", "centroid", "rotor", "estimate" */
 int  pimc_kernel_time(pimc_ctx* ctx, const char* family, double* ms, long* launches);
 int  pimc_synchronize(pimc_ctx* ctx);
 /* pair evaluations performed by one kernel family since creation */

@@ -205,7 +205,6 @@ struct pimc_ctx {
     MlpPacked* d_mlp = nullptr;     // the sampler's packed copy
     bool mlp_ready = false;
     std::vector<float> theta, norm;  // host copies: weights, then xm[4] xs[4] ym ys
-    size_t lds_rot3_da = 0;
     long long nbeads = 0, nrotb = 0, nhist = 0;
     hipGraphExec_t graph_exec = nullptr;
     bool graph_ok = false;
@@ -550,40 +549,63 @@ int pimc_add_rot3d(pimc_ctx* c, const double* rho, const double* erot, const dou
 int pimc_set_replica(pimc_ctx* c, int r, const pimc_replica_desc* d) {
     if (c->committed) return fail(c, "replicas must be set before pimc_commit");
     if (r < 0 || r >= c->nrep) return fail(c, "replica index out of range");
-    if (d->ntypes < 1 || d->ntypes > PIMC_MAX_TYPES) return fail(c, "ntypes out of range");
-    if (d->P < 2 || d->P > 65535) return fail(c, "P out of range");
-    int natoms = 0, nmolt = 0, natt = 0;
-    for (int t = 0; t < d->ntypes; ++t) {
-        if (d->numb[t] < 1) return fail(c, "every type needs at least one atom");
-        if (d->molecule[t] == 0 && ++natt > 1)
-            return fail(c, "No more then one atom/molecule type: densities and potential energy");
-        if (d->molecule[t] != 0 && t != d->ntypes - 1) return fail(c, "Molecules should follow atoms in input file");
-        if (d->molecule[t] == 2) {
-            ++nmolt;
-            if (d->numb[t] > 2) return fail(c, "GetRotE3D supports at most two non-linear rotors");
-            if (d->ntypes > 1 && d->pot3d < 0) return fail(c, "non-linear rotor without a 3D potential table");
-            if (d->Prot > 0 && d->rot3d < 0) return fail(c, "rotating non-linear rotor without rho/E/E^2 tables");
-        }
-        if (d->molecule[t] < 0 || d->molecule[t] > 2) return fail(c, "unknown molecule kind");
-        if (d->molecule[t] == 1) {
-            ++nmolt;
-            if (d->numb[t] != 1) return fail(c, "No more than one linear dopant molecule so far");
-        }
-        if (d->levels[t] < 1 || (1 << d->levels[t]) >= d->P)
-            return fail(c, "Segment size is larger then a number of time slices");
-        natoms += d->numb[t];
-    }
-    if (natoms > 4096) return fail(c, "too many atoms");
-    if (nmolt > 1) return fail(c, "one molecule type per replica");
-    if (d->Prot > 0 && d->P % d->Prot) return fail(c, "NumbTimes is not proportional to NumbRotTimes");
-    if (d->Prot == 1) return fail(c, "rotational sampling needs at least two rotational slices");
+    if (!d) return fail(c, "pimc_set_replica: null description");
+    const std::string m = check_replica(*d);
+    if (!m.empty()) return fail(c, m);
     c->desc[r] = *d;
     c->desc_set[r] = 1;
     return 0;
 }
 
+int pimc_check_replica(const pimc_replica_desc* d, char* msg, int len) {
+    const std::string m = d ? check_replica(*d) : std::string("null description");
+    if (msg && len > 0) std::snprintf(msg, (size_t)len, "%s", m.c_str());
+    return m.empty() ? 0 : -1;
+}
+
+int pimc_plan_rotor3d_split(int protmax, int nchains, int nrot, int blocks_per_cu, int lds_per_cu, int ncu,
+                            int cus_per_xcd) {
+    if (protmax < 2 || nchains < 1 || nrot < 1 || nrot > 2 || ncu < 1 || cus_per_xcd < 1) return -1;
+    return plan_rotor3d_split(
+        protmax, nchains, ncu, cus_per_xcd, [&](int ns) { return rotor3d_part_lds(ns, nrot, sizeof(Frame)); },
+        [&](size_t b) { return std::min(blocks_per_cu, (int)((size_t)lds_per_cu / std::max<size_t>(b, 1))); });
+}
+
+int pimc_plan_rotor_parts(int nchains, const int* prot, const int* rot_ratio, const int* natoms, int fp64, int lds_kb,
+                          int part_slices_max, int blocks_per_cu, int lds_per_cu, int cus_per_xcd, int* parts_per_chain,
+                          int* slices_per_part, long* lds_bytes, char* msg, int len) {
+    std::string err;
+    std::vector<RotorChain> ch;
+    for (int i = 0; i < nchains; ++i) {
+        if (prot[i] < 2 || prot[i] % 2 || rot_ratio[i] < 1 || natoms[i] < 2) err = "chain " + std::to_string(i) + " out of range";
+        ch.push_back(RotorChain{i, prot[i], rot_ratio[i], natoms[i]});
+    }
+    std::vector<RotPart> parts;
+    size_t lds = 0;
+    int rc = err.empty() ? 0 : -1;
+    if (rc == 0)
+        rc = plan_linear_rotor(ch, fp64 ? sizeof(double4) : sizeof(float4), lds_kb, part_slices_max, cus_per_xcd,
+                               [&](size_t b) { return std::min(blocks_per_cu, (int)((size_t)lds_per_cu / std::max<size_t>(b, 1))); },
+                               parts, lds, err);
+    if (rc == 0) {
+        for (const RotPart& q : parts)
+            if (q.w == 0) {
+                parts_per_chain[q.r] = q.W;
+                slices_per_part[q.r] = q.k1 - q.k0;
+            }
+        *lds_bytes = (long)lds;
+    }
+    if (msg && len > 0) std::snprintf(msg, (size_t)len, "%s", err.c_str());
+    return rc;
+}
+
 int pimc_set_option(pimc_ctx* c, const char* name, long value) {
     if (!c || !name) return fail(c, "pimc_set_option: null argument");
+    if (std::strcmp(name, "profile") == 0) {  // at any time: the next pass is issued accordingly
+        if (value != 0 && value != 1) return fail(c, "profile is 0 or 1");
+        c->profile = value == 1;
+        return 0;
+    }
     if (c->committed) return fail(c, "options are set before pimc_commit");
     if (std::strcmp(name, "rotor_part_slices") == 0) {
         if (value < 0 || value > (1 << 20)) return fail(c, "rotor_part_slices out of range");
@@ -622,34 +644,24 @@ int pimc_set_option(pimc_ctx* c, const char* name, long value) {
 // Called at commit and again by the autotuner, which compares budgets.
 static int plan_rotor_parts(pimc_ctx* c, int lds_kb) {
     const size_t geo_bytes = c->fp64 ? sizeof(double4) : sizeof(float4);
-    const long long budget = (long long)lds_kb * 1024;
+    hipDeviceProp_t prop;
+    HIPCHECK(c, hipGetDeviceProperties(&prop, 0));
+    constexpr int NXCD = 8;
+    std::vector<RotorChain> chains;
+    for (int r : c->rot_chains) chains.push_back(RotorChain{r, c->rep[r].Prot, c->rep[r].RotRatio, c->rep[r].natoms});
+    // resident parts per CU at a launch's LDS size, as the runtime computes it for k_rotor
+    auto occupancy = [&](size_t lds) {
+        int occ = 0;
+        hipError_t e = c->fp64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_rotor<double>, 256, lds)
+                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_rotor<float>, 256, lds);
+        return e == hipSuccess ? occ : 0;
+    };
     std::vector<RotPart> parts;
     size_t lds_r = 0;
-    for (int r : c->rot_chains) {
-        const Rep& p = c->rep[r];
-        const long long per_slice = (long long)p.RotRatio * (p.natoms - 1) * (long long)geo_bytes;
-        int ns = (int)std::max<long long>(2, (budget - 16 * (long long)sizeof(double)) /
-                                                 (7 * (long long)sizeof(double) + per_slice)) & ~1;
-        if (c->opt_rotor_part_slices > 0) ns = std::max(2, std::min(ns, c->opt_rotor_part_slices & ~1));
-        if (ns >= p.Prot) ns = p.Prot;
-        int W = (p.Prot + ns - 1) / ns;
-        ns = ((p.Prot + W - 1) / W + 1) & ~1;  // even, balanced
-        W = (p.Prot + ns - 1) / ns;
-        const int fbase = (int)parts.size();
-        for (int w = 0; w < W; ++w) {
-            RotPart q{};
-            q.r = r;
-            q.k0 = w * ns;
-            q.k1 = std::min(p.Prot, (w + 1) * ns);
-            q.w = w;
-            q.W = W;
-            q.fbase = fbase;
-            parts.push_back(q);
-            const int n = q.k1 - q.k0;
-            lds_r = std::max(lds_r, (size_t)((7 * (n + 2) + 3) & ~3) * sizeof(double) + (size_t)(n * per_slice));
-        }
-    }
-    if (lds_r > 160 * 1024) return fail(c, "rotor chain exceeds LDS");
+    std::string err;
+    if (plan_linear_rotor(chains, geo_bytes, lds_kb, c->opt_rotor_part_slices, prop.multiProcessorCount / NXCD, occupancy,
+                          parts, lds_r, err))
+        return fail(c, err);
     const int nflags = (int)parts.size();
     parts = xcd_order(parts, [&](int r) {
         const Rep& p = c->rep[r];
@@ -694,6 +706,8 @@ static int plan_centroid_parts(pimc_ctx* c, int target_parts) {
         HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_centroid_parts<float, 1>, 256, c->lds_knots));
     }
     const double capacity = (double)ncu * std::max(1, std::min(occ0, occ1));
+    // the parts of a replica wait on one another on one XCD: at most what one XCD holds
+    const int per_xcd = std::max(1, std::min(occ0, occ1)) * (ncu / 8);
     const double target = target_parts > 0 ? std::min((double)target_parts, capacity / 2.0) : std::min(1024.0, capacity / 2.0);
     std::vector<RotPart> cp[2];
     double total = 0.0;
@@ -709,7 +723,7 @@ static int plan_centroid_parts(pimc_ctx* c, int target_parts) {
         const Rep& p = c->rep[r];
         const double w = (double)p.natoms * p.natoms * p.P;
         int W = (int)std::ceil(target * w / std::max(total, 1.0));
-        W = std::max(1, std::min({W, 64, std::max(1, p.P / 8)}));
+        W = std::max(1, std::min({W, 64, per_xcd, std::max(1, p.P / 8)}));
         for (int q = 0; q < W; ++q) {
             RotPart x{};
             x.r = r;
@@ -836,8 +850,9 @@ int pimc_commit(pimc_ctx* c) {
                 return fail(c, "non-linear rotor without a 3D potential table");
             if (p.rotkind == 2 && (p.rot3d_id < 0 || p.rot3d_id >= c->nr3))
                 return fail(c, "rotating non-linear rotor without rho/E/E^2 tables");
-            if (p.rotkind == 2 && c->opt_da && (p.nrot != 1 || p.natoms < 2 || p.pot3d_id < 0))
-                return fail(c, "delayed acceptance needs one non-linear rotor among atoms with a 3D potential");
+            if (p.rotkind == 2 && c->opt_da && !(p.natoms > p.nrot ? p.pot3d_id >= 0 : p.nrot == 2))
+                return fail(c, "delayed acceptance needs atoms with a 3D potential (rotor-atom network) or two "
+                               "rotors (rotor-rotor network)");
             if (p.rotkind == 2) {
                 c->bead_moves_per_pass += (double)p.P * p.Prot * p.nrot;
                 rot3.push_back(r);
@@ -966,9 +981,9 @@ int pimc_commit(pimc_ctx* c) {
     }
     // Non-linear chains split by slice range when whole chains would leave CUs idle: as many
     // parts (powers of two, at least 16 slices each) as the device holds resident at once, since
-    // the parts of a chain wait for one another.  Not with delayed acceptance, whose kernel runs
-    // whole chains.
-    if (!c->opt_da && !rot3.empty()) {
+    // the parts of a chain wait for one another.  With delayed acceptance every chain runs as
+    // parts of k_rotor3d_da, a short chain as one part.
+    if (!rot3.empty()) {
         hipDeviceProp_t prop;
         HIPCHECK(c, hipGetDeviceProperties(&prop, 0));
         int protmax = 2, nrmax = 1;
@@ -976,31 +991,31 @@ int pimc_commit(pimc_ctx* c) {
             protmax = std::max(protmax, c->rep[r].Prot);
             nrmax = std::max(nrmax, c->rep[r].nrot);
         }
+        const bool da = c->opt_da;
         auto lds_for = [&](int ns) {
-            const int maxmoves = (ns + 1) / 2;
-            int G = 1;
-            while (G < 64 && G * 2 * maxmoves <= 256) G *= 2;
-            return (size_t)((nrmax * (ns + 1) + 1) & ~1) * sizeof(double) + (size_t)2 * nrmax * sizeof(Frame) +
-                   (size_t)(256 / G) * (nrmax + 1) * sizeof(Frame);
+            return da ? rotor3d_da_part_lds(ns, nrmax, sizeof(Frame), sizeof(MlpPacked))
+                      : rotor3d_part_lds(ns, nrmax, sizeof(Frame));
         };
-        int W = 1;
-        for (int w = 2; w <= 64; w *= 2) {
-            const int ns = ((protmax + w - 1) / w + 1) & ~1;
-            if (ns < 16 || lds_for(ns) > 160 * 1024) break;
+        auto occupancy = [&](size_t lds) {
             int occ = 0;
-            if (c->fp64) HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_rotor3d_parts<double>, 256, lds_for(ns)));
-            else HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_rotor3d_parts<float>, 256, lds_for(ns)));
-            if ((long long)w * (long long)rot3.size() > (long long)occ * prop.multiProcessorCount) break;
-            W = w;
-        }
-        if (W > 1) {
+            hipError_t e;
+            if (da) e = c->fp64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_rotor3d_da<double>, 256, lds)
+                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_rotor3d_da<float>, 256, lds);
+            else e = c->fp64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_rotor3d_parts<double>, 256, lds)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_rotor3d_parts<float>, 256, lds);
+            return e == hipSuccess ? occ : 0;
+        };
+        const int W = plan_rotor3d_split(protmax, (int)rot3.size(), prop.multiProcessorCount,
+                                         prop.multiProcessorCount / 8, lds_for, occupancy);
+        if (W > 1 || da) {
             std::vector<RotPart> p3;
             std::vector<int> whole;
             for (int r : rot3) {
                 const Rep& p = c->rep[r];
-                const int ns = ((p.Prot + W - 1) / W + 1) & ~1;
+                int ns = ((p.Prot + W - 1) / W + 1) & ~1;
+                if (ns > p.Prot) ns = p.Prot;
                 const int Wr = (p.Prot + ns - 1) / ns;
-                if (Wr < 2) {
+                if (Wr < 2 && !da) {
                     whole.push_back(r);
                     continue;
                 }
@@ -1017,6 +1032,7 @@ int pimc_commit(pimc_ctx* c) {
                     c->lds_parts3 = std::max(c->lds_parts3, lds_for(q.k1 - q.k0));
                 }
             }
+            if (c->lds_parts3 > ROTOR_LDS_MAX) return fail(c, "non-linear rotor part exceeds LDS");
             rot3.swap(whole);
             const int nflags3 = (int)p3.size();
             // two records per part, each 2 words per double of nrmax frames, then nrmax link densities
@@ -1183,7 +1199,6 @@ int pimc_commit(pimc_ctx* c) {
         };
         c->rot3_block = lds_for(512) <= 64 * 1024 ? 512 : 256;
         c->lds_rot3 = lds_for(c->rot3_block);
-        c->lds_rot3_da = sizeof(MlpPacked) + (size_t)((protmax + 1) & ~1) * sizeof(double) + 4 * sizeof(Frame);
         if (!rot3.empty() && c->lds_rot3 > 160 * 1024) return fail(c, "non-linear rotor chain exceeds LDS");
     }
     // bisection block: enough lanes for the widest level of the largest window
@@ -1404,14 +1419,15 @@ void launch_rotor(pimc_ctx* c, int u) {
     if (c->nparts)
         hipLaunchKernelGGL(k_rotor<R>, dim3(c->nparts), dim3(256), c->lds_rotor, c->stream, c->d_rep, c->d_mut,
                            c->d_beads, c->d_rotb, c->tables(), c->tables3(), c->d_parts, c->d_flags, c->d_xrec, run);
-    if (c->nparts3)
+    if (c->nparts3 && c->opt_da)
+        hipLaunchKernelGGL(k_rotor3d_da<R>, dim3(c->nparts3), dim3(256), c->lds_parts3, c->stream, c->d_rep, c->d_mut,
+                           c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_parts3, c->d_flags3,
+                           c->d_pub3, c->pub3_stride, run, c->d_mlp);
+    else if (c->nparts3)
         hipLaunchKernelGGL(k_rotor3d_parts<R>, dim3(c->nparts3), dim3(256), c->lds_parts3, c->stream, c->d_rep,
                            c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_parts3,
                            c->d_flags3, c->d_pub3, c->pub3_stride, run);
-    if (c->nrot3 && c->opt_da)
-        hipLaunchKernelGGL(k_rotor3d_da<R>, dim3(c->nrot3), dim3(256), c->lds_rot3_da, c->stream, c->d_rep, c->d_mut,
-                           c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rot3, run, c->d_mlp);
-    else if (c->nrot3 && c->rot3_block == 512)
+    if (c->nrot3 && c->rot3_block == 512)
         hipLaunchKernelGGL((k_rotor3d<R, 512>), dim3(c->nrot3), dim3(512), c->lds_rot3, c->stream, c->d_rep,
                            c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rot3, run);
     else if (c->nrot3)
@@ -1515,7 +1531,7 @@ extern "C" {
 
 int pimc_run(pimc_ctx* c, int n_passes, int sample_every) {
     if (!c->committed) return fail(c, "pimc_commit first");
-    if (c->opt_da && c->nrot3 && !c->mlp_ready)
+    if (c->opt_da && c->nparts3 && !c->mlp_ready)
         return fail(c, "delayed acceptance needs a surrogate: pimc_surrogate_train or pimc_surrogate_set first");
     for (int p = 0; p < n_passes; ++p) {
         bool use_graph = !c->profile;
@@ -1763,32 +1779,53 @@ int pimc_get_work(pimc_ctx* c, double* pair_evals, double* bead_moves) {
     return 0;
 }
 
-int pimc_pot_energy_slice(pimc_ctx* c, int nq, const int* qr, const int* qa, const int* qit, const double* qpos,
-                          double* out) {
+}  // extern "C"
+
+namespace {
+
+int check_queries(pimc_ctx* c, int nq, const int* qr, const int* qa, const int* qit) {
     if (!c->committed) return fail(c, "pimc_commit first");
+    if (nq < 0) return fail(c, "query count is negative");
     for (int q = 0; q < nq; ++q) {
         if (qr[q] < 0 || qr[q] >= c->nrep) return fail(c, "query replica out of range");
         const Rep& p = c->rep[qr[q]];
         if (qa[q] < 0 || qa[q] >= p.natoms || qit[q] < 0 || qit[q] >= p.P) return fail(c, "query out of range");
     }
+    return 0;
+}
+
+// k_pot_slice on device copies of the queries, then the results back to out
+int pot_slice_on(pimc_ctx* c, int nq, const int* qr, const int* qa, const int* qit, const double* qpos, double* out,
+                 int* d_r, int* d_a, int* d_i, double* d_p, double* d_o) {
+    HIPCHECK(c, hipMemcpyAsync(d_r, qr, sizeof(int) * nq, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(d_a, qa, sizeof(int) * nq, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(d_i, qit, sizeof(int) * nq, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(d_p, qpos, sizeof(double) * 3 * nq, hipMemcpyHostToDevice, c->stream));
+    if (c->fp64)
+        hipLaunchKernelGGL(k_pot_slice<double>, dim3((nq + 127) / 128), dim3(128), 0, c->stream, c->d_rep, c->d_beads,
+                           c->d_rotb, c->d_frames, c->tables(), c->tables3(), nq, d_r, d_a, d_i, d_p, d_o);
+    else
+        hipLaunchKernelGGL(k_pot_slice<float>, dim3((nq + 127) / 128), dim3(128), 0, c->stream, c->d_rep, c->d_beads,
+                           c->d_rotb, c->d_frames, c->tables(), c->tables3(), nq, d_r, d_a, d_i, d_p, d_o);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipMemcpyAsync(out, d_o, sizeof(double) * nq, hipMemcpyDeviceToHost, c->stream));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, "pot_energy_slice failed");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pimc_pot_energy_slice(pimc_ctx* c, int nq, const int* qr, const int* qa, const int* qit, const double* qpos,
+                          double* out) {
+    if (check_queries(c, nq, qr, qa, qit)) return -1;
+    if (nq == 0) return 0;
     int *d_r = dalloc<int>(nq), *d_a = dalloc<int>(nq), *d_i = dalloc<int>(nq);
     double *d_p = dalloc<double>(3 * (size_t)nq), *d_o = dalloc<double>(nq);
     int rc = 0;
     if (!d_r || !d_a || !d_i || !d_p || !d_o) rc = fail(c, "hipMalloc failed");
-    if (!rc) {
-        (void)hipMemcpy(d_r, qr, sizeof(int) * nq, hipMemcpyHostToDevice);
-        (void)hipMemcpy(d_a, qa, sizeof(int) * nq, hipMemcpyHostToDevice);
-        (void)hipMemcpy(d_i, qit, sizeof(int) * nq, hipMemcpyHostToDevice);
-        (void)hipMemcpy(d_p, qpos, sizeof(double) * 3 * nq, hipMemcpyHostToDevice);
-        if (c->fp64)
-            hipLaunchKernelGGL(k_pot_slice<double>, dim3((nq + 127) / 128), dim3(128), 0, c->stream, c->d_rep,
-                               c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), nq, d_r, d_a, d_i, d_p, d_o);
-        else
-            hipLaunchKernelGGL(k_pot_slice<float>, dim3((nq + 127) / 128), dim3(128), 0, c->stream, c->d_rep,
-                               c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), nq, d_r, d_a, d_i, d_p, d_o);
-        if (hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, "pot_energy_slice failed");
-        else (void)hipMemcpy(out, d_o, sizeof(double) * nq, hipMemcpyDeviceToHost);
-    }
+    if (!rc) rc = pot_slice_on(c, nq, qr, qa, qit, qpos, out, d_r, d_a, d_i, d_p, d_o);
     void* ps[] = {d_r, d_a, d_i, d_p, d_o};
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -1935,9 +1972,10 @@ int pimc_surrogate_collect(pimc_ctx* c, int r, float* x, float* v, int max_rows)
     if (!c->committed) return fail(c, "pimc_commit first");
     if (r < 0 || r >= c->nrep) return fail(c, "replica index out of range");
     const Rep& p = c->rep[r];
-    if (p.fam != 1 || p.nrot != 1 || p.natoms < 2 || p.pot3d_id < 0)
-        return fail(c, "surrogate rows need one non-linear rotor among atoms with a 3D potential");
-    const long long n = (long long)p.Prot * p.RotRatio * (p.natoms - 1);
+    const int nat = p.natoms - p.nrot;
+    if (p.fam != 1 || p.nrot < 1 || !(nat > 0 ? p.pot3d_id >= 0 : p.nrot == 2))
+        return fail(c, "surrogate rows need non-linear rotors among atoms with a 3D potential, or two rotors");
+    const long long n = (long long)p.nrot * p.Prot * p.RotRatio * (nat > 0 ? nat : p.nrot - 1);
     if (!x || !v || max_rows < n) return fail(c, "surrogate_collect: buffer smaller than " + std::to_string(n) + " rows");
     float* dx = dalloc<float>((size_t)n * MLP_IN);
     float* dv = dalloc<float>((size_t)n);

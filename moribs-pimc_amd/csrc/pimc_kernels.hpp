@@ -29,6 +29,7 @@
 #include "pimc_device.hpp"
 #include "pimc_nl.hpp"
 #include "pimc_mlp.hpp"
+#include "pimc_plan.hpp"
 
 namespace pimc {
 
@@ -378,9 +379,7 @@ __global__ __launch_bounds__(256) void k_bisect_nl(const Rep* __restrict__ reps,
 // stores to complete before it stores the flag: the exchange needs no cache write-back or
 // invalidation, which on a multi-XCD part would flush the L2 holding everyone's tables.
 // Results do not depend on the split: draws are counter-based per (sweep, slice).
-struct RotPart {
-    int r, k0, k1, w, W, fbase, pad0, pad1;
-};
+// (RotPart: pimc_plan.hpp)
 
 // Polls of a numbered word before a wait gives up and sets the part's error bit (each poll
 // sleeps between loads, so this bounds a wait at seconds, far beyond any normal exchange)
@@ -526,6 +525,7 @@ __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ 
             }
         }
         __syncthreads();
+        if (s_err) break;  // a part did not arrive: the later atoms' waits would fail as well
         const double deltav = s_dv;
         bool accepted = false;
         if (deltav < 0.0) accepted = true;
@@ -754,6 +754,13 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
                 }
             }
             __syncthreads();
+            // a wait that gave up: leave at once instead of waiting again at every later half
+            // (every lane reads s_err after the barrier; it is written again only after the
+            // barrier that closes this half)
+            if (s_err) {
+                sw = nsweep;
+                break;
+            }
             const int nmoves = (ns - parity + 1) / 2;  // k0 is even, so local slice kl has parity (kl - 1) & 1
             for (int mv = g; mv < nmoves; mv += groups) {
                 const int kl = 2 * mv + parity + 1;
@@ -1060,8 +1067,11 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
     while (G < 64 && G * 2 * maxmoves <= (int)blockDim.x) G *= 2;
     const int groups = blockDim.x / G;
     const int g = threadIdx.x / G, gl = threadIdx.x % G;
+    const int nat = rp.nl_atom;  // atoms come first: 0 .. nat-1
+    const int items_a = RR * nat, items_r = RR * (nrot - 1);
     double* Lk = smem;
-    Frame* fnb = reinterpret_cast<Frame*>(smem + ((nrot * L + 1) & ~1));
+    double* Ea = Lk + nrot * L;  // rotor-atom energy of every own slice at its current orientation
+    Frame* fnb = reinterpret_cast<Frame*>(smem + ((nrot * L + nrot * ns + 1) & ~1));
     Frame* gf = fnb + 2 * nrot + (long long)g * (nrot + 1);
     unsigned* myflag = flags + pt.fbase + pt.w;  // half-sweeps done by the launches before this one
     constexpr int FD = (int)(sizeof(Frame) / sizeof(double));  // doubles per frame
@@ -1094,6 +1104,24 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
         const Frame& fb = kl == ns ? fnb[nrot + q] : own(q, k0 + kl);
         if (rotden_rho(fa, fb, tab, Lk[i])) err = 1;
     }
+    // rotor-atom sums at the current orientations: atoms do not move during the sweep, so a
+    // move's old sum is this one or the new sum of the last move accepted at the slice (the same
+    // lanes add the same terms in the same order either way)
+    for (int kl = 1 + g; kl <= ns; kl += groups) {
+        const int k = k0 + kl - 1;
+        for (int q = 0; q < nrot; ++q) {
+            const int gatom = rp.nl_atom + q;
+            const Frame& fq = own(q, k);
+            double e = 0.0;
+            for (int i = gl; i < items_a; i += G) {
+                const int it = k * RR + i / nat, a = i % nat;
+                const double4 xa = bead(beads, rp, gatom, it), xj = bead(beads, rp, a, it);
+                e += (double)vcord_v<R>(l3, fq, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z);
+            }
+            e = group_sum(e, G);
+            if (gl == 0) Ea[q * ns + kl - 1] = e;
+        }
+    }
     __syncthreads();
     for (int sw = 0; sw < nsweep; ++sw) {
         const uint32_t jsweep = (uint32_t)(j0 + sw * jstride);
@@ -1124,6 +1152,13 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
                     else reinterpret_cast<unsigned*>(&Lk[(i - nrot * FD) * L + (parity == 0 ? 0 : ns)])[w & 1] = (unsigned)wv;
                 }
                 __syncthreads();
+                // a wait that gave up: no later wait of this launch can succeed either, so the
+                // part leaves at once (s_err is read by every lane after the barrier, and no lane
+                // writes it again before the barrier that closes this half)
+                if (s_err) {
+                    sw = nsweep;
+                    break;
+                }
             }
             const int nmoves = (ns - parity + 1) / 2;  // k0 is even: local slice kl has parity (kl - 1) & 1
             for (int mv = g; mv < nmoves; mv += groups) {
@@ -1158,20 +1193,29 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
                     }
                     if (G > 1)
                         for (int c = 2; c < 4; ++c) d[c] = __shfl(d[c], (c - 2) % G, G);
-                    double eo = 0.0, en = 0.0;
-                    for (int i = gl; i < items; i += G) {
-                        const int it = k * RR + i / np, jp = i % np;
-                        const int j = jp + (jp >= gatom);
-                        const bool rj = j >= rp.nl_atom;
-                        const double4 xa = bead(beads, rp, gatom, it), xj = bead(beads, rp, j, it);
-                        const Frame& fj = gf[rj ? j - rp.nl_atom : q];
-                        eo += pair_nl<R>(l1, l1.k, l3, true, rj, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, gf[q], fj);
-                        en += pair_nl<R>(l1, l1.k, l3, true, rj, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, gf[nrot], fj);
+                    // rotor-atom terms at the proposal (the old sum is cached), then the terms of
+                    // the other rotors at both orientations (a partner rotor may have moved)
+                    double ea = 0.0, eo = 0.0, en = 0.0;
+                    for (int i = gl; i < items_a; i += G) {
+                        const int it = k * RR + i / nat, a = i % nat;
+                        const double4 xa = bead(beads, rp, gatom, it), xj = bead(beads, rp, a, it);
+                        ea += (double)vcord_v<R>(l3, gf[nrot], xa.x, xa.y, xa.z, xj.x, xj.y, xj.z);
                     }
-                    for (int o = G >> 1; o > 0; o >>= 1) {  // the two sums' shuffles interleave
+                    for (int i = gl; i < items_r; i += G) {
+                        const int it = k * RR + i / (nrot - 1), pp = i % (nrot - 1);
+                        const int p = pp + (pp >= q);
+                        const double4 xa = bead(beads, rp, gatom, it), xj = bead(beads, rp, rp.nl_atom + p, it);
+                        eo += caleng_v(xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, gf[q], gf[p]);
+                        en += caleng_v(xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, gf[nrot], gf[p]);
+                    }
+                    for (int o = G >> 1; o > 0; o >>= 1) {  // the sums' shuffles interleave
+                        ea += __shfl_xor(ea, o, G);
                         eo += __shfl_xor(eo, o, G);
                         en += __shfl_xor(en, o, G);
                     }
+                    const double ea_old = Ea[q * ns + kl - 1];
+                    eo += ea_old;
+                    en += ea;
                     double dold = d[0] * d[1];
                     if (fabs(dold) < RZERO) dold = 0.0;
                     if (dold < 0.0) dold = fabs(dold);
@@ -1196,6 +1240,7 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
                         gf[q] = gf[nrot];
                         Lk[q * L + kl - 1] = d[2];
                         Lk[q * L + kl] = d[3];
+                        Ea[q * ns + kl - 1] = ea;
                     }
                     __builtin_amdgcn_wave_barrier();
                 }
@@ -1228,7 +1273,10 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
         const unsigned long long moves = (unsigned long long)nsweep * ns * nrot;
         atomicAdd(&mut[r].total[rtype][2], moves);
         atomicAdd(&mut[r].accep[rtype][2], sacc);
-        atomicAdd(&mut[r].pair_evals[2], 2ull * moves * (unsigned long long)items);
+        // per move the proposal's rotor-atom terms and both orientations' rotor-rotor terms, and
+        // the cached sums once per launch
+        atomicAdd(&mut[r].pair_evals[2], moves * (unsigned long long)(items_a + 2 * items_r) +
+                                             (unsigned long long)ns * nrot * items_a);
     }
 }
 
@@ -1278,35 +1326,64 @@ __global__ __launch_bounds__(256) void k_rcf(const Rep* __restrict__ reps, const
 }
 
 // ------------------------------------------------------------------------- delayed acceptance
-// MCRotations3D (mc_piqmc.cc:692-779) for one non-linear rotor among atoms, with the one-stage
-// test of MCRot3Dstep (:1166-1178) split in two (delayed acceptance, Christen & Fox 2005).
-// Stage 1 accepts with min(1, rho_new/rho_old * exp(-tau (Vs_new - Vs_old))), Vs the surrogate
-// network summed over the slice's rotor-atom pairs; only proposals that pass are charged the
-// exact PotRotE3D sums, and stage 2 accepts them with
-// min(1, exp(-tau ((V_new - V_old) - (Vs_new - Vs_old)))).  The product of the two tests is
-// in detailed balance with the exact target for any surrogate, so the stationary distribution is
-// that of the one-stage move.  Stage 1 uses the one-stage test's uniform (rand4); stage 2 draws
-// a fifth uniform (counter word 3 = 2).  Proposals and density factors are those of k_rotor3d.
-// One wave per slice move: the RotRatio x (N-1) pair rows at the old and the proposed
-// orientation go through the network 32 at a time on the matrix cores, and the exact pairs of a
-// proposal that passes spread over the 64 lanes.
-// LDS: the packed network, Lk [Prot] doubles (the link density cache of k_rotor3d), then one
-// proposal frame per wave.
+// MCRotations3D (mc_piqmc.cc:692-779) with the one-stage test of MCRot3Dstep (:1166-1178) split
+// in two (delayed acceptance, Christen & Fox 2005).  Stage 1 accepts with
+// min(1, rho_new/rho_old * exp(-tau (dVs + dVx))), where Vs is the surrogate network summed over
+// the slice's network pairs and Vx the exact energy of the other pairs; only proposals that pass
+// are charged the exact sums of the network pairs, and stage 2 accepts them with
+// min(1, exp(-tau (dV_net - dVs))).  The product of the two tests is in detailed balance with
+// the exact target for any surrogate, so the stationary distribution is the one-stage move's.
+// Stage 1 uses the one-stage test's uniform (rand4); stage 2 draws a fifth (counter word 3 = 2).
+// Network pairs: rotor-atom pairs when the replica has atoms (rotor-rotor pairs of two rotors are
+// then exact in stage 1), rotor-rotor pairs when it has none.
+//
+// Chains are split into parts as k_rotor3d_parts splits them, with the same boundary records
+// between half-sweeps (a part of W = 1 is a whole chain and trades with itself).  Within a half,
+// rotor q in turn (a move of rotor q at slice k sees the moves of the rotors before it at k, as
+// in the one-stage kernel); each wave takes its moves in chunks of DA_MV, lane j the chunk's move
+// j: (1) proposal, new link densities and the exact stage-1 terms; (2) the chunk's network rows,
+// 32 per pass through the matrix cores, summed per move in row order; (3) stage 1; (4) stage 2
+// for the moves that passed -- the whole wave sums one move's rotor-atom pairs, or each lane its
+// move's rotor-rotor pairs; (5) updates.  The surrogate and exact rotor-atom sums of every own
+// slice at its current orientation are cached (atoms do not move during the sweep): a move
+// evaluates only its proposal, and an accepted move leaves its new sums as the cached ones.
+// LDS: the packed network; Lk [nrot][ns+1], Vs and Ex [nrot][ns] (doubles); the neighbours'
+// boundary frames [2 nrot]; per wave DA_MV proposal frames and 32 row outputs (DA_MV and the
+// LDS size: pimc_plan.hpp).
+
+// Body-frame features of a rotor-rotor pair for the network: the distance of the centres, the
+// cosines of both body z axes (the dipole axis of the TIP4P geometry) with the unit separation
+// from rotor q to rotor p, and the cosine between the two z axes.
+__device__ __forceinline__ void mlp_rotor_pair_features(const Frame& fq, const Frame& fp, double qx, double qy,
+                                                        double qz, double px, double py, double pz, float out[MLP_IN]) {
+    const double rx = px - qx, ry = py - qy, rz = pz - qz;
+    const double rn = sqrt(rx * rx + ry * ry + rz * rz);
+    const double inv = rn > 0.0 ? 1.0 / rn : 0.0;
+    out[0] = (float)rn;
+    out[1] = (float)((fq.m[6] * rx + fq.m[7] * ry + fq.m[8] * rz) * inv);
+    out[2] = (float)((fp.m[6] * rx + fp.m[7] * ry + fp.m[8] * rz) * inv);
+    out[3] = (float)(fq.m[6] * fp.m[6] + fq.m[7] * fp.m[7] + fq.m[8] * fp.m[8]);
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps, Mut* __restrict__ mut,
                                                     const double4* __restrict__ beads, RotBead* __restrict__ rotb,
                                                     Frame* __restrict__ frames, Tables T, Tables3 T3,
-                                                    const int* __restrict__ replist, Run run,
+                                                    const RotPart* __restrict__ parts, unsigned* __restrict__ flags,
+                                                    unsigned long long* __restrict__ pub, int pub_stride, Run run,
                                                     const MlpPacked* __restrict__ Wg) {
     extern __shared__ uint4 mlp_lds[];
     MlpPacked* W = reinterpret_cast<MlpPacked*>(mlp_lds);
-    double* smem = reinterpret_cast<double*>(W + 1);
-    __shared__ unsigned long long red_a[4], red_p[4];
+    __shared__ unsigned long long red_a[4], red_p[4], red_n[4];
     __shared__ int s_err;
-    const int r = replist[blockIdx.x];
+    const RotPart pt = parts[blockIdx.x];
+    if (pt.r < 0) return;  // padding of the XCD-ordered grid
+    const int r = pt.r;
     const Rep& rp = reps[r];
     if (rp.rotkind != 2 || rp.rot3d_id < 0) return;
-    const int Prot = rp.Prot, RR = rp.RotRatio, P = rp.P, N = rp.natoms;
+    const int Prot = rp.Prot, RR = rp.RotRatio, P = rp.P, nrot = rp.nrot;
+    const int nat = rp.nl_atom;              // atoms come first: 0 .. nat-1
+    const bool net_atoms = nat > 0;          // network pairs: rotor-atom, else rotor-rotor
     int nsweep, j0, jstride;
     if (run.ref_order) {
         if (run.u >= P) return;
@@ -1315,174 +1392,380 @@ __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps
         if (run.u >= rp.S) return;
         nsweep = P / rp.S; j0 = run.u; jstride = rp.S;
     }
-    const int np = N - 1;
-    const int items = RR * np;
-    const int gatom = rp.nl_atom;  // the rotor, last
+    const int k0 = pt.k0, ns = pt.k1 - pt.k0, NP = pt.W;
+    const int L = ns + 1;
     const long long rb0 = rp.rot_off;
     const double4* tab = T3.r3 + (long long)rp.rot3d_id * 3 * ROT3D_SIZE;
     const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
     const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
-    double* Lk = smem;
-    Frame* wf = reinterpret_cast<Frame*>(smem + ((Prot + 1) & ~1)) + wv;
+    double* Lk = reinterpret_cast<double*>(W + 1);
+    double* Vs = Lk + nrot * L;   // cached surrogate sums (rotor-atom network)
+    double* Ex = Vs + nrot * ns;  // cached exact rotor-atom sums
+    Frame* fnb = reinterpret_cast<Frame*>(Lk + ((nrot * L + 2 * nrot * ns + 1) & ~1));
+    Frame* prop = fnb + 2 * nrot + wv * DA_MV;
+    float* rowo = reinterpret_cast<float*>(fnb + 2 * nrot + nw * DA_MV) + wv * 32;
+    unsigned* myflag = flags + pt.fbase + pt.w;
+    constexpr int FD = (int)(sizeof(Frame) / sizeof(double));
+    const int nwords = 2 * nrot * (FD + 1);
+    const int half_rec = pub_stride / 2;
+    unsigned long long* mypub = pub + (long long)(pt.fbase + pt.w) * pub_stride;
+    const unsigned long long* prevlast = pub + (long long)(pt.fbase + (pt.w + NP - 1) % NP) * pub_stride + half_rec;
+    const unsigned long long* nextfirst = pub + (long long)(pt.fbase + (pt.w + 1) % NP) * pub_stride;
+    const unsigned base = __hip_atomic_load(myflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t pass = mut[r].pass;
     const int rtype = rp.imtype;
     const double twopi = 2.0 * PI_D;
-    const double ys = (double)W->ys;
-    const int nbatch = (items + 31) / 32;
+    const double ys = (double)Wg->ys;
+    // network rows per move: RR x atoms (proposal only; the old sum is cached), or RR x other
+    // rotors x (old, new)
+    const int rpm = net_atoms ? RR * nat : RR * (nrot - 1) * 2;
+    const int items_x = RR * nat;  // exact rotor-atom pairs of a slice
+    auto own = [&](int q, int k) -> const Frame& { return frames[rb0 + (long long)q * Prot + k]; };
     if (threadIdx.x == 0) s_err = 0;
-    unsigned long long nacc = 0, npass = 0;
+    unsigned long long nacc = 0, npass = 0, nnet = 0;
     int err = 0;
     mlp_stage(Wg, W);
-    for (int i = threadIdx.x; i < Prot; i += blockDim.x) {
-        if (rotden_rho(frames[rb0 + i], frames[rb0 + (i + 1 == Prot ? 0 : i + 1)], tab, Lk[i])) err = 1;
+    for (int i = threadIdx.x; i < 2 * nrot; i += blockDim.x) {
+        const int side = i / nrot, q = i - side * nrot;
+        int k = side ? k0 + ns : k0 - 1;
+        if (k < 0) k += Prot;
+        if (k >= Prot) k -= Prot;
+        fnb[i] = own(q, k);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nrot * L; i += blockDim.x) {
+        const int q = i / L, kl = i - q * L;
+        const Frame& fa = kl == 0 ? fnb[q] : own(q, k0 + kl - 1);
+        const Frame& fb = kl == ns ? fnb[nrot + q] : own(q, k0 + kl);
+        if (rotden_rho(fa, fb, tab, Lk[i])) err = 1;
+    }
+    // The network's rows for slices of rotor q, summed per slice in row order.  Slice list:
+    // lane j < nm holds the slice (local kl) of chunk entry j and, for the proposal rows, its
+    // frame is prop[j]; sums of (old, new) rows return in so/sn on lane j.
+    auto net_rows = [&](int q, int nm, int kl_j, bool proposal, double& so, double& sn) {
+        so = 0.0;
+        sn = 0.0;
+        const int total = nm * rpm;
+        for (int b = 0; b < total; b += 32) {
+            const int row = b + (l & 31);
+            // the row's move and its slice, read from the move's lane with every lane active
+            const int jj = min(row, total - 1) / rpm;
+            const int klr = __shfl(kl_j, jj);
+            float f[MLP_IN] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (row < total) {
+                const int idx = row - jj * rpm;
+                const int k = k0 + klr - 1;
+                const int gq = rp.nl_atom + q;
+                if (net_atoms) {
+                    const int s = idx / nat, a = idx - s * nat;
+                    const int it = k * RR + s;
+                    const double4 xa = bead(beads, rp, gq, it), xj = bead(beads, rp, a, it);
+                    const Frame& fq = proposal ? prop[jj] : own(q, k);
+                    mlp_pair_features(fq.m, fq.zn, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, f);
+                } else {
+                    const int which = idx & 1, rest = idx >> 1;
+                    const int s = rest / (nrot - 1), pp = rest - s * (nrot - 1);
+                    const int p = pp + (pp >= q);
+                    const int it = k * RR + s;
+                    const double4 xq = bead(beads, rp, gq, it), xp = bead(beads, rp, rp.nl_atom + p, it);
+                    const Frame& fq = (proposal && which) ? prop[jj] : own(q, k);
+                    mlp_rotor_pair_features(fq, own(p, k), xq.x, xq.y, xq.z, xp.x, xp.y, xp.z, f);
+                }
+#pragma unroll
+                for (int c = 0; c < MLP_IN; ++c) f[c] = (f[c] - W->xm[c]) * W->xinv[c];
+            }
+            // an opaque zero offset keeps the weight reads inside the loop (hoisted out they
+            // would hold ~130 registers of weights per lane)
+            int z = 0;
+            __asm__ volatile("" : "+s"(z));
+            const float o = mlp_rows32(reinterpret_cast<const MlpPacked*>(reinterpret_cast<const char*>(W) + z), f);
+            if (l < 32) rowo[l] = o;
+            __builtin_amdgcn_wave_barrier();
+            if (l < nm) {  // the rows of move l that fall in this batch, in row order
+                const int r0 = max(l * rpm, b), r1 = min((l + 1) * rpm, b + 32);
+                for (int rr = r0; rr < r1; ++rr) {
+                    const int idx = rr - l * rpm;
+                    const double v = (double)rowo[rr - b];
+                    if (net_atoms || (idx & 1)) sn += v;
+                    else so += v;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    };
+    // exact rotor-atom sum of rotor q at local slice kl with frame f, over the whole wave
+    auto exact_atoms = [&](int q, int kl, const Frame& f) -> double {
+        const int k = k0 + kl - 1, gq = rp.nl_atom + q;
+        double e = 0.0;
+        for (int i = l; i < items_x; i += 64) {
+            const int s = i / nat, a = i - s * nat;
+            const int it = k * RR + s;
+            const double4 xa = bead(beads, rp, gq, it), xj = bead(beads, rp, a, it);
+            e += (double)vcord_v<R>(l3, f, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z);
+        }
+        return wave_sum(e);
+    };
+    // cached sums of every own slice at its current orientation (rotor-atom network only)
+    if (net_atoms) {
+        for (int q = 0; q < nrot; ++q) {
+            for (int c0 = wv * DA_MV; c0 < ns; c0 += nw * DA_MV) {
+                const int nm = min(DA_MV, ns - c0);
+                double so, sn;
+                net_rows(q, nm, c0 + l + 1, false, so, sn);
+                if (l < nm) Vs[q * ns + c0 + l] = sn;
+                for (int j = 0; j < nm; ++j) {
+                    const double e = exact_atoms(q, c0 + j + 1, own(q, k0 + c0 + j));
+                    if (l == 0) Ex[q * ns + c0 + j] = e;
+                }
+            }
+        }
     }
     __syncthreads();
     for (int sw = 0; sw < nsweep; ++sw) {
         const uint32_t jsweep = (uint32_t)(j0 + sw * jstride);
         for (int parity = 0; parity < 2; ++parity) {
-            const int nmoves = (Prot - parity + 1) / 2;
-            for (int mv = wv; mv < nmoves; mv += nw) {
-                const int k = 2 * mv + parity;
-                int k0 = k - 1, k2 = k + 1;
-                if (k0 < 0) k0 += Prot;
-                if (k2 >= Prot) k2 -= Prot;
-                const long long ik = rb0 + k;
-                const RotBead cur = rotb[ik];
-                const uint4 w0 = draw(run.seed, rp.key, pass, K_ROT3D, jsweep, (uint32_t)gatom, (uint32_t)k, 0);
-                const uint4 w1 = draw(run.seed, rp.key, pass, K_ROT3D, jsweep, (uint32_t)gatom, (uint32_t)k, 1);
-                const uint4 w2 = draw(run.seed, rp.key, pass, K_ROT3D, jsweep, (uint32_t)gatom, (uint32_t)k, 2);
-                const double rand1 = u53(w0.x, w0.y), rand2 = u53(w0.z, w0.w);
-                const double rand3 = u53(w1.x, w1.y), rand4 = u53(w1.z, w1.w);
-                const double rand5 = u53(w2.x, w2.y);
-                double cost = cur.cth + rp.rtstep * (rand1 - 0.5);
-                double phi = cur.phi + twopi * (rp.rtstep * (rand2 - 0.5));
-                double chi = cur.chi + twopi * (rp.rtstep * (rand3 - 0.5));
-                if (phi < 0.0) phi = twopi + phi;
-                if (chi < 0.0) chi = twopi + chi;
-                phi = fmod(phi, twopi);
-                chi = fmod(chi, twopi);
-                if (cost > 1.0) cost = 2.0 - cost;
-                if (cost < -1.0) cost = -2.0 - cost;
-                const Frame& fo = frames[ik];
-                if (l == 0) *wf = make_frame(phi, cost, chi);
-                __builtin_amdgcn_wave_barrier();
-                const Frame& fn = *wf;
-                // new density factors (t0,t1') on lane 0 and (t1',t2) on lane 1
-                double dl = 0.0;
-                if (l < 2) {
-                    if (rotden_rho(l == 0 ? frames[rb0 + k0] : fn, l == 0 ? fn : frames[rb0 + k2], tab, dl))
-                        err = 1;
-                }
-                const double d2 = __shfl(dl, 0), d3 = __shfl(dl, 1);
-                double dold = Lk[k0] * Lk[k];
-                if (fabs(dold) < RZERO) dold = 0.0;
-                if (dold < 0.0) dold = fabs(dold);
-                double dnew = d2 * d3;
-                if (fabs(dnew) < RZERO) dnew = 0.0;
-                if (dnew < 0.0) dnew = fabs(dnew);
-                const double ratio = (dold > RZERO) ? dnew / dold : 1.0;
-                // stage 1: surrogate sums at the old and the proposed orientation
-                float so = 0.0f, sn = 0.0f;
-#pragma unroll 1
-                for (int t = 0; t < 2 * nbatch; ++t) {  // (old, new) x batches of 32 rows
-                    const int which = t & 1, row = (t >> 1) * 32 + (l & 31);
-                    float f4[MLP_IN] = {0.0f, 0.0f, 0.0f, 0.0f};
-                    if (row < items) {
-                        const int it = k * RR + row / np, jp = row % np;
-                        const int j = jp + (jp >= gatom);
-                        const double4 xa = bead(beads, rp, gatom, it), xj = bead(beads, rp, j, it);
-                        const Frame& f = which ? fn : fo;
-                        mlp_pair_features(f.m, f.zn, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, f4);
-                        for (int q = 0; q < MLP_IN; ++q) f4[q] = (f4[q] - W->xm[q]) * W->xinv[q];
+            const unsigned h = 2u * (unsigned)sw + (unsigned)parity;
+            if (!(sw == 0 && parity == 0)) {
+                if (NP == 1) {
+                    // one part: the boundary slices are its own, moved in the last half
+                    if (threadIdx.x < nrot) {
+                        const int q = threadIdx.x;
+                        if (parity == 0) {
+                            fnb[q] = own(q, Prot - 1);
+                            Lk[q * L] = Lk[q * L + ns];
+                        } else {
+                            fnb[nrot + q] = own(q, 0);
+                            Lk[q * L + ns] = Lk[q * L];
+                        }
                     }
-                    // an opaque zero offset keeps the weight reads inside the loop (hoisted out
-                    // they would hold ~130 registers of weights per lane)
-                    int z = 0;
-                    __asm__ volatile("" : "+s"(z));
-                    const float o = mlp_rows32(reinterpret_cast<const MlpPacked*>(reinterpret_cast<const char*>(W) + z), f4);
-                    if (l < 32 && row < items) {
-                        if (which) sn += o;
-                        else so += o;
+                } else {
+                    const unsigned long long* src = parity == 0 ? prevlast : nextfirst;
+                    unsigned* dstf = reinterpret_cast<unsigned*>(fnb + (parity == 0 ? 0 : nrot));
+                    const unsigned want = base + h;
+                    for (int w = threadIdx.x; w < nwords; w += blockDim.x) {
+                        unsigned long long wvv;
+                        unsigned spins = 0;
+                        for (;;) {
+                            wvv = __hip_atomic_load(src + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            if ((unsigned)(wvv >> 32) == want) break;
+                            if (++spins > ROT_SPIN_LIMIT) {
+                                s_err = 2;
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(2);
+                        }
+                        const int i = w >> 1;
+                        if (i < nrot * FD) dstf[w] = (unsigned)wvv;
+                        else reinterpret_cast<unsigned*>(&Lk[(i - nrot * FD) * L + (parity == 0 ? 0 : ns)])[w & 1] = (unsigned)wvv;
                     }
                 }
-                so = wave_sum(so);
-                sn = wave_sum(sn);
-                const double dvs = ys * ((double)sn - (double)so);
-                const double rd1 = ratio * exp(-rp.tau * dvs);
-                if (rd1 > 1.0 || rd1 > rand4) {  // the same on every lane
-                    double eo = 0.0, en = 0.0;
-                    for (int i = l; i < 2 * items; i += 64) {
-                        const bool isnew = i >= items;
-                        const int ii = isnew ? i - items : i;
-                        const int it = k * RR + ii / np, jp = ii % np;
-                        const int j = jp + (jp >= gatom);
-                        const double4 xa = bead(beads, rp, gatom, it), xj = bead(beads, rp, j, it);
-                        const double v = pair_nl<R>(l1, l1.k, l3, true, false, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z,
-                                                    isnew ? fn : fo, fo);
-                        if (isnew) en += v;
-                        else eo += v;
-                    }
-                    eo = wave_sum(eo);
-                    en = wave_sum(en);
-                    const double rd2 = exp(-rp.tau * ((en - eo) - dvs));
-                    if (l == 0) ++npass;
-                    if ((rd2 > 1.0 || rd2 > rand5) && l == 0) {
-                        ++nacc;
-                        const double sint = sqrt(1.0 - cost * cost);
-                        RotBead b = cur;
-                        b.cth = cost;
-                        b.phi = phi;
-                        b.chi = chi;
-                        b.nx = sint * cos(phi);
-                        b.ny = sint * sin(phi);
-                        b.nz = cost;
-                        rotb[ik] = b;
-                        frames[ik] = *wf;
-                        Lk[k0] = d2;
-                        Lk[k] = d3;
-                    }
+                __syncthreads();
+                if (s_err) {  // a wait gave up: leave at once (see k_rotor3d_parts)
+                    sw = nsweep;
+                    break;
                 }
-                __builtin_amdgcn_wave_barrier();
             }
-            __syncthreads();
+            const int nmoves = (ns - parity + 1) / 2;  // k0 is even: local slice kl has parity (kl - 1) & 1
+            const int mine = nmoves > wv ? (nmoves - wv + nw - 1) / nw : 0;  // moves wv, wv + nw, ...
+            for (int q = 0; q < nrot; ++q) {
+                const int gq = rp.nl_atom + q;
+                for (int c0 = 0; c0 < mine; c0 += DA_MV) {
+                    const int nm = min(DA_MV, mine - c0);
+                    // (1) proposal of move l
+                    const int mv = wv + (c0 + l) * nw;
+                    const int kl = 2 * mv + parity + 1, k = k0 + kl - 1;
+                    const long long ik = rb0 + (long long)q * Prot + k;
+                    double cost = 0.0, phi = 0.0, chi = 0.0, ratio = 1.0, d2 = 0.0, d3 = 0.0, dvx = 0.0;
+                    double rand4 = 0.0, rand5 = 0.0;
+                    RotBead cur{};
+                    if (l < nm) {
+                        cur = rotb[ik];
+                        const uint4 w0 = draw(run.seed, rp.key, pass, K_ROT3D, jsweep, (uint32_t)gq, (uint32_t)k, 0);
+                        const uint4 w1 = draw(run.seed, rp.key, pass, K_ROT3D, jsweep, (uint32_t)gq, (uint32_t)k, 1);
+                        const uint4 w2 = draw(run.seed, rp.key, pass, K_ROT3D, jsweep, (uint32_t)gq, (uint32_t)k, 2);
+                        const double rand1 = u53(w0.x, w0.y), rand2 = u53(w0.z, w0.w), rand3 = u53(w1.x, w1.y);
+                        rand4 = u53(w1.z, w1.w);
+                        rand5 = u53(w2.x, w2.y);
+                        cost = cur.cth + rp.rtstep * (rand1 - 0.5);
+                        phi = cur.phi + twopi * (rp.rtstep * (rand2 - 0.5));
+                        chi = cur.chi + twopi * (rp.rtstep * (rand3 - 0.5));
+                        if (phi < 0.0) phi = twopi + phi;
+                        if (chi < 0.0) chi = twopi + chi;
+                        phi = fmod(phi, twopi);
+                        chi = fmod(chi, twopi);
+                        if (cost > 1.0) cost = 2.0 - cost;
+                        if (cost < -1.0) cost = -2.0 - cost;
+                        const Frame fn = make_frame(phi, cost, chi);
+                        prop[l] = fn;
+                        const Frame& fa = kl == 1 ? fnb[q] : own(q, k - 1);
+                        const Frame& fb = kl == ns ? fnb[nrot + q] : own(q, k + 1);
+                        if (rotden_rho(fa, fn, tab, d2)) err = 1;
+                        if (rotden_rho(fn, fb, tab, d3)) err = 1;
+                        double dold = Lk[q * L + kl - 1] * Lk[q * L + kl];
+                        if (fabs(dold) < RZERO) dold = 0.0;
+                        if (dold < 0.0) dold = fabs(dold);
+                        double dnew = d2 * d3;
+                        if (fabs(dnew) < RZERO) dnew = 0.0;
+                        if (dnew < 0.0) dnew = fabs(dnew);
+                        ratio = (dold > RZERO) ? dnew / dold : 1.0;
+                        if (net_atoms) {  // rotor-rotor pairs exact in stage 1
+                            const Frame& fo = own(q, k);
+                            for (int p = 0; p < nrot; ++p) {
+                                if (p == q) continue;
+                                for (int s = 0; s < RR; ++s) {
+                                    const int it = k * RR + s;
+                                    const double4 xa = bead(beads, rp, gq, it), xp = bead(beads, rp, rp.nl_atom + p, it);
+                                    const Frame& fp = own(p, k);
+                                    dvx += caleng_v(xa.x, xa.y, xa.z, xp.x, xp.y, xp.z, fn, fp) -
+                                           caleng_v(xa.x, xa.y, xa.z, xp.x, xp.y, xp.z, fo, fp);
+                                }
+                            }
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    // (2) network rows of the chunk
+                    double so, sn;
+                    net_rows(q, nm, kl, true, so, sn);
+                    // (3) stage 1
+                    bool pass1 = false;
+                    double dvs = 0.0;
+                    if (l < nm) {
+                        if (net_atoms) so = Vs[q * ns + kl - 1];
+                        dvs = ys * (sn - so);
+                        const double rd1 = ratio * exp(-rp.tau * (dvs + dvx));
+                        pass1 = rd1 > 1.0 || rd1 > rand4;
+                    }
+                    // (4) stage 2: exact network-pair sums of the moves that passed
+                    double dve = 0.0, ex_new = 0.0;
+                    if (net_atoms) {
+                        unsigned long long todo = __ballot(pass1);
+                        while (todo) {
+                            const int jj = __ffsll((long long)todo) - 1;
+                            todo &= todo - 1;
+                            const int klj = __shfl(kl, jj);
+                            const double e = exact_atoms(q, klj, prop[jj]);
+                            if (l == jj) {
+                                ex_new = e;
+                                dve = e - Ex[q * ns + klj - 1];
+                            }
+                        }
+                    } else if (pass1) {
+                        const Frame& fo = own(q, k);
+                        for (int p = 0; p < nrot; ++p) {
+                            if (p == q) continue;
+                            for (int s = 0; s < RR; ++s) {
+                                const int it = k * RR + s;
+                                const double4 xa = bead(beads, rp, gq, it), xp = bead(beads, rp, rp.nl_atom + p, it);
+                                const Frame& fp = own(p, k);
+                                dve += caleng_v(xa.x, xa.y, xa.z, xp.x, xp.y, xp.z, prop[l], fp) -
+                                       caleng_v(xa.x, xa.y, xa.z, xp.x, xp.y, xp.z, fo, fp);
+                            }
+                        }
+                    }
+                    // (5) stage 2 and the update
+                    if (l < nm) {
+                        nnet += (unsigned long long)rpm;
+                        if (pass1) {
+                            ++npass;
+                            const double rd2 = exp(-rp.tau * (dve - dvs));
+                            if (rd2 > 1.0 || rd2 > rand5) {
+                                ++nacc;
+                                const double sint = sqrt(1.0 - cost * cost);
+                                RotBead b = cur;
+                                b.cth = cost;
+                                b.phi = phi;
+                                b.chi = chi;
+                                b.nx = sint * cos(phi);
+                                b.ny = sint * sin(phi);
+                                b.nz = cost;
+                                rotb[ik] = b;
+                                frames[ik] = prop[l];
+                                Lk[q * L + kl - 1] = d2;
+                                Lk[q * L + kl] = d3;
+                                if (net_atoms) {
+                                    Vs[q * ns + kl - 1] = sn;
+                                    Ex[q * ns + kl - 1] = ex_new;
+                                }
+                            }
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+                __syncthreads();  // rotor q + 1 sees rotor q's moves at the same slices
+            }
+            // publish the boundary slice this half moved (k_rotor3d_parts' record)
+            if (NP > 1) {
+                const int ks = parity == 0 ? k0 : k0 + ns - 1;
+                unsigned long long* dst = mypub + (parity == 0 ? 0 : half_rec);
+                const unsigned long long tag = (unsigned long long)(base + h + 1u) << 32;
+                for (int w = threadIdx.x; w < nwords; w += blockDim.x) {
+                    const int i = w >> 1;
+                    const double v = i < nrot * FD ? reinterpret_cast<const double*>(&own(i / FD, ks))[i % FD]
+                                                   : Lk[(i - nrot * FD) * L + (parity == 0 ? 0 : ns)];
+                    const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+                    __hip_atomic_store(dst + w, tag | ((w & 1) ? (bits >> 32) : (bits & 0xffffffffull)), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
         }
     }
-    if (err) s_err = 1;
+    if (threadIdx.x == 0 && NP > 1)  // every part of the chain ran the same halves
+        __hip_atomic_store(myflag, base + 2u * (unsigned)nsweep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (err) atomicOr(&mut[r].err, 4);
     const unsigned long long sa = block_sum(nacc, red_a);
     const unsigned long long sp = block_sum(npass, red_p);
+    const unsigned long long sn_ = block_sum(nnet, red_n);
     if (threadIdx.x == 0) {
-        if (s_err) atomicOr(&mut[r].err, 4);
-        const unsigned long long moves = (unsigned long long)nsweep * Prot;
+        if (s_err) atomicOr(&mut[r].err, s_err);
+        const unsigned long long moves = (unsigned long long)nsweep * ns * nrot;
         atomicAdd(&mut[r].total[rtype][2], moves);
         atomicAdd(&mut[r].accep[rtype][2], sa);
         atomicAdd(&mut[r].da[0], moves);
         atomicAdd(&mut[r].da[1], sp);
         atomicAdd(&mut[r].da[2], sa);
-        atomicAdd(&mut[r].pair_evals[2], 2ull * sp * (unsigned long long)items);
-        atomicAdd(&mut[r].surrogate_evals, 2ull * moves * (unsigned long long)items);
+        // exact network-pair evaluations of the moves that reached stage 2
+        atomicAdd(&mut[r].pair_evals[2], (net_atoms ? 1ull : 2ull) * sp * (unsigned long long)(net_atoms ? items_x : RR * (nrot - 1)));
+        atomicAdd(&mut[r].surrogate_evals, sn_);
     }
 }
 
-// Training rows of the surrogate from the current configuration of replica r: each (rotor
-// slice k, translational slice it of k, atom partner j) gives the body-frame features of the
-// pair and its exact energy (vcord on the 3D table in FP64, PotRotE3D's atom term).
+// Training rows of the surrogate from the current configuration of replica r, for the pairs the
+// delayed-acceptance move gives the network.  With atoms: each (rotor q, rotor slice k,
+// translational slice it of k, atom a) gives the body-frame features of the pair and its exact
+// energy (vcord on the 3D table in FP64, PotRotE3D's atom term); row order q, k, it, a.  Without
+// atoms: each (rotor q, slice k, translational slice it, other rotor p) gives the rotor-pair
+// features and the TIP4P energy of caleng (PotRotE3D's rotor term).
 __global__ void k_surrogate_collect(const Rep* __restrict__ reps, const double4* __restrict__ beads,
                                     const Frame* __restrict__ frames, Tables3 T3, int r, float* __restrict__ X,
                                     float* __restrict__ y, int nrows) {
     const Rep& rp = reps[r];
-    const int np = rp.natoms - 1, RR = rp.RotRatio, gatom = rp.nl_atom;
+    const int RR = rp.RotRatio, nat = rp.nl_atom, nrot = rp.nrot, Prot = rp.Prot;
+    const int npart = nat > 0 ? nat : nrot - 1;  // partners per translational slice
     const Lut3D<double> l3 = lut3d<double>(T3, rp.pot3d_id);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nrows; i += gridDim.x * blockDim.x) {
-        const int k = i / (RR * np), rem = i - k * (RR * np);
-        const int it = k * RR + rem / np, jp = rem % np;
-        const int j = jp + (jp >= gatom);
-        const double4 xa = bead(beads, rp, gatom, it), xj = bead(beads, rp, j, it);
-        const Frame& f = frames[rp.rot_off + k];
+        const int per_q = Prot * RR * npart;
+        const int q = i / per_q, rem0 = i - q * per_q;
+        const int k = rem0 / (RR * npart), rem = rem0 - k * (RR * npart);
+        const int it = k * RR + rem / npart, jp = rem % npart;
+        const int gq = nat + q;
+        const double4 xa = bead(beads, rp, gq, it);
+        const Frame& f = frames[rp.rot_off + (long long)q * Prot + k];
         float fe[MLP_IN];
-        mlp_pair_features(f.m, f.zn, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, fe);
-        for (int q = 0; q < MLP_IN; ++q) X[(long long)i * MLP_IN + q] = fe[q];
-        y[i] = (float)vcord_v<double>(l3, f, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z);
+        double e;
+        if (nat > 0) {
+            const double4 xj = bead(beads, rp, jp, it);
+            mlp_pair_features(f.m, f.zn, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, fe);
+            e = (double)vcord_v<double>(l3, f, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z);
+        } else {
+            const int p = jp + (jp >= q);
+            const double4 xp = bead(beads, rp, nat + p, it);
+            const Frame& fp = frames[rp.rot_off + (long long)p * Prot + k];
+            mlp_rotor_pair_features(f, fp, xa.x, xa.y, xa.z, xp.x, xp.y, xp.z, fe);
+            e = caleng_v(xa.x, xa.y, xa.z, xp.x, xp.y, xp.z, f, fp);
+        }
+        for (int c = 0; c < MLP_IN; ++c) X[(long long)i * MLP_IN + c] = fe[c];
+        y[i] = (float)e;
     }
 }
 

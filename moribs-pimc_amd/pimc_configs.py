@@ -68,7 +68,8 @@ def replica(n_he, P, T, seed, Prot=None):
 
 
 def workload(name, seed=985456376, n_replicas=256):
-    """Tables and replicas of workload `name` ('c0', 'c1', 'c2', or 'tiny')."""
+    """Tables and replicas of workload `name` ('c0', 'c1', 'c2', 'tiny', 'mini' or 'classes': one
+    replica per bisection size class and boundary, N in {20, 30, 63, 64, 100} He, P 256-512)."""
     r1, v1 = lj_table()
     rg, cg, V = morse_legendre_2d()
     tabs = dict(pot1d=[(r1, v1)], pot2d=[(rg, cg, rg[1] - rg[0], cg[1] - cg[0], V)], rot=[])
@@ -86,6 +87,10 @@ def workload(name, seed=985456376, n_replicas=256):
             P = int(rng.choice([16, 32, 64]))
             T = float(fmt_temperature(float(rng.uniform(0.3, 2.0))))
             reps.append(replica(n, P, T, seed + 1 + r))
+    elif name == "classes":  # every bisection size class of the linear family, long chains
+        for r, (n, P, T) in enumerate([(20, 512, 0.41), (30, 512, 0.93), (63, 256, 1.6), (64, 512, 0.75),
+                                       (100, 512, 1.2)]):
+            reps.append(replica(n, P, float(fmt_temperature(T)), seed + 1 + r))
     elif name == "c2":
         rng = np.random.default_rng(seed)
         for r in range(n_replicas):
@@ -136,11 +141,12 @@ def nl_initial_path(n_solvent, n_rotors, P, rng, shell=(5.0, 7.5), sep=3.0):
     return coords, angles
 
 
-def nl_workload(name, seed=985456376, n_replicas=None):
+def nl_workload(name, seed=985456376, n_replicas=None, asymrho=None):
     """'c3': an asymmetric top (A, B, C = 27.88, 14.51, 9.28 cm^-1) in 32 para-H2 (12-6-8-9 pair
     potential), P = P_rot = 256, T = 0.5 K, rotor-H2 potential from a seeded Legendre expansion on
     a 100 x 181 x 91 grid, one replica.  'c4': two rigid TIP4P water rotors (caleng), T = 1 K,
-    P = P_rot = 2048, 128 replicas.  Density tables from asymrho_tables at (T, P_rot)."""
+    P = P_rot = 2048, 128 replicas.  Density tables from asymrho_tables at (T, P_rot), or from
+    asymrho(T, P_rot, A, B, C, maxj) when given (the device generator, pimc_hip.Context.asymrho)."""
     from pimc_tables import asymrho_tables, h2h2_table, legendre_3d_table
     rng = np.random.default_rng(seed)
     if name == "c3":
@@ -159,7 +165,10 @@ def nl_workload(name, seed=985456376, n_replicas=None):
         nsolv, nrot = 0, 2
     else:
         raise ValueError(name)
-    rho, eng, esq, _ = asymrho_tables(T, P, *ABC_H2O, maxj, -1)
+    if asymrho is not None:
+        rho, eng, esq = asymrho(T, P, *ABC_H2O, maxj)
+    else:
+        rho, eng, esq, _ = asymrho_tables(T, P, *ABC_H2O, maxj, -1)
     tabs["rot3d_list"] = [(rho.reshape(-1), eng.reshape(-1), esq.reshape(-1))]
     reps = []
     for r in range(nrep):

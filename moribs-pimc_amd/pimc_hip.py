@@ -65,6 +65,10 @@ SIGNATURES = {
                           _dp, _dp, _dp]),
     "pimc_set_replica": (_i, [ctypes.c_void_p, _i, ctypes.POINTER(ReplicaDesc)]),
     "pimc_commit": (_i, [ctypes.c_void_p]),
+    "pimc_check_replica": (_i, [ctypes.POINTER(ReplicaDesc), ctypes.c_char_p, _i]),
+    "pimc_plan_rotor3d_split": (_i, [_i, _i, _i, _i, _i, _i, _i]),
+    "pimc_plan_rotor_parts": (_i, [_i, _ip, _ip, _ip, _i, _i, _i, _i, _i, _i, _ip, _ip, ctypes.POINTER(ctypes.c_long),
+                                   ctypes.c_char_p, _i]),
     "pimc_set_option": (_i, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_long]),
     "pimc_set_state": (_i, [ctypes.c_void_p, _i, _dp, _dp]),
     "pimc_get_state": (_i, [ctypes.c_void_p, _i, _dp, _dp, _dp]),
@@ -92,6 +96,61 @@ SIGNATURES = {
     "pimc_surrogate_eval": (_i, [ctypes.c_void_p, _fp, _i, _fp]),
     "pimc_get_da_counters": (_i, [ctypes.c_void_p, _i, _dp]),
 }
+
+
+def replica_desc(ntypes, numb, molecule, levels, mcstep, mass, P, Prot, T, rtstep, pot1d=None, pot2d=None, rotdens=-1,
+                 pot3d=-1, rot3d=-1):
+    """A pimc_replica_desc from the qmc.input quantities of one replica."""
+    d = ReplicaDesc()
+    d.ntypes = ntypes
+    for t in range(MAX_TYPES):
+        d.numb[t] = numb[t] if t < ntypes else 0
+        d.molecule[t] = molecule[t] if t < ntypes else 0
+        d.levels[t] = levels[t] if t < ntypes else 0
+        d.mcstep[t] = mcstep[t] if t < ntypes else 0.0
+        d.mass[t] = mass[t] if t < ntypes else 1.0
+        d.pot1d[t] = (pot1d[t] if pot1d is not None and t < ntypes else -1)
+        d.pot2d[t] = (pot2d[t] if pot2d is not None and t < ntypes else -1)
+    d.rotdens = rotdens
+    d.pot3d, d.rot3d = pot3d, rot3d
+    d.P, d.Prot, d.T, d.rtstep = P, Prot, T, rtstep
+    return d
+
+
+def check_replica(desc, lib=None):
+    """pimc_check_replica: '' if the description is accepted, else the reason (no device needed)."""
+    lib = lib or load_library()
+    msg = ctypes.create_string_buffer(256)
+    rc = lib.pimc_check_replica(ctypes.byref(desc), msg, len(msg))
+    return "" if rc == 0 else msg.value.decode()
+
+
+def plan_rotor_parts(prot, rot_ratio, natoms, fp64=False, lds_kb=52, part_slices_max=0, blocks_per_cu=8,
+                     lds_per_cu=160 * 1024, cus_per_xcd=32, lib=None):
+    """pimc_plan_rotor_parts: (parts per chain, slices per part, launch LDS bytes) of the linear-rotor
+    sweep's split, for a device that holds blocks_per_cu parts per CU by registers (no device needed).
+    Raises PimcError with the library's message when no plan fits."""
+    lib = lib or load_library()
+    n = len(prot)
+    a = [np.ascontiguousarray(x, dtype=np.int32) for x in (prot, rot_ratio, natoms)]
+    W = np.zeros(n, np.int32)
+    ns = np.zeros(n, np.int32)
+    lds = ctypes.c_long(0)
+    msg = ctypes.create_string_buffer(256)
+    ip = lambda x: x.ctypes.data_as(_ip)  # noqa: E731
+    rc = lib.pimc_plan_rotor_parts(n, ip(a[0]), ip(a[1]), ip(a[2]), int(bool(fp64)), int(lds_kb), int(part_slices_max),
+                                   int(blocks_per_cu), int(lds_per_cu), int(cus_per_xcd), ip(W), ip(ns),
+                                   ctypes.byref(lds), msg, len(msg))
+    if rc != 0:
+        raise PimcError(msg.value.decode())
+    return W, ns, lds.value
+
+
+def plan_rotor3d_split(protmax, nchains, nrot, blocks_per_cu, lds_per_cu=160 * 1024, ncu=256, cus_per_xcd=32, lib=None):
+    """pimc_plan_rotor3d_split: parts per non-linear rotor chain (no device needed)."""
+    lib = lib or load_library()
+    return lib.pimc_plan_rotor3d_split(int(protmax), int(nchains), int(nrot), int(blocks_per_cu), int(lds_per_cu),
+                                       int(ncu), int(cus_per_xcd))
 
 
 def load_library(path=LIB_PATH):
@@ -197,19 +256,8 @@ class Context:
     # replicas
     def set_replica(self, r, ntypes, numb, molecule, levels, mcstep, mass, P, Prot, T, rtstep, pot1d=None, pot2d=None,
                     rotdens=-1, pot3d=-1, rot3d=-1):
-        d = ReplicaDesc()
-        d.ntypes = ntypes
-        for t in range(MAX_TYPES):
-            d.numb[t] = numb[t] if t < ntypes else 0
-            d.molecule[t] = molecule[t] if t < ntypes else 0
-            d.levels[t] = levels[t] if t < ntypes else 0
-            d.mcstep[t] = mcstep[t] if t < ntypes else 0.0
-            d.mass[t] = mass[t] if t < ntypes else 1.0
-            d.pot1d[t] = (pot1d[t] if pot1d is not None and t < ntypes else -1)
-            d.pot2d[t] = (pot2d[t] if pot2d is not None and t < ntypes else -1)
-        d.rotdens = rotdens
-        d.pot3d, d.rot3d = pot3d, rot3d
-        d.P, d.Prot, d.T, d.rtstep = P, Prot, T, rtstep
+        d = replica_desc(ntypes, numb, molecule, levels, mcstep, mass, P, Prot, T, rtstep, pot1d, pot2d, rotdens, pot3d,
+                         rot3d)
         self.descs[r] = d
         return self._check(self.lib.pimc_set_replica(self.h, r, ctypes.byref(d)))
 
@@ -301,13 +349,15 @@ class Context:
         return pe.value, bm.value
 
     def pot_energy_slice(self, reps, atoms, its, pos):
+        """PotEnergy(atom, pos, it) per query."""
         reps = np.ascontiguousarray(reps, dtype=np.int32)
         atoms = np.ascontiguousarray(atoms, dtype=np.int32)
         its = np.ascontiguousarray(its, dtype=np.int32)
         pos = _f64(pos).reshape(-1)
         out = np.empty(len(reps))
         ip = lambda a: a.ctypes.data_as(_ip)
-        self._check(self.lib.pimc_pot_energy_slice(self.h, len(reps), ip(reps), ip(atoms), ip(its), _dptr(pos), _dptr(out)))
+        self._check(self.lib.pimc_pot_energy_slice(self.h, len(reps), ip(reps), ip(atoms), ip(its), _dptr(pos),
+                                                   _dptr(out)))
         return out
 
     def estimate(self):

@@ -108,6 +108,19 @@ def pair_features(frame_m, zn, centre, partner):
     return np.array([rn, u[0], u[1], u[2] / zn])
 
 
+def rotor_pair_features(frame_q, frame_p, centre_q, centre_p):
+    """(R, zq.u, zp.u, zq.zp) of a rotor-rotor pair: the distance of the centres, the cosines of
+    each rotor's body z axis (the third row of the frame as stored, the third column of the
+    rotation matrix) with the unit separation u from rotor q to rotor p, and the cosine between
+    the two z axes."""
+    zq = np.asarray(frame_q, np.float64).reshape(3, 3)[2]
+    zp = np.asarray(frame_p, np.float64).reshape(3, 3)[2]
+    r = np.asarray(centre_p, np.float64) - np.asarray(centre_q, np.float64)
+    rn = np.linalg.norm(r)
+    u = r / rn
+    return np.array([rn, zq @ u, zp @ u, zq @ zp])
+
+
 def glorot(seed=0):
     """A random weight vector (Glorot-uniform matrices, zero biases) for tests."""
     rng = np.random.default_rng(seed)

@@ -79,3 +79,21 @@ $(OUT)/obj_gold/ref_harness.o: $(dir $(lastword $(MAKEFILE_LIST)))ref_harness.cc
 HARNO := $(addprefix $(OUT)/obj_gold/,$(addsuffix .o,$(filter-out omprng,$(CXXSRC)) $(F77SRC)))
 $(OUT)/libref.so: $(OUT)/obj_gold/ref_harness.o $(OUT)/obj_gold/mc_main.o $(HARNO)
 	$(CXX) -shared -fopenmp -o $@ $^ $(FRT) -lm
+
+# --- link check of the reference-side binding shown in INTEGRATION.md (section 2) ----------
+# The shim is extracted from the document, compiled against the reference headers and linked
+# with the reference objects and the product library into a shared object with no undefined
+# symbols: the documented binding builds as written.
+ODIR    := $(dir $(lastword $(MAKEFILE_LIST)))
+PKGDIR  := $(abspath $(ODIR)../moribs-pimc_amd)
+$(OUT)/shim:
+	mkdir -p $@
+$(OUT)/shim/gpu_shim.cc: $(ODIR)../INTEGRATION.md $(ODIR)shim_extract.py | $(OUT)/shim
+	python3 $(ODIR)shim_extract.py $< $@
+$(OUT)/shim/gpu_shim.o: $(OUT)/shim/gpu_shim.cc $(ODIR)../include/moribs_pimc.h
+	$(CXX) $(GOLD) -fPIC $(DEFS) -I$(ODIR)../include -Wall -Wno-unused-function -c $< -o $@
+$(OUT)/libgpu_shim.so: $(OUT)/shim/gpu_shim.o $(OUT)/obj_gold/mc_main.o $(GOLDO)
+	$(CXX) -shared -fopenmp -o $@ $^ $(FRT) -lm -L$(PKGDIR) -lmoribs_pimc -Wl,-rpath,$(PKGDIR) -Wl,--no-undefined
+
+.PHONY: shim
+shim: $(OUT)/libgpu_shim.so

@@ -6,6 +6,7 @@ tests of the host driver (which run moribs-pimc_amd/pimc on the same files).
                P = Prot = 4) with fewer blocks; its .rot file is the fixture CO2_T100t4.rot
   he4_co2      4 He + CO2 at 0.37 K, P = Prot = 64, lattice start; LJ He-He, Morse-Legendre
                CO2-He and the linden CO2 density, all written in the reference's formats
+  c0_8he       the 8-He workload: 8 He + CO2 at 0.37 K, P = Prot = 128, the same tables
 """
 import os
 import shutil
@@ -50,8 +51,24 @@ MCSKIP_RATIO     12800
 MCSKIP_TOTAL     12800
 MCSKIP_AVERG     64
 """,
+    "c0_8he": """MASTERDIR        ./
+OUTPUTDIR        ./
+FILENAMEPREFIX   c0he8
+DIMENSION     3
+DENSITY       0.02
+TEMPERATURE   0.37
+ATOM       He4  8    BOLTZMANN     0.15     3         He      PRIMITIVE
+MOLECULE   CO2  1    BOLTZMANN     0.08     3         CO2He   PRIMITIVE
+ROTATION   CO2   0.3  128
+NUMBEROFSLICES     128
+NUMBEROFPASSES     100
+NUMBEROFBLOCKS     {blocks}  10
+MCSKIP_RATIO     12800
+MCSKIP_TOTAL     12800
+MCSKIP_AVERG     128
+""",
 }
-BLOCKS = {"co2_100K": 205, "he4_co2": 1010}
+BLOCKS = {"co2_100K": 205, "he4_co2": 1010, "c0_8he": 310}
 
 
 def materialise(name, d, blocks=None):
@@ -64,12 +81,13 @@ def materialise(name, d, blocks=None):
         with open(os.path.join(d, "CO2_fake.pot"), "w") as f:  # the example's empty 2D table
             f.write("0 0\n0 0\n")
     else:
+        P = 128 if name == "c0_8he" else 64
         r, v = pt.lj_table()
         pt.write_pot1d(os.path.join(d, "He.pot"), r, v)
         rg, cg, V = pt.morse_legendre_2d()
         pt.write_pot2d(os.path.join(d, "CO2He.pot"), rg, cg, V)
-        c, rho, er, es = pc.rot_table(0.37, 64)
-        pt.write_rot(os.path.join(d, pt.rot_filename("CO2", 0.37, 64)), c, rho, er, es)
+        c, rho, er, es = pc.rot_table(0.37, P)
+        pt.write_rot(os.path.join(d, pt.rot_filename("CO2", 0.37, P)), c, rho, er, es)
 
 
 # A non-linear rotor deck for the driver's own tests (the reference program is not run on it):

@@ -8,7 +8,7 @@ The program is oracle/_ref/pimc, compiled from the upstream sources by oracle/re
 runs single-threaded in a scratch directory; each deck's .eng lines (one per block after the
 equilibration blocks) are kept whole, so the test can form means and standard errors itself.
 
-    python tests/golden/make_reference_runs.py
+    python tests/golden/make_reference_runs.py [deck ...]
 """
 import json
 import os
@@ -26,7 +26,9 @@ import numpy as np  # noqa: E402
 
 import decks  # noqa: E402
 
-DISTRIBUTIONS = {"co2_100K": (".rcf",), "he4_co2": (".gra", ".gri", ".grt", ".rcf")}
+DISTRIBUTIONS = {"co2_100K": (".rcf",), "he4_co2": (".gra", ".gri", ".grt", ".rcf"),
+                 "c0_8he": (".gra", ".gri", ".grt", ".rcf")}
+PREFIX = {"co2_100K": "CO2_monomer", "he4_co2": "he4co2", "c0_8he": "c0he8"}
 
 
 GROUP = 10  # consecutive blocks averaged before the spread is taken (slow modes)
@@ -71,7 +73,7 @@ def run(name):
         with open(os.path.join(d, "log.txt"), "w") as log:
             subprocess.run([exe], cwd=d, env=env, stdout=log, stderr=subprocess.STDOUT, timeout=3600)
         wall = time.time() - t0
-        prefix = "CO2_monomer" if name == "co2_100K" else "he4co2"
+        prefix = PREFIX[name]
         eng = decks.read_eng(os.path.join(d, prefix + ".eng"))
         measured = [int(b) for b in eng[:, 0]]
         dist = {ext: block_stats(d, prefix, ext, measured) for ext in DISTRIBUTIONS[name]}
@@ -80,13 +82,19 @@ def run(name):
                 program="oracle/_ref/pimc, OMP_NUM_THREADS=1")
 
 
-def main():
-    out = {name: run(name) for name in decks.DECKS}
-    with open(os.path.join(HERE, "reference_runs.json"), "w") as f:
+def main(names):
+    """Runs the named decks (all when none are named) and merges them into the fixture."""
+    path = os.path.join(HERE, "reference_runs.json")
+    out = {}
+    if names and os.path.exists(path):
+        with open(path) as f:
+            out = json.load(f)
+    for name in names or list(decks.DECKS):
+        out[name] = run(name)
+        print(name, len(out[name]["eng"]), "blocks", out[name]["wall_s"], "s", flush=True)
+    with open(path, "w") as f:
         json.dump(out, f, indent=1)
-    for name, r in out.items():
-        print(name, len(r["eng"]), "blocks", r["wall_s"], "s")
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -23,10 +23,11 @@ def rot3d():
     return tables3d.rot3d()
 
 
-def nl_workload(kind, rot3d, n_h2=4, P=8, Prot=8, seed=7):
-    """One replica of the para-H2 + rotor system ("h2o_h2") or the two-rotor system ("water2")."""
+def nl_workload(kind, rot3d, n_h2=4, P=8, Prot=8, seed=7, nch=181):
+    """One replica of the para-H2 + rotor system ("h2o_h2") or the two-rotor system ("water2");
+    nch is the chi grid of the 3D potential (91, 181 or 361 points: vcord.f:50-85 folds chi by it)."""
     r, v = pt.h2h2_table()
-    pot3d = tables3d.pot3d(nch=181)
+    pot3d = tables3d.pot3d(nch=nch)
     if kind == "h2o_h2":
         rp = dict(ntypes=2, numb=[n_h2, 1], molecule=[0, 2], levels=[2, 2], mcstep=[0.2, 0.15],
                   mass=[2.01565, 18.010565], P=P, Prot=Prot, T=0.5, rtstep=0.25, pot1d=[0, -1], pot2d=[-1, -1],
@@ -69,10 +70,11 @@ def gpu_ctx(wl, flags=0):
 
 
 @pytest.mark.parametrize("fp64", [True, False])
-@pytest.mark.parametrize("kind", ["h2o_h2", "water2"])
-def test_nl_pot_energy_slice(rot3d, kind, fp64):
-    """PotEnergy(atom, pos, it) mc_piqmc.cc:1796-1965 with vcord / caleng partners."""
-    wl = nl_workload(kind, rot3d)
+@pytest.mark.parametrize("kind,nch", [("h2o_h2", 181), ("h2o_h2", 91), ("h2o_h2", 361), ("water2", 181)])
+def test_nl_pot_energy_slice(rot3d, kind, nch, fp64):
+    """PotEnergy(atom, pos, it) mc_piqmc.cc:1796-1965 with vcord / caleng partners, on each chi
+    grid vcord folds to (91: C2v quadrant, 181: half turn, 361: full turn; vcord.f:76-85)."""
+    wl = nl_workload(kind, rot3d, nch=nch)
     rp = wl["replicas"][0]
     ctx = gpu_ctx(wl, ph.FLAG_FP64 if fp64 else 0)
     S = oracle_system(wl)
@@ -109,10 +111,10 @@ def test_nl_estimators(rot3d, kind, fp64):
 
 
 @pytest.mark.parametrize("ref_order", [False, True])
-@pytest.mark.parametrize("kind", ["h2o_h2", "water2"])
-def test_nl_trajectory_fp64(rot3d, kind, ref_order):
+@pytest.mark.parametrize("kind,nch", [("h2o_h2", 181), ("h2o_h2", 91), ("h2o_h2", 361), ("water2", 181)])
+def test_nl_trajectory_fp64(rot3d, kind, nch, ref_order):
     """Centroid, bisection and MCRotations3D moves on counter-based draws follow the oracle."""
-    wl = nl_workload(kind, rot3d)
+    wl = nl_workload(kind, rot3d, nch=nch)
     flags = ph.FLAG_FP64 | (ph.FLAG_REF_ORDER if ref_order else 0)
     ctx = gpu_ctx(wl, flags)
     ctx.run(2)

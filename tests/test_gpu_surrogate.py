@@ -144,12 +144,34 @@ def test_da_requires_a_surrogate(rot3d):
         c.run(1)
 
 
-def test_da_refused_for_two_rotors(rot3d):
-    wl = nl_workload("water2", rot3d)
-    c = ph.Context(1, seed=SEED)
-    c.set_option("delayed_acceptance", 1)
-    with pytest.raises(ph.PimcError, match="one non-linear rotor"):
-        pc.load_into_context(c, wl)
+def test_collected_rotor_rows_are_tip4p_energies(rot3d):
+    """Two rotors and no atoms: the rows are rotor-rotor pairs, (R, zq.u, zp.u, zq.zp) and the
+    TIP4P energy of caleng, in the order (rotor q, slice k, other rotor)."""
+    import oracle as O
+    wl = nl_workload("water2", rot3d, P=8, Prot=8)
+    c = make_ctx(wl)
+    X, V = c.surrogate_collect(0)
+    assert X.shape == (2 * 8, 4)
+    rp = wl["replicas"][0]
+    co, an = rp["coords"], rp["angles"]
+    P = rp["P"]
+    L = O.lib()
+    for q in range(2):
+        p = 1 - q
+        for k in range(P):
+            fr = []
+            for a in (q, p):
+                eul = np.array([an[0, a * P + k], np.arccos(an[1, a * P + k]), an[2, a * P + k]])
+                m = np.zeros(9)
+                L.or_matpre(O.P(eul), m.ctypes.data_as(O._dp))
+                fr.append((eul, m))
+            want = M.rotor_pair_features(fr[0][1], fr[1][1], co[:, q * P + k], co[:, p * P + k])
+            row = q * P + k
+            assert np.allclose(X[row], want, rtol=1e-5, atol=1e-6), (row, X[row], want)
+            e = np.zeros(1)
+            L.or_caleng(O.P(co[:, q * P + k]), O.P(co[:, p * P + k]), e.ctypes.data_as(O._dp), O.P(fr[0][0]),
+                        O.P(fr[1][0]))
+            assert abs(V[row] - e[0]) <= 1e-5 * max(1.0, abs(e[0])), (row, V[row], e[0])
 
 
 def chain_means(c, n_rep, burn, passes):
@@ -161,6 +183,73 @@ def chain_means(c, n_rep, burn, passes):
         s = c.samples(r)
         m.append([s[:, 0].mean(), s[:, 1].mean(), s[:, 2].mean(), (s[:, 0] + s[:, 1] + s[:, 2]).mean()])
     return np.array(m)
+
+
+def replicated_water(rot3d, n_rep, P=8):
+    base = nl_workload("water2", rot3d, P=P, Prot=P)
+    return dict(tables=base["tables"], replicas=[dict(base["replicas"][0]) for _ in range(n_rep)])
+
+
+@pytest.fixture(scope="module")
+def trained_water(rot3d):
+    """A rotor-rotor network trained on rows of a plain run of 64 two-rotor chains."""
+    wl = replicated_water(rot3d, 64)
+    c = make_ctx(wl)
+    xs, vs = [], []
+    c.run(20)
+    for _ in range(40):
+        c.run(5)
+        for r in range(len(wl["replicas"])):
+            x, v = c.surrogate_collect(r)
+            xs.append(x)
+            vs.append(v)
+    X, V = np.concatenate(xs), np.concatenate(vs)
+    hold = np.arange(len(V)) % 10 == 0
+    loss = c.surrogate_train(X[~hold], V[~hold], epochs=60, batch=512, lr=3e-3, seed=12)
+    theta, norm = c.surrogate_get()
+    pred = c.surrogate_eval(X[hold])
+    return dict(X=X, V=V, hold=hold, loss=loss, theta=theta, norm=norm, pred=pred)
+
+
+def test_rotor_network_fits_the_run_rows(trained_water):
+    t = trained_water
+    assert t["loss"][-1] < 0.3 * t["loss"][0]
+    err = t["pred"] - t["V"][t["hold"]]
+    rms = np.sqrt(np.mean(err ** 2))
+    print("rotor-rotor network: hold-out rms", rms, "spread", np.std(t["V"]))
+    assert rms < 0.5 * np.std(t["V"])
+
+
+def assert_same_ensemble(plain, da, n_rep):
+    for j, name in enumerate(("K", "V", "E_rot", "E")):
+        d = da[:, j].mean() - plain[:, j].mean()
+        se = np.hypot(da[:, j].std(ddof=1), plain[:, j].std(ddof=1)) / np.sqrt(n_rep)
+        assert abs(d) <= 2.0 * se, (name, da[:, j].mean(), plain[:, j].mean(), se)
+
+
+@pytest.mark.parametrize("P", [8, 64])
+def test_da_two_rotors_keeps_the_ensemble(rot3d, trained_water, P):
+    """Rotor-rotor pairs through the network as stage 1, caleng as stage 2: per-chain means of
+    K, V, E_rot and E over 64 chains agree with plain moves within two standard errors, with
+    whole chains (P = 8) and with chains split into parts (P = 64)."""
+    n_rep, burn, passes = 64, 60, 240 if P == 8 else 60
+    plain = chain_means(make_ctx(replicated_water(rot3d, n_rep, P)), n_rep, burn, passes)
+    c = make_ctx(replicated_water(rot3d, n_rep, P), da=True)
+    c.surrogate_set(trained_water["theta"], trained_water["norm"])
+    da = chain_means(c, n_rep, burn, passes)
+    assert_same_ensemble(plain, da, n_rep)
+    cnt = c.da_counters(0)
+    assert cnt[0] == (burn + passes) * P * P * 2 and cnt[2] <= cnt[1] <= cnt[0]
+
+
+def test_da_split_chain_keeps_the_ensemble(rot3d, trained):
+    """Rotor-atom network, chains of P = 64 split into parts that trade boundary records."""
+    n_rep, burn, passes = 64, 40, 80
+    plain = chain_means(make_ctx(replicated(rot3d, n_rep, P=64)), n_rep, burn, passes)
+    c = make_ctx(replicated(rot3d, n_rep, P=64), da=True)
+    c.surrogate_set(trained["theta"], trained["norm"])
+    da = chain_means(c, n_rep, burn, passes)
+    assert_same_ensemble(plain, da, n_rep)
 
 
 @pytest.mark.parametrize("net", ["trained", "random"])
