@@ -228,27 +228,31 @@ def blocked_se(x):
     return best
 
 
-def da_comparison(passes, rows_wanted=1_000_000, equil=100):
-    """Surrogate delayed acceptance against plain moves on the asymmetric-top workload: the
+def da_comparison(workload, passes, rows_wanted=1_000_000, equil=100):
+    """Surrogate delayed acceptance against plain moves on a non-linear rotor workload: the
     network is trained on exact pair energies of the run's own configurations (rows collected
-    pass by pass after equilibration), frozen, and both chains then run `passes` passes from the
-    same configuration with one E = K + V + E_rot sample per pass.  Efficiency is 1/(time *
-    sigma^2) with sigma the blocked standard error of <E>; the ratio compares the two at equal
-    sigma(<E>)."""
+    pass by pass after equilibration: rotor-atom pairs on the asymmetric top in para-H2,
+    rotor-rotor pairs on the two-water workload), frozen, and both chains then run `passes`
+    passes from the same configuration with one E = K + V + E_rot sample per replica and pass.
+    sigma(<E>) is the blocked standard error of the sample series (one replica) or the spread
+    of the replicas' means over sqrt(replicas) (many).  Efficiency is 1/(time * sigma^2); the
+    ratio compares the two at equal sigma(<E>)."""
     def note(msg):
-        print(f"[da] {msg}", file=sys.stderr, flush=True)
+        print(f"[da {workload}] {msg}", file=sys.stderr, flush=True)
 
-    wl = pc.nl_workload("c3", asymrho=device_asymrho())
-    bm_pass = float(pc.bead_moves_per_pass(wl["replicas"][0]))
-    ctx = pc.load_into_context(ph.Context(1), wl)
+    wl = pc.nl_workload(workload, asymrho=device_asymrho())
+    nrep = len(wl["replicas"])
+    bm_pass = float(sum(pc.bead_moves_per_pass(rp) for rp in wl["replicas"]))
+    ctx = pc.load_into_context(ph.Context(nrep), wl)
     ctx.run(equil)
     xs, vs, n = [], [], 0
     while n < rows_wanted:
         ctx.run(1)
-        x, v = ctx.surrogate_collect(0)
-        xs.append(x)
-        vs.append(v)
-        n += len(v)
+        for r in range(nrep):
+            x, v = ctx.surrogate_collect(r)
+            xs.append(x)
+            vs.append(v)
+            n += len(v)
     X, V = np.concatenate(xs), np.concatenate(vs)
     note(f"{len(V)} rows collected")
     hold = np.arange(len(V)) % 20 == 0
@@ -257,43 +261,62 @@ def da_comparison(passes, rows_wanted=1_000_000, equil=100):
     t_train = time.perf_counter() - t0
     theta, norm = ctx.surrogate_get()
     rms = float(np.sqrt(np.mean((ctx.surrogate_eval(X[hold]) - V[hold]) ** 2)))
-    coords, angles, _ = ctx.get_state(0)
+    state = [ctx.get_state(r) for r in range(nrep)]
     note(f"trained in {t_train:.1f} s, hold-out rms {rms:.3f} K")
+    ctx.close()
 
     def chain(da):
-        c = ph.Context(1)
+        c = ph.Context(nrep)
         if da:
             c.set_option("delayed_acceptance", 1)
         pc.load_into_context(c, wl)
-        c.set_state(0, coords, angles)
+        for r in range(nrep):
+            c.set_state(r, state[r][0], state[r][1])
         if da:
             c.surrogate_set(theta, norm)
-        c.run(5)
+        c.run(3)
         c.synchronize()
         c.clear_samples()
         dt = 0.0
-        for done in range(0, passes, 50):  # chunks with a progress line each
-            n = min(50, passes - done)
+        step = max(1, min(50, passes // 4))
+        for done in range(0, passes, step):  # chunks with a progress line each
+            k = min(step, passes - done)
             t0 = time.perf_counter()
-            c.run(n, sample_every=1)
+            c.run(k, sample_every=1)
             c.synchronize()
             dt += time.perf_counter() - t0
-            note(f"{'delayed acceptance' if da else 'plain'}: {done + n}/{passes} passes, {dt:.1f} s")
-        smp = c.samples(0)
-        e = smp[:, 0] + smp[:, 1] + smp[:, 2]
+            note(f"{'delayed acceptance' if da else 'plain'}: {done + k}/{passes} passes, {dt:.1f} s")
+        es = []
+        for r in range(nrep):
+            smp = c.samples(r)
+            es.append(smp[:, 0] + smp[:, 1] + smp[:, 2])
+        es = np.array(es)
+        if nrep == 1:
+            mean_e, sig = float(es[0].mean()), blocked_se(es[0])
+        else:
+            m = es.mean(axis=1)
+            mean_e, sig = float(m.mean()), float(m.std(ddof=1) / math.sqrt(nrep))
         out = {"seconds": round(dt, 3), "bead_moves_per_s": round(passes * bm_pass / dt, 1),
-               "mean_E": round(float(e.mean()), 4), "sigma_E": round(blocked_se(e), 5)}
+               "mean_E": round(mean_e, 4), "sigma_E": round(sig, 5)}
+        tot = acc = 0.0
+        cnt = np.zeros(4)
+        for r in range(nrep):
+            t_, a_ = c.counters(r)
+            tot += t_[-1][2]
+            acc += a_[-1][2]
+            if da:
+                cnt += c.da_counters(r)
         if da:
-            cnt = c.da_counters(0)
             out.update(stage1_pass=round(cnt[1] / cnt[0], 4), stage2_accept=round(cnt[2] / max(cnt[1], 1), 4),
-                       accept=round(cnt[2] / cnt[0], 4))
-        tot, acc = c.counters(0)
-        out["rotor_accept"] = round(float(acc[-1][2] / tot[-1][2]), 4)
+                       accept=round(cnt[2] / cnt[0], 4), network_rows=int(cnt[3]))
+        out["rotor_accept"] = round(float(acc / tot), 4)
+        c.close()
         return out
 
     plain, da = chain(False), chain(True)
     eff = (plain["seconds"] * plain["sigma_E"] ** 2) / (da["seconds"] * da["sigma_E"] ** 2)
-    return {"workload": DESC["c3"].replace("plain moves", "delayed acceptance"), "passes": passes,
+    return {"workload": DESC[workload].replace("plain moves", "delayed acceptance"), "passes": passes,
+            "network_pairs": "rotor-atom" if workload == "c3" else "rotor-rotor",
             "training": {"rows": int((~hold).sum()), "epochs": len(loss), "seconds": round(t_train, 2),
                          "loss_first_last": [round(float(loss[0]), 5), round(float(loss[-1]), 5)],
                          "holdout_rms_K": round(rms, 4), "energy_spread_K": round(float(np.std(V)), 3)},
@@ -312,8 +335,11 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--shapes", default=None, metavar="JSON",
                     help="launch options to use instead of autotuning, e.g. the launch_shapes of a bench line")
-    ap.add_argument("--da", type=int, default=0, metavar="PASSES",
-                    help="compare surrogate delayed acceptance with plain moves on the asymmetric-top workload")
+    ap.add_argument("--da", type=int, default=200, metavar="PASSES",
+                    help="passes of the surrogate delayed-acceptance comparison on the asymmetric top (0: skip)")
+    ap.add_argument("--da-water", type=int, default=10, metavar="PASSES",
+                    help="passes of the rotor-rotor delayed-acceptance comparison on the two-water workload (0: skip)")
+    ap.add_argument("--da-only", action="store_true", help="only the delayed-acceptance comparisons")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; replicas are independent chains, so N GPUs "
@@ -321,11 +347,13 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
-    if args.da:
-        r = da_comparison(args.da)
-        print(json.dumps({"metric": METRIC, "value": r["da"]["bead_moves_per_s"], "unit": "bead-moves/s",
-                          "n_gpus": 1, "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
-                          "config": {"workload": r["workload"]}, "surrogate_da": r}))
+    if args.da_only:
+        r = {}
+        if args.da:
+            r["c3"] = da_comparison("c3", args.da)
+        if args.da_water:
+            r["c4"] = da_comparison("c4", args.da_water, rows_wanted=500_000, equil=5)
+        print(json.dumps({"metric": METRIC, "surrogate_da": r}))
         return
     nonlinear = args.workload in ("c3", "c4")
     wl = pc.nl_workload(args.workload, asymrho=device_asymrho()) if nonlinear else pc.workload(args.workload)
@@ -420,10 +448,20 @@ def main():
         "profiled_ms_per_step": round(1e3 * (t3 - t2) / args.steps, 4),
         "roofline": roof,
     }
+    ctx.close()
     if not args.no_cpu:
         cb = cpu_baseline_port(wl) if nonlinear else cpu_baseline(wl)
         if cb is not None:
             out["cpu_baseline"] = cb
+    # the metric's second half: surrogate delayed acceptance against plain moves at equal
+    # sigma(<E>), rotor-atom network (asymmetric top) and rotor-rotor network (two waters)
+    da = {}
+    if args.da:
+        da["c3"] = da_comparison("c3", args.da)
+    if args.da_water:
+        da["c4"] = da_comparison("c4", args.da_water, rows_wanted=500_000, equil=5)
+    if da:
+        out["surrogate_da"] = da
     print(json.dumps(out))
 
 

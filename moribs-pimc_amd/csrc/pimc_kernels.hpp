@@ -704,67 +704,32 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
         const uint32_t jsweep = (uint32_t)(j0 + sw * jstride);
         for (int parity = 0; parity < 2; ++parity) {
             const unsigned h = 2u * (unsigned)sw + (unsigned)parity;  // half-sweeps done before this one
-            // the neighbour's boundary slice this half reads: k0-1 (even half), k1 (odd half)
-            if (!(sw == 0 && parity == 0)) {
+            // the neighbour's boundary slice this half reads: k0-1 (even half), k1 (odd half); a
+            // single part copies its own, split parts receive it inside the boundary move below
+            if (W == 1 && !(sw == 0 && parity == 0)) {
                 const int kl = parity == 0 ? 0 : ns + 1;
-                if (W == 1) {
-                    if (threadIdx.x == 0) {
-                        const int src = parity == 0 ? ns : 1;
-                        nv[3 * kl] = nv[3 * src];
-                        nv[3 * kl + 1] = nv[3 * src + 1];
-                        nv[3 * kl + 2] = nv[3 * src + 2];
-                        const int lk = parity == 0 ? 0 : ns;
-                        double p = nv[3 * lk] * nv[3 * lk + 3] + nv[3 * lk + 1] * nv[3 * lk + 4] + nv[3 * lk + 2] * nv[3 * lk + 5];
-                        Lk[lk] = rot_rho(T, trt, p);
-                    }
-                } else if (threadIdx.x < 8) {
-                    // The even half reads what the previous part published after its odd half of
-                    // the last sweep, the odd half what the next part published after its even
-                    // half of this sweep: the orientation and the density of the shared link (the
-                    // same product of the same operands this part would form), as eight words
-                    // that each carry the publishing half's number (base + h) above 32 bits of a
-                    // value.  Polling the words themselves returns the data with its check.  The
-                    // neighbour cannot publish that record again before this part's next half.
-                    const int nb = parity == 0 ? (pt.w + W - 1) % W : (pt.w + 1) % W;
-                    const unsigned long long* rec = xrec + ((size_t)(pt.fbase + nb) * 2 + (parity == 0 ? 1 : 0)) * 8;
-                    const unsigned want = base + h;
-                    unsigned long long wv = 0;
-                    unsigned spins = 0;
-                    for (;;) {
-                        wv = __hip_atomic_load(rec + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        if ((__ballot((unsigned)(wv >> 32) == want) & 0xffull) == 0xffull) break;
-                        if (++spins > ROT_SPIN_LIMIT) {
-                            s_err = 2;
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(2);
-                    }
-                    const unsigned lo = (unsigned)wv;
-                    double v[4];
-                    for (int i = 0; i < 4; ++i) {
-                        const unsigned a = __shfl(lo, 2 * i, 8), b = __shfl(lo, 2 * i + 1, 8);
-                        v[i] = __longlong_as_double((long long)(((unsigned long long)b << 32) | a));
-                    }
-                    if (threadIdx.x == 0) {
-                        nv[3 * kl] = v[0];
-                        nv[3 * kl + 1] = v[1];
-                        nv[3 * kl + 2] = v[2];
-                        Lk[parity == 0 ? 0 : ns] = v[3];
-                    }
+                if (threadIdx.x == 0) {
+                    const int src = parity == 0 ? ns : 1;
+                    nv[3 * kl] = nv[3 * src];
+                    nv[3 * kl + 1] = nv[3 * src + 1];
+                    nv[3 * kl + 2] = nv[3 * src + 2];
+                    const int lk = parity == 0 ? 0 : ns;
+                    double p = nv[3 * lk] * nv[3 * lk + 3] + nv[3 * lk + 1] * nv[3 * lk + 4] + nv[3 * lk + 2] * nv[3 * lk + 5];
+                    Lk[lk] = rot_rho(T, trt, p);
                 }
-            }
-            __syncthreads();
-            // a wait that gave up: leave at once instead of waiting again at every later half
-            // (every lane reads s_err after the barrier; it is written again only after the
-            // barrier that closes this half)
-            if (s_err) {
-                sw = nsweep;
-                break;
+                __syncthreads();
             }
             const int nmoves = (ns - parity + 1) / 2;  // k0 is even, so local slice kl has parity (kl - 1) & 1
+            // the move that reads the neighbour's slice: kl = 1 (even half) or kl = ns (odd half)
+            const bool receive = W > 1 && !(sw == 0 && parity == 0);
+            const int kl_b = parity == 0 ? 1 : ns;
             for (int mv = g; mv < nmoves; mv += groups) {
                 const int kl = 2 * mv + parity + 1;
                 const int k = k0 + kl - 1;
+                const bool bnd = receive && kl == kl_b;
+                // the lane whose link factor needs the neighbour's slice: rho(n_k-1 . n') on the
+                // lane of kl - 1 in the even half, rho(n' . n_k+1) on the lane of kl + 1 in the odd
+                const int dep = bnd ? (G == 1 ? 0 : parity) : -1;
                 // both draws of the move up front (independent, so they interleave)
                 const uint4 w0 = draw(run.seed, rp.key, pass, K_ROTLIN, jsweep, (uint32_t)ar, (uint32_t)k, 0);
                 const uint4 w1 = draw(run.seed, rp.key, pass, K_ROTLIN, jsweep, (uint32_t)ar, (uint32_t)k, 1);
@@ -775,20 +740,56 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
                 double sint = sqrt(1.0 - cost * cost);
                 double nnx = sint * cos(ph), nny = sint * sin(ph), nnz = cost;
                 // new link factors rho(n_k-1 . n') and rho(n' . n_k+1), on two lanes when G >= 2;
-                // their knots are loaded before the partner gathers, so the loads overlap
+                // their knots are loaded before the partner gathers, so the loads overlap (the
+                // factor that needs the neighbour's slice waits for it, after the gathers)
                 double rho0 = 0.0, rho1 = 0.0, pa = 0.0, pb = 0.0;
                 RhoCell ca{}, cb{};
-                if (gl < 2) {
+                auto link_a = [&]() {
                     const int kn = gl == 0 ? kl - 1 : kl + 1;
                     pa = gl == 0 ? nv[3 * kn] * nnx + nv[3 * kn + 1] * nny + nv[3 * kn + 2] * nnz
                                  : nnx * nv[3 * kn] + nny * nv[3 * kn + 1] + nnz * nv[3 * kn + 2];
                     ca = rot_rho_load(T, trt, pa);
-                }
-                if (G == 1) {
+                };
+                auto link_b = [&]() {
                     pb = nnx * nv[3 * kl + 3] + nny * nv[3 * kl + 4] + nnz * nv[3 * kl + 5];
                     cb = rot_rho_load(T, trt, pb);
-                }
+                };
+                if (gl < 2 && gl != dep) link_a();
+                if (G == 1 && !(bnd && parity == 1)) link_b();
                 double enew = slice_energy<R>(l2, geo + (long long)(kl - 1) * items, items, gl, G, nnx, nny, nnz);
+                if (bnd) {
+                    // The even half reads what the previous part published after its odd half of
+                    // the last sweep, the odd half what the next part published after its even
+                    // half of this sweep: the orientation and the density of the shared link (the
+                    // same product of the same operands this part would form), as eight words
+                    // that each carry the publishing half's number (base + h) above 32 bits of a
+                    // value.  Lanes of this move's group poll the words themselves, so the data
+                    // comes with its check; the neighbour cannot publish that record again before
+                    // this part's next half.  Only this move waits, after its partner gathers.
+                    const int nb = parity == 0 ? (pt.w + W - 1) % W : (pt.w + 1) % W;
+                    const unsigned long long* rec = xrec + ((size_t)(pt.fbase + nb) * 2 + (parity == 0 ? 1 : 0)) * 8;
+                    const unsigned want = base + h;
+                    const int klr = parity == 0 ? 0 : ns + 1;
+                    for (int w = gl; w < 8; w += G) {
+                        unsigned long long wv = 0;
+                        unsigned spins = 0;
+                        for (;;) {
+                            wv = __hip_atomic_load(rec + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            if ((unsigned)(wv >> 32) == want) break;
+                            if (++spins > ROT_SPIN_LIMIT) {
+                                s_err = 2;
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(2);
+                        }
+                        const int i = w >> 1;  // doubles: the orientation, then the link density
+                        double* dst = i < 3 ? &nv[3 * klr + i] : &Lk[parity == 0 ? 0 : ns];
+                        reinterpret_cast<unsigned*>(dst)[w & 1] = (unsigned)wv;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    if (gl == dep) link_a();
+                    if (G == 1 && parity == 1) link_b();
+                }
                 if (gl < 2) {
                     if (gl == 0) rho0 = rot_rho_eval(T, trt, pa, ca);
                     else rho1 = rot_rho_eval(T, trt, pa, ca);
@@ -822,6 +823,12 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
                 }
             }
             __syncthreads();
+            // a wait that gave up: leave at once instead of waiting again at every later half
+            // (every lane reads s_err after the barrier; it is written again only in the next half)
+            if (s_err) {
+                sw = nsweep;
+                break;
+            }
             // publish the boundary slice this half moved (k0 after the even half, k1-1 after
             // the odd half), then the count of half-sweeps done
             if (W > 1 && threadIdx.x < 8) {

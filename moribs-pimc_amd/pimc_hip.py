@@ -382,7 +382,10 @@ class Context:
     def surrogate_collect(self, r):
         """(features [n,4], exact pair energies [n]) of replica r's current configuration."""
         d = self.descs[r]
-        n = d.Prot * (d.P // d.Prot) * (sum(d.numb[: d.ntypes]) - 1)
+        nrot = sum(d.numb[t] for t in range(d.ntypes) if d.molecule[t] != 0)
+        nat = sum(d.numb[t] for t in range(d.ntypes) if d.molecule[t] == 0)
+        # rotor-atom rows with atoms, rotor-rotor rows without (pimc_surrogate_collect)
+        n = nrot * d.Prot * (d.P // d.Prot) * (nat if nat > 0 else nrot - 1)
         x = np.zeros((n, MLP_IN), np.float32)
         v = np.zeros(n, np.float32)
         got = self._check(self.lib.pimc_surrogate_collect(self.h, int(r), _fptr(x), _fptr(v), n))

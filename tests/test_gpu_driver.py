@@ -21,9 +21,10 @@ DRIVER = os.path.join(ROOT, "moribs-pimc_amd", "pimc")
 with open(os.path.join(ROOT, "tests", "golden", "reference_runs.json")) as f:
     REF = json.load(f)
 
-GPU_BLOCKS = {"co2_100K": 45, "he4_co2": 130}
+GPU_BLOCKS = {"co2_100K": 45, "he4_co2": 130, "c0_8he": 60}
 CHAINS = 8
-PREFIX = {"co2_100K": "CO2_monomer", "he4_co2": "he4co2"}
+PREFIX = {"co2_100K": "CO2_monomer", "he4_co2": "he4co2", "c0_8he": "c0he8"}
+NEQ = {"co2_100K": 5, "he4_co2": 10, "c0_8he": 10}
 
 
 def mean_se(chains):
@@ -41,7 +42,7 @@ def mean_se(chains):
 def driver_runs(tmp_path_factory):
     """The driver on each deck with CHAINS independent chains: {name: (directory, [eng per chain])}."""
     out = {}
-    for name in ("co2_100K", "he4_co2"):
+    for name in ("co2_100K", "he4_co2", "c0_8he"):
         d = str(tmp_path_factory.mktemp(name))
         decks.materialise(name, d, blocks=GPU_BLOCKS[name])
         r = subprocess.run([DRIVER, "--replicas", str(CHAINS)], cwd=d, capture_output=True, text=True, timeout=600)
@@ -53,10 +54,10 @@ def driver_runs(tmp_path_factory):
     return out
 
 
-@pytest.mark.parametrize("name", ["co2_100K", "he4_co2"])
+@pytest.mark.parametrize("name", ["co2_100K", "he4_co2", "c0_8he"])
 def test_driver_matches_reference_statistics(name, driver_runs):
     d, engs = driver_runs[name]
-    neq = 5 if name == "co2_100K" else 10
+    neq = NEQ[name]
     for e in engs:
         assert e.shape == (GPU_BLOCKS[name] - neq, 10)
         assert np.array_equal(e[:, 0], np.arange(neq + 1, GPU_BLOCKS[name] + 1))
@@ -100,7 +101,8 @@ def grouped_mean_se(v, group):
 
 
 @pytest.mark.parametrize("name,ext", [("he4_co2", ".gra"), ("he4_co2", ".gri"), ("he4_co2", ".grt"),
-                                      ("he4_co2", ".rcf"), ("co2_100K", ".rcf")])
+                                      ("he4_co2", ".rcf"), ("co2_100K", ".rcf"), ("c0_8he", ".gra"),
+                                      ("c0_8he", ".gri"), ("c0_8he", ".grt"), ("c0_8he", ".rcf")])
 def test_distributions_match_reference(name, ext, driver_runs):
     """He-He g(r), the radial and angular He density around the rotor, and the rotor's
     orientational correlation <n(tau).n(0)>: per-bin means of the driver's block files against
@@ -114,7 +116,7 @@ def test_distributions_match_reference(name, ext, driver_runs):
     slow mode (He crossing from one end of the rotor to the other) that neither run converges,
     and the two runs show it with opposite signs."""
     d, engs = driver_runs[name]
-    neq = 5 if name == "co2_100K" else 10
+    neq = NEQ[name]
     ref = REF[name]["distributions"][ext]
     v = block_columns(d, PREFIX[name], ext, range(neq + 1, GPU_BLOCKS[name] + 1))
     if ext == ".grt":  # the symmetric part of every block
