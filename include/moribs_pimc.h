@@ -213,6 +213,15 @@ int  pimc_get_work(pimc_ctx* ctx, double* pair_evals, double* bead_moves);
    the atom's bead at slice it replaced by pos[3q..3q+2] */
 int  pimc_pot_energy_slice(pimc_ctx* ctx, int nq, const int* replica, const int* atom,
                            const int* it, const double* pos, double* out);
+/* pimc_pot_energy_slice without any device allocation.  No counterpart in the reference: it is
+   for callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+   harnesses that forbid hipMalloc).  Queries, positions and results are staged in a device
+   array of the library (704 KiB, PIMC_NOALLOC_MAX_QUERIES queries) that every GPU holds for the
+   rest of the process; more queries per call are refused with an error.  Calls that overlap on
+   one GPU (other threads or contexts) share that array and race; the caller serialises them. */
+#define PIMC_NOALLOC_MAX_QUERIES 16384
+int  pimc_pot_energy_slice_noalloc(pimc_ctx* ctx, int nq, const int* replica, const int* atom,
+                                   const int* it, const double* pos, double* out);
 /* one MCGetAverage sample of every replica on the current state, out[8*replica] */
 int  pimc_estimate(pimc_ctx* ctx, double* out);
 

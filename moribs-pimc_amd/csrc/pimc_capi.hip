@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 #include <map>
+#include <mutex>
 #include <algorithm>
 
 using namespace pimc;
@@ -95,6 +96,42 @@ T* dalloc(size_t n) {
     if (n == 0) n = 1;
     if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
     return p;
+}
+
+// Staging of pimc_pot_energy_slice_noalloc: queries (replica, atom, slice: three int arrays),
+// positions and results, in a file-scope device array of which every GPU holds its own copy
+// until the process ends.  Host code never passes the array's name to a kernel or a copy (on
+// the host it is a stand-in, not a device address); it uses the address hipGetSymbolAddress
+// returned for the device, kept per device in scratch_dev.
+constexpr size_t NA_Q = PIMC_NOALLOC_MAX_QUERIES;
+constexpr size_t NA_OFF_A = NA_Q * sizeof(int), NA_OFF_I = 2 * NA_Q * sizeof(int), NA_OFF_POS = 3 * NA_Q * sizeof(int);
+constexpr size_t NA_OFF_OUT = NA_OFF_POS + 3 * NA_Q * sizeof(double);
+constexpr size_t NA_BYTES = NA_OFF_OUT + NA_Q * sizeof(double);  // 704 KiB
+static_assert(NA_OFF_A % 256 == 0 && NA_OFF_POS % 256 == 0 && NA_OFF_OUT % 256 == 0, "scratch regions stay aligned");
+__device__ __align__(256) static unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+unsigned char* scratch_dev[NA_MAX_DEVICES] = {};
+
+// the address of this device's copy of `scratch`, looked up once per device (device current)
+unsigned char* scratch_for_current_device(std::string& err) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NA_MAX_DEVICES) {
+        err = "no current device";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        scratch_dev[dev] = static_cast<unsigned char*>(p);
+    }
+    return scratch_dev[dev];
 }
 
 }  // namespace
@@ -216,6 +253,8 @@ struct pimc_ctx {
     std::vector<Pending> pending;
     double bead_moves_per_pass = 0;
     double bead_moves = 0;
+    unsigned char* scratch = nullptr;  // this device's copy of `scratch` (pimc_pot_energy_slice_noalloc)
+    std::string scratch_err;
     Tables tables() const {
         Tables T;
         T.k1f = d_k1f;
@@ -279,6 +318,7 @@ int pimc_create(int n_replicas, unsigned flags, unsigned seed, pimc_ctx** out) {
         delete c;
         return -3;
     }
+    c->scratch = scratch_for_current_device(c->scratch_err);  // reported by the entry point that uses it
     *out = c;
     return 0;
 }
@@ -1830,6 +1870,23 @@ int pimc_pot_energy_slice(pimc_ctx* c, int nq, const int* qr, const int* qa, con
     for (void* p : ps)
         if (p) (void)hipFree(p);
     return rc;
+}
+
+// The same queries staged in this device's copy of `scratch` instead of buffers allocated per
+// call: no allocation, at most PIMC_NOALLOC_MAX_QUERIES queries.  Nothing in the staging needs
+// to start at zero (every query slot read is written first, every result written by the kernel).
+int pimc_pot_energy_slice_noalloc(pimc_ctx* c, int nq, const int* qr, const int* qa, const int* qit,
+                                  const double* qpos, double* out) {
+    if (nq > PIMC_NOALLOC_MAX_QUERIES)
+        return fail(c, "pimc_pot_energy_slice_noalloc: at most " + std::to_string(PIMC_NOALLOC_MAX_QUERIES) +
+                           " queries per call (" + std::to_string(nq) + " given)");
+    if (check_queries(c, nq, qr, qa, qit)) return -1;
+    if (!c->scratch) return fail(c, "pimc_pot_energy_slice_noalloc: no staging on this device (" + c->scratch_err + ")");
+    if (nq == 0) return 0;
+    unsigned char* s = c->scratch;
+    return pot_slice_on(c, nq, qr, qa, qit, qpos, out, reinterpret_cast<int*>(s), reinterpret_cast<int*>(s + NA_OFF_A),
+                        reinterpret_cast<int*>(s + NA_OFF_I), reinterpret_cast<double*>(s + NA_OFF_POS),
+                        reinterpret_cast<double*>(s + NA_OFF_OUT));
 }
 
 int pimc_estimate(pimc_ctx* c, double* out) {

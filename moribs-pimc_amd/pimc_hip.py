@@ -84,6 +84,7 @@ SIGNATURES = {
     "pimc_get_work": (_i, [ctypes.c_void_p, _dp, _dp]),
     "pimc_get_rcf": (_i, [ctypes.c_void_p, _i, _dp, _i]),
     "pimc_pot_energy_slice": (_i, [ctypes.c_void_p, _i, _ip, _ip, _ip, _dp, _dp]),
+    "pimc_pot_energy_slice_noalloc": (_i, [ctypes.c_void_p, _i, _ip, _ip, _ip, _dp, _dp]),
     "pimc_estimate": (_i, [ctypes.c_void_p, _dp]),
     "pimc_kernel_time": (_i, [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_long)]),
     "pimc_synchronize": (_i, [ctypes.c_void_p]),
@@ -348,16 +349,17 @@ class Context:
         self._check(self.lib.pimc_get_work(self.h, ctypes.byref(pe), ctypes.byref(bm)))
         return pe.value, bm.value
 
-    def pot_energy_slice(self, reps, atoms, its, pos):
-        """PotEnergy(atom, pos, it) per query."""
+    def pot_energy_slice(self, reps, atoms, its, pos, noalloc=False):
+        """PotEnergy(atom, pos, it) per query; noalloc=True stages the queries in the library's
+        per-device array (pimc_pot_energy_slice_noalloc) instead of allocating."""
         reps = np.ascontiguousarray(reps, dtype=np.int32)
         atoms = np.ascontiguousarray(atoms, dtype=np.int32)
         its = np.ascontiguousarray(its, dtype=np.int32)
         pos = _f64(pos).reshape(-1)
         out = np.empty(len(reps))
         ip = lambda a: a.ctypes.data_as(_ip)
-        self._check(self.lib.pimc_pot_energy_slice(self.h, len(reps), ip(reps), ip(atoms), ip(its), _dptr(pos),
-                                                   _dptr(out)))
+        f = self.lib.pimc_pot_energy_slice_noalloc if noalloc else self.lib.pimc_pot_energy_slice
+        self._check(f(self.h, len(reps), ip(reps), ip(atoms), ip(its), _dptr(pos), _dptr(out)))
         return out
 
     def estimate(self):
