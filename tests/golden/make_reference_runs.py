@@ -40,10 +40,9 @@ def symmetric_part(v):
     return 0.5 * (v[..., :49] + v[..., 48::-1])
 
 
-def block_stats(d, prefix, ext, blocks):
-    """Per-bin mean over the block files <prefix><block><ext>, and its standard error from the
-    spread of averages over groups of GROUP consecutive blocks.  For .grt, the symmetric part
-    of every block is taken first."""
+def block_values(d, prefix, ext, blocks):
+    """Column 1 of the block files <prefix><block><ext>: [blocks, bins], and the bin centres.
+    For .grt, the symmetric part of every block."""
     vals = []
     for b in blocks:
         path = os.path.join(d, f"{prefix}{b:03d}{ext}")
@@ -59,27 +58,54 @@ def block_stats(d, prefix, ext, blocks):
     if ext == ".grt":
         v = symmetric_part(v)
         x = x[:49]
-    g = v[: len(v) // GROUP * GROUP].reshape(-1, GROUP, v.shape[1]).mean(axis=1)
+    return x, v
+
+
+def block_stats(runs):
+    """Per-bin mean over the blocks of every chain, and its standard error from the spread of
+    averages over groups of GROUP consecutive blocks of each chain.  runs: [(x, v)] per chain."""
+    x = runs[0][0]
+    v = np.concatenate([r[1] for r in runs])
+    g = np.concatenate([r[1][: len(r[1]) // GROUP * GROUP].reshape(-1, GROUP, r[1].shape[1]).mean(axis=1)
+                        for r in runs])
     return dict(x=x.tolist(), mean=v.mean(axis=0).tolist(), se=(g.std(axis=0, ddof=1) / np.sqrt(len(g))).tolist(),
                 blocks=len(v), group=GROUP)
 
 
-def run(name):
+def run(name, chains=1):
+    """The reference program on deck `name`, `chains` independent processes at once (its rotor
+    moves draw from a generator seeded by the wall clock, omprng.cc:21-26, so concurrent runs
+    are distinct chains).  The first chain's block lines are kept as `eng`, every chain's as
+    `eng_chains`; the distributions pool all chains."""
     exe = os.path.join(ROOT, "oracle", "_ref", "pimc")
-    with tempfile.TemporaryDirectory() as d:
-        decks.materialise(name, d)
-        env = dict(os.environ, OMP_NUM_THREADS="1")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    prefix = PREFIX[name]
+    with tempfile.TemporaryDirectory() as top:
+        dirs, procs = [], []
         t0 = time.time()
-        with open(os.path.join(d, "log.txt"), "w") as log:
-            subprocess.run([exe], cwd=d, env=env, stdout=log, stderr=subprocess.STDOUT, timeout=3600)
+        for c in range(chains):
+            d = os.path.join(top, f"chain{c}")
+            decks.materialise(name, d)
+            dirs.append(d)
+            log = open(os.path.join(d, "log.txt"), "w")
+            procs.append((subprocess.Popen([exe], cwd=d, env=env, stdout=log, stderr=subprocess.STDOUT), log))
+            time.sleep(0.05)  # distinct wall-clock seeds
+        for p, log in procs:
+            p.wait(timeout=7200)
+            log.close()
         wall = time.time() - t0
-        prefix = PREFIX[name]
-        eng = decks.read_eng(os.path.join(d, prefix + ".eng"))
-        measured = [int(b) for b in eng[:, 0]]
-        dist = {ext: block_stats(d, prefix, ext, measured) for ext in DISTRIBUTIONS[name]}
-    return dict(deck=decks.DECKS[name].format(blocks=decks.BLOCKS[name]), columns=["block", *decks.ENG_COLUMNS],
-                eng=eng.tolist(), distributions=dist, wall_s=round(wall, 1),
-                program="oracle/_ref/pimc, OMP_NUM_THREADS=1")
+        engs = [decks.read_eng(os.path.join(d, prefix + ".eng")) for d in dirs]
+        measured = [int(b) for b in engs[0][:, 0]]
+        dist = {ext: block_stats([block_values(d, prefix, ext, measured) for d in dirs]) for ext in DISTRIBUTIONS[name]}
+    out = dict(deck=decks.DECKS[name].format(blocks=decks.BLOCKS[name]), columns=["block", *decks.ENG_COLUMNS],
+               eng=engs[0].tolist(), distributions=dist, wall_s=round(wall, 1),
+               program="oracle/_ref/pimc, OMP_NUM_THREADS=1")
+    if chains > 1:
+        out["eng_chains"] = [e.tolist() for e in engs]
+    return out
+
+
+CHAINS = {"c0_8he": 8}
 
 
 def main(names):
@@ -90,7 +116,7 @@ def main(names):
         with open(path) as f:
             out = json.load(f)
     for name in names or list(decks.DECKS):
-        out[name] = run(name)
+        out[name] = run(name, CHAINS.get(name, 1))
         print(name, len(out[name]["eng"]), "blocks", out[name]["wall_s"], "s", flush=True)
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
