@@ -150,6 +150,12 @@ int  pimc_plan_rotor3d_split(int protmax, int nchains, int nrot, int blocks_per_
                           as a two-stage test with the surrogate network as the first stage
                           (see the surrogate section below); needs one non-linear rotor among
                           atoms with a 3D potential, and a surrogate loaded before pimc_run.
+     "samples_per_pass"   k >= 1 (default 1): a sampled pass takes k estimator samples inside it,
+                          after every P/k time steps in the reference's order, or every S/k
+                          phases in segment order (S = 2^levels; a phase moves P/S time steps'
+                          worth of slices), as the reference samples every MCSKIP_AVERG time
+                          steps (mc_main.cc:383-386).  Needs the same P and S in every replica
+                          and k dividing P (reference order) or S (segment order).
    One option may be set at any time, also after pimc_commit:
      "profile"            1: time every launch with HIP events (pimc_kernel_time), issuing passes
                           launch by launch; 0: replay the captured pass graph (the default). */

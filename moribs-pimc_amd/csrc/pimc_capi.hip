@@ -243,7 +243,9 @@ struct pimc_ctx {
     bool mlp_ready = false;
     std::vector<float> theta, norm;  // host copies: weights, then xm[4] xs[4] ym ys
     long long nbeads = 0, nrotb = 0, nhist = 0;
-    hipGraphExec_t graph_exec = nullptr;
+    hipGraphExec_t graph_exec = nullptr;    // one pass
+    hipGraphExec_t graph_exec_s = nullptr;  // one pass with samples_per_pass samples inside it
+    int opt_samples_per_pass = 1;
     bool graph_ok = false;
     std::map<std::string, Family> fam;
     struct Pending {
@@ -330,6 +332,7 @@ void pimc_destroy(pimc_ctx* c) {
         (void)hipEventDestroy(p.b);
     }
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph_exec_s) (void)hipGraphExecDestroy(c->graph_exec_s);
     void* ptrs[] = {c->d_rep, c->d_mut, c->d_beads, c->d_rotb, c->d_hist, c->d_samples, c->d_k1f, c->d_k1d,
                     c->d_t1, c->d_v2f, c->d_v2c, c->d_v2d, c->d_g2d, c->d_g2f, c->d_t2, c->d_rk, c->d_re, c->d_tr,
                     c->d_work, c->d_all, c->d_rotreps, c->d_est, c->d_rows, c->d_rowbuf, c->d_parts, c->d_flags, c->d_xrec,
@@ -670,6 +673,11 @@ int pimc_set_option(pimc_ctx* c, const char* name, long value) {
         c->opt_rotor_lds_kb = (int)value;
         return 0;
     }
+    if (std::strcmp(name, "samples_per_pass") == 0) {
+        if (value < 1 || value > 4096) return fail(c, "samples_per_pass is 1 to 4096");
+        c->opt_samples_per_pass = (int)value;
+        return 0;
+    }
     if (std::strcmp(name, "delayed_acceptance") == 0) {
         if (value != 0 && value != 1) return fail(c, "delayed_acceptance is 0 or 1");
         c->opt_da = value == 1;
@@ -968,6 +976,15 @@ int pimc_commit(pimc_ctx* c) {
         c->Smax = std::max(c->Smax, S);
         c->Pmax = std::max(c->Pmax, p.P);
         c->max_types = std::max(c->max_types, p.ntypes);
+    }
+    if (c->opt_samples_per_pass > 1) {  // samples inside a pass: one time grid for every replica
+        const int k = c->opt_samples_per_pass;
+        for (int r = 0; r < c->nrep; ++r)
+            if (c->rep[r].P != c->Pmax || c->rep[r].S != c->Smax)
+                return fail(c, "samples_per_pass > 1 needs the same P and segment size in every replica");
+        if ((c->ref_order ? c->Pmax : c->Smax) % k)
+            return fail(c, c->ref_order ? "samples_per_pass must divide P"
+                                        : "samples_per_pass must divide the segment size 2^levels");
     }
     if (std::max(lds_b, lds_b1) > 160 * 1024)
         return fail(c, "segment window exceeds LDS (too many atoms for this segment size)");
@@ -1525,9 +1542,12 @@ void launch_rotor_linear(pimc_ctx* c, int u) {
                        c->d_beads, c->d_rotb, c->tables(), c->tables3(), c->d_parts, c->d_flags, c->d_xrec, run);
 }
 
-// one pass in the order described at the top of this file
+// one pass in the order described at the top of this file; with `sampled` and samples_per_pass
+// k > 1, an estimator sample after every P/k time steps (reference order) or every S/k phases
+// (segment order), as the reference samples every MCSKIP_AVERG time steps (mc_main.cc:383-386)
 template <typename R>
-void issue_pass(pimc_ctx* c) {
+void issue_pass(pimc_ctx* c, bool sampled) {
+    const int k = sampled ? c->opt_samples_per_pass : 1;
     if (c->ref_order) {
         for (int t = 0; t < c->Pmax; ++t) {
             for (int ty = 0; ty < c->max_types; ++ty) {
@@ -1535,6 +1555,7 @@ void issue_pass(pimc_ctx* c) {
                 launch_bisect<R>(c, t, ty, ty + 1);
             }
             launch_rotor<R>(c, t);
+            if (k > 1 && (t + 1) % (c->Pmax / k) == 0) launch_estimate<R>(c, nullptr);
         }
     } else {
         for (int u = 0; u < c->Smax; ++u) {
@@ -1547,6 +1568,7 @@ void issue_pass(pimc_ctx* c) {
                 launch_bisect<R>(c, u, 0, PIMC_MAX_TYPES);
             }
             launch_rotor<R>(c, u);
+            if (k > 1 && (u + 1) % (c->Smax / k) == 0) launch_estimate<R>(c, nullptr);
         }
     }
     hipLaunchKernelGGL(k_pass_end, dim3((c->nrep + 255) / 256), dim3(256), 0, c->stream, c->d_mut, c->nrep);
@@ -1574,21 +1596,24 @@ int pimc_run(pimc_ctx* c, int n_passes, int sample_every) {
     if (c->opt_da && c->nparts3 && !c->mlp_ready)
         return fail(c, "delayed acceptance needs a surrogate: pimc_surrogate_train or pimc_surrogate_set first");
     for (int p = 0; p < n_passes; ++p) {
+        const bool sampled = sample_every > 0 && (p + 1) % sample_every == 0;
+        const bool inside = sampled && c->opt_samples_per_pass > 1;  // samples within the pass
         bool use_graph = !c->profile;
-        if (use_graph && !c->graph_exec) {
+        hipGraphExec_t& gx = inside ? c->graph_exec_s : c->graph_exec;
+        if (use_graph && !gx) {
             hipGraph_t g;
             HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            if (c->fp64) issue_pass<double>(c);
-            else issue_pass<float>(c);
+            if (c->fp64) issue_pass<double>(c, inside);
+            else issue_pass<float>(c, inside);
             HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
-            HIPCHECK(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+            HIPCHECK(c, hipGraphInstantiate(&gx, g, nullptr, nullptr, 0));
             (void)hipGraphDestroy(g);
         }
-        if (use_graph) HIPCHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
-        else if (c->fp64) issue_pass<double>(c);
-        else issue_pass<float>(c);
+        if (use_graph) HIPCHECK(c, hipGraphLaunch(gx, c->stream));
+        else if (c->fp64) issue_pass<double>(c, inside);
+        else issue_pass<float>(c, inside);
         HIPCHECK(c, hipGetLastError());
-        if (sample_every > 0 && (p + 1) % sample_every == 0) {
+        if (sampled && !inside) {
             if (c->fp64) launch_estimate<double>(c, nullptr);
             else launch_estimate<float>(c, nullptr);
         }
@@ -1708,10 +1733,11 @@ int pimc_autotune(pimc_ctx* c, int passes, long* chosen) {
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     for (void* q : {(void*)sb, (void*)sr, (void*)sm}) (void)hipFree(q);
-    if (c->graph_exec) {  // launch shapes changed: the next pass is captured again
-        (void)hipGraphExecDestroy(c->graph_exec);
-        c->graph_exec = nullptr;
-    }
+    for (hipGraphExec_t* gx : {&c->graph_exec, &c->graph_exec_s})
+        if (*gx) {  // launch shapes changed: the next pass is captured again
+            (void)hipGraphExecDestroy(*gx);
+            *gx = nullptr;
+        }
     if (rc == 0 && chosen) {
         for (int k = 0; k < 3; ++k) chosen[k] = c->bw_lanes[k];
         chosen[3] = c->rotor_lds_kb;

@@ -30,8 +30,10 @@
 //
 // Time steps and passes.  The reference counts "steps" (one time slice of one pass) and
 // samples when the step count is a multiple of MCSKIP_AVERG (mc_main.cc:340-484).  A pass on
-// the device moves every slice at once, so sampling, sum lines and ratio lines are taken at
-// pass granularity: every max(1, MCSKIP_AVERG / NUMBEROFSLICES) passes, and so on.
+// the device moves every slice at once; sampling is taken every max(1, MCSKIP_AVERG /
+// NUMBEROFSLICES) passes, and, when MCSKIP_AVERG is below NUMBEROFSLICES, k times inside each
+// pass (library option samples_per_pass: after every P/k time steps in the reference's order,
+// every S/k segment phases otherwise).  Sum lines and ratio lines are taken at pass granularity.
 //
 // Usage:  pimc [qmc.input] [--seed N] [--fp64] [--ref-order] [--replicas R] [--autotune] [--init-only]
 // --autotune picks launch shapes by timing them on the deck before the first block (results
@@ -858,6 +860,21 @@ int main(int argc, char** argv) {
         die("pimc_create", c.ctx ? pimc_last_error(c.ctx) : "no HIP device");
     pimc_replica_desc desc = load_tables(d, c);
     for (int r = 0; r < nrep; ++r) c.check(pimc_set_replica(c.ctx, r, &desc), "pimc_set_replica");
+    // MCSKIP_AVERG below NUMBEROFSLICES: samples inside a pass, after every P/k time steps in the
+    // reference's order or every S/k phases in segment order; k the largest divisor of that grid
+    // not above P / MCSKIP_AVERG
+    int per_pass = 1;
+    if (d.skip_averg < d.P) {
+        int S = 1;
+        for (auto& s : d.types) S = std::max(S, 1 << s.levels);
+        const long grid = (flags & PIMC_FLAG_REF_ORDER) ? d.P : S;
+        for (long k = std::min<long>(grid, d.P / d.skip_averg); k >= 1; --k)
+            if (grid % k == 0) {
+                per_pass = (int)k;
+                break;
+            }
+        if (per_pass > 1) c.check(pimc_set_option(c.ctx, "samples_per_pass", per_pass), "pimc_set_option");
+    }
     c.check(pimc_commit(c.ctx), "pimc_commit");
     for (int r = 0; r < nrep; ++r) c.check(pimc_set_state(c.ctx, r, coords.data(), angles.data()), "pimc_set_state");
     if (d.restart)
@@ -926,19 +943,19 @@ int main(int argc, char** argv) {
                 Chain& ch = chains[r];
                 int got = 0;
                 if (measured) {
-                    samples.resize(8 * (size_t)(m / every + 1));
-                    got = pimc_get_samples(c.ctx, r, samples.data(), (int)(m / every + 1));
+                    samples.resize(8 * (size_t)(m / every * per_pass + 1));
+                    got = pimc_get_samples(c.ctx, r, samples.data(), (int)(m / every * per_pass + 1));
                     c.check(got, "pimc_get_samples");
                 }
                 int next = 0;
                 bool ratio_line = false;
                 for (long q = done + 1; q <= done + m; ++q) {
                     ratio_line |= crosses(q, d.skip_ratio);
-                    if (measured && (q - done) % every == 0 && next < got) {
-                        ch.block.add(&samples[8 * (size_t)next]);
-                        ch.total.add(&samples[8 * (size_t)next]);
-                        ++next;
-                    }
+                    if (measured && (q - done) % every == 0)
+                        for (int i = 0; i < per_pass && next < got; ++i, ++next) {
+                            ch.block.add(&samples[8 * (size_t)next]);
+                            ch.total.add(&samples[8 * (size_t)next]);
+                        }
                     if (measured && crosses(q, d.skip_total) && ch.block.count > 0) {
                         ch.sums_count += 1.0;
                         save_sum_energy(ch, d, N);

@@ -89,6 +89,7 @@ def lib():
             ("or_rotations_3d", _i, [_vp, _i, _u, _u]),
             ("or_pass", _i, [_vp, _u, _i]),
             ("or_run", _i, [_vp, _u, _i, _i, _i, _dp]),
+            ("or_pass_sampled", _i, [_vp, _u, _i, _i, _dp]),
             ("or_set_tape", _i, [_vp, _i, _dp, _l]),
             ("or_tape_pos", _l, [_vp, _i]),
             ("or_tape_short", _i, [_vp]),
@@ -182,6 +183,14 @@ class System:
         if rc < 0:
             raise RuntimeError(self.L.or_last_error(self.h).decode())
         return out[: 8 * ns].reshape(ns, 8)
+
+    def pass_sampled(self, pass_, order, per_pass):
+        """One pass with per_pass samples inside it (or_pass_sampled): [per_pass, 8]."""
+        out = np.zeros(8 * per_pass)
+        rc = self.L.or_pass_sampled(self.h, pass_, order, per_pass, out.ctypes.data_as(_dp))
+        if rc < 0:
+            raise RuntimeError(self.L.or_last_error(self.h).decode())
+        return out[: 8 * rc].reshape(rc, 8)
 
     def sample(self):
         out = np.zeros(8)

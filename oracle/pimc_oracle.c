@@ -1530,6 +1530,45 @@ void or_sample(or_sys* s, double out[8]) {
     out[4] = sCv; out[5] = sCv_trans; out[6] = sCv_rot; out[7] = rterm;
 }
 
+/* One pass sampled per_pass times inside it, as the reference samples when its count of time
+   steps is a multiple of MCSKIP_AVERG (mc_main.cc:348-386): in reference order after time
+   step t when (t + 1) % (P / per_pass) == 0; in segment order after phase u (the bisections
+   starting at u, u + S, ... and the rotor sweeps u, u + S, ...) when (u + 1) % (S / per_pass)
+   == 0.  Returns the number of samples written to out (8 values each), -1 on error. */
+int or_pass_sampled(or_sys* s, uint32_t pass, int order, int per_pass, double* out) {
+    int P = s->P, ns = 0;
+    if (per_pass < 1) return fail(s, "per_pass must be at least 1");
+    if (order == OR_ORDER_REFERENCE) {
+        if (P % per_pass) return fail(s, "samples per pass must divide P");
+        for (int t = 0; t < P; t++) {
+            for (int type = 0; type < s->ntypes; type++) {
+                if (t == 0) or_molecular_move(s, type, pass);
+                or_bisection_move(s, type, t, pass);
+                if (rot_sweep(s, type, pass, (uint32_t)t) < 0) return -1;
+            }
+            if ((t + 1) % (P / per_pass) == 0) or_sample(s, out + 8 * (ns++));
+        }
+        return ns;
+    }
+    int S = 1;
+    for (int type = 0; type < s->ntypes; ++type)
+        if (s->type[type].mlsegm > S) S = s->type[type].mlsegm;
+    if (P % S != 0) return fail(s, "segment order needs P divisible by the segment size");
+    if (S % per_pass) return fail(s, "samples per pass must divide the segment size");
+    for (int u = 0; u < S; ++u) {
+        for (int type = 0; type < s->ntypes; type++) {
+            if (u == 0) or_molecular_move(s, type, pass);
+            for (int atom = 0; atom < s->type[type].numb; ++atom)
+                for (int k = 0; k < P / S; ++k) or_bisection_atom(s, type, atom, u + k * S, pass);
+        }
+        for (int type = 0; type < s->ntypes; type++)
+            for (int k = 0; k < P / S; ++k)
+                if (rot_sweep(s, type, pass, (uint32_t)(u + k * S)) < 0) return -1;
+        if ((u + 1) % (S / per_pass) == 0) or_sample(s, out + 8 * (ns++));
+    }
+    return ns;
+}
+
 int or_run(or_sys* s, uint32_t pass0, int npasses, int order, int every, double* out) {
     int ns = 0;
     for (int n = 0; n < npasses; ++n) {

@@ -118,6 +118,9 @@ int  or_rotations_3d(or_sys* s, int type, uint32_t pass, uint32_t sweep);
 int  or_pass(or_sys* s, uint32_t pass, int order);
 /* samples every `every` passes after each pass; out [npasses/every][8] (may be NULL) */
 int  or_run(or_sys* s, uint32_t pass0, int npasses, int order, int every, double* out);
+/* one pass with per_pass samples inside it (mc_main.cc:348-386 with MCSKIP_AVERG = P/per_pass
+   time steps); returns the samples written to out [per_pass][8] */
+int  or_pass_sampled(or_sys* s, uint32_t pass, int order, int per_pass, double* out);
 
 /* tape mode: draws taken from per-stream tapes in the reference's call order
    (1 rnd1, 2 rnd2, 3 rnd3, 8 gauss r1, 9 gauss r2, 15 runif) */
