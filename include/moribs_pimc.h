@@ -147,9 +147,10 @@ int  pimc_plan_rotor3d_split(int protmax, int nchains, int nrot, int blocks_per_
                           24, at most 64 and more atoms; defaults 64, 128, 256.  Results do not
                           depend on them.
      "delayed_acceptance" 1: the non-linear rotor move (MCRot3Dstep mc_piqmc.cc:938-1199) runs
-                          as a two-stage test with the surrogate network as the first stage
-                          (see the surrogate section below); needs one non-linear rotor among
-                          atoms with a 3D potential, and a surrogate loaded before pimc_run.
+                          its test as three Metropolis factors: the density, the surrogate
+                          network, then the exact correction (see the surrogate section below);
+                          needs atoms with a 3D potential (rotor-atom network) or two rotors
+                          (rotor-rotor network), and a surrogate loaded before pimc_run.
      "samples_per_pass"   k >= 1 (default 1): a sampled pass takes k estimator samples inside it,
                           after every P/k time steps in the reference's order, or every S/k
                           phases in segment order (S = 2^levels; a phase moves P/S time steps'
@@ -262,7 +263,8 @@ int  pimc_surrogate_set(pimc_ctx* ctx, const float* theta, const float* norm);
 int  pimc_surrogate_get(pimc_ctx* ctx, float* theta, float* norm);
 /* the sampler's evaluation (BF16 matrix cores) on raw features: out[n] energies in Kelvin */
 int  pimc_surrogate_eval(pimc_ctx* ctx, const float* x, int n, float* out);
-/* out[4]: delayed-acceptance proposals, stage-1 passes, accepted moves, network evaluations */
+/* out[4]: delayed-acceptance proposals, proposals that passed stages 0 and 1 (charged the exact
+   network-pair sums), accepted moves, network evaluations */
 int  pimc_get_da_counters(pimc_ctx* ctx, int replica, double* out);
 
 #ifdef __cplusplus

@@ -1333,14 +1333,18 @@ __global__ __launch_bounds__(256) void k_rcf(const Rep* __restrict__ reps, const
 }
 
 // ------------------------------------------------------------------------- delayed acceptance
-// MCRotations3D (mc_piqmc.cc:692-779) with the one-stage test of MCRot3Dstep (:1166-1178) split
-// in two (delayed acceptance, Christen & Fox 2005).  Stage 1 accepts with
-// min(1, rho_new/rho_old * exp(-tau (dVs + dVx))), where Vs is the surrogate network summed over
-// the slice's network pairs and Vx the exact energy of the other pairs; only proposals that pass
-// are charged the exact sums of the network pairs, and stage 2 accepts them with
-// min(1, exp(-tau (dV_net - dVs))).  The product of the two tests is in detailed balance with
-// the exact target for any surrogate, so the stationary distribution is the one-stage move's.
-// Stage 1 uses the one-stage test's uniform (rand4); stage 2 draws a fifth (counter word 3 = 2).
+// MCRotations3D (mc_piqmc.cc:692-779) with the one-stage test of MCRot3Dstep (:1166-1178),
+// min(1, rho_new/rho_old * exp(-tau dV)), split into three factors (multi-stage delayed
+// acceptance, Christen & Fox 2005), each a Metropolis test of its own:
+//   stage 0  min(1, rho_new/rho_old), the density factor, which the move has in hand anyway;
+//   stage 1  min(1, exp(-tau (dVs + dVx))), Vs the surrogate network summed over the slice's
+//            network pairs and Vx the exact energy of the other pairs;
+//   stage 2  min(1, exp(-tau (dV_net - dVs))), the exact network-pair sums.
+// Each factor is in detailed balance with its own part of the target, so their product is in
+// detailed balance with rho * exp(-tau V) for any surrogate: the stationary distribution is the
+// one-stage move's.  A proposal is charged the network only when it passes stage 0, and the exact
+// network-pair sums only when it passes stage 1.  Stage 0 takes its uniform from counter word 3
+// = 2 (second half), stage 1 the one-stage test's (rand4), stage 2 counter word 3 = 2 (first).
 // Network pairs: rotor-atom pairs when the replica has atoms (rotor-rotor pairs of two rotors are
 // then exact in stage 1), rotor-rotor pairs when it has none.
 //
@@ -1348,12 +1352,13 @@ __global__ __launch_bounds__(256) void k_rcf(const Rep* __restrict__ reps, const
 // between half-sweeps (a part of W = 1 is a whole chain and trades with itself).  Within a half,
 // rotor q in turn (a move of rotor q at slice k sees the moves of the rotors before it at k, as
 // in the one-stage kernel); each wave takes its moves in chunks of DA_MV, lane j the chunk's move
-// j: (1) proposal, new link densities and the exact stage-1 terms; (2) the chunk's network rows,
-// 32 per pass through the matrix cores, summed per move in row order; (3) stage 1; (4) stage 2
-// for the moves that passed -- the whole wave sums one move's rotor-atom pairs, or each lane its
-// move's rotor-rotor pairs; (5) updates.  The surrogate and exact rotor-atom sums of every own
-// slice at its current orientation are cached (atoms do not move during the sweep): a move
-// evaluates only its proposal, and an accepted move leaves its new sums as the cached ones.
+// j: (1) proposal, new link densities and the exact stage-1 terms; (2) stage 0, then the network
+// rows of the moves that passed it, listed first, 32 per pass through the matrix cores, summed
+// per move in row order; (3) stage 1; (4) stage 2 for the moves that passed -- the whole wave
+// sums one move's rotor-atom pairs, or each lane its move's rotor-rotor pairs; (5) updates.  The
+// surrogate and exact rotor-atom sums of every own slice at its current orientation are cached
+// (atoms do not move during the sweep): a move evaluates only its proposal, and an accepted move
+// leaves its new sums as the cached ones.
 // LDS: the packed network; Lk [nrot][ns+1], Vs and Ex [nrot][ns] (doubles); the neighbours'
 // boundary frames [2 nrot]; per wave DA_MV proposal frames and 32 row outputs (DA_MV and the
 // LDS size: pimc_plan.hpp).
@@ -1447,10 +1452,10 @@ __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps
         const Frame& fb = kl == ns ? fnb[nrot + q] : own(q, k0 + kl);
         if (rotden_rho(fa, fb, tab, Lk[i])) err = 1;
     }
-    // The network's rows for slices of rotor q, summed per slice in row order.  Slice list:
-    // lane j < nm holds the slice (local kl) of chunk entry j and, for the proposal rows, its
-    // frame is prop[j]; sums of (old, new) rows return in so/sn on lane j.
-    auto net_rows = [&](int q, int nm, int kl_j, bool proposal, double& so, double& sn) {
+    // The network's rows for slices of rotor q, summed per slice in row order.  Entry j < nm of
+    // the list is the move of lane src (src_j on lane j): its slice is that lane's kl_j and, for
+    // the proposal rows, its frame prop[src]; sums of (old, new) rows return in so/sn on lane j.
+    auto net_rows = [&](int q, int nm, int kl_j, int src_j, bool proposal, double& so, double& sn) {
         so = 0.0;
         sn = 0.0;
         const int total = nm * rpm;
@@ -1458,7 +1463,8 @@ __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps
             const int row = b + (l & 31);
             // the row's move and its slice, read from the move's lane with every lane active
             const int jj = min(row, total - 1) / rpm;
-            const int klr = __shfl(kl_j, jj);
+            const int sl = __shfl(src_j, jj);
+            const int klr = __shfl(kl_j, sl);
             float f[MLP_IN] = {0.0f, 0.0f, 0.0f, 0.0f};
             if (row < total) {
                 const int idx = row - jj * rpm;
@@ -1468,7 +1474,7 @@ __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps
                     const int s = idx / nat, a = idx - s * nat;
                     const int it = k * RR + s;
                     const double4 xa = bead(beads, rp, gq, it), xj = bead(beads, rp, a, it);
-                    const Frame& fq = proposal ? prop[jj] : own(q, k);
+                    const Frame& fq = proposal ? prop[sl] : own(q, k);
                     mlp_pair_features(fq.m, fq.zn, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, f);
                 } else {
                     const int which = idx & 1, rest = idx >> 1;
@@ -1476,7 +1482,7 @@ __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps
                     const int p = pp + (pp >= q);
                     const int it = k * RR + s;
                     const double4 xq = bead(beads, rp, gq, it), xp = bead(beads, rp, rp.nl_atom + p, it);
-                    const Frame& fq = (proposal && which) ? prop[jj] : own(q, k);
+                    const Frame& fq = (proposal && which) ? prop[sl] : own(q, k);
                     mlp_rotor_pair_features(fq, own(p, k), xq.x, xq.y, xq.z, xp.x, xp.y, xp.z, f);
                 }
 #pragma unroll
@@ -1519,7 +1525,7 @@ __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps
             for (int c0 = wv * DA_MV; c0 < ns; c0 += nw * DA_MV) {
                 const int nm = min(DA_MV, ns - c0);
                 double so, sn;
-                net_rows(q, nm, c0 + l + 1, false, so, sn);
+                net_rows(q, nm, c0 + l + 1, l, false, so, sn);
                 if (l < nm) Vs[q * ns + c0 + l] = sn;
                 for (int j = 0; j < nm; ++j) {
                     const double e = exact_atoms(q, c0 + j + 1, own(q, k0 + c0 + j));
@@ -1584,7 +1590,7 @@ __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps
                     const int kl = 2 * mv + parity + 1, k = k0 + kl - 1;
                     const long long ik = rb0 + (long long)q * Prot + k;
                     double cost = 0.0, phi = 0.0, chi = 0.0, ratio = 1.0, d2 = 0.0, d3 = 0.0, dvx = 0.0;
-                    double rand4 = 0.0, rand5 = 0.0;
+                    double rand4 = 0.0, rand5 = 0.0, rand6 = 0.0;
                     RotBead cur{};
                     if (l < nm) {
                         cur = rotb[ik];
@@ -1594,6 +1600,7 @@ __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps
                         const double rand1 = u53(w0.x, w0.y), rand2 = u53(w0.z, w0.w), rand3 = u53(w1.x, w1.y);
                         rand4 = u53(w1.z, w1.w);
                         rand5 = u53(w2.x, w2.y);
+                        rand6 = u53(w2.z, w2.w);
                         cost = cur.cth + rp.rtstep * (rand1 - 0.5);
                         phi = cur.phi + twopi * (rp.rtstep * (rand2 - 0.5));
                         chi = cur.chi + twopi * (rp.rtstep * (rand3 - 0.5));
@@ -1631,16 +1638,28 @@ __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps
                         }
                     }
                     __builtin_amdgcn_wave_barrier();
-                    // (2) network rows of the chunk
-                    double so, sn;
-                    net_rows(q, nm, kl, true, so, sn);
-                    // (3) stage 1
+                    // (2) stage 0, the density factor alone; the network rows of the moves that
+                    // pass it, listed first in lane order (slot: a lane's place in that order,
+                    // src: the lane whose move fills this lane's place)
+                    const bool pass0 = l < nm && (ratio > 1.0 || ratio > rand6);
+                    const unsigned long long m0 = __ballot(pass0);
+                    const int n0 = __popcll(m0);
+                    const int below = __popcll(m0 & ((1ull << l) - 1ull));
+                    const int slot = pass0 ? below : n0 + (l - below);
+                    const int src = __builtin_amdgcn_ds_permute(slot << 2, l);
+                    double so = 0.0, sn = 0.0;
+                    if (n0) {
+                        net_rows(q, n0, kl, src, true, so, sn);
+                        so = __shfl(so, slot);
+                        sn = __shfl(sn, slot);
+                    }
+                    // (3) stage 1, the surrogate factor
                     bool pass1 = false;
                     double dvs = 0.0;
-                    if (l < nm) {
+                    if (pass0) {
                         if (net_atoms) so = Vs[q * ns + kl - 1];
                         dvs = ys * (sn - so);
-                        const double rd1 = ratio * exp(-rp.tau * (dvs + dvx));
+                        const double rd1 = exp(-rp.tau * (dvs + dvx));
                         pass1 = rd1 > 1.0 || rd1 > rand4;
                     }
                     // (4) stage 2: exact network-pair sums of the moves that passed
@@ -1671,7 +1690,7 @@ __global__ __launch_bounds__(256) void k_rotor3d_da(const Rep* __restrict__ reps
                         }
                     }
                     // (5) stage 2 and the update
-                    if (l < nm) {
+                    if (pass0) {
                         nnet += (unsigned long long)rpm;
                         if (pass1) {
                             ++npass;

@@ -159,7 +159,7 @@ inline size_t rotor3d_part_lds(int ns, int nrot, size_t frame_bytes) {
 // packed network (mlp_bytes), the link densities and the two cached sums per own slice
 // (nrot x (3 ns + 1) doubles, padded to two), the neighbours' boundary frames and DA_MV proposal
 // frames per wave, then 32 row outputs per wave.
-constexpr int DA_MV = 16;  // moves a wave takes at once in the delayed-acceptance sweep
+constexpr int DA_MV = 64;  // moves a wave takes at once in the delayed-acceptance sweep: one per lane
 inline size_t rotor3d_da_part_lds(int ns, int nrot, size_t frame_bytes, size_t mlp_bytes) {
     return mlp_bytes + (size_t)((nrot * (ns + 1) + 2 * nrot * ns + 1) & ~1) * sizeof(double) +
            (size_t)(2 * nrot + 4 * DA_MV) * frame_bytes + (size_t)4 * 32 * sizeof(float);

@@ -2,9 +2,9 @@
 
 Kernels against the FP64 oracle (oracle/mlp_oracle.py): the loss and gradient of the training
 GEMMs (BF16 operands, FP32 sums), and the sampler's matrix-core evaluation.  Then the learned
-part in use: rows collected from the run's own configurations, training, the two-stage move
-keeping the plain move's ensemble (within two standard errors) whatever the network, and what
-a trained network saves against an untrained one."""
+part in use: rows collected from the run's own configurations, training, the delayed-acceptance
+move (density, network and exact factors) keeping the plain move's ensemble (within two standard
+errors) whatever the network, and what a trained network saves against an untrained one."""
 import numpy as np
 import pytest
 
@@ -234,7 +234,7 @@ def test_da_two_rotors_keeps_the_ensemble(rot3d, trained_water, P):
     whole chains (P = 8) and with chains split into parts (P = 64).  Both methods continue the
     same equilibrated chains (a plain run from the cold start): the four features of a rotor
     pair leave the twist of the hydrogens about the dipole axes out, so the network misjudges
-    some downhill moves of the cold start and the two-stage move relaxes from it more slowly
+    some downhill moves of the cold start and the delayed-acceptance move relaxes from it more slowly
     than the plain move, although both keep the same equilibrium."""
     n_rep, (equil, burn, passes) = 64, ((300, 20, 240) if P == 8 else (600, 20, 300))
     wl = replicated_water(rot3d, n_rep, P)
@@ -294,9 +294,12 @@ def test_da_keeps_the_ensemble(rot3d, trained, net):
 
 
 def test_trained_network_saves_exact_evaluations(rot3d, trained):
-    """Exact PotRotE3D sums per accepted move: a trained first stage lets through few proposals
-    that the exact stage then rejects; with an untrained (constant) network stage 1 is the
-    density test alone."""
+    """Exact PotRotE3D sums per accepted move: a trained network lets through few proposals
+    that the exact stage then rejects; with an untrained (constant) network the move is the
+    density factor followed by the exact one, and every proposal that passes the density is
+    charged the exact sums.  The network's factor can only lower the acceptance (a product of
+    three Metropolis factors accepts at most as often as the product of two), and it lowers it
+    little when the network is accurate."""
     out = {}
     for net in ("trained", "constant"):
         c = make_ctx(replicated(rot3d, 32), da=True)
@@ -309,6 +312,5 @@ def test_trained_network_saves_exact_evaluations(rot3d, trained):
         out[net] = (cnt[1] / cnt[2], cnt[2] / cnt[1], cnt[2] / cnt[0])
     print("exact sums per accepted move, stage-2 acceptance, acceptance:", out)
     assert out["trained"][1] > 0.9
-    assert out["trained"][0] < out["constant"][0]
-    # the two-stage acceptance approaches the one-stage one as the first stage improves
-    assert out["trained"][2] >= out["constant"][2] - 0.01
+    assert out["trained"][0] < 0.85 * out["constant"][0]
+    assert out["constant"][2] - 0.03 <= out["trained"][2] <= out["constant"][2] + 0.005
