@@ -219,3 +219,19 @@ def test_c4_fp32_total_v(c4):
     ctx = gpu_ctx(c4, 0)
     ctx.run(1)
     assert_fp32_v(ctx, c4, _nl_tabs(c4), angle_rows=slice(0, 3))
+
+
+@pytest.fixture(scope="module")
+def c4_many(generator):
+    return pc.nl_workload("c4", n_replicas=128, asymrho=_asymrho(generator))
+
+
+def test_c4_128_replicas_trajectory_fp64(c4_many):
+    """The full two-water workload: 128 chains of 2048 slices split into a few long parts, with
+    one lane per move and several moves per lane in each half-sweep.  A replica's chain after one
+    pass against the oracle's."""
+    ctx = gpu_ctx(c4_many, ph.FLAG_FP64)
+    ctx.run(1)
+    r = 77
+    st, cn = oracle_run("c4_many", c4_many, r, 1, O.ORDER_SEGMENTS, _nl_tabs(c4_many))
+    assert_same_chain(ctx, r, st, cn, c4_many["replicas"][r], angle_rows=slice(0, 3))
