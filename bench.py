@@ -113,7 +113,10 @@ def cpu_baseline(wl, budget_s=12.0):
     if not os.path.exists(exe):
         return None
     cores = max(1, min(16, os.cpu_count() or 1))
-    reps = wl["replicas"][:cores]
+    # one deck per core: the first replicas, or a workload's replicas repeated when it has fewer
+    # (the same work per process; the reference seeds its translational moves alike, so repeated
+    # decks are copies of one chain, which changes nothing in the time)
+    reps = [wl["replicas"][i % len(wl["replicas"])] for i in range(cores)]
     rate_guess = 2.5e5  # bead moves per second per core, only to size the runs
     jobs = []
     tmp = tempfile.mkdtemp(prefix="pimc_cpu_")
@@ -145,8 +148,9 @@ def cpu_baseline(wl, budget_s=12.0):
         bm_sum += pc.bead_moves_per_pass(rp)
         t_sum += max(per_pass, 1e-9)
     return dict(value=cores * bm_sum / t_sum, unit="bead-moves/s", cores=cores, kind="reference", cpu=cpu_model(),
-                sample=f"first {len(reps)} replicas of the workload, upstream pimc (-Ofast, 1 thread each), "
-                       f"{cores} concurrent processes; per-pass time from two pass counts")
+                sample=f"{'first ' + str(cores) + ' replicas' if len(wl['replicas']) >= cores else 'the workload repeated to ' + str(cores) + ' decks'}"
+                       f" of the workload, upstream pimc (-Ofast, 1 thread each), {cores} concurrent processes; "
+                       f"per-pass time from two pass counts")
 
 
 def _reap(procs, times):
