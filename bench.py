@@ -355,8 +355,8 @@ def main():
     ap.add_argument("--da-only", action="store_true", help="only the delayed-acceptance comparisons")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; replicas are independent chains, so N GPUs "
-                                   "would each run their own share with no communication (see DESIGN.md)",
+        print(json.dumps({"error": "bench.py measures one GPU; the replicas are independent chains, so N GPUs "
+                                   "would each run their own share of them with no communication",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
