@@ -1,7 +1,7 @@
 """Potential and density-matrix tables in MoRiBS-PIMC's file formats.
 
 Generators (closed forms the workloads are defined on) and readers/writers that keep the
-reference layouts, so the same files drive the reference program and this build:
+reference layouts, so the same files drive the reference program and the host driver here:
   1D pair potential  '.pot'  two columns r [Angstrom], V [K]; lines whose first token is '#'
                              are skipped (get_filesize/read_datafile, mc_poten.cc:731-788)
   2D rotor potential '.pot'  'nr nc / dr dc / r grid / cos grid / V[r][cos]' (init_pot2D,
