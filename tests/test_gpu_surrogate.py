@@ -262,6 +262,28 @@ def test_da_two_rotors_keeps_the_ensemble(rot3d, trained_water, P):
     assert cnt[0] == (burn + passes) * P * P * 2 and cnt[2] <= cnt[1] <= cnt[0]
 
 
+def test_da_fp64_keeps_the_ensemble(rot3d, trained):
+    """The FP64 build of the delayed-acceptance sweep (pair terms and table look-ups in FP64, the
+    network unchanged) against FP64 plain moves, rotor-atom network, 64 chains."""
+    n_rep, burn, passes = 64, 100, 300
+
+    def ctx(da):
+        c = ph.Context(n_rep, flags=ph.FLAG_FP64, seed=SEED)
+        if da:
+            c.set_option("delayed_acceptance", 1)
+        pc.load_into_context(c, replicated(rot3d, n_rep))
+        if da:
+            c.surrogate_set(trained["theta"], trained["norm"])
+        return c
+
+    plain = chain_means(ctx(False), n_rep, burn, passes)
+    c = ctx(True)
+    da = chain_means(c, n_rep, burn, passes)
+    assert_same_ensemble(plain, da, n_rep)
+    cnt = c.da_counters(0)
+    assert cnt[0] == (burn + passes) * 8 * 8 and cnt[2] <= cnt[1] <= cnt[0] and cnt[3] > 0
+
+
 def test_da_split_chain_keeps_the_ensemble(rot3d, trained):
     """Rotor-atom network, chains of P = 64 split into parts that trade boundary records."""
     n_rep, burn, passes = 64, 40, 80
