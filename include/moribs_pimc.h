@@ -189,7 +189,9 @@ int  pimc_run(pimc_ctx* ctx, int n_passes, int sample_every);
    centroid part target. */
 int  pimc_autotune(pimc_ctx* ctx, int passes, long* chosen);
 /* samples of one replica, 8 doubles each: K, V, E_rot, E_rot^2 term (ErotSQ), Cv, Cv_trans,
-   Cv_rot, Erot_termSQ (mc_main.cc:551-616).  Returns the number copied. */
+   Cv_rot, Erot_termSQ (mc_main.cc:551-616).  Returns the number copied.  A replica keeps up to
+   4096 samples between pimc_clear_samples calls; if it took more, the later ones were not
+   stored and this call fails instead of returning a truncated series. */
 int  pimc_get_samples(pimc_ctx* ctx, int replica, double* out, int max_samples);
 int  pimc_clear_samples(pimc_ctx* ctx);
 /* MCTotal/MCAccep of one replica, [ntypes][3] */

@@ -1756,6 +1756,10 @@ int pimc_get_samples(pimc_ctx* c, int r, double* out, int max_samples) {
     Mut m;
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     HIPCHECK(c, hipMemcpy(&m, c->d_mut + r, sizeof(Mut), hipMemcpyDeviceToHost));
+    if ((long long)m.nsamples > c->sample_cap)
+        return fail(c, "replica " + std::to_string(r) + " took " + std::to_string(m.nsamples) + " samples, more than the " +
+                           std::to_string(c->sample_cap) + " its buffer holds: read and clear them (pimc_clear_samples) " +
+                           "at least every " + std::to_string(c->sample_cap) + " samples");
     int n = std::min<int>(std::min<int>((int)m.nsamples, c->sample_cap), max_samples);
     if (n > 0)
         HIPCHECK(c, hipMemcpy(out, c->d_samples + c->rep[r].sample_off, sizeof(double) * 8 * n, hipMemcpyDeviceToHost));

@@ -905,7 +905,9 @@ int main(int argc, char** argv) {
     const long P = d.P;
     const long every = std::max(1L, d.skip_averg / P);
     const long ratio_passes = std::max(1L, d.skip_ratio / P);
-    const long chunk = every * std::max(1L, std::min(1024L, ratio_passes / every));
+    // passes per library call: at most 1024 sampled passes, and no more samples than a replica's
+    // buffer holds (4096, read and cleared after each call)
+    const long chunk = every * std::max(1L, std::min(std::min(1024L, 4096L / per_pass), ratio_passes / every));
     auto crosses = [&](long q, long skip) { return (q * P) / skip > ((q - 1) * P) / skip; };
 
     const int nt = (int)d.types.size();

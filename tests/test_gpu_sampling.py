@@ -53,3 +53,16 @@ def test_samples_per_pass_must_divide_the_grid():
     c.set_option("samples_per_pass", 3)
     with pytest.raises(ph.PimcError, match="divide"):
         pc.load_into_context(c, wl)
+
+
+def test_reading_more_samples_than_the_buffer_holds_fails():
+    """A replica stores 4096 samples between clears; reading after more were taken raises
+    instead of returning a truncated series, and a clear starts over."""
+    wl = pc.workload("tiny")
+    c = ctx_with(wl, ph.FLAG_REF_ORDER, 16)  # a sample after every time step of P = 16
+    c.run(257, sample_every=1)  # 4112 samples
+    with pytest.raises(ph.PimcError, match="4096"):
+        c.samples(0)
+    c.clear_samples()
+    c.run(2, sample_every=1)
+    assert c.samples(0).shape == (32, 8)
