@@ -157,9 +157,9 @@ def _reap(procs, times):
     """The reference's main() returns 1 after a complete run (mc_main.cc:507); completion is
     told by the block energies it writes at the end of the block."""
     while True:
-        for k, (i, n, t0, p, d) in enumerate(procs):
+        for k, (i, n, t0, p, d, *eng) in enumerate(procs):
             if p.poll() is not None:
-                if p.returncode not in (0, 1) or not os.path.exists(os.path.join(d, "gr.eng")):
+                if p.returncode not in (0, 1) or not os.path.exists(os.path.join(d, eng[0] if eng else "gr.eng")):
                     raise RuntimeError(f"reference pimc failed on replica {i} ({p.returncode})")
                 times[(i, n)] = time.perf_counter() - t0
                 procs.pop(k)
@@ -186,6 +186,94 @@ def cpu_baseline_port(wl, budget_s=15.0):
     dt = time.perf_counter() - t0
     return dict(value=n * pc.bead_moves_per_pass(rp) / dt, unit="bead-moves/s", cores=1, kind="port", cpu=cpu_model(),
                 sample=f"{n} pass(es) of the first replica on the oracle (FP64, one host core)")
+
+
+def write_deck_nl(d, rp, passes, shared):
+    """qmc.input and xyz.init of a non-linear rotor replica for the reference program, with links
+    to the table files written once in `shared` (init_pot1D, init_pot3D, init_rot3D)."""
+    names = ["H2" if t == "pH2" else t for t in rp["type_names"]]  # the reference's mass table
+    lines = ["MASTERDIR ./", "OUTPUTDIR ./", "FILENAMEPREFIX nl", "DIMENSION 3", "DENSITY 0.02",
+             f"TEMPERATURE {pt.fmt_temperature(rp['T'])}"]
+    rotor = None
+    for t in range(rp["ntypes"]):
+        m = rp["molecule"][t]
+        kw = "ATOM" if m == 0 else ("MOLECULE" if m == 1 else "NONLINEAR")
+        fpot = "pH2" if m == 0 else ("rotor3d" if rp.get("pot3d", -1) >= 0 else "none")
+        lines.append(f"{kw} {names[t]} {rp['numb'][t]} BOLTZMANN {rp['mcstep'][t]} {rp['levels'][t]} {fpot} PRIMITIVE")
+        if m == 2:
+            rotor = names[t]
+    P = rp["P"]
+    lines += [f"ROTATION {rotor} {rp['rtstep']} {rp['Prot']}", f"NUMBEROFSLICES {P}", f"NUMBEROFPASSES {passes}",
+              "NUMBEROFBLOCKS 1 0", "MCSKIP_RATIO 1000000000", "MCSKIP_TOTAL 1000000000", f"MCSKIP_AVERG {P}",
+              "READMCCOORDS"]
+    with open(os.path.join(d, "qmc.input"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    for name in os.listdir(shared):
+        os.symlink(os.path.join(shared, name), os.path.join(d, name))
+    co, an = rp["coords"], rp["angles"]
+    with open(os.path.join(d, "xyz.init"), "w") as f:
+        f.write("0\ncomment\n")
+        for i in range(co.shape[1]):
+            f.write(f"X {co[0,i]:.17g} {an[0,i]:.17g} {co[1,i]:.17g} {an[1,i]:.17g} {co[2,i]:.17g} {an[2,i]:.17g}\n")
+    return rotor
+
+
+def cpu_baseline_reference_nl(wl, budget_s=60.0):
+    """The reference program on a non-linear rotor workload: its density tables (181*361*361
+    values, three files) and 3D potential written once as the text it reads, then one single-
+    threaded process per host core on the first replica's deck, per-pass time from two pass
+    counts as for the linear workloads.  Opt-in (--cpu-reference): the tables are 1.7 GB of text
+    and each process parses them first."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "pimc")
+    if not os.path.exists(exe):
+        return None
+    cores = max(1, min(16, os.cpu_count() or 1))
+    rp = wl["replicas"][0]
+    t = wl["tables"]
+    tmp = tempfile.mkdtemp(prefix="pimc_cpu_nl_")
+    shared = os.path.join(tmp, "tables")
+    os.makedirs(shared)
+    if t["pot1d"]:
+        pt.write_pot1d(os.path.join(shared, "pH2.pot"), *t["pot1d"][0])
+    if t.get("pot3d_list"):
+        pt.write_pot3d(os.path.join(shared, "rotor3d.pot"), *t["pot3d_list"][0])
+    rotor = [n for n, m in zip(rp["type_names"], rp["molecule"]) if m == 2][0]
+    pt.write_rot3d(os.path.join(shared, f"{rotor}_T{pt.fmt_temperature(rp['T'])}t{rp['Prot']}"), *t["rot3d_list"][0])
+    print("[cpu reference] tables written", file=sys.stderr, flush=True)
+    bm = pc.bead_moves_per_pass(rp)
+    # about budget_s of passes beyond the first, so that the difference outweighs the spread of
+    # the table parsing between processes
+    n1 = 1
+    n2 = n1 + max(2, int(budget_s * 2.0e5 / bm))
+    jobs = []
+    for i in range(cores):
+        for n in (n1, n2):
+            d = os.path.join(tmp, f"r{i}_{n}")
+            os.makedirs(d)
+            write_deck_nl(d, rp, n, shared)
+            jobs.append((i, n, d))
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    times, procs = {}, []
+
+    def reap():
+        _reap(procs, times)
+        print(f"[cpu reference] {len(times)}/{len(jobs)} runs done", file=sys.stderr, flush=True)
+
+    for i, n, d in jobs:
+        procs.append((i, n, time.perf_counter(), subprocess.Popen([exe], cwd=d, env=env, stdout=subprocess.DEVNULL,
+                                                                  stderr=subprocess.DEVNULL), d, "nl.eng"))
+        if len(procs) >= cores:
+            reap()
+    while procs:
+        reap()
+    shutil.rmtree(tmp, ignore_errors=True)
+    t_sum = 0.0
+    for i in range(cores):
+        (m1, t1), (m2, t2) = sorted((n, x) for (j, n), x in times.items() if j == i)
+        t_sum += max((t2 - t1) / (m2 - m1), 1e-9)
+    return dict(value=cores * cores * bm / t_sum, unit="bead-moves/s", cores=cores, kind="reference", cpu=cpu_model(),
+                sample=f"the first replica's deck repeated to {cores} decks, upstream pimc (-Ofast, 1 thread each), "
+                       f"{cores} concurrent processes after parsing the tables; per-pass time from {n1} and {n2} passes")
 
 
 def traffic_per_launch(workload_tag, kernel):
@@ -345,6 +433,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-reference", action="store_true",
+                    help="non-linear workloads: time the reference program (writes 1.7 GB of text tables) "
+                         "instead of the oracle on one core")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--shapes", default=None, metavar="JSON",
                     help="launch options to use instead of autotuning, e.g. the launch_shapes of a bench line")
@@ -463,7 +554,10 @@ def main():
     }
     ctx.close()
     if not args.no_cpu:
-        cb = cpu_baseline_port(wl) if nonlinear else cpu_baseline(wl)
+        if nonlinear:
+            cb = cpu_baseline_reference_nl(wl) if args.cpu_reference else cpu_baseline_port(wl)
+        else:
+            cb = cpu_baseline(wl)
         if cb is not None:
             out["cpu_baseline"] = cb
     # the metric's second half: surrogate delayed acceptance against plain moves at equal

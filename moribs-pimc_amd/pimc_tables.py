@@ -321,6 +321,25 @@ def write_pot2d(path, rgrid, cgrid, V, dr=None, dc=None):
         np.savetxt(f, np.asarray(V).reshape(-1), fmt="%.17g")
 
 
+def write_pot3d(path, nr, nth, nch, rmin, rmax, v):
+    """The 3D rotor-partner potential as init_pot3D reads it (mc_poten.cc:283-297): the header
+    'Rgrd THgrd CHgrd Rvmin Rvmax', then one value a line in the table's order."""
+    with open(path, "w") as f:
+        f.write(f"{nr} {nth} {nch} {rmin!r} {rmax!r}\n")
+        np.asarray(v, dtype=np.float64).reshape(-1).tofile(f, sep="\n", format="%.17g")
+        f.write("\n")
+
+
+def write_rot3d(stem, rho, erot, esq):
+    """An asymmetric top's density, energy and energy-square tables as init_rot3D reads them
+    (mc_poten.cc:440-500): stem + '.rho', '.eng', '.esq', 181*361*361 values each in the
+    (theta, phi, chi) order of rotpro_sub.f:7-26, whitespace-separated."""
+    for ext, a in ((".rho", rho), (".eng", erot), (".esq", esq)):
+        with open(stem + ext, "w") as f:
+            np.asarray(a, dtype=np.float64).reshape(-1).tofile(f, sep="\n", format="%.17g")
+            f.write("\n")
+
+
 def read_pot2d(path):
     with open(path) as f:
         tok = f.read().split()
