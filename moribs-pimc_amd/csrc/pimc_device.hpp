@@ -122,10 +122,11 @@ struct Tables3 {
 
 // rotation matrix of the Euler angles of one rotor slice (matpre, rotden.f:136-163), Fortran
 // column order m[(j-1)*3 + (i-1)] = rotmat(i, j), so the body axes are its three columns;
-// zn = |z axis| as vcord's dotang takes it (vcord.f:176-188)
-struct Frame {
+// zn = |z axis| as vcord's dotang takes it (vcord.f:176-188).  80 bytes, 16-byte aligned: five
+// 16-byte loads per frame, and the split water sweep's LDS copies leave room for two parts per CU
+struct alignas(16) Frame {
     double m[9];
-    double zn, pad0, pad1;
+    double zn;
 };
 
 constexpr long long ROT3D_SIZE = 181LL * 361 * 361;  // rotpro_sub.f:7-26, mc_setup.h:116

@@ -1212,8 +1212,15 @@ __global__ __launch_bounds__(256) void k_rotor3d_parts(const Rep* __restrict__ r
                         const int it = k * RR + i / (nrot - 1), pp = i % (nrot - 1);
                         const int p = pp + (pp >= q);
                         const double4 xa = bead(beads, rp, gatom, it), xj = bead(beads, rp, rp.nl_atom + p, it);
-                        eo += caleng_v(xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, gf[q], gf[p]);
-                        en += caleng_v(xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, gf[nrot], gf[p]);
+                        // the two orientations in turn, not interleaved: one caleng's sites
+                        // live at a time, which keeps the kernel within 256 registers (two
+                        // waves per SIMD, so two parts of a water chain share a CU)
+#pragma unroll 1
+                        for (int o = 0; o < 2; ++o) {
+                            const double e = caleng_v(xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, gf[o ? nrot : q], gf[p]);
+                            if (o) en += e;
+                            else eo += e;
+                        }
                     }
                     for (int o = G >> 1; o > 0; o >>= 1) {  // the sums' shuffles interleave
                         ea += __shfl_xor(ea, o, G);

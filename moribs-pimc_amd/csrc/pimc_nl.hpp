@@ -30,7 +30,6 @@ __host__ __device__ inline Frame make_frame(double phi, double cth, double chi) 
     double d = 0.0;
     for (int i = 6; i < 9; ++i) d = d + f.m[i] * f.m[i];
     f.zn = sqrt(d);
-    f.pad0 = f.pad1 = 0.0;
     return f;
 }
 

@@ -116,9 +116,10 @@ def test_rotor3d_split_fits_one_xcd():
     other for ever."""
     assert ph.plan_rotor3d_split(2048, 1, 2, blocks_per_cu=2) == 64
     assert ph.plan_rotor3d_split(2048, 1, 2, blocks_per_cu=1) == 32
-    # 128 chains: parts of 1024 slices need 90 KB of LDS (one per CU), and 2 x 128 parts fill the
-    # device's 256 CUs; 4 parts of 512 slices (80 KB) would need 512
-    assert ph.plan_rotor3d_split(2048, 128, 2, blocks_per_cu=2) == 2
+    # 128 chains: parts of 512 slices need 76 KB of LDS, so two share a CU when the registers allow
+    # two waves per SIMD, and 4 x 128 parts fill the device's 256 CUs twice; at one block per CU
+    # 2 x 128 parts of 1024 slices (92 KB) fill it once
+    assert ph.plan_rotor3d_split(2048, 128, 2, blocks_per_cu=2) == 4
     assert ph.plan_rotor3d_split(2048, 128, 2, blocks_per_cu=1) == 2
     # short chains: parts keep at least 16 slices
     assert ph.plan_rotor3d_split(64, 1, 1, blocks_per_cu=2) == 4
