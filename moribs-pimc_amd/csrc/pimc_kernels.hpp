@@ -608,8 +608,8 @@ __device__ __forceinline__ double slice_energy(const Lut2D<R>& l2, const vec4_t<
 // phi, cth, nv[3], E, Lk over ns+2 entries (double), then the geometry block
 // geo[(it - k0*RR) * np + jp] (vec4).  E[kl] is the partner energy of slice kl at its current
 // axis and Lk[kl] the density factor rho(n_kl . n_kl+1) of the link kl -> kl+1: the "old"
-// values of a move are the "new" values of the last accepted move that changed them, or are
-// recomputed from the same operands when a neighbour's slice changes.
+// values of a move are the "new" values of the last accepted move that changed them (for a
+// link to a neighbour part, the value that part published with its slice).
 template <typename R>
 __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut* __restrict__ mut,
                                                const double4* __restrict__ beads, RotBead* __restrict__ rotb,
@@ -704,21 +704,10 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
         const uint32_t jsweep = (uint32_t)(j0 + sw * jstride);
         for (int parity = 0; parity < 2; ++parity) {
             const unsigned h = 2u * (unsigned)sw + (unsigned)parity;  // half-sweeps done before this one
-            // the neighbour's boundary slice this half reads: k0-1 (even half), k1 (odd half); a
-            // single part copies its own, split parts receive it inside the boundary move below
-            if (W == 1 && !(sw == 0 && parity == 0)) {
-                const int kl = parity == 0 ? 0 : ns + 1;
-                if (threadIdx.x == 0) {
-                    const int src = parity == 0 ? ns : 1;
-                    nv[3 * kl] = nv[3 * src];
-                    nv[3 * kl + 1] = nv[3 * src + 1];
-                    nv[3 * kl + 2] = nv[3 * src + 2];
-                    const int lk = parity == 0 ? 0 : ns;
-                    double p = nv[3 * lk] * nv[3 * lk + 3] + nv[3 * lk + 1] * nv[3 * lk + 4] + nv[3 * lk + 2] * nv[3 * lk + 5];
-                    Lk[lk] = rot_rho(T, trt, p);
-                }
-                __syncthreads();
-            }
+            // the neighbour's boundary slice this half reads: k0-1 (even half), k1 (odd half).
+            // Split parts receive it inside the boundary move below.  A single part is its own
+            // neighbour: local slices 0 and ns are one slice, as are 1 and ns + 1, and links 0 and
+            // ns one link, so the move that changes one copy writes the other (below)
             const int nmoves = (ns - parity + 1) / 2;  // k0 is even, so local slice kl has parity (kl - 1) & 1
             // the move that reads the neighbour's slice: kl = 1 (even half) or kl = ns (odd half)
             const bool receive = W > 1 && !(sw == 0 && parity == 0);
@@ -818,6 +807,13 @@ __global__ __launch_bounds__(256) void k_rotor(const Rep* __restrict__ reps, Mut
                             E[kl] = enew;
                             Lk[kl - 1] = rho0;
                             Lk[kl] = rho1;
+                            if (W == 1 && (kl == 1 || kl == ns)) {  // the other copy of the slice and link
+                                const int km = kl == 1 ? ns + 1 : 0;
+                                nv[3 * km] = nnx;
+                                nv[3 * km + 1] = nny;
+                                nv[3 * km + 2] = nnz;
+                                Lk[kl == 1 ? ns : 0] = kl == 1 ? rho0 : rho1;
+                            }
                         }
                     }
                 }
