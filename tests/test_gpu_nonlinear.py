@@ -218,6 +218,33 @@ def test_nl_trajectory_split_chain_fp64(rot3d, kind, ref_order):
     assert np.abs(ag[:, rot] - ao[:, rot]).max() < 1e-9
 
 
+@pytest.mark.parametrize("kind,P,ratio,ref_order", [("h2o_h2", 16, 2, False), ("h2o_h2", 16, 4, True),
+                                                    ("water2", 16, 2, True), ("h2o_h2", 128, 2, False),
+                                                    ("water2", 128, 4, False)])
+def test_nl_trajectory_rotor_ratio_fp64(rot3d, kind, P, ratio, ref_order):
+    """NumbRotTimes = NumbTimes / ratio (RotRatio, mc_setup.cc:379-384): a rotor slice's partner
+    energy sums over its `ratio` translational slices (mc_piqmc.cc:1071-1072); at P = 128 the
+    rotor chain is split into parts.  FP64 trajectories follow the oracle move for move
+    (orientations compared on the NumbRotTimes rotor slices)."""
+    wl = nl_workload(kind, rot3d, n_h2=4, P=P, Prot=P // ratio, seed=9)
+    rp = wl["replicas"][0]
+    flags = ph.FLAG_FP64 | (ph.FLAG_REF_ORDER if ref_order else 0)
+    ctx = gpu_ctx(wl, flags)
+    ctx.run(2)
+    S = oracle_system(wl)
+    S.run(0, 2, order=O.ORDER_REFERENCE if ref_order else O.ORDER_SEGMENTS)
+    cg, ag, _ = ctx.get_state(0)
+    co, ao, _ = S.state()
+    tg, acg = ctx.counters(0)
+    to, aco = S.counters()
+    assert np.array_equal(tg, to) and np.array_equal(acg, aco), (acg, aco)
+    assert np.abs(cg - co).max() < 1e-9
+    nat = sum(n for n, m in zip(rp["numb"], rp["molecule"]) if m == 0)
+    for q in range(sum(rp["numb"]) - nat):
+        a0 = (nat + q) * P
+        assert np.abs(ag[:, a0:a0 + rp["Prot"]] - ao[:, a0:a0 + rp["Prot"]]).max() < 1e-9, q
+
+
 def test_split_chain_restore_continues_exactly(rot3d):
     wl = nl_workload("water2", rot3d, P=64, Prot=64, seed=5)
     one = gpu_ctx(wl)

@@ -136,6 +136,41 @@ def test_trajectory_fp64_rotor_parts(name, slices, ref_order):
         assert np.abs(ag[:2] - ao[:2]).max() < 1e-9, r
 
 
+def ratio_workload(name, ratio):
+    """Workload `name` with NumbRotTimes = NumbTimes / ratio (RotRatio, mc_setup.cc:379-384): a
+    rotor slice spans `ratio` translational slices, and its partner energy sums over all of them
+    (mc_piqmc.cc:854-855); the density table is the one of (T, NumbRotTimes)."""
+    wl = pc.workload(name)
+    rp = wl["replicas"][0]
+    rp["Prot"] = rp["P"] // ratio
+    wl["tables"]["rot"] = [pc.rot_table(rp["T"], rp["Prot"])]
+    rp["rotdens"] = 0
+    return wl
+
+
+@pytest.mark.parametrize("name,ratio,slices,ref_order", [("tiny", 2, 0, False), ("tiny", 4, 0, True),
+                                                         ("tiny", 2, 4, False), ("c0", 4, 0, False),
+                                                         ("c0", 2, 8, True)])
+def test_trajectory_fp64_rotor_ratio(name, ratio, slices, ref_order):
+    """Fewer rotor slices than translational slices, whole and split rotor chains: the GPU chain
+    follows the oracle's (orientations compared on the NumbRotTimes rotor slices)."""
+    wl = ratio_workload(name, ratio)
+    rp = wl["replicas"][0]
+    flags = ph.FLAG_FP64 | (ph.FLAG_REF_ORDER if ref_order else 0)
+    ctx = gpu_ctx(wl, flags, **({"rotor_part_slices": slices} if slices else {}))
+    ctx.run(2)
+    S = O.System(rp, wl["tables"], seed=SEED, replica=0)
+    S.run(0, 2, order=O.ORDER_REFERENCE if ref_order else O.ORDER_SEGMENTS)
+    cg, ag, _ = ctx.get_state(0)
+    co, ao, _ = S.state()
+    assert np.array_equal(ctx.counters(0)[0], S.counters()[0])
+    assert np.array_equal(ctx.counters(0)[1], S.counters()[1]), (ctx.counters(0)[1], S.counters()[1])
+    assert np.abs(cg - co).max() < 1e-9
+    rot = sum(rp["numb"]) - 1  # the rotor is the last atom
+    cols = slice(rot * rp["P"], rot * rp["P"] + rp["Prot"])
+    assert np.abs(ag[:2, cols] - ao[:2, cols]).max() < 1e-9
+
+
 def test_rotor_parts_fp32_match_single_part():
     """Production arithmetic: the split changes nothing but the launch shape."""
     wl = pc.workload("c0")

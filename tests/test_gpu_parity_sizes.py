@@ -221,17 +221,15 @@ def test_c4_fp32_total_v(c4):
     assert_fp32_v(ctx, c4, _nl_tabs(c4), angle_rows=slice(0, 3))
 
 
-@pytest.fixture(scope="module")
-def c4_many(generator):
-    return pc.nl_workload("c4", n_replicas=128, asymrho=_asymrho(generator))
-
-
-def test_c4_128_replicas_trajectory_fp64(c4_many):
-    """The full two-water workload: 128 chains of 2048 slices split into a few long parts, with
-    one lane per move and several moves per lane in each half-sweep.  A replica's chain after one
-    pass against the oracle's."""
-    ctx = gpu_ctx(c4_many, ph.FLAG_FP64)
+@pytest.mark.parametrize("nrep", [128, 256])
+def test_c4_many_replicas_trajectory_fp64(generator, nrep):
+    """The full two-water workload (128 chains of 2048 slices: four parts of 512 slices per chain,
+    two parts per CU, one lane per move) and twice as many chains (two parts of 1024 slices, so
+    each lane makes two moves per half-sweep).  A replica's chain after one pass against the
+    oracle's."""
+    wl = pc.nl_workload("c4", n_replicas=nrep, asymrho=_asymrho(generator))
+    ctx = gpu_ctx(wl, ph.FLAG_FP64)
     ctx.run(1)
     r = 77
-    st, cn = oracle_run("c4_many", c4_many, r, 1, O.ORDER_SEGMENTS, _nl_tabs(c4_many))
-    assert_same_chain(ctx, r, st, cn, c4_many["replicas"][r], angle_rows=slice(0, 3))
+    st, cn = oracle_run(f"c4_{nrep}", wl, r, 1, O.ORDER_SEGMENTS, _nl_tabs(wl))
+    assert_same_chain(ctx, r, st, cn, wl["replicas"][r], angle_rows=slice(0, 3))
