@@ -245,6 +245,31 @@ def test_nl_trajectory_rotor_ratio_fp64(rot3d, kind, P, ratio, ref_order):
         assert np.abs(ag[:, a0:a0 + rp["Prot"]] - ao[:, a0:a0 + rp["Prot"]]).max() < 1e-9, q
 
 
+@pytest.mark.parametrize("kind,ratio", [("h2o_h2", 2), ("water2", 4)])
+def test_nl_estimators_fp32_rotor_ratio(rot3d, kind, ratio):
+    """Production arithmetic with fewer rotor slices (P = 128, split chains): after three FP32
+    passes, V of the reached configuration equals the FP64 oracle's to 1e-6 relative, and K and
+    the rotational terms to 1e-10."""
+    P = 128
+    wl = nl_workload(kind, rot3d, n_h2=4, P=P, Prot=P // ratio, seed=9)
+    rp = wl["replicas"][0]
+    ctx = gpu_ctx(wl, 0)
+    ctx.run(3)
+    got = ctx.estimate()[0]
+    cg, ag, _ = ctx.get_state(0)
+    angles = np.array(rp["angles"], copy=True)
+    nat = sum(n for n, m in zip(rp["numb"], rp["molecule"]) if m == 0)
+    for q in range(sum(rp["numb"]) - nat):
+        a0 = (nat + q) * P
+        angles[:, a0:a0 + rp["Prot"]] = ag[:, a0:a0 + rp["Prot"]]
+    S = oracle_system(wl)
+    S.set_state(cg, angles)
+    ref = S.sample()
+    for k in (0, 2, 3, 7):  # K and the rotational terms are FP64 on both paths
+        assert abs(got[k] - ref[k]) <= 1e-10 * max(1.0, abs(ref[k])), (k, got[k], ref[k])
+    assert abs(got[1] - ref[1]) <= 1e-6 * max(1.0, abs(ref[1])), (got[1], ref[1])
+
+
 def test_split_chain_restore_continues_exactly(rot3d):
     wl = nl_workload("water2", rot3d, P=64, Prot=64, seed=5)
     one = gpu_ctx(wl)

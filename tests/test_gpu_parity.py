@@ -171,6 +171,28 @@ def test_trajectory_fp64_rotor_ratio(name, ratio, slices, ref_order):
     assert np.abs(ag[:2, cols] - ao[:2, cols]).max() < 1e-9
 
 
+@pytest.mark.parametrize("name,ratio,slices", [("tiny", 2, 0), ("c0", 4, 8)])
+def test_estimators_fp32_rotor_ratio(name, ratio, slices):
+    """Production arithmetic with fewer rotor slices: after three FP32 passes, K and E_rot of the
+    reached configuration equal the FP64 oracle's to 1e-10 and V to 1e-6 relative."""
+    wl = ratio_workload(name, ratio)
+    rp = wl["replicas"][0]
+    ctx = gpu_ctx(wl, 0, **({"rotor_part_slices": slices} if slices else {}))
+    ctx.run(3)
+    got = ctx.estimate()[0]
+    cg, ag, _ = ctx.get_state(0)
+    rot = sum(rp["numb"]) - 1
+    cols = slice(rot * rp["P"], rot * rp["P"] + rp["Prot"])
+    angles = np.array(rp["angles"], copy=True)
+    angles[:2, cols] = ag[:2, cols]
+    S = O.System(rp, wl["tables"])
+    S.set_state(cg, angles)
+    ref = S.sample()
+    for k in (0, 2, 3, 7):
+        assert abs(got[k] - ref[k]) <= 1e-10 * (abs(ref[k]) + 1.0), (k, got[k], ref[k])
+    assert abs(got[1] - ref[1]) <= 1e-6 * abs(ref[1]), (got[1], ref[1])
+
+
 def test_rotor_parts_fp32_match_single_part():
     """Production arithmetic: the split changes nothing but the launch shape."""
     wl = pc.workload("c0")
