@@ -111,6 +111,14 @@ int  pimc_set_replica(pimc_ctx* ctx, int replica, const pimc_replica_desc* desc)
    even rotor slice count for the checkerboard sweeps (mc_piqmc.cc:490-512, :728-760).  Returns 0
    if accepted, else -1 with the reason in msg (len bytes, may be NULL). */
 int  pimc_check_replica(const pimc_replica_desc* desc, char* msg, int len);
+/* Minimum image (the MINIMAGE keyword, mc_input.cc:298-300) of a set replica in a periodic box
+   of edges box[0..2] (BoxSize, mc_setup.cc:343-366), before pimc_commit: the separation of every
+   atom pair in the moves and the potential estimator, and every kinetic link, is taken to its
+   nearest image, dr - BoxSize rint(dr / BoxSize) (mc_piqmc.cc:1239-1240,1832-1833;
+   mc_estim.cc:536-537,916-917).  Pairs with a non-linear rotor take the positions themselves, as
+   vcord and caleng do there.  box == NULL turns it off.  Refused for a rotating linear rotor: the
+   reference's rotor move stops with "MIN IMAGE for orient pot" (mc_piqmc.cc:2013-2016). */
+int  pimc_set_minimum_image(pimc_ctx* ctx, int replica, const double* box);
 int  pimc_commit(pimc_ctx* ctx);   /* allocates device state after all replicas are set */
 /* How pimc_commit splits linear-rotor chains over workgroups, without a device (no counterpart
    in the reference, whose sweep is one OpenMP loop per chain, mc_piqmc.cc:490-512).  Chain i has

@@ -17,6 +17,7 @@
 #include <map>
 #include <mutex>
 #include <algorithm>
+#include <array>
 
 using namespace pimc;
 
@@ -143,6 +144,7 @@ struct pimc_ctx {
     std::string err;
     std::vector<pimc_replica_desc> desc;
     std::vector<char> desc_set;
+    std::vector<std::array<double, 3>> box;  // minimum-image box per replica, zeros when off
     std::vector<Rep> rep;
     // host tables
     std::vector<Tab1D> t1;
@@ -316,6 +318,7 @@ int pimc_create(int n_replicas, unsigned flags, unsigned seed, pimc_ctx** out) {
     c->profile = flags & PIMC_FLAG_PROFILE;
     c->desc.resize(n_replicas);
     c->desc_set.assign(n_replicas, 0);
+    c->box.assign(n_replicas, std::array<double, 3>{0.0, 0.0, 0.0});
     if (hipStreamCreate(&c->stream) != hipSuccess) {
         delete c;
         return -3;
@@ -600,6 +603,29 @@ int pimc_set_replica(pimc_ctx* c, int r, const pimc_replica_desc* d) {
     return 0;
 }
 
+// a rotating linear rotor: the reference's rotor move stops under MINIMAGE (mc_piqmc.cc:2013-2016)
+static bool rotating_linear(const pimc_replica_desc& d) {
+    for (int t = 0; t < d.ntypes; ++t)
+        if (d.molecule[t] == 1 && d.Prot > 0) return true;
+    return false;
+}
+
+int pimc_set_minimum_image(pimc_ctx* c, int r, const double* box) {
+    if (c->committed) return fail(c, "the minimum image is set before pimc_commit");
+    if (r < 0 || r >= c->nrep) return fail(c, "replica index out of range");
+    if (!c->desc_set[r]) return fail(c, "pimc_set_minimum_image: pimc_set_replica first");
+    if (!box) {
+        c->box[r] = {0.0, 0.0, 0.0};
+        return 0;
+    }
+    for (int i = 0; i < 3; ++i)
+        if (!(box[i] > 0.0) || !std::isfinite(box[i])) return fail(c, "minimum image: box edges must be positive");
+    if (rotating_linear(c->desc[r]))
+        return fail(c, "MIN IMAGE for orient pot: the minimum image is not applied to a rotating linear rotor");
+    c->box[r] = {box[0], box[1], box[2]};
+    return 0;
+}
+
 int pimc_check_replica(const pimc_replica_desc* d, char* msg, int len) {
     const std::string m = d ? check_replica(*d) : std::string("null description");
     if (msg && len > 0) std::snprintf(msg, (size_t)len, "%s", m.c_str());
@@ -846,6 +872,12 @@ int pimc_commit(pimc_ctx* c) {
         p.nrot = 0;
         p.pot3d_id = d.pot3d;
         p.rot3d_id = d.rot3d;
+        if (c->box[r][0] > 0.0) {
+            if (rotating_linear(d))
+                return fail(c, "MIN IMAGE for orient pot: the minimum image is not applied to a rotating linear rotor");
+            p.minimage = 1;
+            for (int i = 0; i < 3; ++i) p.box[i] = c->box[r][i];
+        }
         int g = 0, S = 1;
         for (int t = 0; t < d.ntypes; ++t) {
             p.tnumb[t] = d.numb[t];

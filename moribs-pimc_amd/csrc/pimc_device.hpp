@@ -42,8 +42,10 @@ struct Rep {
     int est_ts;    // slices per estimator tile (a power of two)
     int h3_off;    // (r, theta, chi) bins around a non-linear rotor, from hist_off; -1 if none
     int hrel_off;  // relative Euler angle bins (theta, phi, chi) of rotor links, from hist_off; -1 if none
+    int minimage;  // 1: separations of atom pairs and kinetic links take the nearest periodic image
     double lambda[MAXT], mcstep[MAXT];
     double T, beta, tau, rtstep;
+    double box[3];  // BoxSize (mc_setup.cc:343-366), used when minimage is set
     long long bead_off, rot_off, hist_off, sample_off;
     long long row_off;  // first estimator tile partial of the replica
     long long rcf_off;  // [RCF_L][Prot] accumulators of the rotational correlation functions, -1 if none
@@ -536,20 +538,32 @@ __device__ __forceinline__ R pair_d(const Lut1D<R>& l1, const void* __restrict__
     else return spot1d<R>(l1, static_cast<const vec4_t<R>*>(k1), r);
 }
 
+// MINIMAGE (mc_piqmc.cc:1239-1240,1832-1833; mc_estim.cc:536-537,916-917): component id of a
+// separation moved to its nearest periodic image, d - BoxSize * rint(d / BoxSize); box == nullptr
+// (no minimum image) leaves it as it is
+template <typename T>
+__device__ __forceinline__ T nearest_image(T d, const double* box, int id) {
+    if (!box) return d;
+    const T L = (T)box[id];
+    return d - L * rint(d / L);
+}
+
 // positions in FP64: the separation is formed in FP64 and rounded once
 template <typename R>
 __device__ __forceinline__ R pair_v(const Lut1D<R>& l1, const vec4_t<R>* __restrict__ k1, const Lut2D<R>& l2,
                                     bool rot_a, bool rot_j, double xa, double ya, double za, double xj, double yj,
-                                    double zj, double nx, double ny, double nz) {
-    return pair_d<R>(l1, k1, l2, rot_a, rot_j, (R)(xa - xj), (R)(ya - yj), (R)(za - zj), nx, ny, nz);
+                                    double zj, double nx, double ny, double nz, const double* box = nullptr) {
+    return pair_d<R>(l1, k1, l2, rot_a, rot_j, (R)nearest_image(xa - xj, box, 0), (R)nearest_image(ya - yj, box, 1),
+                     (R)nearest_image(za - zj, box, 2), nx, ny, nz);
 }
 
 // positions already in the pair precision (the bisection window)
 template <typename R, bool HALF = false>
 __device__ __forceinline__ R pair_vr(const Lut1D<R>& l1, const void* __restrict__ k1, const Lut2D<R>& l2,
                                      bool rot_a, bool rot_j, R xa, R ya, R za, R xj, R yj, R zj, double nx, double ny,
-                                     double nz) {
-    return pair_d<R, HALF>(l1, k1, l2, rot_a, rot_j, xa - xj, ya - yj, za - zj, nx, ny, nz);
+                                     double nz, const double* box = nullptr) {
+    return pair_d<R, HALF>(l1, k1, l2, rot_a, rot_j, nearest_image(xa - xj, box, 0), nearest_image(ya - yj, box, 1),
+                           nearest_image(za - zj, box, 2), nx, ny, nz);
 }
 
 // One DPP lane move within rows of 16 lanes (ctrl: quad_perm 0x00-0xff, row_mirror 0x140,

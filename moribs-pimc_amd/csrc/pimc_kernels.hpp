@@ -76,15 +76,18 @@ __device__ __forceinline__ void rotor_axis(const RotBead* rotb, const Rep& rp, i
 
 // PotEnergy's dispatch (mc_piqmc.cc:1847-1958) for the non-linear family: pairs of the atom
 // type by SPot1D, atom / non-linear rotor pairs by vcord with the rotor's frame at that slice
-// (rcom = the rotor, rpt = the atom), rotor / rotor pairs by caleng.
+// (rcom = the rotor, rpt = the atom), rotor / rotor pairs by caleng.  vcord and caleng take the
+// positions themselves, so the minimum image (box) reaches the atom pairs only, as there.
 template <typename R>
 __device__ __forceinline__ double pair_nl(const Lut1D<R>& l1, const vec4_t<R>* __restrict__ k1, const Lut3D<R>& l3,
                                           bool rot_a, bool rot_j, double xa, double ya, double za, double xj,
-                                          double yj, double zj, const Frame& fa, const Frame& fj) {
+                                          double yj, double zj, const Frame& fa, const Frame& fj,
+                                          const double* box = nullptr) {
     if (rot_a && rot_j) return caleng_v(xa, ya, za, xj, yj, zj, fa, fj);
     if (rot_a) return (double)vcord_v<R>(l3, fa, xa, ya, za, xj, yj, zj);
     if (rot_j) return (double)vcord_v<R>(l3, fj, xj, yj, zj, xa, ya, za);
-    const R dx = (R)(xa - xj), dy = (R)(ya - yj), dz = (R)(za - zj);
+    const R dx = (R)nearest_image(xa - xj, box, 0), dy = (R)nearest_image(ya - yj, box, 1),
+            dz = (R)nearest_image(za - zj, box, 2);
     return (double)spot1d<R>(l1, k1, sqrt(dx * dx + dy * dy + dz * dz));
 }
 
@@ -139,6 +142,7 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
     const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
     const Lut3D<R> l3 = lut3d<R>(T3, FAM == 1 ? rp.pot3d_id : -1);
     const bool half_knots = FAM == 0 && is_fp32<R>() && l1.uniform;
+    const double* box = rp.minimage ? rp.box : nullptr;  // MINIMAGE (not with a rotating linear rotor)
     if (half_knots) {
         float2* k2 = reinterpret_cast<float2*>(kn);
         for (int i = tid; i < l1.n; i += nth) k2[i] = make_float2((float)l1.k[i].y, (float)l1.k[i].z);
@@ -291,23 +295,23 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                         if constexpr (FAM == 1) {
                             const Frame& fa = fr[(rot_a ? a - rp.nl_atom : 0) * W + slot];
                             const Frame& fj = fr[(rj ? j - rp.nl_atom : 0) * W + slot];
-                            acc += pair_nl<R>(l1, kn, l3, rot_a, rj, xn, yn, zn, pj.x, pj.y, pj.z, fa, fj) -
-                                   pair_nl<R>(l1, kn, l3, rot_a, rj, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, fa, fj);
+                            acc += pair_nl<R>(l1, kn, l3, rot_a, rj, xn, yn, zn, pj.x, pj.y, pj.z, fa, fj, box) -
+                                   pair_nl<R>(l1, kn, l3, rot_a, rj, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, fa, fj, box);
                         } else if (split) {
                             if (rj) {
                                 const P4 ax = nhf[slot];
                                 const double nx = ax.x, ny = ax.y, nz = ax.z;
-                                acc += (double)pair_vr<R, H>(l1, kn, l2, false, true, xn, yn, zn, pj.x, pj.y, pj.z, nx, ny, nz) -
-                                       (double)pair_vr<R, H>(l1, kn, l2, false, true, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, nx, ny, nz);
+                                acc += (double)pair_vr<R, H>(l1, kn, l2, false, true, xn, yn, zn, pj.x, pj.y, pj.z, nx, ny, nz, box) -
+                                       (double)pair_vr<R, H>(l1, kn, l2, false, true, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, nx, ny, nz, box);
                             } else {
-                                acc += (double)pair_vr<R, H>(l1, kn, l2, false, false, xn, yn, zn, pj.x, pj.y, pj.z, 0.0, 0.0, 1.0) -
-                                       (double)pair_vr<R, H>(l1, kn, l2, false, false, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, 0.0, 0.0, 1.0);
+                                acc += (double)pair_vr<R, H>(l1, kn, l2, false, false, xn, yn, zn, pj.x, pj.y, pj.z, 0.0, 0.0, 1.0, box) -
+                                       (double)pair_vr<R, H>(l1, kn, l2, false, false, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, 0.0, 0.0, 1.0, box);
                             }
                         } else {
                             const P4 ax = nhf[slot];
                             const double nx = ax.x, ny = ax.y, nz = ax.z;
-                            acc += (double)pair_vr<R, H>(l1, kn, l2, rot_a, rj, xn, yn, zn, pj.x, pj.y, pj.z, nx, ny, nz) -
-                                   (double)pair_vr<R, H>(l1, kn, l2, rot_a, rj, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, nx, ny, nz);
+                            acc += (double)pair_vr<R, H>(l1, kn, l2, rot_a, rj, xn, yn, zn, pj.x, pj.y, pj.z, nx, ny, nz, box) -
+                                   (double)pair_vr<R, H>(l1, kn, l2, rot_a, rj, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, nx, ny, nz, box);
                         }
                     }
                     return acc;
@@ -422,6 +426,7 @@ __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ 
     __syncthreads();
     unsigned* myflag = flags + pt.fbase + pt.w;
     const unsigned base = __hip_atomic_load(myflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double* box = rp.minimage ? rp.box : nullptr;  // MINIMAGE (not with a rotating linear rotor)
     if (threadIdx.x == 0) s_err = 0;
     unsigned long long acc_n = 0;
     for (int ia = 0; ia < rp.tnumb[ty]; ++ia) {
@@ -459,12 +464,13 @@ __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ 
                 if (rj) rotor_axis(rotb, rp, it, nx, ny, nz);
                 R vn, vo;
                 if (rj) {
-                    vn = pair_v<R>(l1, kn, l2, false, true, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, nx, ny, nz);
-                    vo = pair_v<R>(l1, kn, l2, false, true, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz);
+                    vn = pair_v<R>(l1, kn, l2, false, true, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, nx, ny, nz,
+                                   box);
+                    vo = pair_v<R>(l1, kn, l2, false, true, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz, box);
                 } else {
                     vn = pair_v<R>(l1, kn, l2, false, false, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, 0.0,
-                                   0.0, 1.0);
-                    vo = pair_v<R>(l1, kn, l2, false, false, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, 0.0, 0.0, 1.0);
+                                   0.0, 1.0, box);
+                    vo = pair_v<R>(l1, kn, l2, false, false, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, 0.0, 0.0, 1.0, box);
                 }
                 acc += (double)vn - (double)vo;
             }
@@ -479,15 +485,15 @@ __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ 
                 const Frame& fa = frame_at(frames, rp, rot_a ? a : rp.nl_atom, it);
                 const Frame& fj = frame_at(frames, rp, rot_j ? j : rp.nl_atom, it);
                 const double vn = pair_nl<R>(l1, kn, l3, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y,
-                                             xj.z, fa, fj);
-                const double vo = pair_nl<R>(l1, kn, l3, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, fa, fj);
+                                             xj.z, fa, fj, box);
+                const double vo = pair_nl<R>(l1, kn, l3, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, fa, fj, box);
                 acc += vn - vo;
             } else {
                 double nx, ny, nz;
                 rotor_axis(rotb, rp, it, nx, ny, nz);
                 const R vn = pair_v<R>(l1, kn, l2, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z,
-                                       nx, ny, nz);
-                const R vo = pair_v<R>(l1, kn, l2, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz);
+                                       nx, ny, nz, box);
+                const R vo = pair_v<R>(l1, kn, l2, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz, box);
                 acc += (double)vn - (double)vo;
             }
         }
@@ -1850,11 +1856,13 @@ __global__ __launch_bounds__(256) void k_est_tiles(const Rep* __restrict__ reps,
     for (int i = threadIdx.x; i < BINSR; i += blockDim.x) g1s[i] = 0u;
     __syncthreads();
     // kinetic links (a, s) - (a, s + 1)
+    const double* box = rp.minimage ? rp.box : nullptr;  // MINIMAGE (not with a rotating linear rotor)
     double kin = 0.0;
     for (int i = threadIdx.x; i < N * nts; i += blockDim.x) {
         const int a = i / nts, w = i - a * nts;
         const int j = a * W + w;
-        const double dx = lx[j] - lx[j + 1], dy = ly[j] - ly[j + 1], dz = lz[j] - lz[j + 1];
+        const double dx = nearest_image(lx[j] - lx[j + 1], box, 0), dy = nearest_image(ly[j] - ly[j + 1], box, 1),
+                     dz = nearest_image(lz[j] - lz[j + 1], box, 2);
         kin += (dx * dx + dy * dy + dz * dz) / (4.0 * rp.beta * rp.lambda[kind[a] & 255]);
     }
     const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
@@ -1877,7 +1885,8 @@ __global__ __launch_bounds__(256) void k_est_tiles(const Rep* __restrict__ reps,
         const int k0 = kind[a0], k1 = kind[a1];
         const int t0 = k0 & 255, t1 = k1 & 255, m0 = k0 >> 8, m1 = k1 >> 8;
         const int j0 = a0 * W + w, j1 = a1 * W + w;
-        const double ex = lx[j0] - lx[j1], ey = ly[j0] - ly[j1], ez = lz[j0] - lz[j1];
+        const double ex = nearest_image(lx[j0] - lx[j1], box, 0), ey = nearest_image(ly[j0] - ly[j1], box, 1),
+                     ez = nearest_image(lz[j0] - lz[j1], box, 2);
         const R dx = (R)ex, dy = (R)ey, dz = (R)ez;
         const R rr = sqrt(dx * dx + dy * dy + dz * dz);
         if constexpr (FAM == 1) {
@@ -2037,6 +2046,7 @@ __global__ void k_pot_slice(const Rep* __restrict__ reps, const double4* __restr
     const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
     double nx, ny, nz;
     rotor_axis(rotb, rp, it, nx, ny, nz);
+    const double* box = rp.minimage ? rp.box : nullptr;
     double s = 0.0;
     for (int j = 0; j < rp.natoms; ++j) {
         if (j == a) continue;
@@ -2045,10 +2055,11 @@ __global__ void k_pot_slice(const Rep* __restrict__ reps, const double4* __restr
             const bool ra = rp.nl_atom >= 0 && a >= rp.nl_atom, rj = rp.nl_atom >= 0 && j >= rp.nl_atom;
             const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
             s += pair_nl<R>(l1, l1.k, l3, ra, rj, qpos[3 * q], qpos[3 * q + 1], qpos[3 * q + 2], xj.x, xj.y, xj.z,
-                            frame_at(frames, rp, ra ? a : rp.nl_atom, it), frame_at(frames, rp, rj ? j : rp.nl_atom, it));
+                            frame_at(frames, rp, ra ? a : rp.nl_atom, it), frame_at(frames, rp, rj ? j : rp.nl_atom, it),
+                            box);
         } else {
             s += (double)pair_v<R>(l1, l1.k, l2, a == rp.lin_atom, j == rp.lin_atom, qpos[3 * q], qpos[3 * q + 1],
-                                   qpos[3 * q + 2], xj.x, xj.y, xj.z, nx, ny, nz);
+                                   qpos[3 * q + 2], xj.x, xj.y, xj.z, nx, ny, nz, box);
         }
     }
     out[q] = s;

@@ -76,7 +76,7 @@ struct Deck {
     std::vector<Species> types;
     int P = 0, Prot = 0;
     long passes = 0, blocks = 0, eqblocks = 0, startblock = 0;
-    bool rotation = false, read_coords = false, restart = false;
+    bool rotation = false, read_coords = false, restart = false, minimage = false;
     std::string rot_type;
     double rot_step = 0.0;
     long skip_ratio = 100000, skip_total = 10000, skip_averg = 1;  // mc_setup.cc:91-93
@@ -162,7 +162,7 @@ Deck read_deck(const std::string& path) {
         }
         else if (key == "ROTSYM") die("IOReadParams", "ROTSYM is not supported");
         else if (key == "WORM") die("IOReadParams", "the worm algorithm is not supported");
-        else if (key == "MINIMAGE") die("IOReadParams", "periodic boundaries (MINIMAGE) are not supported");
+        else if (key == "MINIMAGE") d.minimage = true;  // mc_input.cc:298-300
         else if (key == "CRYSTAL") die("IOReadParams", "crystal lattices are not supported");
         else if (key == "READMCCOORDS") d.read_coords = true;
         else if (key == "MCSKIP_RATIO") { in >> d.skip_ratio; need(in, key); }
@@ -189,6 +189,9 @@ Deck read_deck(const std::string& path) {
             }
         if (!found) die("IOReadParams", "Can't find a particle type to sample rotational degrees of freedom");
     }
+    if (d.minimage && d.rotation)  // the reference's linear-rotor move stops here (mc_piqmc.cc:2013-2016)
+        for (auto& s : d.types)
+            if (s.type == d.rot_type && s.molecule == 1) die("PotRotEnergy", "MIN IMAGE for orient pot");
     if (d.P <= 0 || d.passes <= 0 || d.blocks <= 0) die("IOReadParams", "NUMBEROFSLICES/PASSES/BLOCKS must be positive");
     if (d.skip_ratio <= 0 || d.skip_total <= 0 || d.skip_averg <= 0) die("IOReadParams", "MCSKIP_* must be positive");
     for (auto& s : d.types) s.mass = mass_of(s.type);
@@ -860,6 +863,11 @@ int main(int argc, char** argv) {
         die("pimc_create", c.ctx ? pimc_last_error(c.ctx) : "no HIP device");
     pimc_replica_desc desc = load_tables(d, c);
     for (int r = 0; r < nrep; ++r) c.check(pimc_set_replica(c.ctx, r, &desc), "pimc_set_replica");
+    if (d.minimage) {  // BoxSize of MCInit (mc_setup.cc:343-366): the volume per particle from DENSITY
+        if (d.density <= 0.0) die("MCInit", "MINIMAGE needs a positive DENSITY for the box");
+        const double L = std::pow((double)d.natoms() / d.density, 1.0 / 3.0), box[3] = {L, L, L};
+        for (int r = 0; r < nrep; ++r) c.check(pimc_set_minimum_image(c.ctx, r, box), "pimc_set_minimum_image");
+    }
     // MCSKIP_AVERG below NUMBEROFSLICES: samples inside a pass, after every P/k time steps in the
     // reference's order or every S/k phases in segment order; k the largest divisor of that grid
     // not above P / MCSKIP_AVERG

@@ -208,6 +208,8 @@ def load_into_context(ctx, wl):
         ctx.set_replica(i, rp["ntypes"], rp["numb"], rp["molecule"], rp["levels"], rp["mcstep"], rp["mass"], rp["P"],
                         rp["Prot"], rp["T"], rp["rtstep"], pot1d=rp["pot1d"], pot2d=rp["pot2d"], rotdens=rp["rotdens"],
                         pot3d=rp.get("pot3d", -1), rot3d=rp.get("rot3d", -1))
+        if rp.get("box") is not None:  # MINIMAGE with this box
+            ctx.set_minimum_image(i, rp["box"])
     ctx.commit()
     for i, rp in enumerate(wl["replicas"]):
         ctx.set_state(i, rp["coords"], rp["angles"])

@@ -66,6 +66,7 @@ SIGNATURES = {
     "pimc_set_replica": (_i, [ctypes.c_void_p, _i, ctypes.POINTER(ReplicaDesc)]),
     "pimc_commit": (_i, [ctypes.c_void_p]),
     "pimc_check_replica": (_i, [ctypes.POINTER(ReplicaDesc), ctypes.c_char_p, _i]),
+    "pimc_set_minimum_image": (_i, [ctypes.c_void_p, _i, _dp]),
     "pimc_plan_rotor3d_split": (_i, [_i, _i, _i, _i, _i, _i, _i]),
     "pimc_plan_rotor_parts": (_i, [_i, _ip, _ip, _ip, _i, _i, _i, _i, _i, _i, _ip, _ip, ctypes.POINTER(ctypes.c_long),
                                    ctypes.c_char_p, _i]),
@@ -261,6 +262,13 @@ class Context:
                          rot3d)
         self.descs[r] = d
         return self._check(self.lib.pimc_set_replica(self.h, r, ctypes.byref(d)))
+
+    def set_minimum_image(self, r, box):
+        """pimc_set_minimum_image: periodic box edges (3 values, or one for a cube), None for off."""
+        if box is None:
+            return self._check(self.lib.pimc_set_minimum_image(self.h, r, None))
+        b = np.broadcast_to(np.asarray(box, dtype=np.float64), (3,)).copy()
+        return self._check(self.lib.pimc_set_minimum_image(self.h, r, _dptr(b)))
 
     def set_option(self, name, value):
         self._check(self.lib.pimc_set_option(self.h, name.encode(), int(value)))

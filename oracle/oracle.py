@@ -60,6 +60,7 @@ def lib():
             ("or_set_rotdens", _i, [_vp, _i, _i, _dp, _dp, _dp, _dp]),
             ("or_set_pot3d", _i, [_vp, _i, _i, _i, _d, _d, _dp]),
             ("or_set_rot3d", _i, [_vp, _dp, _dp, _dp]),
+            ("or_set_minimage", None, [_vp, _i, _dp]),
             ("or_set_state", None, [_vp, _dp, _dp]),
             ("or_get_state", None, [_vp, _dp, _dp, _dp]),
             ("or_get_counters", None, [_vp, _dp, _dp]),
@@ -152,6 +153,8 @@ class System:
                 self._keep += [rho, er, es]
                 self.L.or_set_rot3d(self.h, rho.ctypes.data_as(_dp), er.ctypes.data_as(_dp), es.ctypes.data_as(_dp))
         self.L.or_set_rng(self.h, seed & 0xFFFFFFFF, replica)
+        if rp.get("box") is not None:  # MINIMAGE with this box
+            self.L.or_set_minimage(self.h, 1, P(np.broadcast_to(np.asarray(rp["box"], dtype=np.float64), (3,)).copy()))
         if "coords" in rp:
             self.set_state(rp["coords"], rp["angles"])
 
@@ -217,6 +220,7 @@ def ref():
         L = ctypes.CDLL(REF_SO)
         for name, res, args in [
             ("h_setup", _i, [_i, ctypes.c_char_p, _ip, _ip, _ip, _dp, _dp, _i, _i, _d, _d, ctypes.c_char_p, ctypes.c_char_p]),
+            ("h_set_minimage", None, [_i, _dp]),
             ("h_set_state", None, [_dp, _dp]),
             ("h_set_newcoords", None, [_dp]),
             ("h_get_state", None, [_dp, _dp, _dp]),

@@ -187,6 +187,11 @@ void h_set_pot3d(int nr, int nth, int nch, double rmin, double rmax, double* v) 
     vtable = v;
 }
 
+// minimum image (MINIMAGE, mc_input.cc:298-300) with the given BoxSize (mc_setup.cc:343-366)
+void h_set_minimage(int on, const double* box) {
+    MINIMAGE = on != 0;
+    for (int id = 0; id < NDIM; ++id) BoxSize[id] = box[id];
+}
 double h_spot1d(double r, int type) { return SPot1D(r, type); }
 double h_lpot2d(double r, double c, int type) { return LPot2D(r, c, type); }
 double h_srotdens(double g, int type) { return SRotDens(g, type); }

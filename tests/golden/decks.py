@@ -7,6 +7,8 @@ tests of the host driver (which run moribs-pimc_amd/pimc on the same files).
   he4_co2      4 He + CO2 at 0.37 K, P = Prot = 64, lattice start; LJ He-He, Morse-Legendre
                CO2-He and the linden CO2 density, all written in the reference's formats
   c0_8he       the 8-He workload: 8 He + CO2 at 0.37 K, P = Prot = 128, the same tables
+  he16_pbc     16 He at liquid density (0.0218 Angstrom^-3, a periodic box of 9 Angstrom) under
+               MINIMAGE at 2 K, P = 16, LJ He-He
 """
 import os
 import shutil
@@ -67,8 +69,23 @@ MCSKIP_RATIO     12800
 MCSKIP_TOTAL     12800
 MCSKIP_AVERG     128
 """,
+    "he16_pbc": """MASTERDIR        ./
+OUTPUTDIR        ./
+FILENAMEPREFIX   he16pbc
+DIMENSION     3
+DENSITY       0.0218
+TEMPERATURE   2.0
+ATOM       He4  16   BOLTZMANN     0.40     3         He      PRIMITIVE
+MINIMAGE
+NUMBEROFSLICES     16
+NUMBEROFPASSES     100
+NUMBEROFBLOCKS     {blocks}  10
+MCSKIP_RATIO     1600
+MCSKIP_TOTAL     1600
+MCSKIP_AVERG     16
+""",
 }
-BLOCKS = {"co2_100K": 205, "he4_co2": 1010, "c0_8he": 310}
+BLOCKS = {"co2_100K": 205, "he4_co2": 1010, "c0_8he": 310, "he16_pbc": 1010}
 
 
 def materialise(name, d, blocks=None):
@@ -80,6 +97,9 @@ def materialise(name, d, blocks=None):
         shutil.copy(os.path.join(HERE, "CO2_T100t4.rot"), d)
         with open(os.path.join(d, "CO2_fake.pot"), "w") as f:  # the example's empty 2D table
             f.write("0 0\n0 0\n")
+    elif name == "he16_pbc":
+        r, v = pt.lj_table()
+        pt.write_pot1d(os.path.join(d, "He.pot"), r, v)
     else:
         P = 128 if name == "c0_8he" else 64
         r, v = pt.lj_table()

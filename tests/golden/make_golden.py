@@ -24,6 +24,11 @@ Sections
              GetPotEnergy_Densities (with its 3D bins and the relative Euler angle bins) and
              two reference-order passes on tapes
   water2     2 non-linear rotors (TIP4P via caleng), P = Prot = 8: the same quantities
+  periodic   16 He in a cubic box of 8.5 Angstrom under MINIMAGE, P = 16, on helium.pot:
+             PotEnergy (slice, path, on MCCoords and on newcoords), the estimators V (with its
+             g(r) bins) and K, and two reference-order passes on tapes
+  h2o_h2_pbc the h2o_h2 system under MINIMAGE in a box of 11 Angstrom (para-H2 pairs take
+             their nearest image, the vcord pairs the positions): the h2o_h2 quantities
 """
 import json
 import os
@@ -292,7 +297,7 @@ def section_nonlinear(arrays):
 
 
 # ----------------------------------------------------------------------------- non-linear systems
-def _system3d(arrays, tag, ntypes, names, numb, molecule, pots, P, Prot, T, seed, with_pot3d):
+def _system3d(arrays, tag, ntypes, names, numb, molecule, pots, P, Prot, T, seed, with_pot3d, box=None):
     import ctypes
     import pimc_tables as pt
     import tables3d
@@ -310,6 +315,8 @@ def _system3d(arrays, tag, ntypes, names, numb, molecule, pots, P, Prot, T, seed
     assert rc == 0
     rho, erot, esq = tables3d.rot3d()
     R.h_set_rot3d(O.P(rho), O.P(erot), O.P(esq))
+    if box is not None:
+        R.h_set_minimage(1, O.P([box] * 3))
     rng = np.random.default_rng(seed)
     natoms = sum(numb)
     n = natoms * P
@@ -369,8 +376,64 @@ def section_water2(arrays):
     return _system3d(arrays, "water2", 1, b"H2O\0", [2], [2], b"none\0", 8, 8, 1.0, SEED + 4, False)
 
 
+def section_h2o_h2_pbc(arrays):
+    return _system3d(arrays, "h2o_h2_pbc", 2, b"H2\0H2O\0", [4, 1], [0, 2], b"pH2\0H2Opot\0", 8, 8, 0.5, SEED + 3,
+                     True, box=11.0)
+
+
+# ----------------------------------------------------------------------------- periodic helium
+def section_periodic(arrays):
+    import pimc_tables as pt
+    O, R = _lib()
+    pot = os.path.join(REFTREE, "examples/MF_1He_0.37K_512_128/helium.pot")
+    P, N, L = 16, 16, 8.5
+    d = tempfile.mkdtemp(prefix="golden_pbc_")
+    with open(pot) as fi, open(os.path.join(d, "helium.pot"), "w") as fo:
+        fo.write(fi.read())
+    rc = R.h_setup(1, b"He4\0", O.I([N]), O.I([0]), O.I([3]), O.P([0.6]), O.P([4.0026032497]), P, 0, 1.5, 0.0,
+                   d.encode(), b"helium\0")
+    assert rc == 0
+    R.h_set_minimage(1, O.P([L, L, L]))
+    rng = np.random.default_rng(SEED + 5)
+    centres = []
+    while len(centres) < N:  # atoms anywhere in the box, at least 2.6 Angstrom from every image of the others
+        x = rng.uniform(-0.5 * L, 0.5 * L, 3)
+        if all(np.linalg.norm(x - c - L * np.rint((x - c) / L)) > 2.6 for c in centres):
+            centres.append(x)
+    coords = np.zeros((3, N * P))
+    for a, c in enumerate(centres):
+        coords[:, a * P:(a + 1) * P] = c[:, None] + rng.normal(0.0, 0.25, (3, P))
+    angles = np.zeros_like(coords)
+    newc = coords + rng.normal(0.0, 0.2, coords.shape)
+    arrays["periodic_coords"], arrays["periodic_newcoords"] = coords, newc
+    R.h_set_state(O.P(coords), O.P(angles))
+    R.h_set_newcoords(O.P(newc))
+    out = {"box": L,
+           "pot_slice": [[R.h_pot_energy_slice(a, 0, it) for it in range(P)] for a in range(N)],
+           "pot_slice_new": [[R.h_pot_energy_slice(a, 1, it) for it in range(P)] for a in range(N)],
+           "pot_path": [R.h_pot_energy_path(a, 0) for a in range(N)],
+           "pot_path_new": [R.h_pot_energy_path(a, 1) for a in range(N)]}
+    est, gr1, _ = estimators(R)
+    out["kin"], out["pot"] = est["kin"], est["pot"]
+    arrays["periodic_gr1d"] = gr1
+    tapes = make_tapes(rng, 40000)
+    for k, t in tapes.items():
+        R.h_set_tape(k, O.P(t), len(t))
+    R.h_reset_counters()
+    run_passes(R, P, 1, 2)
+    out["tape_used"] = tape_record(R, tapes, "periodic", arrays)
+    c1, _, _ = get_state(R, N * P)
+    arrays["periodic_pass_coords"] = c1
+    out["pass_total"], out["pass_accep"] = counters(R, 1)
+    est, gr1, _ = estimators(R)
+    out["pass_kin"], out["pass_pot"] = est["kin"], est["pot"]
+    arrays["periodic_pass_gr1d"] = gr1
+    return out
+
+
 SECTIONS = dict(linear=section_linear, helium=section_helium, nonlinear=section_nonlinear,
-                h2o_h2=section_h2o_h2, water2=section_water2)
+                h2o_h2=section_h2o_h2, water2=section_water2, periodic=section_periodic,
+                h2o_h2_pbc=section_h2o_h2_pbc)
 
 
 def main():

@@ -27,8 +27,8 @@ import numpy as np  # noqa: E402
 import decks  # noqa: E402
 
 DISTRIBUTIONS = {"co2_100K": (".rcf",), "he4_co2": (".gra", ".gri", ".grt", ".rcf"),
-                 "c0_8he": (".gra", ".gri", ".grt", ".rcf")}
-PREFIX = {"co2_100K": "CO2_monomer", "he4_co2": "he4co2", "c0_8he": "c0he8"}
+                 "c0_8he": (".gra", ".gri", ".grt", ".rcf"), "he16_pbc": ()}
+PREFIX = {"co2_100K": "CO2_monomer", "he4_co2": "he4co2", "c0_8he": "c0he8", "he16_pbc": "he16pbc"}
 
 
 GROUP = 10  # consecutive blocks averaged before the spread is taken (slow modes)

@@ -87,13 +87,22 @@ def test_xyz_init_is_read(tmp_path):
     assert np.allclose(cos[0], [st * np.cos(0.5), st * np.sin(0.5), 0.25], atol=1e-6)
 
 
-@pytest.mark.parametrize("extra,msg", [("WORM He4 0.13 16", "worm"), ("MINIMAGE", "MINIMAGE"),
+@pytest.mark.parametrize("extra,msg", [("WORM He4 0.13 16", "worm"), ("MINIMAGE", "MIN IMAGE for orient pot"),
                                        ("CRYSTAL FCC 1 1 1", "crystal"),
                                        ("REFLECTY 1", "REFLECTY"), ("ROTDENSI 1 -1 0 1 1 1 1", "ROTDENSI")])
 def test_methods_outside_the_path_are_refused(tmp_path, extra, msg):
     write(tmp_path, extra=extra)
     r = run(tmp_path, "--init-only")
     assert r.returncode == 1 and msg in r.stderr, r.stderr
+
+
+def test_minimum_image_accepted_without_a_rotating_linear_rotor(tmp_path):
+    """MINIMAGE is read (mc_input.cc:298-300) for a deck whose rotor does not rotate; with the
+    rotation line it is refused as the reference's rotor move stops (above)."""
+    with open(tmp_path / "qmc.input", "w") as f:
+        f.write(DECK.format(nhe=2, P=4, extra="MINIMAGE").replace("ROTATION CO2 0.3 4\n", ""))
+    r = run(tmp_path, "--init-only")
+    assert r.returncode == 0, r.stderr
 
 
 def test_bose_statistics_refused(tmp_path):

@@ -22,10 +22,10 @@ DRIVER = os.path.join(ROOT, "moribs-pimc_amd", "pimc")
 with open(os.path.join(ROOT, "tests", "golden", "reference_runs.json")) as f:
     REF = json.load(f)
 
-GPU_BLOCKS = {"co2_100K": 45, "he4_co2": 130, "c0_8he": 60}
+GPU_BLOCKS = {"co2_100K": 45, "he4_co2": 130, "c0_8he": 60, "he16_pbc": 130}
 CHAINS = 8
-PREFIX = {"co2_100K": "CO2_monomer", "he4_co2": "he4co2", "c0_8he": "c0he8"}
-NEQ = {"co2_100K": 5, "he4_co2": 10, "c0_8he": 10}
+PREFIX = {"co2_100K": "CO2_monomer", "he4_co2": "he4co2", "c0_8he": "c0he8", "he16_pbc": "he16pbc"}
+NEQ = {"co2_100K": 5, "he4_co2": 10, "c0_8he": 10, "he16_pbc": 10}
 
 
 def mean_se(chains):
@@ -45,7 +45,7 @@ def mean_se(chains):
 def driver_runs(tmp_path_factory):
     """The driver on each deck with CHAINS independent chains: {name: (directory, [eng per chain])}."""
     out = {}
-    for name in ("co2_100K", "he4_co2", "c0_8he"):
+    for name in ("co2_100K", "he4_co2", "c0_8he", "he16_pbc"):
         d = str(tmp_path_factory.mktemp(name))
         decks.materialise(name, d, blocks=GPU_BLOCKS[name])
         r = subprocess.run([DRIVER, "--replicas", str(CHAINS)], cwd=d, capture_output=True, text=True, timeout=600)
@@ -57,7 +57,7 @@ def driver_runs(tmp_path_factory):
     return out
 
 
-@pytest.mark.parametrize("name", ["co2_100K", "he4_co2", "c0_8he"])
+@pytest.mark.parametrize("name", ["co2_100K", "he4_co2", "c0_8he", "he16_pbc"])
 def test_driver_matches_reference_statistics(name, driver_runs):
     d, engs = driver_runs[name]
     neq = NEQ[name]

@@ -195,6 +195,76 @@ def test_helium_fixture(G, A):
     assert np.array_equal(gr1, A["helium_gr1d"])
 
 
+# ----------------------------------------------------------------------------- minimum image
+def periodic_system(A, **extra):
+    r, v = pt.read_pot1d(os.path.join(GOLD, "helium.pot"))
+    rp = dict(ntypes=1, numb=[16], molecule=[0], levels=[3], mcstep=[0.6], mass=[4.0026032497], P=16, Prot=0,
+              T=1.5, rtstep=0.0, pot1d=[0], pot2d=[-1], rotdens=-1, box=8.5, **extra)
+    S = O.System(rp, dict(pot1d=[(r, v)], pot2d=[], rot=[]))
+    S.set_state(A["periodic_coords"], np.zeros_like(A["periodic_coords"]))
+    return S, rp, (r, v)
+
+
+def test_periodic_energies(G, A):
+    """MINIMAGE (mc_piqmc.cc:1239-1240,1832-1833; mc_estim.cc:536-537,916-917): PotEnergy on the
+    path and on newcoords, V with its g(r) bins and K equal the compiled reference's."""
+    S, rp, _ = periodic_system(A)
+    g = G["periodic"]
+    new = A["periodic_newcoords"]
+    close([[S.L.or_pot_energy_slice(S.h, a, None, it) for it in range(16)] for a in range(16)], g["pot_slice"])
+    close([[S.L.or_pot_energy_slice(S.h, a, O.P(new), it) for it in range(16)] for a in range(16)], g["pot_slice_new"])
+    close([S.L.or_pot_energy_path(S.h, a, None) for a in range(16)], g["pot_path"])
+    close([S.L.or_pot_energy_path(S.h, a, O.P(new)) for a in range(16)], g["pot_path_new"])
+    close(S.L.or_kin_energy(S.h), g["kin"])
+    gr1 = np.zeros(300)
+    close(S.L.or_pot_energy_densities(S.h, gr1.ctypes.data_as(O._dp), None), g["pot"])
+    assert np.array_equal(gr1, A["periodic_gr1d"])
+
+
+def test_periodic_minimum_image_brute_force(A):
+    """V under the minimum image equals a direct sum over pairs and slices that takes, for every
+    pair, the closest of its 27 nearest periodic images; without the minimum image V differs."""
+    S, rp, (r, v) = periodic_system(A)
+    L, P, N = 8.5, 16, 16
+    c = A["periodic_coords"].reshape(3, N, P)
+    shifts = np.array([[i, j, k] for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)], float) * L
+    V = 0.0
+    for a in range(N):
+        for b in range(a + 1, N):
+            for it in range(P):
+                d = c[:, a, it] - c[:, b, it]
+                rr = np.min(np.linalg.norm(d[None, :] + shifts, axis=1))
+                V += S.L.or_spot1d(S.h, rr, 0)
+    V /= P
+    gr1 = np.zeros(300)
+    close(S.L.or_pot_energy_densities(S.h, gr1.ctypes.data_as(O._dp), None), V, rtol=1e-12)
+    plain, _, _ = periodic_system(A)
+    plain.L.or_set_minimage(plain.h, 0, None)
+    assert abs(plain.L.or_pot_energy_densities(plain.h, gr1.ctypes.data_as(O._dp), None) - V) > 1.0
+
+
+def test_periodic_reference_order_passes(G, A):
+    """Two passes under MINIMAGE on the reference's random numbers: the same path, counters and
+    stream consumption, and the same V and K afterwards."""
+    S, rp, _ = periodic_system(A)
+    g = G["periodic"]
+    S.set_tapes(tapes_of(A, "periodic"))
+    S.L.or_reset_counters(S.h)
+    for p in range(2):
+        assert S.L.or_pass(S.h, p, O.ORDER_REFERENCE) == 0
+    assert S.L.or_tape_short(S.h) == 0
+    for k, n in g["tape_used"].items():
+        assert S.L.or_tape_pos(S.h, int(k)) == n
+    c, _, _ = S.state()
+    close(c.reshape(-1), A["periodic_pass_coords"], rtol=1e-13)
+    t, acc = S.counters()
+    assert t.reshape(-1).tolist() == g["pass_total"] and acc.reshape(-1).tolist() == g["pass_accep"]
+    close(S.L.or_kin_energy(S.h), g["pass_kin"])
+    gr1 = np.zeros(300)
+    close(S.L.or_pot_energy_densities(S.h, gr1.ctypes.data_as(O._dp), None), g["pass_pot"])
+    assert np.array_equal(gr1, A["periodic_pass_gr1d"])
+
+
 def test_helium_potential_brute_force(A):
     """V of the estimator equals a direct double loop over pairs and slices."""
     S = helium_system(A)
@@ -321,10 +391,12 @@ def test_caleng_symmetry():
 
 # ----------------------------------------------------------------------------- non-linear systems
 def system3d(tag, rot3d):
-    if tag == "h2o_h2":
+    if tag in ("h2o_h2", "h2o_h2_pbc"):
         r, v = pt.h2h2_table()
         rp = dict(ntypes=2, numb=[4, 1], molecule=[0, 2], levels=[2, 2], mcstep=[0.2, 0.2], mass=[2.01565, 18.010565],
                   P=8, Prot=8, T=0.5, rtstep=0.25, pot1d=[0, -1], pot2d=[-1, -1], rotdens=-1)
+        if tag == "h2o_h2_pbc":
+            rp["box"] = 11.0
         tabs = dict(pot1d=[(r, v)], pot2d=[], rot=[], pot3d=tables3d.pot3d(nch=181), rot3d=rot3d)
     else:
         rp = dict(ntypes=1, numb=[2], molecule=[2], levels=[2], mcstep=[0.2], mass=[18.010565], P=8, Prot=8, T=1.0,
@@ -333,7 +405,7 @@ def system3d(tag, rot3d):
     return O.System(rp, tabs)
 
 
-@pytest.mark.parametrize("tag", ["h2o_h2", "water2"])
+@pytest.mark.parametrize("tag", ["h2o_h2", "water2", "h2o_h2_pbc"])
 def test_nonlinear_energies(G, A, rot3d, tag):
     S = system3d(tag, rot3d)
     g = G[tag]
@@ -350,7 +422,7 @@ def test_nonlinear_energies(G, A, rot3d, tag):
     assert np.array_equal(gr1, A[f"{tag}_gr1d"])
 
 
-@pytest.mark.parametrize("tag", ["h2o_h2", "water2"])
+@pytest.mark.parametrize("tag", ["h2o_h2", "water2", "h2o_h2_pbc"])
 def test_nonlinear_reference_order_passes(G, A, rot3d, tag):
     S = system3d(tag, rot3d)
     g = G[tag]
@@ -380,7 +452,7 @@ def nl_bins(S):
     return g3, rel
 
 
-@pytest.mark.parametrize("tag", ["h2o_h2", "water2"])
+@pytest.mark.parametrize("tag", ["h2o_h2", "water2", "h2o_h2_pbc"])
 def test_nonlinear_density_bins(G, A, rot3d, tag):
     """The (r, theta, chi) bins around the rotor and the relative Euler angle bins equal the
     compiled reference's _gr3D and _relthe/_relphi/_relchi_sum on the same path."""
@@ -393,7 +465,7 @@ def test_nonlinear_density_bins(G, A, rot3d, tag):
     assert np.array_equal(g3, ref[0])  # density type 0: the atom type (h2o_h2) or nothing (water2)
     assert np.all(ref[1:] == 0.0)
     assert np.array_equal(rel, A[f"{tag}_rel"])
-    nrot = 1 if tag == "h2o_h2" else 2
+    nrot = 2 if tag == "water2" else 1
     assert rel[:50].sum() == rel[50:150].sum() == rel[150:].sum() == nrot * 8
 
 
