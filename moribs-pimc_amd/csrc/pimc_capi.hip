@@ -145,6 +145,7 @@ struct pimc_ctx {
     std::vector<pimc_replica_desc> desc;
     std::vector<char> desc_set;
     std::vector<std::array<double, 3>> box;  // minimum-image box per replica, zeros when off
+    bool any_mi = false;                       // a replica uses it: launch the kernels built for it
     std::vector<Rep> rep;
     // host tables
     std::vector<Tab1D> t1;
@@ -773,11 +774,19 @@ static int plan_centroid_parts(pimc_ctx* c, int target_parts) {
     HIPCHECK(c, hipGetDeviceProperties(&prop, 0));
     const int ncu = prop.multiProcessorCount;
     if (c->fp64) {
-        HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, k_centroid_parts<double, 0>, 256, c->lds_knots));
-        HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_centroid_parts<double, 1>, 256, c->lds_knots));
+        HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &occ0, (c->any_mi ? k_centroid_parts<double, 0, true> : k_centroid_parts<double, 0, false>), 256,
+                        c->lds_knots));
+        HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &occ1, (c->any_mi ? k_centroid_parts<double, 1, true> : k_centroid_parts<double, 1, false>), 256,
+                        c->lds_knots));
     } else {
-        HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, k_centroid_parts<float, 0>, 256, c->lds_knots));
-        HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, k_centroid_parts<float, 1>, 256, c->lds_knots));
+        HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &occ0, (c->any_mi ? k_centroid_parts<float, 0, true> : k_centroid_parts<float, 0, false>), 256,
+                        c->lds_knots));
+        HIPCHECK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &occ1, (c->any_mi ? k_centroid_parts<float, 1, true> : k_centroid_parts<float, 1, false>), 256,
+                        c->lds_knots));
     }
     const double capacity = (double)ncu * std::max(1, std::min(occ0, occ1));
     // the parts of a replica wait on one another on one XCD: at most what one XCD holds
@@ -876,6 +885,7 @@ int pimc_commit(pimc_ctx* c) {
             if (rotating_linear(d))
                 return fail(c, "MIN IMAGE for orient pot: the minimum image is not applied to a rotating linear rotor");
             p.minimage = 1;
+            c->any_mi = true;
             for (int i = 0; i < 3; ++i) p.box[i] = c->box[r][i];
         }
         int g = 0, S = 1;
@@ -1476,13 +1486,13 @@ void launch_bisect(pimc_ctx* c, int u, int tb, int te) {
             s = c->side[nside];
             (void)hipStreamWaitEvent(s, c->ev_fork, 0);
         }
-        hipLaunchKernelGGL(k_bisect<R>, dim3(c->bw_n[k]), dim3(c->bw_lanes[k]), c->bw_lds[k], s, c->d_rep, c->d_mut,
+        hipLaunchKernelGGL((c->any_mi ? k_bisect<R, true> : k_bisect<R, false>), dim3(c->bw_n[k]), dim3(c->bw_lanes[k]), c->bw_lds[k], s, c->d_rep, c->d_mut,
                            c->d_beads, c->d_rotb, c->tables(), c->d_work + c->bw_off[k], run);
         if (k != top) (void)hipEventRecord(c->ev_join[nside++], s);
     }
     for (int i = 0; i < nside; ++i) (void)hipStreamWaitEvent(c->stream, c->ev_join[i], 0);
     if (c->nwork1)
-        hipLaunchKernelGGL(k_bisect_nl<R>, dim3(c->nwork1), dim3(c->bisect_block), c->lds_bisect1, c->stream,
+        hipLaunchKernelGGL((c->any_mi ? k_bisect_nl<R, true> : k_bisect_nl<R, false>), dim3(c->nwork1), dim3(c->bisect_block), c->lds_bisect1, c->stream,
                            c->d_rep, c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_work1, run);
 }
 
@@ -1491,11 +1501,11 @@ void launch_centroid(pimc_ctx* c, int type) {
     Run run{c->seed, 0, c->ref_order ? 1 : 0, type, type + 1};
     Timed t(c, "centroid");
     if (c->ncp0)
-        hipLaunchKernelGGL((k_centroid_parts<R, 0>), dim3(c->ncp0), dim3(256), c->lds_knots, c->stream, c->d_rep, c->d_mut,
+        hipLaunchKernelGGL((c->any_mi ? k_centroid_parts<R, 0, true> : k_centroid_parts<R, 0, false>), dim3(c->ncp0), dim3(256), c->lds_knots, c->stream, c->d_rep, c->d_mut,
                            c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_cparts0, c->d_cflags,
                            c->d_cslots, c->ncflags, run);
     if (c->ncp1)
-        hipLaunchKernelGGL((k_centroid_parts<R, 1>), dim3(c->ncp1), dim3(256), c->lds_knots, c->stream, c->d_rep, c->d_mut,
+        hipLaunchKernelGGL((c->any_mi ? k_centroid_parts<R, 1, true> : k_centroid_parts<R, 1, false>), dim3(c->ncp1), dim3(256), c->lds_knots, c->stream, c->d_rep, c->d_mut,
                            c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_cparts1, c->d_cflags,
                            c->d_cslots, c->ncflags, run);
 }
@@ -1528,11 +1538,11 @@ template <typename R>
 void launch_estimate(pimc_ctx* c, double* out) {
     Timed t(c, "estimate");
     if (c->nrows)
-        hipLaunchKernelGGL((k_est_tiles<R, 0>), dim3(c->nrows), dim3(256), c->lds_est, c->stream, c->d_rep,
+        hipLaunchKernelGGL((c->any_mi ? k_est_tiles<R, 0, true> : k_est_tiles<R, 0, false>), dim3(c->nrows), dim3(256), c->lds_est, c->stream, c->d_rep,
                            c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rows,
                            out ? nullptr : c->d_hist, c->d_rowbuf);
     if (c->nrows1)
-        hipLaunchKernelGGL((k_est_tiles<R, 1>), dim3(c->nrows1), dim3(256), c->lds_est, c->stream, c->d_rep,
+        hipLaunchKernelGGL((c->any_mi ? k_est_tiles<R, 1, true> : k_est_tiles<R, 1, false>), dim3(c->nrows1), dim3(256), c->lds_est, c->stream, c->d_rep,
                            c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_rows1,
                            out ? nullptr : c->d_hist, c->d_rowbuf);
     if (!out && c->nrcf)
@@ -1548,7 +1558,7 @@ void launch_estimate(pimc_ctx* c, double* out) {
 template <typename R>
 void launch_bisect_class(pimc_ctx* c, int k, int u) {
     Run run{c->seed, u, c->ref_order ? 1 : 0, 0, c->max_types};
-    hipLaunchKernelGGL(k_bisect<R>, dim3(c->bw_n[k]), dim3(c->bw_lanes[k]), c->bw_lds[k], c->stream, c->d_rep,
+    hipLaunchKernelGGL((c->any_mi ? k_bisect<R, true> : k_bisect<R, false>), dim3(c->bw_n[k]), dim3(c->bw_lanes[k]), c->bw_lds[k], c->stream, c->d_rep,
                        c->d_mut, c->d_beads, c->d_rotb, c->tables(), c->d_work + c->bw_off[k], run);
 }
 
@@ -1557,11 +1567,11 @@ void launch_centroid_all(pimc_ctx* c) {
     for (int ty = 0; ty < c->max_types; ++ty) {
         Run run{c->seed, 0, c->ref_order ? 1 : 0, ty, ty + 1};
         if (c->ncp0)
-            hipLaunchKernelGGL((k_centroid_parts<R, 0>), dim3(c->ncp0), dim3(256), c->lds_knots, c->stream, c->d_rep,
+            hipLaunchKernelGGL((c->any_mi ? k_centroid_parts<R, 0, true> : k_centroid_parts<R, 0, false>), dim3(c->ncp0), dim3(256), c->lds_knots, c->stream, c->d_rep,
                                c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_cparts0,
                                c->d_cflags, c->d_cslots, c->ncflags, run);
         if (c->ncp1)
-            hipLaunchKernelGGL((k_centroid_parts<R, 1>), dim3(c->ncp1), dim3(256), c->lds_knots, c->stream, c->d_rep,
+            hipLaunchKernelGGL((c->any_mi ? k_centroid_parts<R, 1, true> : k_centroid_parts<R, 1, false>), dim3(c->ncp1), dim3(256), c->lds_knots, c->stream, c->d_rep,
                                c->d_mut, c->d_beads, c->d_rotb, c->d_frames, c->tables(), c->tables3(), c->d_cparts1,
                                c->d_cflags, c->d_cslots, c->ncflags, run);
     }

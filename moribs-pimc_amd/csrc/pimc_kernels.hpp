@@ -105,7 +105,9 @@ __host__ __device__ constexpr int bisect_ring(int W) { return BIS_RING > 3 * W +
 // (float4 in FP32 runs, double4 in FP64 runs); new positions are built in FP64 from FP64 end
 // points, and accepted ones go to HBM in FP64 at once.  blockDim a multiple of 64.
 // FAM 1 (non-linear rotors) adds the rotors' frames at the window's slots: fr [nrot*(S+1)].
-template <typename R, int FAM>
+// MI: built for contexts that hold a replica under the minimum image (its pairs then take their
+// nearest image); without it the kernels carry no trace of it
+template <typename R, int FAM, bool MI>
 __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* __restrict__ mut,
                                             double4* __restrict__ beads, const RotBead* __restrict__ rotb,
                                             const Frame* __restrict__ frames, Tables T, Tables3 T3,
@@ -142,7 +144,7 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
     const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
     const Lut3D<R> l3 = lut3d<R>(T3, FAM == 1 ? rp.pot3d_id : -1);
     const bool half_knots = FAM == 0 && is_fp32<R>() && l1.uniform;
-    const double* box = rp.minimage ? rp.box : nullptr;  // MINIMAGE (not with a rotating linear rotor)
+    const double* box = MI && rp.minimage ? rp.box : nullptr;  // MINIMAGE (not with a rotating linear rotor)
     if (half_knots) {
         float2* k2 = reinterpret_cast<float2*>(kn);
         for (int i = tid; i < l1.n; i += nth) k2[i] = make_float2((float)l1.k[i].y, (float)l1.k[i].z);
@@ -258,14 +260,18 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                 // the rotor terms in one index space, rotor terms last, so a wave runs one
                 // table's code except at the seam
                 const bool split = FAM == 0 && !rot_a && rp.lin_atom == N - 1;
-                auto item_sum = [&](auto half_c) -> double {
+                auto item_sum = [&](auto half_c, auto mi_c) -> double {
                     constexpr bool H = decltype(half_c)::value;
+                    // the minimum image as its own instance of the loop, so the plain one is unchanged
+                    const double* bx = decltype(mi_c)::value ? box : nullptr;
+                    // (that instance takes every pair through the general branch)
+                    const bool spl = !decltype(mi_c)::value && split;
                     double acc = 0.0;
                     // item i = (midpoint qa, partner index jp) with den partners per midpoint in
                     // the atom range; a lane's items advance by nth, so (qa, jp) is carried
                     // along with one division per level
-                    const int den = split ? np - 1 : np;
-                    const int n1 = split ? nmid * den : items;  // items of the atom range
+                    const int den = spl ? np - 1 : np;
+                    const int n1 = spl ? nmid * den : items;  // items of the atom range
                     const int dd = den > 0 ? den : 1;
                     const int dq = nth / dd, dj = nth - dq * dd;
                     int qa = tid / dd, jp = tid - qa * dd;
@@ -279,7 +285,7 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                         } else {
                             q = qa;
                             j = jp + (jp >= a);
-                            rj = split ? false
+                            rj = spl ? false
                                        : (FAM == 1 ? (rp.nl_atom >= 0 && j >= rp.nl_atom) : (j == rp.lin_atom));
                         }
                         jp += dj;
@@ -295,29 +301,35 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
                         if constexpr (FAM == 1) {
                             const Frame& fa = fr[(rot_a ? a - rp.nl_atom : 0) * W + slot];
                             const Frame& fj = fr[(rj ? j - rp.nl_atom : 0) * W + slot];
-                            acc += pair_nl<R>(l1, kn, l3, rot_a, rj, xn, yn, zn, pj.x, pj.y, pj.z, fa, fj, box) -
-                                   pair_nl<R>(l1, kn, l3, rot_a, rj, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, fa, fj, box);
-                        } else if (split) {
+                            acc += pair_nl<R>(l1, kn, l3, rot_a, rj, xn, yn, zn, pj.x, pj.y, pj.z, fa, fj, bx) -
+                                   pair_nl<R>(l1, kn, l3, rot_a, rj, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, fa, fj, bx);
+                        } else if (spl) {
                             if (rj) {
                                 const P4 ax = nhf[slot];
                                 const double nx = ax.x, ny = ax.y, nz = ax.z;
-                                acc += (double)pair_vr<R, H>(l1, kn, l2, false, true, xn, yn, zn, pj.x, pj.y, pj.z, nx, ny, nz, box) -
-                                       (double)pair_vr<R, H>(l1, kn, l2, false, true, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, nx, ny, nz, box);
+                                acc += (double)pair_vr<R, H>(l1, kn, l2, false, true, xn, yn, zn, pj.x, pj.y, pj.z, nx, ny, nz, bx) -
+                                       (double)pair_vr<R, H>(l1, kn, l2, false, true, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, nx, ny, nz, bx);
                             } else {
-                                acc += (double)pair_vr<R, H>(l1, kn, l2, false, false, xn, yn, zn, pj.x, pj.y, pj.z, 0.0, 0.0, 1.0, box) -
-                                       (double)pair_vr<R, H>(l1, kn, l2, false, false, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, 0.0, 0.0, 1.0, box);
+                                acc += (double)pair_vr<R, H>(l1, kn, l2, false, false, xn, yn, zn, pj.x, pj.y, pj.z, 0.0, 0.0, 1.0, bx) -
+                                       (double)pair_vr<R, H>(l1, kn, l2, false, false, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, 0.0, 0.0, 1.0, bx);
                             }
                         } else {
                             const P4 ax = nhf[slot];
                             const double nx = ax.x, ny = ax.y, nz = ax.z;
-                            acc += (double)pair_vr<R, H>(l1, kn, l2, rot_a, rj, xn, yn, zn, pj.x, pj.y, pj.z, nx, ny, nz, box) -
-                                   (double)pair_vr<R, H>(l1, kn, l2, rot_a, rj, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, nx, ny, nz, box);
+                            acc += (double)pair_vr<R, H>(l1, kn, l2, rot_a, rj, xn, yn, zn, pj.x, pj.y, pj.z, nx, ny, nz, bx) -
+                                   (double)pair_vr<R, H>(l1, kn, l2, rot_a, rj, pa.x, pa.y, pa.z, pj.x, pj.y, pj.z, nx, ny, nz, bx);
                         }
                     }
                     return acc;
                 };
-                double acc = half_knots ? item_sum(std::integral_constant<bool, true>{})
-                                        : item_sum(std::integral_constant<bool, false>{});
+                using T1 = std::integral_constant<bool, true>;
+                using F1 = std::integral_constant<bool, false>;
+                double acc;
+                if constexpr (MI)
+                    acc = half_knots ? (box ? item_sum(T1{}, T1{}) : item_sum(T1{}, F1{}))
+                                     : (box ? item_sum(F1{}, T1{}) : item_sum(F1{}, F1{}));
+                else
+                    acc = half_knots ? item_sum(T1{}, F1{}) : item_sum(F1{}, F1{});
                 acc = block_sum(acc, red);
                 evals += 2ull * (unsigned long long)items;
                 // sum over all points up to this level; level acceptance of mc_piqmc.cc:263-273
@@ -355,19 +367,19 @@ __device__ __forceinline__ void bisect_body(const Rep* __restrict__ reps, Mut* _
 }
 
 // the linear family keeps four waves per SIMD; the non-linear family needs more registers
-template <typename R>
+template <typename R, bool MI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bisect(
     const Rep* __restrict__ reps, Mut* __restrict__ mut, double4* __restrict__ beads, const RotBead* __restrict__ rotb,
     Tables T, const int2* __restrict__ work, Run run) {
-    bisect_body<R, 0>(reps, mut, beads, rotb, nullptr, T, Tables3{}, work, run);
+    bisect_body<R, 0, MI>(reps, mut, beads, rotb, nullptr, T, Tables3{}, work, run);
 }
 
-template <typename R>
+template <typename R, bool MI>
 __global__ __launch_bounds__(256) void k_bisect_nl(const Rep* __restrict__ reps, Mut* __restrict__ mut,
                                                    double4* __restrict__ beads, const RotBead* __restrict__ rotb,
                                                    const Frame* __restrict__ frames, Tables T, Tables3 T3,
                                                    const int2* __restrict__ work, Run run) {
-    bisect_body<R, 1>(reps, mut, beads, rotb, frames, T, T3, work, run);
+    bisect_body<R, 1, MI>(reps, mut, beads, rotb, frames, T, T3, work, run);
 }
 
 // ------------------------------------------------------------------------- linear rotor
@@ -399,7 +411,7 @@ constexpr unsigned ROT_SPIN_LIMIT = 1u << 22;
 // change only inside that part, so no bead data crosses parts.  Partials are double-buffered by
 // atom parity: a part can overwrite a buffer only after every part has published the following
 // atom, that is after every part has read it.
-template <typename R, int FAM>
+template <typename R, int FAM, bool MI>
 __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ reps, Mut* __restrict__ mut,
                                                         double4* __restrict__ beads, const RotBead* __restrict__ rotb,
                                                         const Frame* __restrict__ frames, Tables T, Tables3 T3,
@@ -426,7 +438,7 @@ __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ 
     __syncthreads();
     unsigned* myflag = flags + pt.fbase + pt.w;
     const unsigned base = __hip_atomic_load(myflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const double* box = rp.minimage ? rp.box : nullptr;  // MINIMAGE (not with a rotating linear rotor)
+    const double* box = MI && rp.minimage ? rp.box : nullptr;  // MINIMAGE (not with a rotating linear rotor)
     if (threadIdx.x == 0) s_err = 0;
     unsigned long long acc_n = 0;
     for (int ia = 0; ia < rp.tnumb[ty]; ++ia) {
@@ -436,67 +448,77 @@ __global__ __launch_bounds__(256) void k_centroid_parts(const Rep* __restrict__ 
         const double dx = rp.mcstep[ty] * (u53(w0.x, w0.y) - 0.5);
         const double dy = rp.mcstep[ty] * (u53(w0.z, w0.w) - 0.5);
         const double dz = rp.mcstep[ty] * (u53(w1.x, w1.y) - 0.5);
-        double acc = 0.0;
         const int items = nt * np;
         const bool rot_a = FAM == 1 ? (rp.nl_atom >= 0 && a >= rp.nl_atom) : (a == rp.lin_atom);
         // an atom with the linear rotor last among its partners: atom-atom and rotor terms in
         // separate loops (no wave runs both tables' code)
         const bool split = FAM == 0 && !rot_a && rp.lin_atom == N - 1;
-        if (split) {  // items: nt*(np-1) atom-atom terms first, then the nt rotor terms
-            const int np1 = np - 1, n1 = nt * np1;
-            // item i = (slice t0 + q, partner index jp); a lane's items advance by blockDim,
-            // so (q, jp) is carried along with one division per atom
-            const int dd = np1 > 0 ? np1 : 1, nth = (int)blockDim.x;
-            const int dq = nth / dd, dj = nth - dq * dd;
-            int qa = (int)threadIdx.x / dd, jpa = (int)threadIdx.x - qa * dd;
-            for (int i = threadIdx.x; i < items; i += blockDim.x) {
-                const bool rj = i >= n1;
-                const int it = t0 + (rj ? i - n1 : qa), jp = rj ? 0 : jpa;
-                const int j = rj ? N - 1 : jp + (jp >= a);
-                jpa += dj;
-                qa += dq;
-                if (jpa >= dd) {
-                    jpa -= dd;
-                    ++qa;
+        // the minimum image as its own instance of the loops, so the plain one is unchanged
+        auto items_sum = [&](auto mi_c) -> double {
+            const double* bx = decltype(mi_c)::value ? box : nullptr;
+            double acc = 0.0;
+            if (split) {  // items: nt*(np-1) atom-atom terms first, then the nt rotor terms
+                const int np1 = np - 1, n1 = nt * np1;
+                // item i = (slice t0 + q, partner index jp); a lane's items advance by blockDim,
+                // so (q, jp) is carried along with one division per atom
+                const int dd = np1 > 0 ? np1 : 1, nth = (int)blockDim.x;
+                const int dq = nth / dd, dj = nth - dq * dd;
+                int qa = (int)threadIdx.x / dd, jpa = (int)threadIdx.x - qa * dd;
+                for (int i = threadIdx.x; i < items; i += blockDim.x) {
+                    const bool rj = i >= n1;
+                    const int it = t0 + (rj ? i - n1 : qa), jp = rj ? 0 : jpa;
+                    const int j = rj ? N - 1 : jp + (jp >= a);
+                    jpa += dj;
+                    qa += dq;
+                    if (jpa >= dd) {
+                        jpa -= dd;
+                        ++qa;
+                    }
+                    const double4 xa = bead(beads, rp, a, it), xj = bead(beads, rp, j, it);
+                    double nx = 0.0, ny = 0.0, nz = 1.0;
+                    if (rj) rotor_axis(rotb, rp, it, nx, ny, nz);
+                    R vn, vo;
+                    if (rj) {
+                        vn = pair_v<R>(l1, kn, l2, false, true, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, nx, ny, nz,
+                                       bx);
+                        vo = pair_v<R>(l1, kn, l2, false, true, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz, bx);
+                    } else {
+                        vn = pair_v<R>(l1, kn, l2, false, false, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, 0.0,
+                                       0.0, 1.0, bx);
+                        vo = pair_v<R>(l1, kn, l2, false, false, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, 0.0, 0.0, 1.0, bx);
+                    }
+                    acc += (double)vn - (double)vo;
                 }
+            }
+            for (int i = split ? items : (int)threadIdx.x; i < items; i += blockDim.x) {
+                const int it = t0 + i / np, jp = i % np;
+                const int j = jp + (jp >= a);
+                const bool rot_j = FAM == 1 ? (rp.nl_atom >= 0 && j >= rp.nl_atom) : (j == rp.lin_atom);
                 const double4 xa = bead(beads, rp, a, it), xj = bead(beads, rp, j, it);
-                double nx = 0.0, ny = 0.0, nz = 1.0;
-                if (rj) rotor_axis(rotb, rp, it, nx, ny, nz);
-                R vn, vo;
-                if (rj) {
-                    vn = pair_v<R>(l1, kn, l2, false, true, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, nx, ny, nz,
-                                   box);
-                    vo = pair_v<R>(l1, kn, l2, false, true, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz, box);
+                if constexpr (FAM == 1) {
+                    const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
+                    const Frame& fa = frame_at(frames, rp, rot_a ? a : rp.nl_atom, it);
+                    const Frame& fj = frame_at(frames, rp, rot_j ? j : rp.nl_atom, it);
+                    const double vn = pair_nl<R>(l1, kn, l3, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y,
+                                                 xj.z, fa, fj, bx);
+                    const double vo = pair_nl<R>(l1, kn, l3, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, fa, fj, bx);
+                    acc += vn - vo;
                 } else {
-                    vn = pair_v<R>(l1, kn, l2, false, false, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z, 0.0,
-                                   0.0, 1.0, box);
-                    vo = pair_v<R>(l1, kn, l2, false, false, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, 0.0, 0.0, 1.0, box);
+                    double nx, ny, nz;
+                    rotor_axis(rotb, rp, it, nx, ny, nz);
+                    const R vn = pair_v<R>(l1, kn, l2, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z,
+                                           nx, ny, nz, bx);
+                    const R vo = pair_v<R>(l1, kn, l2, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz, bx);
+                    acc += (double)vn - (double)vo;
                 }
-                acc += (double)vn - (double)vo;
             }
-        }
-        for (int i = split ? items : (int)threadIdx.x; i < items; i += blockDim.x) {
-            const int it = t0 + i / np, jp = i % np;
-            const int j = jp + (jp >= a);
-            const bool rot_j = FAM == 1 ? (rp.nl_atom >= 0 && j >= rp.nl_atom) : (j == rp.lin_atom);
-            const double4 xa = bead(beads, rp, a, it), xj = bead(beads, rp, j, it);
-            if constexpr (FAM == 1) {
-                const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
-                const Frame& fa = frame_at(frames, rp, rot_a ? a : rp.nl_atom, it);
-                const Frame& fj = frame_at(frames, rp, rot_j ? j : rp.nl_atom, it);
-                const double vn = pair_nl<R>(l1, kn, l3, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y,
-                                             xj.z, fa, fj, box);
-                const double vo = pair_nl<R>(l1, kn, l3, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, fa, fj, box);
-                acc += vn - vo;
-            } else {
-                double nx, ny, nz;
-                rotor_axis(rotb, rp, it, nx, ny, nz);
-                const R vn = pair_v<R>(l1, kn, l2, rot_a, rot_j, xa.x + dx, xa.y + dy, xa.z + dz, xj.x, xj.y, xj.z,
-                                       nx, ny, nz, box);
-                const R vo = pair_v<R>(l1, kn, l2, rot_a, rot_j, xa.x, xa.y, xa.z, xj.x, xj.y, xj.z, nx, ny, nz, box);
-                acc += (double)vn - (double)vo;
-            }
-        }
+            return acc;
+        };
+        double acc;
+        if constexpr (MI)
+            acc = box ? items_sum(std::integral_constant<bool, true>{}) : items_sum(std::integral_constant<bool, false>{});
+        else
+            acc = items_sum(std::integral_constant<bool, false>{});
         const double partial = block_sum(acc, red);
         unsigned long long* buf = slots + ((size_t)(ia & 1) * nslots + pt.fbase) * 2;
         if (W == 1) {
@@ -1818,7 +1840,7 @@ __global__ void k_pass_end(Mut* mut, int n) {
 // part[row_off + tile] = (pair potential of the tile, its kinetic links / (4 beta lambda)).
 // The potential and the bins follow GetPotEnergy_Densities (mc_estim.cc:500-687), the kinetic
 // term GetKinEnergy (mc_estim.cc:876-937).
-template <typename R, int FAM>
+template <typename R, int FAM, bool MI>
 __global__ __launch_bounds__(256) void k_est_tiles(const Rep* __restrict__ reps, const double4* __restrict__ beads,
                                                    const RotBead* __restrict__ rotb, const Frame* __restrict__ frames,
                                                    Tables T, Tables3 T3, const int2* __restrict__ tiles,
@@ -1855,16 +1877,8 @@ __global__ __launch_bounds__(256) void k_est_tiles(const Rep* __restrict__ reps,
     unsigned* g3 = (g1 && rp.h3_off >= 0) ? g1 + rp.h3_off : nullptr;
     for (int i = threadIdx.x; i < BINSR; i += blockDim.x) g1s[i] = 0u;
     __syncthreads();
-    // kinetic links (a, s) - (a, s + 1)
-    const double* box = rp.minimage ? rp.box : nullptr;  // MINIMAGE (not with a rotating linear rotor)
+    const double* box = MI && rp.minimage ? rp.box : nullptr;  // MINIMAGE (not with a rotating linear rotor)
     double kin = 0.0;
-    for (int i = threadIdx.x; i < N * nts; i += blockDim.x) {
-        const int a = i / nts, w = i - a * nts;
-        const int j = a * W + w;
-        const double dx = nearest_image(lx[j] - lx[j + 1], box, 0), dy = nearest_image(ly[j] - ly[j + 1], box, 1),
-                     dz = nearest_image(lz[j] - lz[j + 1], box, 2);
-        kin += (dx * dx + dy * dy + dz * dz) / (4.0 * rp.beta * rp.lambda[kind[a] & 255]);
-    }
     const Lut1D<R> l1 = lut1d<R>(T, rp.pot1d_id);
     const Lut2D<R> l2 = lut2d<R>(T, rp.pot2d_id);
     const double drad = MAX_RADIUS / (double)BINSR;
@@ -1873,67 +1887,85 @@ __global__ __launch_bounds__(256) void k_est_tiles(const Rep* __restrict__ reps,
     const int npairs = N * (N - 1) / 2;
     const float invN = 1.0f / (float)N;
     double pot = 0.0;
-    for (int i = threadIdx.x; i < (npairs << lts); i += blockDim.x) {
-        const int w = i & (TS - 1);
-        if (w >= nts) continue;
-        const int p = i >> lts;
-        // p = N (d - 1) + a; the float quotient is exact while N^2 < 2^23
-        const int d1 = N < 2896 ? (int)(((float)p + 0.5f) * invN) : p / N;
-        const int a0 = p - d1 * N;
-        int a1 = a0 + d1 + 1;
-        if (a1 >= N) a1 -= N;
-        const int k0 = kind[a0], k1 = kind[a1];
-        const int t0 = k0 & 255, t1 = k1 & 255, m0 = k0 >> 8, m1 = k1 >> 8;
-        const int j0 = a0 * W + w, j1 = a1 * W + w;
-        const double ex = nearest_image(lx[j0] - lx[j1], box, 0), ey = nearest_image(ly[j0] - ly[j1], box, 1),
-                     ez = nearest_image(lz[j0] - lz[j1], box, 2);
-        const R dx = (R)ex, dy = (R)ey, dz = (R)ez;
-        const R rr = sqrt(dx * dx + dy * dy + dz * dz);
-        if constexpr (FAM == 1) {
-            if (m0 == 2 || m1 == 2) {  // mc_estim.cc:582-672: vcord or caleng
-                const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
-                const int it = s0 + w;
-                const Frame& f0 = frame_at(frames, rp, m0 == 2 ? a0 : rp.nl_atom, it);
-                const Frame& f1 = frame_at(frames, rp, m1 == 2 ? a1 : rp.nl_atom, it);
-                pot += pair_nl<R>(l1, l1.k, l3, m0 == 2, m1 == 2, lx[j0], ly[j0], lz[j0], lx[j1], ly[j1], lz[j1], f0, f1);
-                if (g3 && m0 != m1) {  // bin_3Ddensity of vcord's FP64 geometry (mc_estim.cc:637,1252-1273)
-                    double rad, the, chi;
-                    if (m0 == 2) vcord_geometry(f0, lx[j0], ly[j0], lz[j0], lx[j1], ly[j1], lz[j1], rad, the, chi);
-                    else vcord_geometry(f1, lx[j1], ly[j1], lz[j1], lx[j0], ly[j0], lz[j0], rad, the, chi);
-                    const int br = (int)floor(rad / drad);
-                    if (br < BINSR && br >= 0) {
-                        const int bt = (int)floor(the / dth);
-                        if (bt < BINST && bt >= 0) {
-                            const int bc = (int)floor(chi / dch);
-                            if (bc < BINSC && bc >= 0) atomicAdd(&g3[(br * BINST + bt) * BINSC + bc], 1u);
+    // kinetic links (a, s) - (a, s + 1), then the pairs; the minimum image as its own instance
+    // of the loops, so the plain one is unchanged
+    auto tile_sums = [&](auto mi_c) {
+        const double* bx = decltype(mi_c)::value ? box : nullptr;
+        for (int i = threadIdx.x; i < N * nts; i += blockDim.x) {
+            const int a = i / nts, w = i - a * nts;
+            const int j = a * W + w;
+            const double dx = nearest_image(lx[j] - lx[j + 1], bx, 0), dy = nearest_image(ly[j] - ly[j + 1], bx, 1),
+                         dz = nearest_image(lz[j] - lz[j + 1], bx, 2);
+            kin += (dx * dx + dy * dy + dz * dz) / (4.0 * rp.beta * rp.lambda[kind[a] & 255]);
+        }
+        for (int i = threadIdx.x; i < (npairs << lts); i += blockDim.x) {
+            const int w = i & (TS - 1);
+            if (w >= nts) continue;
+            const int p = i >> lts;
+            // p = N (d - 1) + a; the float quotient is exact while N^2 < 2^23
+            const int d1 = N < 2896 ? (int)(((float)p + 0.5f) * invN) : p / N;
+            const int a0 = p - d1 * N;
+            int a1 = a0 + d1 + 1;
+            if (a1 >= N) a1 -= N;
+            const int k0 = kind[a0], k1 = kind[a1];
+            const int t0 = k0 & 255, t1 = k1 & 255, m0 = k0 >> 8, m1 = k1 >> 8;
+            const int j0 = a0 * W + w, j1 = a1 * W + w;
+            const double ex = nearest_image(lx[j0] - lx[j1], bx, 0), ey = nearest_image(ly[j0] - ly[j1], bx, 1),
+                         ez = nearest_image(lz[j0] - lz[j1], bx, 2);
+            const R dx = (R)ex, dy = (R)ey, dz = (R)ez;
+            const R rr = sqrt(dx * dx + dy * dy + dz * dz);
+            if constexpr (FAM == 1) {
+                if (m0 == 2 || m1 == 2) {  // mc_estim.cc:582-672: vcord or caleng
+                    const Lut3D<R> l3 = lut3d<R>(T3, rp.pot3d_id);
+                    const int it = s0 + w;
+                    const Frame& f0 = frame_at(frames, rp, m0 == 2 ? a0 : rp.nl_atom, it);
+                    const Frame& f1 = frame_at(frames, rp, m1 == 2 ? a1 : rp.nl_atom, it);
+                    pot += pair_nl<R>(l1, l1.k, l3, m0 == 2, m1 == 2, lx[j0], ly[j0], lz[j0], lx[j1], ly[j1], lz[j1], f0, f1);
+                    if (g3 && m0 != m1) {  // bin_3Ddensity of vcord's FP64 geometry (mc_estim.cc:637,1252-1273)
+                        double rad, the, chi;
+                        if (m0 == 2) vcord_geometry(f0, lx[j0], ly[j0], lz[j0], lx[j1], ly[j1], lz[j1], rad, the, chi);
+                        else vcord_geometry(f1, lx[j1], ly[j1], lz[j1], lx[j0], ly[j0], lz[j0], rad, the, chi);
+                        const int br = (int)floor(rad / drad);
+                        if (br < BINSR && br >= 0) {
+                            const int bt = (int)floor(the / dth);
+                            if (bt < BINST && bt >= 0) {
+                                const int bc = (int)floor(chi / dch);
+                                if (bc < BINSC && bc >= 0) atomicAdd(&g3[(br * BINST + bt) * BINSC + bc], 1u);
+                            }
                         }
                     }
-                }
-                continue;
-            }
-        }
-        if (m0 == 1 || m1 == 1) {  // axis to the partner: from the rotor to the atom
-            const double nx = ax[w], ny = ay[w], nz = az[w];
-            R cost = ((R)nx * dx + (R)ny * dy + (R)nz * dz) / rr;
-            if (m0 == 1) cost = -cost;
-            pot += (double)lpot2d<R>(l2, rr, cost);
-            if (g2h) {  // bins from FP64 geometry, as bin_2Ddensity gets it (mc_estim.cc:1233-1250)
-                const double rd = sqrt(ex * ex + ey * ey + ez * ez);
-                double cd = (nx * ex + ny * ey + nz * ez) / rd;
-                if (m0 == 1) cd = -cd;
-                const int br = (int)floor(rd / drad);
-                if (br < BINSR && br >= 0) {
-                    const int bt = (int)floor(acos(cd) / dth);
-                    if (bt < BINST && bt >= 0) atomicAdd(&g2h[br * BINST + bt], 1u);
+                    continue;
                 }
             }
-        } else if (t0 == t1 && m0 == 0) {
-            pot += (double)spot1d<R>(l1, l1.k, rr);
-            if (g1) {  // bin_1Ddensity (mc_estim.cc:1275-1286) on the FP64 distance
-                const int br = (int)floor(sqrt(ex * ex + ey * ey + ez * ez) / drad);
-                if (br < BINSR && br >= 0) atomicAdd(&g1s[br], 1u);
+            if (m0 == 1 || m1 == 1) {  // axis to the partner: from the rotor to the atom
+                const double nx = ax[w], ny = ay[w], nz = az[w];
+                R cost = ((R)nx * dx + (R)ny * dy + (R)nz * dz) / rr;
+                if (m0 == 1) cost = -cost;
+                pot += (double)lpot2d<R>(l2, rr, cost);
+                if (g2h) {  // bins from FP64 geometry, as bin_2Ddensity gets it (mc_estim.cc:1233-1250)
+                    const double rd = sqrt(ex * ex + ey * ey + ez * ez);
+                    double cd = (nx * ex + ny * ey + nz * ez) / rd;
+                    if (m0 == 1) cd = -cd;
+                    const int br = (int)floor(rd / drad);
+                    if (br < BINSR && br >= 0) {
+                        const int bt = (int)floor(acos(cd) / dth);
+                        if (bt < BINST && bt >= 0) atomicAdd(&g2h[br * BINST + bt], 1u);
+                    }
+                }
+            } else if (t0 == t1 && m0 == 0) {
+                pot += (double)spot1d<R>(l1, l1.k, rr);
+                if (g1) {  // bin_1Ddensity (mc_estim.cc:1275-1286) on the FP64 distance
+                    const int br = (int)floor(sqrt(ex * ex + ey * ey + ez * ez) / drad);
+                    if (br < BINSR && br >= 0) atomicAdd(&g1s[br], 1u);
+                }
             }
         }
+    };
+    if constexpr (MI) {
+        if (box) tile_sums(std::integral_constant<bool, true>{});
+        else tile_sums(std::integral_constant<bool, false>{});
+    } else {
+        tile_sums(std::integral_constant<bool, false>{});
     }
     const double sp = block_sum(pot, red);
     const double sk = block_sum(kin, red);
