@@ -292,10 +292,10 @@ def traffic_per_launch(workload_tag, kernel):
 
 # the linear family's bisection is one fork-join launch of its size classes; the traffic file
 # holds that launch's bytes as k_bisect<float>[launch]
-KERNEL_NAMES = {"bisect": "k_bisect<float>[launch]", "centroid": "k_centroid_parts<float,0>", "rotor": "k_rotor<float>",
-                "estimate": "k_est_tiles<float,0>"}
-KERNEL_NAMES_NL = {"bisect": "k_bisect_nl<float>", "centroid": "k_centroid_parts<float,1>", "rotor": "k_rotor3d_parts<float>",
-                   "estimate": "k_est_tiles<float,1>"}
+KERNEL_NAMES = {"bisect": "k_bisect<float,false>[launch]", "centroid": "k_centroid_parts<float,0,false>",
+                "rotor": "k_rotor<float>", "estimate": "k_est_tiles<float,0,false>"}
+KERNEL_NAMES_NL = {"bisect": "k_bisect_nl<float,false>", "centroid": "k_centroid_parts<float,1,false>",
+                   "rotor": "k_rotor3d_parts<float>", "estimate": "k_est_tiles<float,1,false>"}
 WORKLOAD_TAG = {"c2": "c2_256replicas", "c1": "c1_64he", "c0": "c0_8he", "c3": "c3_asymtop_32h2",
                 "c4": "c4_water2_128replicas"}
 
