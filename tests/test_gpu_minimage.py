@@ -152,3 +152,11 @@ def test_minimum_image_refused_for_a_rotating_linear_rotor():
     wl["replicas"][0]["box"] = 10.0
     with pytest.raises(RuntimeError, match="MIN IMAGE for orient pot"):
         gpu_ctx(wl)
+
+
+@pytest.mark.parametrize("box", [0.0, -8.5, float("inf"), float("nan")])
+def test_minimum_image_box_must_be_positive_and_finite(box):
+    wl = helium_box(4, 16, 8.5, seed=1)
+    wl["replicas"][0]["box"] = box
+    with pytest.raises(RuntimeError, match="box edges must be positive"):
+        gpu_ctx(wl)
