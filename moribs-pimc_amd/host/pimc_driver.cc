@@ -1004,7 +1004,8 @@ int main(int argc, char** argv) {
                 std::ostringstream bn;
                 bn << std::setw(3) << std::setfill('0') << block;
                 const std::string fname = ch.stem + bn.str();
-                save_block_densities(ch, fname, ch.block.count, d, n_plain, linear);
+                // the block densities only with a molecule (MCSaveBlockAverages mc_main.cc:716-735)
+                if (linear || nonlinear) save_block_densities(ch, fname, ch.block.count, d, n_plain, linear);
                 if (nonlinear) {  // MCSaveBlockAverages mc_main.cc:722-735
                     save_rho1d(ch.gr3, fname, ch.block.count, d, nullptr);
                     save_gtc(ch.gr3, fname + ".gtc", ch.block.count, d);

@@ -57,6 +57,17 @@ def driver_runs(tmp_path_factory):
     return out
 
 
+def test_atoms_only_deck_writes_the_reference_files(driver_runs):
+    """Without a molecule the reference writes no block densities, only the accumulated g(r)
+    (mc_main.cc:446-451,716-735)."""
+    d, _ = driver_runs["he16_pbc"]
+    names = os.listdir(d)
+    assert "he16pbc_sum.gra" in names and "he16pbc.eng" in names
+    assert not [n for n in names if n.startswith("he16pbc0") and n.endswith(".gra")]
+    gra = np.loadtxt(os.path.join(d, "he16pbc_sum.gra"))
+    assert gra.shape == (300, 2) and gra[:, 1].max() > 0.0
+
+
 @pytest.mark.parametrize("name", ["co2_100K", "he4_co2", "c0_8he", "he16_pbc"])
 def test_driver_matches_reference_statistics(name, driver_runs):
     d, engs = driver_runs[name]
