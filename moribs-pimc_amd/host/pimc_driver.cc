@@ -42,8 +42,10 @@
 // before any device work.  With R > 1 the same deck runs as R independent chains in one context; chain r > 0 writes
 // its files under the prefix <OUTPUTDIR><FILENAMEPREFIX>_r<r>.
 #include "moribs_pimc.h"
+#include "sci_format.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -53,6 +55,7 @@
 #include <iostream>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -599,27 +602,39 @@ void save_gtc(const std::vector<double>& g3, const std::string& name, double aco
 }
 
 // SaveDensities3D (mc_estim.cc:1728-1820): r, theta, chi and rho / (r^2 sin theta) per column,
-// one line per bin, chi fastest.  Formatted into a buffer: the file has 1.5 million lines.
+// one line per bin, chi fastest.  Formatted into a buffer: the file has 1.5 million lines.  The
+// r, theta and chi fields repeat, so each is formatted once; the densities go through
+// sci_format::e14_6, which writes printf's "%14.6e" bytes.
 void save_g3d(const std::vector<double>& g3, const std::string& name, double acount, const Deck& d) {
     const auto cols = dens3d_columns(d, g3);
     auto f = open_out(name, "SaveDensities3D");
     const double norm5 = DR * DTH * DCH * d.P * acount;
+    auto field = [](double x) {
+        char num[32];
+        const int n = std::snprintf(num, sizeof num, "%14.6e   ", x);
+        if (n < 0 || n >= (int)sizeof num) die("SaveDensities3D", "number format overflow");
+        return std::string(num, (size_t)n);
+    };
+    std::vector<std::string> fr(NBR), ft(NBT), fc(NBC);
+    for (int ir = 0; ir < NBR; ++ir) fr[ir] = field((ir + 0.5) * DR);
+    for (int it = 0; it < NBT; ++it) ft[it] = field((it + 0.5) * DTH * 180.0 / M_PI);
+    for (int ic = 0; ic < NBC; ++ic) fc[ic] = field((ic + 0.5) * DCH * 180.0 / M_PI);
     std::string buf;
     buf.reserve(1 << 22);
-    char num[128];  // four fields of 14 characters and 3 blanks
+    char num[32];  // one density field (snprintf's longest is 15 characters)
     for (int ir = 0; ir < NBR; ++ir) {
         const double r = (ir + 0.5) * DR;
         for (int it = 0; it < NBT; ++it) {
-            const double th = (it + 0.5) * DTH;
+            const double sth = std::sin((it + 0.5) * DTH);
             for (int ic = 0; ic < NBC; ++ic) {
-                const double ch = (ic + 0.5) * DCH;
                 const size_t k = ((size_t)ir * NBT + it) * NBC + ic;
                 for (auto& c : cols) {
-                    const double v = c.bins ? (*c.bins)[k] / (norm5 * r * r * std::sin(th)) : 0.0;
-                    const int n = std::snprintf(num, sizeof num, "%14.6e   %14.6e   %14.6e   %14.6e   ", r,
-                                                th * 180.0 / M_PI, ch * 180.0 / M_PI, v);
-                    if (n < 0 || n >= (int)sizeof num) die("SaveDensities3D", "number format overflow");
-                    buf.append(num, (size_t)n);
+                    const double v = c.bins ? (*c.bins)[k] / (norm5 * r * r * sth) : 0.0;
+                    buf += fr[ir];
+                    buf += ft[it];
+                    buf += fc[ic];
+                    buf.append(num, (size_t)sci_format::e14_6(v, num));
+                    buf += "   ";
                 }
                 buf.push_back('\n');
             }
@@ -631,6 +646,24 @@ void save_g3d(const std::vector<double>& g3, const std::string& name, double aco
     }
     f.write(buf.data(), (std::streamsize)buf.size());
     if (!f.good()) die("SaveDensities3D", "cannot write " + name);
+}
+
+// Runs job(r) for r = 0..n-1 on up to 16 threads.
+template <typename F>
+void for_each_chain(int n, F job) {
+    const int hw = (int)std::thread::hardware_concurrency();
+    const int nt = std::max(1, std::min(std::min(n, 16), hw));
+    if (nt == 1) {
+        for (int r = 0; r < n; ++r) job(r);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&] {
+            for (int r = next++; r < n; r = next++) job(r);
+        });
+    for (auto& th : pool) th.join();
 }
 
 void set_sci(std::ostream& o) { o << std::setprecision(6) << std::scientific; }
@@ -996,8 +1029,23 @@ int main(int argc, char** argv) {
             if (measured) c.check(pimc_clear_samples(c.ctx), "pimc_clear_samples");
             done += m;
         }
-        std::vector<double> cg(3 * (size_t)n), ag(3 * (size_t)n), sg(3 * (size_t)n);
+        // every chain's configuration from the device first (device calls stay on this thread),
+        // then each chain's files on a worker thread: the chains write disjoint files
+        std::vector<std::vector<double>> cgs(nrep, std::vector<double>(3 * (size_t)n)), sgs = cgs;
+        std::vector<double> ag(3 * (size_t)n);
         for (int r = 0; r < nrep; ++r) {
+            std::vector<double>& sg = sgs[r];
+            c.check(pimc_get_state(c.ctx, r, cgs[r].data(), ag.data(), sg.data()), "pimc_get_state");
+            // beads without an orientation on the device keep their initial cosines
+            int first = 0;
+            for (auto& s : d.types) {
+                if (!(d.rotation && s.type == d.rot_type))
+                    for (int i = first * d.P; i < (first + s.numb) * d.P; ++i)
+                        for (int id = 0; id < 3; ++id) sg[(size_t)id * n + i] = cos0[(size_t)id * n + i];
+                first += s.numb;
+            }
+        }
+        for_each_chain(nrep, [&](int r) {
             Chain& ch = chains[r];
             if (measured && ch.block.count > 0) {
                 // MCSaveBlockAverages, then the accumulated files (mc_main.cc:446-462)
@@ -1039,17 +1087,8 @@ int main(int argc, char** argv) {
                 }
                 if (d.rotation) save_rcf(ch.rcf_sum, ch.stem + "_sum.rcf", ch.total.count, d);
             }
-            c.check(pimc_get_state(c.ctx, r, cg.data(), ag.data(), sg.data()), "pimc_get_state");
-            // beads without an orientation on the device keep their initial cosines
-            int first = 0;
-            for (auto& s : d.types) {
-                if (!(d.rotation && s.type == d.rot_type))
-                    for (int i = first * d.P; i < (first + s.numb) * d.P; ++i)
-                        for (int id = 0; id < 3; ++id) sg[(size_t)id * n + i] = cos0[(size_t)id * n + i];
-                first += s.numb;
-            }
-            save_xyz(ch, d, cg, sg);
-        }
+            save_xyz(ch, d, cgs[r], sgs[r]);
+        });
         // checkpoint of every chain after the block (its exact device state)
         Checkpoint k;
         k.block = block;

@@ -148,3 +148,60 @@ def test_new_run_keeps_an_existing_checkpoint(tmp_path):
     (tmp_path / "yw001.stat").write_text("STARTBLOCK 3\n")
     r = run(tmp_path)
     assert r.returncode == 1 and "RESTART" in r.stderr, r.stderr
+
+
+FORMAT_CHECK = r"""
+#include "sci_format.hpp"
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+int main() {
+    long n = 0, bad = 0;
+    auto check = [&](double x) {
+        char a[64] = {0}, b[64] = {0};
+        const int na = sci_format::e14_6(x, a);
+        const int nb = std::snprintf(b, sizeof b, "%14.6e", x);
+        ++n;
+        if (na != nb || std::memcmp(a, b, (size_t)nb)) {
+            if (bad++ < 5) std::printf("%.17g [%.*s] [%s]\n", x, na, a, b);
+        }
+    };
+    std::mt19937_64 g(7);
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    for (long i = 0; i < 3000000; ++i) check((u(g) < 0.5 ? -1.0 : 1.0) * std::pow(10.0, 44.0 * u(g) - 22.0));
+    for (int e = -18; e <= 18; ++e)  // seven-digit ties, decades and their neighbours
+        for (long m = 999990; m <= 1000010; ++m)
+            for (double h : {0.0, 0.5}) {
+                const double x = (m + h) * std::pow(10.0, e);
+                for (double y : {x, std::nextafter(x, 0.0), std::nextafter(x, 1e300)}) {
+                    check(y);
+                    check(-y);
+                }
+            }
+    for (long m = 9999990; m <= 10000010; ++m)
+        for (int e = -12; e <= 12; ++e) check((m + 0.5) * std::pow(10.0, e - 7));
+    const double special[] = {0.0, -0.0, 5e-324, 1e-300, 1e300, 1e100, 9.9999995e99, 1e16, 1e-16,
+                              1234567.5, 0.5, 2.5, INFINITY, -INFINITY, NAN};
+    for (double x : special) {
+        check(x);
+        check(-x);
+    }
+    std::printf("%ld %ld\n", n, bad);
+    return bad != 0;
+}
+"""
+
+
+def test_density_field_format_equals_printf(tmp_path):
+    """The 3D density file's fast "%14.6e" (host/sci_format.hpp) against snprintf on random
+    magnitudes, seven-digit ties, decades, signed zeros and non-finite values."""
+    src = tmp_path / "check.cc"
+    src.write_text(FORMAT_CHECK)
+    exe = tmp_path / "check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "moribs-pimc_amd", "host"), str(src), "-o",
+                    str(exe)], check=True, timeout=120)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+    n, bad = map(int, r.stdout.split()[-2:])
+    assert bad == 0 and n > 3000000
