@@ -9,6 +9,11 @@ tests of the host driver (which run moribs-pimc_amd/pimc on the same files).
   c0_8he       the 8-He workload: 8 He + CO2 at 0.37 K, P = Prot = 128, the same tables
   he16_pbc     16 He at liquid density (0.0218 Angstrom^-3, a periodic box of 9 Angstrom) under
                MINIMAGE at 2 K, P = 16, LJ He-He
+  h2o_ph2      4 para-H2 + one rotating H2O at 0.5 K, P = Prot = 16, blocks of 500 passes (both
+               programs rewrite a 1.5-million-line 3D density file after every block): the H2-H2 table of
+               pimc_tables.h2h2_table, the stand-in 3D potential and density tables of tables3d
+               (H2Opot.pot, H2O_T0.5t16.rho/.eng/.esq, 1.4 GB of text written once per directory
+               or shared between directories)
 """
 import os
 import shutil
@@ -84,16 +89,57 @@ MCSKIP_RATIO     1600
 MCSKIP_TOTAL     1600
 MCSKIP_AVERG     16
 """,
+    "h2o_ph2": """MASTERDIR        ./
+OUTPUTDIR        ./
+FILENAMEPREFIX   h2oph2
+DIMENSION     3
+DENSITY       0.02
+TEMPERATURE   0.5
+ATOM       H2   4    BOLTZMANN     0.30     2         pH2      PRIMITIVE
+NONLINEAR  H2O  1    BOLTZMANN     0.05     2         H2Opot   PRIMITIVE
+ROTATION   H2O   0.3  16
+NUMBEROFSLICES     16
+NUMBEROFPASSES     500
+NUMBEROFBLOCKS     {blocks}  10
+MCSKIP_RATIO     8000
+MCSKIP_TOTAL     8000
+MCSKIP_AVERG     16
+""",
 }
-BLOCKS = {"co2_100K": 205, "he4_co2": 1010, "c0_8he": 310, "he16_pbc": 1010}
+BLOCKS = {"co2_100K": 205, "he4_co2": 1010, "c0_8he": 310, "he16_pbc": 1010, "h2o_ph2": 130}
+ROT3D_STEM = "H2O_T0.5t16"  # init_rot3D's file stem for H2O at 0.5 K with 16 rotor slices
 
 
-def materialise(name, d, blocks=None):
-    """Writes the deck and its table files into directory d."""
+def write_nl_tables(d, rotation):
+    """The H2-H2 table, the stand-in 3D potential and, with rotation moves, the stand-in density,
+    energy and energy-square tables of tables3d, as the reference reads them."""
+    try:
+        from golden import tables3d
+    except ImportError:  # run from tests/golden itself
+        import tables3d
+    os.makedirs(d, exist_ok=True)
+    r, v = pt.h2h2_table()
+    pt.write_pot1d(os.path.join(d, "pH2.pot"), r, v)
+    tables3d.write_pot3d(os.path.join(d, "H2Opot.pot"), *tables3d.pot3d(nch=181))
+    if rotation:
+        pt.write_rot3d(os.path.join(d, ROT3D_STEM), *tables3d.rot3d())
+
+
+def materialise(name, d, blocks=None, shared=None):
+    """Writes the deck and its table files into directory d.  For h2o_ph2 the tables are
+    written once into `shared` when given (and linked from d), else into d."""
     os.makedirs(d, exist_ok=True)
     with open(os.path.join(d, "qmc.input"), "w") as f:
         f.write(DECKS[name].format(blocks=blocks or BLOCKS[name]))
-    if name == "co2_100K":
+    if name == "h2o_ph2":
+        if shared is None:
+            write_nl_tables(d, rotation=True)
+            return
+        if not os.path.exists(os.path.join(shared, ROT3D_STEM + ".esq")):
+            write_nl_tables(shared, rotation=True)
+        for f in ("pH2.pot", "H2Opot.pot", ROT3D_STEM + ".rho", ROT3D_STEM + ".eng", ROT3D_STEM + ".esq"):
+            os.symlink(os.path.join(shared, f), os.path.join(d, f))
+    elif name == "co2_100K":
         shutil.copy(os.path.join(HERE, "CO2_T100t4.rot"), d)
         with open(os.path.join(d, "CO2_fake.pot"), "w") as f:  # the example's empty 2D table
             f.write("0 0\n0 0\n")
@@ -132,16 +178,10 @@ MCSKIP_AVERG     16
 
 def materialise_nl(d, blocks=3):
     """Writes NL_DECK and its tables into directory d."""
-    try:
-        from golden import tables3d
-    except ImportError:  # run from tests/golden itself
-        import tables3d
     os.makedirs(d, exist_ok=True)
     with open(os.path.join(d, "qmc.input"), "w") as f:
         f.write(NL_DECK.format(blocks=blocks))
-    r, v = pt.h2h2_table()
-    pt.write_pot1d(os.path.join(d, "pH2.pot"), r, v)
-    tables3d.write_pot3d(os.path.join(d, "H2Opot.pot"), *tables3d.pot3d(nch=181))
+    write_nl_tables(d, rotation=False)
 
 
 ENG_COLUMNS = ("kin", "pot", "kin+pot", "rot", "rotsq", "total", "Cv", "Cv_trans", "Cv_rot")

@@ -27,8 +27,11 @@ import numpy as np  # noqa: E402
 import decks  # noqa: E402
 
 DISTRIBUTIONS = {"co2_100K": (".rcf",), "he4_co2": (".gra", ".gri", ".grt", ".rcf"),
-                 "c0_8he": (".gra", ".gri", ".grt", ".rcf"), "he16_pbc": ()}
-PREFIX = {"co2_100K": "CO2_monomer", "he4_co2": "he4co2", "c0_8he": "c0he8", "he16_pbc": "he16pbc"}
+                 "c0_8he": (".gra", ".gri", ".grt", ".rcf"), "he16_pbc": (),
+                 "h2o_ph2": (".gra", ".gri", ".grt", ".grc", ".rcf")}
+PREFIX = {"co2_100K": "CO2_monomer", "he4_co2": "he4co2", "c0_8he": "c0he8", "he16_pbc": "he16pbc",
+          "h2o_ph2": "h2oph2"}
+SYMMETRIC = ("he4_co2", "c0_8he")  # decks whose .grt is kept as its symmetric part (linear rotor)
 
 
 GROUP = 10  # consecutive blocks averaged before the spread is taken (slow modes)
@@ -40,9 +43,9 @@ def symmetric_part(v):
     return 0.5 * (v[..., :49] + v[..., 48::-1])
 
 
-def block_values(d, prefix, ext, blocks):
+def block_values(d, prefix, ext, blocks, symmetric=True):
     """Column 1 of the block files <prefix><block><ext>: [blocks, bins], and the bin centres.
-    For .grt, the symmetric part of every block."""
+    For .grt, the symmetric part of every block when `symmetric`."""
     vals = []
     for b in blocks:
         path = os.path.join(d, f"{prefix}{b:03d}{ext}")
@@ -55,7 +58,7 @@ def block_values(d, prefix, ext, blocks):
         vals.append(a[:, 1])
         x = a[:, 0]
     v = np.array(vals)
-    if ext == ".grt":
+    if ext == ".grt" and symmetric:
         v = symmetric_part(v)
         x = x[:49]
     return x, v
@@ -82,10 +85,12 @@ def run(name, chains=1):
     prefix = PREFIX[name]
     with tempfile.TemporaryDirectory() as top:
         dirs, procs = [], []
+        shared = os.path.join(top, "tables")  # tables too large to write per chain
+        decks.materialise(name, os.path.join(top, "first"), shared=shared)
         t0 = time.time()
         for c in range(chains):
             d = os.path.join(top, f"chain{c}")
-            decks.materialise(name, d)
+            decks.materialise(name, d, shared=shared)
             dirs.append(d)
             log = open(os.path.join(d, "log.txt"), "w")
             procs.append((subprocess.Popen([exe], cwd=d, env=env, stdout=log, stderr=subprocess.STDOUT), log))
@@ -96,7 +101,8 @@ def run(name, chains=1):
         wall = time.time() - t0
         engs = [decks.read_eng(os.path.join(d, prefix + ".eng")) for d in dirs]
         measured = [int(b) for b in engs[0][:, 0]]
-        dist = {ext: block_stats([block_values(d, prefix, ext, measured) for d in dirs]) for ext in DISTRIBUTIONS[name]}
+        dist = {ext: block_stats([block_values(d, prefix, ext, measured, name in SYMMETRIC) for d in dirs])
+                for ext in DISTRIBUTIONS[name]}
     out = dict(deck=decks.DECKS[name].format(blocks=decks.BLOCKS[name]), columns=["block", *decks.ENG_COLUMNS],
                eng=engs[0].tolist(), distributions=dist, wall_s=round(wall, 1),
                program="oracle/_ref/pimc, OMP_NUM_THREADS=1")
@@ -105,7 +111,7 @@ def run(name, chains=1):
     return out
 
 
-CHAINS = {"c0_8he": 8}
+CHAINS = {"c0_8he": 8, "h2o_ph2": 8}
 
 
 def main(names):

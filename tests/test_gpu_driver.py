@@ -22,10 +22,12 @@ DRIVER = os.path.join(ROOT, "moribs-pimc_amd", "pimc")
 with open(os.path.join(ROOT, "tests", "golden", "reference_runs.json")) as f:
     REF = json.load(f)
 
-GPU_BLOCKS = {"co2_100K": 45, "he4_co2": 130, "c0_8he": 60, "he16_pbc": 130}
+GPU_BLOCKS = {"co2_100K": 45, "he4_co2": 130, "c0_8he": 60, "he16_pbc": 130, "h2o_ph2": 40}
 CHAINS = 8
-PREFIX = {"co2_100K": "CO2_monomer", "he4_co2": "he4co2", "c0_8he": "c0he8", "he16_pbc": "he16pbc"}
-NEQ = {"co2_100K": 5, "he4_co2": 10, "c0_8he": 10, "he16_pbc": 10}
+PREFIX = {"co2_100K": "CO2_monomer", "he4_co2": "he4co2", "c0_8he": "c0he8", "he16_pbc": "he16pbc",
+          "h2o_ph2": "h2oph2"}
+NEQ = {"co2_100K": 5, "he4_co2": 10, "c0_8he": 10, "he16_pbc": 10, "h2o_ph2": 10}
+SYMMETRIC = ("he4_co2", "c0_8he")  # linear-rotor decks: .grt compared through its symmetric part
 
 
 def mean_se(chains):
@@ -41,20 +43,29 @@ def mean_se(chains):
     return x.mean(), se
 
 
+def run_driver(name, d):
+    """The driver on deck `name` in directory d with CHAINS independent chains: (d, [eng per chain])."""
+    decks.materialise(name, d, blocks=GPU_BLOCKS[name])
+    r = subprocess.run([DRIVER, "--replicas", str(CHAINS)], cwd=d, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    engs = []
+    for c in range(CHAINS):
+        engs.append(decks.read_eng(os.path.join(d, PREFIX[name] + (f"_r{c}" if c else "") + ".eng")))
+    return d, engs
+
+
 @pytest.fixture(scope="module")
 def driver_runs(tmp_path_factory):
     """The driver on each deck with CHAINS independent chains: {name: (directory, [eng per chain])}."""
-    out = {}
-    for name in ("co2_100K", "he4_co2", "c0_8he", "he16_pbc"):
-        d = str(tmp_path_factory.mktemp(name))
-        decks.materialise(name, d, blocks=GPU_BLOCKS[name])
-        r = subprocess.run([DRIVER, "--replicas", str(CHAINS)], cwd=d, capture_output=True, text=True, timeout=600)
-        assert r.returncode == 0, r.stderr[-2000:]
-        engs = []
-        for c in range(CHAINS):
-            engs.append(decks.read_eng(os.path.join(d, PREFIX[name] + (f"_r{c}" if c else "") + ".eng")))
-        out[name] = (d, engs)
-    return out
+    return {name: run_driver(name, str(tmp_path_factory.mktemp(name)))
+            for name in ("co2_100K", "he4_co2", "c0_8he", "he16_pbc")}
+
+
+@pytest.fixture(scope="module")
+def nl_runs(tmp_path_factory):
+    """The driver on the rotating-H2O deck (its density tables, 1.4 GB of text, are written here
+    and parsed by the driver first): {"h2o_ph2": (directory, [eng per chain])}."""
+    return {"h2o_ph2": run_driver("h2o_ph2", str(tmp_path_factory.mktemp("h2o_ph2")))}
 
 
 def test_atoms_only_deck_writes_the_reference_files(driver_runs):
@@ -70,7 +81,17 @@ def test_atoms_only_deck_writes_the_reference_files(driver_runs):
 
 @pytest.mark.parametrize("name", ["co2_100K", "he4_co2", "c0_8he", "he16_pbc"])
 def test_driver_matches_reference_statistics(name, driver_runs):
-    d, engs = driver_runs[name]
+    compare_energies(name, *driver_runs[name])
+
+
+@pytest.mark.timeout(900)
+def test_nonlinear_driver_matches_reference_statistics(nl_runs):
+    """4 para-H2 around a rotating H2O (translations of both, rotor moves of MCRot3Dstep,
+    GetRotE3D): K, V, E_rot and E against the reference program's eight chains."""
+    compare_energies("h2o_ph2", *nl_runs["h2o_ph2"])
+
+
+def compare_energies(name, d, engs):
     neq = NEQ[name]
     for e in engs:
         assert e.shape == (GPU_BLOCKS[name] - neq, 10)
@@ -131,11 +152,24 @@ def test_distributions_match_reference(name, ext, driver_runs):
     only even Legendre terms, so the exact density is symmetric; its antisymmetric part is a
     slow mode (He crossing from one end of the rotor to the other) that neither run converges,
     and the two runs show it with opposite signs."""
-    d, engs = driver_runs[name]
+    compare_distribution(name, ext, driver_runs[name][0])
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("ext", [".gra", ".gri", ".grt", ".grc", ".rcf"])
+def test_nonlinear_distributions_match_reference(ext, nl_runs):
+    """Around the rotating H2O: H2-H2 g(r), the H2 density in r, theta and chi of the rotor's
+    frame (SaveRho1D mc_estim.cc:1233-1286) and the correlation of the rotor's axis, by the
+    criterion of test_distributions_match_reference (.grt whole: the potential holds cos theta)."""
+    compare_distribution("h2o_ph2", ext, nl_runs["h2o_ph2"][0])
+
+
+def compare_distribution(name, ext, d):
+    """Per-bin z of the driver's block files against the fixture's, and the acceptance criterion."""
     neq = NEQ[name]
     ref = REF[name]["distributions"][ext]
     v = block_columns(d, PREFIX[name], ext, range(neq + 1, GPU_BLOCKS[name] + 1))
-    if ext == ".grt":  # the symmetric part of every block
+    if ext == ".grt" and name in SYMMETRIC:  # the symmetric part of every block
         v = 0.5 * (v[..., :49] + v[..., 48::-1])
     mg, sg = grouped_mean_se(v, ref["group"])
     mr, sr = np.array(ref["mean"]), np.array(ref["se"])
