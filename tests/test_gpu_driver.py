@@ -65,6 +65,8 @@ def driver_runs(tmp_path_factory):
 def nl_runs(tmp_path_factory):
     """The driver on the rotating-H2O deck (its density tables, 1.4 GB of text, are written here
     and parsed by the driver first): {"h2o_ph2": (directory, [eng per chain])}."""
+    if "h2o_ph2" not in REF:
+        pytest.skip("reference_runs.json holds no runs of the rotating-H2O deck (make_reference_runs.py h2o_ph2)")
     return {"h2o_ph2": run_driver("h2o_ph2", str(tmp_path_factory.mktemp("h2o_ph2")))}
 
 
