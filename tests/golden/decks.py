@@ -9,11 +9,14 @@ tests of the host driver (which run moribs-pimc_amd/pimc on the same files).
   c0_8he       the 8-He workload: 8 He + CO2 at 0.37 K, P = Prot = 128, the same tables
   he16_pbc     16 He at liquid density (0.0218 Angstrom^-3, a periodic box of 9 Angstrom) under
                MINIMAGE at 2 K, P = 16, LJ He-He
-  h2o_ph2      4 para-H2 + one rotating H2O at 0.5 K, P = Prot = 16, blocks of 500 passes (both
-               programs rewrite a 1.5-million-line 3D density file after every block): the H2-H2 table of
-               pimc_tables.h2h2_table, the stand-in 3D potential and density tables of tables3d
-               (H2Opot.pot, H2O_T0.5t16.rho/.eng/.esq, 1.4 GB of text written once per directory
-               or shared between directories)
+  h2o_ph2      2 para-H2 + one rotating H2O at 1 K, P = Prot = 16, blocks of 500 passes (both
+               programs rewrite a 1.5-million-line 3D density file after every block): the H2-H2
+               table of pimc_tables.h2h2_table, the stand-in 3D potential and density tables of
+               tables3d (H2Opot.pot, H2O_T1t16.rho/.eng/.esq, 1.4 GB of text written once per
+               directory or shared between directories).  With 4 para-H2 at 0.5 K each chain of
+               the reference program stayed in its own configuration for 60,000 passes (K per
+               chain from 33 to 52 K); with 2 at 1 K chains started apart agree within their
+               blocking errors.
 """
 import os
 import shutil
@@ -94,8 +97,8 @@ OUTPUTDIR        ./
 FILENAMEPREFIX   h2oph2
 DIMENSION     3
 DENSITY       0.02
-TEMPERATURE   0.5
-ATOM       H2   4    BOLTZMANN     0.30     2         pH2      PRIMITIVE
+TEMPERATURE   1.0
+ATOM       H2   2    BOLTZMANN     0.30     2         pH2      PRIMITIVE
 NONLINEAR  H2O  1    BOLTZMANN     0.05     2         H2Opot   PRIMITIVE
 ROTATION   H2O   0.3  16
 NUMBEROFSLICES     16
@@ -107,7 +110,7 @@ MCSKIP_AVERG     16
 """,
 }
 BLOCKS = {"co2_100K": 205, "he4_co2": 1010, "c0_8he": 310, "he16_pbc": 1010, "h2o_ph2": 130}
-ROT3D_STEM = "H2O_T0.5t16"  # init_rot3D's file stem for H2O at 0.5 K with 16 rotor slices
+ROT3D_STEM = "H2O_T1t16"  # init_rot3D's file stem for H2O at 1 K with 16 rotor slices
 
 
 def write_nl_tables(d, rotation):

@@ -88,7 +88,7 @@ def test_driver_matches_reference_statistics(name, driver_runs):
 
 @pytest.mark.timeout(900)
 def test_nonlinear_driver_matches_reference_statistics(nl_runs):
-    """4 para-H2 around a rotating H2O (translations of both, rotor moves of MCRot3Dstep,
+    """2 para-H2 around a rotating H2O (translations of both, rotor moves of MCRot3Dstep,
     GetRotE3D): K, V, E_rot and E against the reference program's eight chains."""
     compare_energies("h2o_ph2", *nl_runs["h2o_ph2"])
 
