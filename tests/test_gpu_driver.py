@@ -189,6 +189,27 @@ def compare_distribution(name, ext, d):
     assert np.abs(z).max() < 4.5
 
 
+def test_chain_files_do_not_depend_on_the_other_chains(tmp_path):
+    """Each chain's block and accumulated files are written on a worker thread of their own:
+    chain 0 of a four-chain run writes, byte for byte, the files of a one-chain run (around a
+    non-linear rotor, the 1.5-million-line _sum.g3d included), and the other chains differ."""
+    out = {}
+    for n in (1, 4):
+        d = str(tmp_path / f"c{n}")
+        decks.materialise_nl(d, blocks=3)
+        r = subprocess.run([DRIVER, "--replicas", str(n)], cwd=d, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        out[n] = d
+    names = sorted(f for f in os.listdir(out[1]) if f.startswith("h2oh2"))
+    assert "h2oh2_sum.g3d" in names and "h2oh2.eng" in names and len(names) > 20
+    for f in names:
+        with open(os.path.join(out[1], f), "rb") as a, open(os.path.join(out[4], f), "rb") as b:
+            assert a.read() == b.read(), f
+    for c in (1, 2, 3):
+        with open(os.path.join(out[4], "h2oh2.eng")) as a, open(os.path.join(out[4], f"h2oh2_r{c}.eng")) as b:
+            assert a.read() != b.read()
+
+
 def test_driver_output_layouts(tmp_path):
     """.eng: block number and nine columns per measured block; _sum.eng: one line per
     MCSKIP_TOTAL steps; .xyz: natoms*P bead lines; the accept-ratio line on stdout."""
